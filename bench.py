@@ -1,0 +1,292 @@
+#!/usr/bin/env python3
+"""Benchmark of the MCPNet hot path on one MI355X.
+
+Workload (default): 20000 genes x 2000 samples of the seeded GRN generator.  One step is the
+pipeline the reference runs in mcpnet (apps/pipeline/mcpnet.cpp:286-515): all-pairs B-spline MI
+(10 bins, order 3) with split-BF16 MFMA contraction, the MI diagonal set to 0, then MCP3 =
+ratio(MI, MI (x) (MI (x) MI)) with two max-min products.  Inputs are resident in HBM before the
+timed region; stage times come from HIP events on the library's own stream, per-kernel launch
+durations from the library's event bracketing of each launch.
+
+Prints ONE JSON line.  value = MI gene-pairs/s of the MI stage; mcp3_maxmin_gops = 1.6e13
+algorithmic max-min ops / MCP3 stage time (BASELINE's unit: one (min, max) per (i, k, j) triple,
+M*N*K per product).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+# MI355X peaks (vendor figures): HBM3E 8 TB/s; dense MFMA BF16 2.5 PFLOP/s, FP32 157.3 TFLOP/s.
+PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_FP32_TFLOPS = 157.3
+# VALU issue: 256 CUs x 64 lanes x 2.4 GHz; the max-min inner loop spends 1.5 instructions per
+# (i, k, j) triple (v_min_i32 per triple + v_max3_i32 per two triples).
+CUS, LANES, CLOCK_GHZ = 256, 64, 2.4
+PEAK_MAXMIN_GOPS = CUS * LANES * CLOCK_GHZ / 1.5
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def omp_threads():
+    try:
+        return int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    except ValueError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baselines(X, S, bins, mi_host, budget_s):
+    """Bounded CPU samples: the FP32 B-spline MI port (reference inner product) and the
+    reference's own max_of_pairmin (oracle/_ref) on the same data."""
+    import oracle
+
+    G = X.shape[0]
+    out = {}
+    # ---- B-spline MI: rows [0, R) against genes [0, C) of the same workload (same S, bins)
+    C = min(G, 2000)
+    W = oracle.bspline_weights_f32(oracle.minmax_scale_f32(X[:C]), bins=bins, order=3)
+    rows = 8
+    t0 = time.perf_counter()
+    oracle.port_bspline_mi_rows_f32(W, S, bins, 0, rows)
+    t_probe = time.perf_counter() - t0
+    rows = int(min(C, max(8, rows * (0.5 * budget_s) / max(t_probe, 1e-3))))
+    t0 = time.perf_counter()
+    oracle.port_bspline_mi_rows_f32(W, S, bins, 0, rows)
+    t = time.perf_counter() - t0
+    out["mi"] = {
+        "value": rows * C / t, "unit": "gene-pairs/s", "cores": omp_threads(), "kind": "port",
+        "sample": f"FP32 B-spline MI inner product (100 dots of S={S} + log2 per pair, OpenMP) on "
+                  f"{rows} x {C} gene pairs of the same data, {t:.1f} s; host {cpu_model()}",
+    }
+    # ---- max-min: rows [0, R) of MI (x) MI with the reference's max_of_pairmin
+    if mi_host is not None and oracle.ref_lib() is not None:
+        A = mi_host
+        N = A.shape[0]
+        r = 4
+        t0 = time.perf_counter()
+        oracle.ref_maxmin(A[:r], A)
+        t_probe = time.perf_counter() - t0
+        r = int(min(N, max(4, r * (0.5 * budget_s) / max(t_probe, 1e-3))))
+        t0 = time.perf_counter()
+        oracle.ref_maxmin(A[:r], A)
+        t = time.perf_counter() - t0
+        out["maxmin"] = {
+            "value": r * N * N / t / 1e9, "unit": "Gop/s", "cores": omp_threads(), "kind": "reference",
+            "sample": f"reference max_of_pairmin (simd_maxmin.hpp, float = scalar path) over rows 0..{r - 1} "
+                      f"of MI (x) MI, N = K = {N}, {t:.1f} s",
+        }
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--genes", type=int, default=20000)
+    ap.add_argument("--samples", type=int, default=2000)
+    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16", "bf16x3"])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of each bounded CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 MI comparison pass")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(see DESIGN.md for how the work would shard); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import mcpnet_amd
+    from mcpnet_amd import grn
+
+    G, S, bins = args.genes, args.samples, 10
+    cid = 3 if (G, S) == (20000, 2000) else 0
+    seed = grn.config_seed(cid if cid else 3)
+    log(f"bench: generating {G} x {S} GRN expression (seed {seed:#x})")
+    t0 = time.perf_counter()
+    X = grn.grn_expression(G, S, seed)[0]
+    log(f"bench: data ready in {time.perf_counter() - t0:.1f} s")
+
+    dev = torch.device("cuda:0")
+    h = mcpnet_amd.Handle(0)
+    h.set_profiling(True)
+    stream = torch.cuda.ExternalStream(h.stream)
+    Xd = torch.from_numpy(X).to(dev)
+    MI = torch.empty((G, G), dtype=torch.float32, device=dev)
+    MCP = torch.empty((G, G), dtype=torch.float32, device=dev)
+    MM = torch.empty((G, G), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    kern = {}
+
+    def step(record):
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(stream)
+        h.bspline_mi(Xd, bins=bins, precision=args.precision, out=MI)
+        e1.record(stream)
+        if record:
+            for name, ms in h.last_timings():
+                kern.setdefault(name, []).append(ms)
+        MI.fill_diagonal_(0.0)   # mcpnet.cpp:299-308, default diagonal 0
+        h.mcp(MI, 3, out=MCP, maxmin_out=MM)
+        e2.record(stream)
+        if record:
+            for name, ms in h.last_timings():
+                kern.setdefault(name, []).append(ms)
+        return e0, e1, e2
+
+    for i in range(args.warmup):
+        log(f"bench: warmup step {i + 1}/{args.warmup}")
+        step(False)
+    torch.cuda.synchronize()
+    h.synchronize()
+    events = []
+    t_wall = time.perf_counter()
+    for i in range(args.steps):
+        events.append(step(True))
+        log(f"bench: step {i + 1}/{args.steps} done")
+    torch.cuda.synchronize()
+    h.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    t_mi = np.mean([a.elapsed_time(b) for a, b, _ in events]) / 1e3
+    t_mcp = np.mean([b.elapsed_time(c) for _, b, c in events]) / 1e3
+    t_step = np.mean([a.elapsed_time(c) for a, _, c in events]) / 1e3
+
+    P = G * (G - 1) / 2
+    mi_rate = P / t_mi
+    mcp3_ops = 2.0 * G * G * G
+    mcp3_gops = mcp3_ops / t_mcp / 1e9
+
+    # ---- per-kernel rooflines (algorithmic work per launch / mean launch duration)
+    def mean_ms(name):
+        v = kern.get(name)
+        return float(np.mean(v)) if v else None
+
+    mi_name = f"bspline_mi_kernel<{args.precision}>"
+    mi_ms = mean_ms(mi_name)
+    mi_flops = 2.0 * bins * bins * S * P          # 2*B^2*S per unordered pair
+    peak_mi = PEAK_FP32_TFLOPS if args.precision == "fp32" else PEAK_BF16_TFLOPS
+    roof = {}
+    if mi_ms:
+        ach = mi_flops / (mi_ms / 1e3) / 1e12
+        roof[mi_name] = {"bound": "mfma", "achieved": ach, "peak": peak_mi, "unit": "TFLOP/s", "frac": ach / peak_mi,
+                         "traffic": None, "mean_ms": mi_ms, "work_per_launch": mi_flops,
+                         "work_unit": "algorithmic flops 2*B^2*S per unordered gene pair (split products not credited)"}
+    for name, tri in (("maxmin_tile_kernel<f32,tri>", True), ("maxmin_tile_kernel<f32>", False)):
+        ms = mean_ms(name)
+        if not ms:
+            continue
+        work = (G * (G + 1) / 2 if tri else G * G) * G
+        ach = work / (ms / 1e3) / 1e9
+        roof[name] = {"bound": "valu", "achieved": ach, "peak": PEAK_MAXMIN_GOPS, "unit": "Gop/s",
+                      "frac": ach / PEAK_MAXMIN_GOPS, "traffic": None, "mean_ms": ms, "work_per_launch": work,
+                      "work_unit": "(min,max) triples over the output the launch must produce"
+                                   + (" (upper triangle, mirrored)" if tri else "")}
+    # dominant kernel = largest time per step
+    per_step = {n: float(np.sum(v)) / args.steps for n, v in kern.items()}
+    dominant = max(per_step, key=per_step.get) if per_step else None
+    traffic_file = ROOT / "profiles" / "traffic.json"
+    if traffic_file.exists():
+        try:
+            tr = json.loads(traffic_file.read_text())
+            for n in roof:
+                if n in tr:
+                    roof[n]["traffic"] = tr[n]
+        except (OSError, ValueError):
+            pass
+
+    # ---- FP32 MI pass for the BF16-split / FP32 speed ratio and accuracy on sampled pairs
+    extra = {}
+    if not args.no_fp32 and args.precision != "fp32":
+        MI32 = torch.empty((G, G), dtype=torch.float32, device=dev)
+        h.bspline_mi(Xd, bins=bins, precision="fp32", out=MI32)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        h.bspline_mi(Xd, bins=bins, precision="fp32", out=MI32)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t32 = e0.elapsed_time(e1) / 1e3
+        extra["mi_fp32_pairs_per_s"] = P / t32
+        extra["mi_speedup_vs_fp32"] = t32 / t_mi
+        del MI32
+
+    cpu = {}
+    if not args.no_cpu:
+        # CPU leg: the oracle is the checker (MI error vs FP64 on sampled pairs) and the timed
+        # CPU baselines; nothing here feeds the GPU numbers above.
+        import oracle
+
+        log("bench: accuracy check against the FP64 oracle")
+        rng = np.random.default_rng(7)
+        genes = np.sort(rng.choice(G, min(G, 200), replace=False))
+        li, lj = np.triu_indices(len(genes), k=1)
+        h.bspline_mi(Xd, bins=bins, precision=args.precision, out=MI)
+        got = MI[torch.from_numpy(genes[li]).long().to(dev), torch.from_numpy(genes[lj]).long().to(dev)].cpu().numpy()
+        ref = oracle.bspline_mi_pairs_f64(X[genes], li.astype(np.int32), lj.astype(np.int32))
+        err = np.abs(got.astype(np.float64) - ref)
+        extra["mi_err_vs_fp64"] = {"max": float(err.max()), "mean": float(err.mean()),
+                                   "pairs": int(len(li)), "sample": "all pairs of 200 seeded random genes"}
+        log("bench: CPU baselines")
+        MI.fill_diagonal_(0.0)
+        cpu = cpu_baselines(X, S, bins, MI.cpu().numpy(), args.cpu_seconds)
+
+    result = {
+        "metric": "MI gene-pairs/s (split-BF16 B-spline MI) & MCP3 max-min Gop/s @20k genes",
+        "value": mi_rate,
+        "unit": "gene-pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{args.precision} MFMA (fp32 accumulate) MI; f32 max-min",
+        "data": "synthetic: seeded linear-Gaussian GRN (3 parents/gene), exp, per-gene min-max",
+        "config": {"workload": f"{G} genes x {S} samples: B-spline MI (10 bins, order 3, {args.precision}) "
+                               f"-> diagonal 0 -> MCP3 (two max-min products + ratio)",
+                   "genes": G, "samples": S, "bins": bins, "spline_order": 3, "parallelism": "single GPU"},
+        "mcp3_maxmin_gops": mcp3_gops,
+        "stage_ms": {"mi": t_mi * 1e3, "mcp3": t_mcp * 1e3},
+        "wall_s_timed_region": t_wall,
+        "roofline": dict(roof.get(dominant, {}), kernel=dominant) if dominant in roof else None,
+        "roofline_kernels": roof,
+        "kernel_ms_per_step": per_step,
+        "cpu_baseline": cpu.get("mi"),
+        "cpu_baseline_maxmin": cpu.get("maxmin"),
+        **extra,
+    }
+    if cpu.get("mi"):
+        result["speedup_vs_cpu_mi"] = mi_rate / cpu["mi"]["value"]
+    if cpu.get("maxmin"):
+        result["speedup_vs_cpu_maxmin"] = mcp3_gops / cpu["maxmin"]["value"]
+    print(json.dumps(result))
+    h.close()
+
+
+if __name__ == "__main__":
+    main()

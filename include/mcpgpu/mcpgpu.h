@@ -1,0 +1,183 @@
+/*
+ * mcpgpu.h -- C ABI of the MI355X (gfx950) implementation of MCPNet's two hot paths:
+ * all-pairs mutual information (B-spline and adaptive partitioning) and the max-min
+ * ("maximum capacity path") matrix products behind the MCP2/3/4 scores.
+ *
+ * Every entry point replaces one splash pattern call of the reference, specialised on the
+ * reference's kernel functor; the comment above each declaration names the reference code
+ * (path:line, relative to the MCPNet source tree) whose result it reproduces.
+ *
+ * Conventions
+ *   - Matrices are row-major, rows = genes; leading dimensions (ld*) are in ELEMENTS.
+ *   - `ptrs` says where the matrix pointers live: MCPG_HOST (pageable or pinned host memory;
+ *     the call copies in, computes and copies out before returning) or MCPG_DEVICE
+ *     (device memory on the handle's device; work is ordered on the handle's stream and the
+ *     call returns after the stream has drained, so errors are reported synchronously).
+ *   - Return value: MCPG_OK (0) or a negative mcpg_status; nothing is thrown across the ABI.
+ *     mcpg_last_error() returns a message for the last failing call on a handle.
+ *   - A handle owns a device, a HIP stream and a grow-only device workspace.  A handle is
+ *     not thread-safe: use one per host thread (the reference copies its kernel functors
+ *     per thread for the same reason).
+ */
+#ifndef MCPGPU_H_
+#define MCPGPU_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mcpg_ctx* mcpg_handle;
+
+enum mcpg_status {
+  MCPG_OK = 0,
+  MCPG_ERR_INVALID_ARG = -1,   /* null pointer, bad enum, bins < order, ld too small ... */
+  MCPG_ERR_SHAPE = -2,         /* dimension mismatch (the reference asserts, BSplineMI.hpp:332, :493) */
+  MCPG_ERR_NONFINITE = -3,     /* NaN/Inf in an input that must be finite */
+  MCPG_ERR_OUT_OF_MEMORY = -4, /* device allocation failed */
+  MCPG_ERR_HIP = -5,           /* any other HIP runtime error (text in mcpg_last_error) */
+  MCPG_ERR_NO_DEVICE = -6,     /* no HIP device / bad device ordinal */
+  MCPG_ERR_UNSUPPORTED = -7    /* size outside what the kernels support (e.g. S too large) */
+};
+
+enum mcpg_ptr_kind { MCPG_HOST = 0, MCPG_DEVICE = 1 };
+
+/* Arithmetic of the B-spline joint-table contraction. */
+enum mcpg_precision {
+  MCPG_FP32 = 0,   /* FP32 weights, FP32 MFMA (v_mfma_f32_16x16x4_f32) */
+  MCPG_BF16 = 1,   /* BF16 weights, BF16 MFMA, FP32 accumulation */
+  MCPG_BF16X3 = 2  /* hi+lo BF16 split, 3 MFMA products (hi*hi + hi*lo + lo*hi), FP32 accumulation */
+};
+
+/* ------------------------------------------------------------------ handle */
+int mcpg_create(mcpg_handle* h, int device);
+int mcpg_destroy(mcpg_handle h);
+const char* mcpg_status_string(int status);
+const char* mcpg_last_error(mcpg_handle h);
+int mcpg_synchronize(mcpg_handle h);
+/* The hipStream_t the handle launches on (returned as void*), for callers that time or
+ * order work around the library's kernels. */
+int mcpg_get_stream(mcpg_handle h, void** stream);
+/* Number of kernel launches the most recent call made and, for the dominant kernel of that
+ * call, its name (a static string) -- used by profiling harnesses. */
+int mcpg_last_launch_info(mcpg_handle h, int* launches, const char** dominant_kernel);
+/* Profiling (no counterpart in the reference, whose stage timers are host wall clocks,
+ * mcpnet.cpp:289-296): when enabled, every kernel launch of a call is bracketed by HIP events on
+ * the handle's stream; mcpg_last_timings returns the kernel names (static strings) and durations
+ * (ms) of the most recent call's launches, up to max_entries, and their total count. */
+int mcpg_set_profiling(mcpg_handle h, int enable);
+int mcpg_last_timings(mcpg_handle h, int max_entries, const char** names, float* ms, int* count);
+
+/* ------------------------------------------------------------------ B-spline MI */
+
+/* MinMaxScale<float> row transform: y = (x - min) * (1/(max-min)), a constant row is copied.
+ * Replaces GlobalTransform<..., mcp::kernel::MinMaxScale<T>> (include/mcp/correlation/BSplineMI.hpp:75-116;
+ * called at apps/pipeline/mcpnet_helper.hpp:60-64, apps/correlation/mi.cpp:198-204). */
+int mcpg_minmax_scale_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx,
+                          float* Y, int64_t ldy);
+
+/* BSplineWeightsKernel<float> row transform on already-scaled rows: W[g][b*S+s] = B_{b,order}(x_gs)
+ * (bin-major, the reference layout) and W[g][bins*S] = marginal entropy H_g (computed in FP64,
+ * stored as float).  ldw >= bins*S+1.
+ * Replaces Transform<..., mcp::kernel::BSplineWeightsKernel<T>> (BSplineMI.hpp:119-274;
+ * mcpnet_helper.hpp:70-80, mi.cpp:214-222). */
+int mcpg_bspline_weights_f32(mcpg_handle h, int ptrs, const float* Xs, int64_t G, int64_t S, int64_t ldx,
+                             int bins, int order, float* W, int64_t ldw);
+
+/* All-pairs B-spline MI from reference-layout weight rows (the output of
+ * mcpg_bspline_weights_f32): MI[i][j] = W[i][bins*S] + W[j][bins*S] - H(i,j) for all i, j,
+ * with H(i,j) = -sum_{b,c} p log2 p, p = (1/S) <W_i,b , W_j,c>.  The output is exactly symmetric.
+ * Replaces InnerProduct<..., mcp::kernel::BSplineMIKernel<T>, ..., true> (BSplineMI.hpp:276-336,
+ * :492-499; mcpnet_helper.hpp:90-98; the commented-out call at mi.cpp:251). */
+int mcpg_bspline_mi_weights_f32(mcpg_handle h, int ptrs, const float* W, int64_t G, int64_t S, int64_t ldw,
+                                int bins, int precision, float* MI, int64_t ldmi);
+
+/* Fused compute_mi (apps/pipeline/mcpnet_helper.hpp:56-99): min-max scale, B-spline weights with
+ * FP64 marginal entropies, all-pairs MI with the chosen contraction precision; the G x G MI
+ * matrix is exactly symmetric.  Weight panels never leave the device. */
+int mcpg_bspline_mi_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx,
+                        int bins, int order, int precision, float* MI, int64_t ldmi);
+
+/* Adjoint of the B-spline MI (no counterpart in the reference, which has no gradients):
+ * for L = sum_n m[n] * MI(pi[n], pj[n]) (pairs with pi != pj; the MI is the MCPG_FP32 forward),
+ * writes dX[g][s] = dL/dX[g][s] through the min-max scaling (sub-gradient at the first arg-min /
+ * arg-max), the B-spline basis derivative and the FP32 joint tables, which are recomputed from
+ * X rather than stored.  pair arrays are in host memory; X and dX follow `ptrs`.
+ * If L_out is non-null it receives L (FP64 accumulation of the FP32 forward). */
+int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx,
+                                 int bins, int order, const int32_t* pi, const int32_t* pj, const float* m,
+                                 int64_t npairs, float* dX, int64_t lddx, double* L_out);
+
+/* ------------------------------------------------------------------ adaptive-partitioning MI */
+
+/* add_noise: out = in + U[0, 1e-5) jitter from a counter-based SplitMix64 stream
+ * (element (g,s) draws counter g*S+s of stream `seed`).  Replaces add_noise
+ * (include/mcp/correlation/AdaptivePartitioningMI.hpp:40-114; mi.cpp:125-132), whose random
+ * stream depends on the thread count and is not reproduced. */
+int mcpg_add_noise_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx,
+                       uint64_t seed, float* Y, int64_t ldy);
+
+/* Descending rank per row (largest value -> rank 0), ties broken by ascending column index.
+ * Replaces Transform<..., splash::kernel::Rank<T,size_t,0,false>> (mi.cpp:141-147,
+ * mcpnet_helper.hpp:127-133). */
+int mcpg_rank_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx,
+                  int32_t* R, int64_t ldr);
+
+/* IntegralCumulativeHistogramKernel<int,0>: H[g][i] = #{s : R[g][s] < i}, i = 0..S (ldh >= S+1).
+ * (AdaptivePartitioningMI.hpp:414-442; mi.cpp:156-162). */
+int mcpg_cumhist_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,
+                     int32_t* H, int64_t ldh);
+
+/* All-pairs adaptive-partitioning MI on rank rows (values in [0,S)): replaces
+ * InnerProduct<..., AdaptivePartitionRankMIKernel2<IT,T,0>, ..., true> (AdaptivePartitioningMI.hpp:445-647;
+ * mi.cpp:170-177, mcpnet_helper.hpp:156-162).  MI[i][j] for i <= j is computed as kernel(x=i, y=j)
+ * and mirrored.  Integer cell counts follow the reference exactly; the per-pair sum is FP64 in
+ * the reference's LIFO window order. */
+int mcpg_ap_mi_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,
+                   float* MI, int64_t ldmi);
+
+/* Debug/parity view of the same kernel: for each requested pair (px[n], py[n]) the leaf windows
+ * in the order their terms enter the FP64 sum, as (nxy, nx, ny) int32 triples, at most
+ * max_leaves per pair (cells[n*max_leaves*3 + 3*l + {0,1,2}]); nleaves[n] gets the true count
+ * (which may exceed max_leaves) and mi[n] the pair's MI.  Pair arrays and outputs are host memory. */
+int mcpg_ap_mi_cells_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,
+                         const int32_t* px, const int32_t* py, int64_t npairs, int32_t max_leaves,
+                         int32_t* cells, int32_t* nleaves, double* mi);
+
+/* ------------------------------------------------------------------ max-min (MCP) */
+
+/* C[i][j] = max_k min(A[i][k], B[j][k])  (row . row, like splash InnerProduct), i < M, j < N, k < K.
+ * kmask (nullable, device/host per `ptrs`, length K): MASKED kernel, the term becomes
+ * min(kmask[k], A[i][k], B[j][k]).  Empty K gives -inf (f32) / -DBL_MAX (f64) as in the reference.
+ * symmetric != 0 requires A == B (same pointer) and M == N and computes only j >= i, mirroring.
+ * Replaces InnerProduct<..., mcp::kernel::mcp2_maxmin_kernel<T,MASKED,true>, ...>
+ * (include/mcp/filter/mcp.hpp:72-93 -> include/mcp/filter/simd_maxmin.hpp:30-46, :299-450).
+ * Exact: bit-identical to the reference for inputs without NaN or -0. */
+int mcpg_maxmin_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, const float* B, int64_t ldb,
+                    const float* kmask, int64_t M, int64_t N, int64_t K, int symmetric, float* C, int64_t ldc);
+int mcpg_maxmin_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, const double* B, int64_t ldb,
+                    const double* kmask, int64_t M, int64_t N, int64_t K, int symmetric, double* C, int64_t ldc);
+
+/* ratio_kernel<T,T> (include/mcp/transform/combine.hpp:210-222; compute_ratio mcpnet_helper.hpp:172-184):
+ * out = |a| < eps ? 0 : |m| < eps ? T_MAX : (T)((double)a / (double)m), eps = numeric_limits<T>::epsilon(). */
+int mcpg_ratio_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, const float* Mm, int64_t ldm,
+                   int64_t M, int64_t N, float* out, int64_t ldo);
+int mcpg_ratio_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, const double* Mm, int64_t ldm,
+                   int64_t M, int64_t N, double* out, int64_t ldo);
+
+/* compute_mcp{2,3,4} (apps/pipeline/mcpnet_helper.hpp:198-205, :270-277, :357-364) on an N x N
+ * matrix A (MI with its diagonal already set): k = 2: M = A(x)A; k = 3: M = A(x)(A(x)A);
+ * k = 4: M = (A(x)A)(x)(A(x)A); out = ratio(A, M).  maxmin_out (nullable) receives M (the
+ * reference's -x/--maxmin-output).  Intermediates stay on the device; the last product fuses
+ * the ratio.  Products whose result is exactly symmetric are computed on one triangle. */
+int mcpg_mcp_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N, int k,
+                 float* maxmin_out, int64_t ldm, float* mcp_out, int64_t ldo);
+int mcpg_mcp_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, int k,
+                 double* maxmin_out, int64_t ldm, double* mcp_out, int64_t ldo);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif  /* MCPGPU_H_ */

@@ -1,0 +1,381 @@
+"""ctypes binding of libmcpgpu.so (the C ABI declared in include/mcpgpu/mcpgpu.h).
+
+Arrays are accepted either as numpy arrays (host memory: the library copies in and out) or as
+torch tensors on a ROCm device (device memory: pointers are passed straight through).  There is
+no CPU fallback: if the library is not built, or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, byref, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_uint64, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libmcpgpu.so"
+
+MCPG_HOST, MCPG_DEVICE = 0, 1
+PRECISIONS = {"fp32": 0, "bf16": 1, "bf16x3": 2}
+STATUS = {
+    0: "ok", -1: "invalid argument", -2: "shape mismatch", -3: "non-finite input", -4: "out of device memory",
+    -5: "HIP runtime error", -6: "no usable HIP device", -7: "unsupported size",
+}
+
+
+class MCPGError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str = ""):
+        self.status = status
+        msg = f"{where}: {STATUS.get(status, 'unknown status')} ({status})"
+        if detail:
+            msg += f": {detail}"
+        super().__init__(msg)
+
+
+_lib = None
+
+# name -> argtypes (restype is c_int for every entry point except the string getters)
+_P = c_void_p
+_SIGS = {
+    "mcpg_create": [POINTER(c_void_p), c_int],
+    "mcpg_destroy": [_P],
+    "mcpg_synchronize": [_P],
+    "mcpg_get_stream": [_P, POINTER(c_void_p)],
+    "mcpg_last_launch_info": [_P, POINTER(c_int), POINTER(c_char_p)],
+    "mcpg_set_profiling": [_P, c_int],
+    "mcpg_last_timings": [_P, c_int, POINTER(c_char_p), POINTER(c_float), POINTER(c_int)],
+    "mcpg_minmax_scale_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_bspline_weights_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
+    "mcpg_bspline_mi_weights_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
+    "mcpg_bspline_mi_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, c_int, _P, c_int64],
+    "mcpg_bspline_mi_backward_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, _P, _P, c_int64,
+                                     _P, c_int64, POINTER(c_double)],
+    "mcpg_add_noise_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_uint64, _P, c_int64],
+    "mcpg_rank_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_cumhist_i32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_ap_mi_i32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_ap_mi_cells_i32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, _P, c_int64, c_int32, _P, _P, _P],
+    "mcpg_maxmin_f32": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, c_int, _P, c_int64],
+    "mcpg_maxmin_f64": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, c_int, _P, c_int64],
+    "mcpg_ratio_f32": [_P, c_int, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_ratio_f64": [_P, c_int, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_mcp_f32": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64, _P, c_int64],
+    "mcpg_mcp_f64": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64, _P, c_int64],
+}
+EXPORTS = ["mcpg_status_string", "mcpg_last_error", *_SIGS.keys()]
+
+
+def load() -> ctypes.CDLL:
+    """Loads libmcpgpu.so (built in-tree); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get("MCPG_LIBRARY", LIB_PATH))
+    if not path.exists():
+        raise RuntimeError(f"{path} is not built; run `python -m mcpnet_amd.build` (HIP toolchain required)")
+    lib = ctypes.CDLL(str(path))
+    for name, argtypes in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = c_int
+    lib.mcpg_status_string.argtypes = [c_int]
+    lib.mcpg_status_string.restype = c_char_p
+    lib.mcpg_last_error.argtypes = [c_void_p]
+    lib.mcpg_last_error.restype = c_char_p
+    _lib = lib
+    return lib
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+class _Mat:
+    """A 2-D (or 1-D) operand: pointer, leading dimension and where it lives."""
+
+    def __init__(self, x, dtype, name):
+        self.obj = x
+        if _is_torch(x):
+            import torch
+            want = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32}[dtype]
+            if x.dtype != want:
+                raise TypeError(f"{name}: expected {want}, got {x.dtype}")
+            if not x.is_cuda:
+                raise TypeError(f"{name}: torch tensors must be on the GPU (use numpy for host memory)")
+            if x.dim() == 2:
+                if x.stride(1) != 1:
+                    raise ValueError(f"{name}: rows must be contiguous")
+                self.ld = x.stride(0)
+            else:
+                if x.dim() != 1 or x.stride(0) != 1:
+                    raise ValueError(f"{name}: expected a contiguous vector")
+                self.ld = x.shape[0]
+            self.ptr = x.data_ptr()
+            self.kind = MCPG_DEVICE
+        else:
+            a = np.asarray(x)
+            if a.dtype != dtype:
+                raise TypeError(f"{name}: expected {np.dtype(dtype)}, got {a.dtype}")
+            if a.ndim == 2:
+                if a.strides[1] != a.itemsize:
+                    raise ValueError(f"{name}: rows must be contiguous")
+                self.ld = a.strides[0] // a.itemsize if a.shape[0] > 1 else a.shape[1]
+            else:
+                a = np.ascontiguousarray(a)
+                self.ld = a.shape[0] if a.ndim == 1 else 0
+            self.obj = a
+            self.ptr = a.ctypes.data
+            self.kind = MCPG_HOST
+        self.shape = tuple(self.obj.shape)
+
+
+def _empty_like_kind(kind, shape, dtype, ref):
+    if kind == MCPG_DEVICE:
+        import torch
+        tdt = {np.float32: torch.float32, np.float64: torch.float64, np.int32: torch.int32}[dtype]
+        return torch.empty(shape, dtype=tdt, device=ref.device)
+    return np.empty(shape, dtype=dtype)
+
+
+def _same_kind(*mats):
+    kinds = {m.kind for m in mats if m is not None}
+    if len(kinds) > 1:
+        raise TypeError("mixing host (numpy) and device (torch) operands in one call is not supported")
+    return kinds.pop()
+
+
+class Handle:
+    """One libmcpgpu handle (device, HIP stream, workspace).  Not thread-safe."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = c_void_p()
+        st = self.lib.mcpg_create(byref(h), int(device))
+        if st != 0:
+            raise MCPGError(st, "mcpg_create")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mcpg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------------ helpers
+    def _check(self, st: int, where: str):
+        if st != 0:
+            detail = self.lib.mcpg_last_error(self.h)
+            raise MCPGError(st, where, detail.decode() if detail else "")
+
+    def _pre(self, kind):
+        if kind == MCPG_DEVICE:
+            import torch
+            torch.cuda.current_stream().synchronize()
+
+    @property
+    def stream(self) -> int:
+        s = c_void_p()
+        self._check(self.lib.mcpg_get_stream(self.h, byref(s)), "mcpg_get_stream")
+        return s.value or 0
+
+    def last_launch_info(self):
+        n = c_int()
+        name = c_char_p()
+        self._check(self.lib.mcpg_last_launch_info(self.h, byref(n), byref(name)), "mcpg_last_launch_info")
+        return n.value, (name.value or b"").decode()
+
+    def set_profiling(self, enable: bool = True):
+        self._check(self.lib.mcpg_set_profiling(self.h, int(enable)), "mcpg_set_profiling")
+
+    def last_timings(self):
+        """[(kernel name, ms)] for the launches of the most recent call (profiling on)."""
+        n = c_int()
+        self._check(self.lib.mcpg_last_timings(self.h, 0, None, None, byref(n)), "mcpg_last_timings")
+        names = (c_char_p * max(n.value, 1))()
+        ms = (c_float * max(n.value, 1))()
+        self._check(self.lib.mcpg_last_timings(self.h, n.value, names, ms, byref(n)), "mcpg_last_timings")
+        return [(names[i].decode(), float(ms[i])) for i in range(n.value)]
+
+    # ------------------------------------------------------------------ B-spline MI
+    def minmax_scale(self, X, out=None):
+        x = _Mat(X, np.float32, "X")
+        G, S = x.shape
+        out = _empty_like_kind(x.kind, (G, S), np.float32, X) if out is None else out
+        y = _Mat(out, np.float32, "out")
+        self._pre(x.kind)
+        self._check(self.lib.mcpg_minmax_scale_f32(self.h, x.kind, x.ptr, G, S, x.ld, y.ptr, y.ld), "mcpg_minmax_scale_f32")
+        return out
+
+    def bspline_weights(self, Xs, bins=10, order=3, out=None):
+        x = _Mat(Xs, np.float32, "Xs")
+        G, S = x.shape
+        out = _empty_like_kind(x.kind, (G, bins * S + 1), np.float32, Xs) if out is None else out
+        w = _Mat(out, np.float32, "out")
+        self._pre(x.kind)
+        self._check(self.lib.mcpg_bspline_weights_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order, w.ptr, w.ld),
+                    "mcpg_bspline_weights_f32")
+        return out
+
+    def bspline_mi_weights(self, W, S, bins=10, precision="fp32", out=None):
+        w = _Mat(W, np.float32, "W")
+        G = w.shape[0]
+        out = _empty_like_kind(w.kind, (G, G), np.float32, W) if out is None else out
+        m = _Mat(out, np.float32, "out")
+        self._pre(w.kind)
+        self._check(self.lib.mcpg_bspline_mi_weights_f32(self.h, w.kind, w.ptr, G, S, w.ld, bins,
+                                                         PRECISIONS[precision], m.ptr, m.ld),
+                    "mcpg_bspline_mi_weights_f32")
+        return out
+
+    def bspline_mi(self, X, bins=10, order=3, precision="fp32", out=None):
+        x = _Mat(X, np.float32, "X")
+        G, S = x.shape
+        out = _empty_like_kind(x.kind, (G, G), np.float32, X) if out is None else out
+        m = _Mat(out, np.float32, "out")
+        _same_kind(x, m)
+        self._pre(x.kind)
+        self._check(self.lib.mcpg_bspline_mi_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order,
+                                                 PRECISIONS[precision], m.ptr, m.ld), "mcpg_bspline_mi_f32")
+        return out
+
+    def bspline_mi_backward(self, X, pi, pj, m, bins=10, order=3, out=None):
+        """dL/dX for L = sum_n m[n] * MI(pi[n], pj[n]); returns (dX, L)."""
+        x = _Mat(X, np.float32, "X")
+        G, S = x.shape
+        pi = np.ascontiguousarray(pi, dtype=np.int32)
+        pj = np.ascontiguousarray(pj, dtype=np.int32)
+        mm = np.ascontiguousarray(m, dtype=np.float32)
+        if not (pi.shape == pj.shape == mm.shape):
+            raise ValueError("pair arrays must have the same length")
+        out = _empty_like_kind(x.kind, (G, S), np.float32, X) if out is None else out
+        d = _Mat(out, np.float32, "out")
+        L = c_double()
+        self._pre(x.kind)
+        self._check(self.lib.mcpg_bspline_mi_backward_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order,
+                                                          pi.ctypes.data, pj.ctypes.data, mm.ctypes.data, len(pi),
+                                                          d.ptr, d.ld, byref(L)),
+                    "mcpg_bspline_mi_backward_f32")
+        return out, L.value
+
+    # ------------------------------------------------------------------ adaptive partitioning
+    def add_noise(self, X, seed, out=None):
+        x = _Mat(X, np.float32, "X")
+        G, S = x.shape
+        out = _empty_like_kind(x.kind, (G, S), np.float32, X) if out is None else out
+        y = _Mat(out, np.float32, "out")
+        self._pre(x.kind)
+        self._check(self.lib.mcpg_add_noise_f32(self.h, x.kind, x.ptr, G, S, x.ld, c_uint64(seed), y.ptr, y.ld),
+                    "mcpg_add_noise_f32")
+        return out
+
+    def rank(self, X, out=None):
+        x = _Mat(X, np.float32, "X")
+        G, S = x.shape
+        out = _empty_like_kind(x.kind, (G, S), np.int32, X) if out is None else out
+        r = _Mat(out, np.int32, "out")
+        self._pre(x.kind)
+        self._check(self.lib.mcpg_rank_f32(self.h, x.kind, x.ptr, G, S, x.ld, r.ptr, r.ld), "mcpg_rank_f32")
+        return out
+
+    def cumhist(self, R, out=None):
+        r = _Mat(R, np.int32, "R")
+        G, S = r.shape
+        out = _empty_like_kind(r.kind, (G, S + 1), np.int32, R) if out is None else out
+        hm = _Mat(out, np.int32, "out")
+        self._pre(r.kind)
+        self._check(self.lib.mcpg_cumhist_i32(self.h, r.kind, r.ptr, G, S, r.ld, hm.ptr, hm.ld), "mcpg_cumhist_i32")
+        return out
+
+    def ap_mi(self, R, out=None):
+        r = _Mat(R, np.int32, "R")
+        G, S = r.shape
+        out = _empty_like_kind(r.kind, (G, G), np.float32, R) if out is None else out
+        m = _Mat(out, np.float32, "out")
+        self._pre(r.kind)
+        self._check(self.lib.mcpg_ap_mi_i32(self.h, r.kind, r.ptr, G, S, r.ld, m.ptr, m.ld), "mcpg_ap_mi_i32")
+        return out
+
+    def ap_mi_cells(self, R, px, py, max_leaves=4096):
+        """Leaf (nxy, nx, ny) triples per requested pair in summation order; returns
+        (list of int32 arrays of shape (n_leaves, 3), float64 MI per pair)."""
+        r = _Mat(R, np.int32, "R")
+        G, S = r.shape
+        px = np.ascontiguousarray(px, dtype=np.int32)
+        py = np.ascontiguousarray(py, dtype=np.int32)
+        n = len(px)
+        cells = np.zeros((n, max_leaves, 3), dtype=np.int32)
+        nl = np.zeros(n, dtype=np.int32)
+        mi = np.zeros(n, dtype=np.float64)
+        self._pre(r.kind)
+        self._check(self.lib.mcpg_ap_mi_cells_i32(self.h, r.kind, r.ptr, G, S, r.ld, px.ctypes.data, py.ctypes.data, n,
+                                                  max_leaves, cells.ctypes.data, nl.ctypes.data, mi.ctypes.data),
+                    "mcpg_ap_mi_cells_i32")
+        if (nl > max_leaves).any():
+            raise ValueError(f"a pair has more than max_leaves={max_leaves} leaves")
+        return [cells[k, : nl[k]].copy() for k in range(n)], mi
+
+    # ------------------------------------------------------------------ max-min / MCP
+    def maxmin(self, A, B=None, kmask=None, symmetric=False, out=None):
+        dt = np.float64 if str(getattr(A, "dtype", "")).endswith("float64") else np.float32
+        a = _Mat(A, dt, "A")
+        b = a if B is None else _Mat(B, dt, "B")
+        km = None if kmask is None else _Mat(kmask, dt, "kmask")
+        M, K = a.shape
+        N = b.shape[0]
+        if b.shape[1] != K:
+            raise ValueError("A and B must have the same number of columns (row . row product)")
+        if km is not None and km.shape[0] != K:
+            raise ValueError("kmask length must equal K")
+        kind = _same_kind(a, b, km)
+        out = _empty_like_kind(kind, (M, N), dt, A) if out is None else out
+        c = _Mat(out, dt, "out")
+        fn = self.lib.mcpg_maxmin_f64 if dt == np.float64 else self.lib.mcpg_maxmin_f32
+        self._pre(kind)
+        self._check(fn(self.h, kind, a.ptr, a.ld, b.ptr, b.ld, km.ptr if km else None, M, N, K, int(symmetric),
+                       c.ptr, c.ld), "mcpg_maxmin")
+        return out
+
+    def ratio(self, A, Mm, out=None):
+        dt = np.float64 if str(getattr(A, "dtype", "")).endswith("float64") else np.float32
+        a = _Mat(A, dt, "A")
+        m = _Mat(Mm, dt, "M")
+        if a.shape != m.shape:
+            raise ValueError("ratio operands must have the same shape")
+        kind = _same_kind(a, m)
+        out = _empty_like_kind(kind, a.shape, dt, A) if out is None else out
+        o = _Mat(out, dt, "out")
+        fn = self.lib.mcpg_ratio_f64 if dt == np.float64 else self.lib.mcpg_ratio_f32
+        self._pre(kind)
+        self._check(fn(self.h, kind, a.ptr, a.ld, m.ptr, m.ld, a.shape[0], a.shape[1], o.ptr, o.ld), "mcpg_ratio")
+        return out
+
+    def mcp(self, A, k, want_maxmin=False, out=None, maxmin_out=None):
+        """MCP_k score: ratio(A, A^(x)k); returns (mcp, maxmin-or-None)."""
+        dt = np.float64 if str(getattr(A, "dtype", "")).endswith("float64") else np.float32
+        a = _Mat(A, dt, "A")
+        N = a.shape[0]
+        if a.shape[1] != N:
+            raise ValueError("MCP needs a square matrix")
+        out = _empty_like_kind(a.kind, (N, N), dt, A) if out is None else out
+        o = _Mat(out, dt, "out")
+        mm = None
+        if want_maxmin or maxmin_out is not None:
+            maxmin_out = _empty_like_kind(a.kind, (N, N), dt, A) if maxmin_out is None else maxmin_out
+            mm = _Mat(maxmin_out, dt, "maxmin_out")
+        fn = self.lib.mcpg_mcp_f64 if dt == np.float64 else self.lib.mcpg_mcp_f32
+        self._pre(a.kind)
+        self._check(fn(self.h, a.kind, a.ptr, a.ld, N, int(k), mm.ptr if mm else None, mm.ld if mm else 0, o.ptr, o.ld),
+                    "mcpg_mcp")
+        return out, maxmin_out

@@ -1,0 +1,512 @@
+// Adaptive-partitioning mutual information on rank-transformed rows, plus the rank, jitter and
+// cumulative-histogram transforms that feed it.
+//
+// One wavefront evaluates one gene pair at a time.  The pair's points are bucketed by y rank
+// into LDS (xGroupedByY), and the window recursion of the reference runs as a LIFO stack held
+// in the wave's registers (entry k lives in lane k % 64, slot k / 64), so every control
+// decision is wave-uniform.  A popped window is split at its mid ranks and its four quadrant
+// counts come from one pass over the window's y range: each lane tests one point and the
+// counts are ballot + popcount over the wave.  Counts are integers and match the reference
+// exactly; leaf terms enter the FP64 sum in the reference's LIFO order.
+#include <cmath>
+
+#include "internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+using mcpg::ceil_div;
+
+constexpr int kMaxWaves = 4;       // waves per workgroup (fewer when S is large)
+constexpr int64_t kLdsBudget = 64 * 1024;
+constexpr int64_t kMaxS = kLdsBudget / 8 - 1;  // one wave: 2 x (S+1) ints of LDS
+constexpr int kStackSlots = 2;     // stack capacity = 64 * kStackSlots windows
+
+struct Win {
+  int x1, y1, x2, y2, n;
+};
+
+__device__ __forceinline__ int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// SplitMix64 (Steele, Lea, Flood 2014), counter-based: draw `ctr` of stream `seed`.
+__host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t seed, uint64_t ctr) {
+  uint64_t z = seed + (ctr + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__global__ void noise_kernel(const float* __restrict__ X, int64_t ldx, int64_t G, int64_t S, uint64_t seed,
+                             float* __restrict__ Y, int64_t ldy) {
+  const int64_t total = G * S;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = t / S, s = t % S;
+    const double u = (double)(splitmix64(seed, (uint64_t)t) >> 11) * 0x1.0p-53;
+    const float j = (float)(u * 1e-5);
+    Y[g * ldy + s] = X[g * ldx + s] + j;
+  }
+}
+
+// Descending rank, ties by ascending index: r_s = #{t : x_t > x_s} + #{t < s : x_t == x_s}.
+__global__ __launch_bounds__(256) void rank_kernel(const float* __restrict__ X, int64_t ldx, int64_t S,
+                                                    int32_t* __restrict__ R, int64_t ldr, int* flags) {
+  extern __shared__ float xsh[];
+  const int64_t g = blockIdx.x;
+  for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
+    const float v = X[g * ldx + s];
+    if (!isfinite(v)) atomicOr(flags, 1);
+    xsh[s] = v;
+  }
+  __syncthreads();
+  for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
+    const float v = xsh[s];
+    int r = 0;
+    for (int64_t t = 0; t < S; ++t) {
+      const float u = xsh[t];
+      r += (u > v) || (u == v && t < s);
+    }
+    R[g * ldr + s] = r;
+  }
+}
+
+// IntegralCumulativeHistogramKernel (AdaptivePartitioningMI.hpp:422-439) for one row per block.
+__global__ __launch_bounds__(256) void cumhist_kernel(const int32_t* __restrict__ R, int64_t ldr, int64_t S,
+                                                       int32_t* __restrict__ H, int64_t ldh, int* flags) {
+  extern __shared__ int hsh[];
+  const int64_t g = blockIdx.x;
+  for (int64_t i = threadIdx.x; i <= S; i += blockDim.x) hsh[i] = 0;
+  __syncthreads();
+  for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
+    const int r = R[g * ldr + s];
+    if (r < 0 || r >= S) { atomicOr(flags, 2); continue; }
+    atomicAdd(&hsh[r + 1], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int64_t i = 2; i <= S; ++i) hsh[i] += hsh[i - 1];
+  }
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i <= S; i += blockDim.x) H[g * ldh + i] = hsh[i];
+}
+
+__device__ __forceinline__ void tri_decode(int64_t Lid, int64_t nb, int64_t* bm, int64_t* bn) {
+  double a = 2.0 * (double)nb + 1.0;
+  int64_t r = (int64_t)((a - sqrt(a * a - 8.0 * (double)Lid)) * 0.5);
+  if (r < 0) r = 0;
+  if (r > nb - 1) r = nb - 1;
+  auto start = [nb](int64_t rr) { return rr * nb - rr * (rr - 1) / 2; };
+  while (r > 0 && start(r) > Lid) --r;
+  while (r + 1 < nb && start(r + 1) <= Lid) ++r;
+  *bm = r;
+  *bn = r + (Lid - start(r));
+}
+
+struct APArgs {
+  const int32_t* R;
+  int64_t ldr;
+  const int32_t* H;   // cumulative histograms, G x (S+1)
+  int64_t ldh;
+  int64_t G, S;
+  float* MI;
+  int64_t ldmi;
+  // debug / cells mode
+  const int32_t* px;
+  const int32_t* py;
+  int64_t npairs;     // > 0: cells mode over the listed pairs
+  int32_t max_leaves;
+  int32_t* cells;
+  int32_t* nleaves;
+  double* mi_out;
+  int* flags;
+};
+
+// Quadrant counts of window w split at (mx, my) over xg[b0 .. b1), lower/upper y halves split at bm.
+__device__ __forceinline__ void count_quadrants(const int* __restrict__ xg, int b0, int bm, int b1, int x1, int mx,
+                                                int x2, int lane, int& n0, int& n1, int& n2, int& n3) {
+  n0 = n1 = n2 = n3 = 0;
+  for (int base = b0; base < b1; base += 64) {
+    const int idx = base + lane;
+    const bool valid = idx < b1;
+    const int x = valid ? xg[idx] : -1;
+    const bool inL = valid && x >= x1 && x < mx;
+    const bool inR = valid && x >= mx && x < x2;
+    const bool low = idx < bm;
+    n0 += __popcll(__ballot(inL && low));
+    n1 += __popcll(__ballot(inR && low));
+    n2 += __popcll(__ballot(inL && !low));
+    n3 += __popcll(__ballot(inR && !low));
+  }
+}
+
+// Evaluates AdaptivePartitionRankMIKernel2::operator()(x=gx, y=gy) (AdaptivePartitioningMI.hpp:553-645).
+// Returns the FP64 value mi/S + log2(S).  If rec != nullptr, leaf (nxy, nx, ny) triples are
+// recorded in summation order (up to max_leaves) and *nleaves receives the total count.
+__device__ double ap_pair(const APArgs& a, int64_t gx, int64_t gy, int* xg, int* cnt, int lane, int32_t* rec,
+                          int32_t* nleaves) {
+  const int S = (int)a.S;
+  const int32_t* __restrict__ cHX = a.H + gx * a.ldh;
+  const int32_t* __restrict__ cHY = a.H + gy * a.ldh;
+  const int32_t* __restrict__ xr = a.R + gx * a.ldr;
+  const int32_t* __restrict__ yr = a.R + gy * a.ldr;
+
+  // _createSparseMatrix (:462-493): bucket x ranks by y rank
+  for (int i = lane; i < S; i += 64) cnt[i] = 0;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  for (int i = lane; i < S; i += 64) {
+    const int y = yr[i];
+    const int off = cHY[y] + atomicAdd(&cnt[y], 1);
+    xg[off] = xr[i];
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+
+  // register-resident LIFO stack
+  int sx1[kStackSlots], sy1[kStackSlots], sx2[kStackSlots], sy2[kStackSlots], sn[kStackSlots];
+  int sp = 0;  // number of entries
+  bool whole = true;
+  auto push = [&](const Win& w) {
+    const int slot = sp >> 6, ln = sp & 63;
+#pragma unroll
+    for (int k = 0; k < kStackSlots; ++k)
+      if (k == slot && lane == ln) { sx1[k] = w.x1; sy1[k] = w.y1; sx2[k] = w.x2; sy2[k] = w.y2; sn[k] = w.n; }
+    ++sp;
+  };
+  auto top = [&]() {
+    const int idx = sp - 1;
+    const int slot = idx >> 6, ln = idx & 63;
+    Win w{0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < kStackSlots; ++k)
+      if (k == slot) {
+        w.x1 = __builtin_amdgcn_readlane(sx1[k], ln);
+        w.y1 = __builtin_amdgcn_readlane(sy1[k], ln);
+        w.x2 = __builtin_amdgcn_readlane(sx2[k], ln);
+        w.y2 = __builtin_amdgcn_readlane(sy2[k], ln);
+        w.n = __builtin_amdgcn_readlane(sn[k], ln);
+      }
+    return w;
+  };
+  auto term = [&](const Win& w, int32_t& leaf) -> double {
+    const int64_t nx = (int64_t)(cHX[w.x2] - cHX[w.x1]);
+    const int64_t ny = (int64_t)(cHY[w.y2] - cHY[w.y1]);
+    if (rec && lane == 0 && leaf < a.max_leaves) {
+      rec[3 * leaf + 0] = w.n;
+      rec[3 * leaf + 1] = (int32_t)nx;
+      rec[3 * leaf + 2] = (int32_t)ny;
+    }
+    ++leaf;
+    return (double)w.n * log2((double)w.n / (double)(nx * ny));
+  };
+
+  push(Win{0, 0, S, S, S});
+  double mi = 0.0;
+  int32_t leaf = 0;
+  int overflow = 0;
+  while (sp > 0) {
+    const Win w = top();
+    const int mx = (w.x1 + w.x2) >> 1;
+    const int my = (w.y1 + w.y2) >> 1;
+    Win q[4] = {Win{w.x1, w.y1, mx, my, 0}, Win{mx, w.y1, w.x2, my, 0}, Win{w.x1, my, mx, w.y2, 0},
+                Win{mx, my, w.x2, w.y2, 0}};
+    const int b0 = rfl(cHY[w.y1]), bm = rfl(cHY[my]), b1 = rfl(cHY[w.y2]);
+    count_quadrants(xg, b0, bm, b1, w.x1, mx, w.x2, lane, q[0].n, q[1].n, q[2].n, q[3].n);
+    const double e = (double)w.n / 4.0;
+    double chi = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double d = (double)q[i].n - e;
+      chi += d * d;
+    }
+    chi /= e;
+    if (chi > 7.815 || whole) {
+      --sp;
+      whole = false;  // only the first window is `whole`; pushed children are copies with whole = false
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const Win& c = q[i];
+        const bool splittable = ((c.x1 + 1 < c.x2) || (c.y1 + 1 < c.y2)) && (c.n > 2);
+        if (splittable) {
+          if (sp >= 64 * kStackSlots) { overflow = 1; continue; }
+          push(c);
+        } else if (c.n > 0) {
+          mi += term(c, leaf);
+        }
+      }
+    } else if (w.n > 0) {
+      mi += term(w, leaf);
+      --sp;
+    } else {
+      --sp;  // unreachable for pushed windows (n > 2); keeps the loop finite
+    }
+  }
+  if (overflow && lane == 0) atomicOr(a.flags, 4);
+  if (nleaves) *nleaves = leaf;
+  return mi / (double)S + log2((double)S);
+}
+
+__global__ __launch_bounds__(64 * kMaxWaves) void ap_mi_kernel(APArgs a) {
+  extern __shared__ int lds[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+  int* xg = lds + (int64_t)w * 2 * (a.S + 1);
+  int* cnt = xg + (a.S + 1);
+  const int64_t total_waves = (int64_t)gridDim.x * waves;
+  const int64_t wave_id = (int64_t)blockIdx.x * waves + w;
+  if (a.npairs > 0) {
+    for (int64_t n = wave_id; n < a.npairs; n += total_waves) {
+      int32_t nl = 0;
+      const double v = ap_pair(a, a.px[n], a.py[n], xg, cnt, lane, a.cells + n * 3 * (int64_t)a.max_leaves, &nl);
+      if (lane == 0) {
+        a.nleaves[n] = nl;
+        a.mi_out[n] = v;
+      }
+    }
+    return;
+  }
+  const int64_t P = a.G * (a.G + 1) / 2;
+  for (int64_t p = wave_id; p < P; p += total_waves) {
+    int64_t i, j;
+    tri_decode(p, a.G, &i, &j);
+    const double v = ap_pair(a, i, j, xg, cnt, lane, nullptr, nullptr);
+    if (lane == 0) {
+      const float f = (float)v;
+      a.MI[i * a.ldmi + j] = f;
+      a.MI[j * a.ldmi + i] = f;
+    }
+  }
+}
+
+int check_flags(mcpg_ctx* h, const char* what) {
+  int flags[1];
+  int st = mcpg::read_flags(h, flags, 1);
+  if (st) return st;
+  if (flags[0] & 1) return mcpg::set_error(h, MCPG_ERR_NONFINITE, std::string(what) + ": non-finite input value");
+  if (flags[0] & 2) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, std::string(what) + ": rank outside [0, S)");
+  if (flags[0] & 4) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, std::string(what) + ": window stack overflow");
+  return MCPG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcpg_add_noise_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, uint64_t seed,
+                       float* Y, int64_t ldy) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0 || S == 0) return MCPG_OK;
+  if (!X || !Y) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldx < S || ldy < S) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  int st;
+  const float* dX = X;
+  float* dY = Y;
+  int64_t lx = ldx, ly = ldy;
+  if (ptrs == MCPG_HOST) {
+    mcpg::Sizer sz;
+    sz.add<float>(G * S);
+    sz.add<float>(G * S);
+    void* base = mcpg::workspace(h, sz.off, &st);
+    if (!base) return st;
+    mcpg::Carver cv(base);
+    float* a = cv.take<float>(G * S);
+    float* b = cv.take<float>(G * S);
+    if ((st = mcpg::copy_2d(h, a, S * 4, X, ldx * 4, S * 4, G, hipMemcpyHostToDevice))) return st;
+    dX = a; dY = b; lx = ly = S;
+  }
+  {
+    mcpg::LaunchScope ls(h, "noise_kernel");
+    hipLaunchKernelGGL(noise_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(G * S, 256), 65536)), dim3(256), 0,
+                       h->stream, dX, lx, G, S, seed, dY, ly);
+  }
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, Y, ldy * 4, dY, S * 4, S * 4, G, hipMemcpyDeviceToHost))) return st;
+  return mcpg::finish(h, "mcpg_add_noise_f32");
+}
+
+int mcpg_rank_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int32_t* R, int64_t ldr) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0 || S == 0) return MCPG_OK;
+  if (!X || !R) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldx < S || ldr < S) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (S > 16384) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "rank supports S <= 16384");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  int st;
+  const float* dX = X;
+  int32_t* dR = R;
+  int64_t lx = ldx, lr = ldr;
+  if (ptrs == MCPG_HOST) {
+    mcpg::Sizer sz;
+    sz.add<float>(G * S);
+    sz.add<int32_t>(G * S);
+    void* base = mcpg::workspace(h, sz.off, &st);
+    if (!base) return st;
+    mcpg::Carver cv(base);
+    float* a = cv.take<float>(G * S);
+    dR = cv.take<int32_t>(G * S);
+    if ((st = mcpg::copy_2d(h, a, S * 4, X, ldx * 4, S * 4, G, hipMemcpyHostToDevice))) return st;
+    dX = a; lx = lr = S;
+  }
+  if ((st = mcpg::reset_flags(h))) return st;
+  {
+    mcpg::LaunchScope ls(h, "rank_kernel");
+    hipLaunchKernelGGL(rank_kernel, dim3((unsigned)G), dim3(256), S * sizeof(float), h->stream, dX, lx, S, dR, lr,
+                       h->d_flags);
+  }
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, R, ldr * 4, dR, S * 4, S * 4, G, hipMemcpyDeviceToHost))) return st;
+  if ((st = mcpg::finish(h, "mcpg_rank_f32"))) return st;
+  return check_flags(h, "mcpg_rank_f32");
+}
+
+int mcpg_cumhist_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, int32_t* H,
+                     int64_t ldh) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0) return MCPG_OK;
+  if (!R || !H) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldr < S || ldh < S + 1) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (S > 16 * 1024 * 1024) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "S too large");
+  if (S + 1 > 16384) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "cumulative histogram supports S < 16384");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  int st;
+  const int32_t* dR = R;
+  int32_t* dH = H;
+  int64_t lr = ldr, lh = ldh;
+  if (ptrs == MCPG_HOST) {
+    mcpg::Sizer sz;
+    sz.add<int32_t>(G * S);
+    sz.add<int32_t>(G * (S + 1));
+    void* base = mcpg::workspace(h, sz.off, &st);
+    if (!base) return st;
+    mcpg::Carver cv(base);
+    int32_t* a = cv.take<int32_t>(G * S);
+    dH = cv.take<int32_t>(G * (S + 1));
+    if (S > 0 && (st = mcpg::copy_2d(h, a, S * 4, R, ldr * 4, S * 4, G, hipMemcpyHostToDevice))) return st;
+    dR = a; lr = S; lh = S + 1;
+  }
+  if ((st = mcpg::reset_flags(h))) return st;
+  {
+    mcpg::LaunchScope ls(h, "cumhist_kernel");
+    hipLaunchKernelGGL(cumhist_kernel, dim3((unsigned)G), dim3(256), (S + 1) * sizeof(int), h->stream, dR, lr, S, dH, lh,
+                       h->d_flags);
+  }
+  if (ptrs == MCPG_HOST &&
+      (st = mcpg::copy_2d(h, H, ldh * 4, dH, (S + 1) * 4, (S + 1) * 4, G, hipMemcpyDeviceToHost)))
+    return st;
+  if ((st = mcpg::finish(h, "mcpg_cumhist_i32"))) return st;
+  return check_flags(h, "mcpg_cumhist_i32");
+}
+
+static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, float* MI, int64_t ldmi,
+                  const int32_t* px, const int32_t* py, int64_t npairs, int32_t max_leaves, int32_t* cells,
+                  int32_t* nleaves, double* miv) {
+  MCPG_HIP(h, hipSetDevice(h->device));
+  int st;
+  const bool cells_mode = npairs > 0;
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) sz.add<int32_t>(G * S);
+  sz.add<int32_t>(G * (S + 1));
+  if (!cells_mode && ptrs == MCPG_HOST) sz.add<float>(G * G);
+  if (cells_mode) {
+    sz.add<int32_t>(npairs); sz.add<int32_t>(npairs);
+    sz.add<int32_t>(npairs * 3 * (int64_t)max_leaves); sz.add<int32_t>(npairs); sz.add<double>(npairs);
+  }
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const int32_t* dR = R;
+  int64_t lr = ldr;
+  if (ptrs == MCPG_HOST) {
+    int32_t* t = cv.take<int32_t>(G * S);
+    if ((st = mcpg::copy_2d(h, t, S * 4, R, ldr * 4, S * 4, G, hipMemcpyHostToDevice))) return st;
+    dR = t; lr = S;
+  }
+  int32_t* dH = cv.take<int32_t>(G * (S + 1));
+  float* dMI = MI;
+  int64_t lm = ldmi;
+  if (!cells_mode && ptrs == MCPG_HOST) { dMI = cv.take<float>(G * G); lm = G; }
+  if ((st = mcpg::reset_flags(h))) return st;
+  {
+    mcpg::LaunchScope ls(h, "cumhist_kernel");
+    hipLaunchKernelGGL(cumhist_kernel, dim3((unsigned)G), dim3(256), (S + 1) * sizeof(int), h->stream, dR, lr, S, dH,
+                       S + 1, h->d_flags);
+  }
+  // ranks index the histograms: refuse out-of-range ranks before the pair kernel reads them
+  if ((st = check_flags(h, "mcpg_ap_mi_i32"))) return st;
+  APArgs a{};
+  a.R = dR; a.ldr = lr; a.H = dH; a.ldh = S + 1; a.G = G; a.S = S; a.MI = dMI; a.ldmi = lm; a.flags = h->d_flags;
+  int32_t *d_px = nullptr, *d_py = nullptr, *d_cells = nullptr, *d_nl = nullptr;
+  double* d_mi = nullptr;
+  if (cells_mode) {
+    d_px = cv.take<int32_t>(npairs); d_py = cv.take<int32_t>(npairs);
+    d_cells = cv.take<int32_t>(npairs * 3 * (int64_t)max_leaves); d_nl = cv.take<int32_t>(npairs);
+    d_mi = cv.take<double>(npairs);
+    MCPG_HIP(h, hipMemcpyAsync(d_px, px, npairs * 4, hipMemcpyHostToDevice, h->stream));
+    MCPG_HIP(h, hipMemcpyAsync(d_py, py, npairs * 4, hipMemcpyHostToDevice, h->stream));
+    a.px = d_px; a.py = d_py; a.npairs = npairs; a.max_leaves = max_leaves; a.cells = d_cells; a.nleaves = d_nl;
+    a.mi_out = d_mi;
+  }
+  const int64_t work = cells_mode ? npairs : G * (G + 1) / 2;
+  const int64_t per_wave = 2 * (S + 1) * (int64_t)sizeof(int);
+  const int waves = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxWaves, kLdsBudget / per_wave));
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, waves), 256 * 64 / waves));
+  const size_t lds = (size_t)waves * per_wave;
+  {
+    mcpg::LaunchScope ls(h, "ap_mi_kernel");
+    hipLaunchKernelGGL(ap_mi_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds, h->stream, a);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mcpg::hip_error(h, e, "ap_mi_kernel launch");
+  if (cells_mode) {
+    MCPG_HIP(h, hipMemcpyAsync(cells, d_cells, npairs * 3 * (int64_t)max_leaves * 4, hipMemcpyDeviceToHost, h->stream));
+    MCPG_HIP(h, hipMemcpyAsync(nleaves, d_nl, npairs * 4, hipMemcpyDeviceToHost, h->stream));
+    MCPG_HIP(h, hipMemcpyAsync(miv, d_mi, npairs * 8, hipMemcpyDeviceToHost, h->stream));
+  } else if (ptrs == MCPG_HOST) {
+    if ((st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, G * 4, G * 4, G, hipMemcpyDeviceToHost))) return st;
+  }
+  if ((st = mcpg::finish(h, "mcpg_ap_mi_i32"))) return st;
+  return check_flags(h, "mcpg_ap_mi_i32");
+}
+
+int mcpg_ap_mi_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, float* MI,
+                   int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (S > kMaxS) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "adaptive-partitioning MI supports S <= 8191");
+  if (!R || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldr < S || ldmi < G) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  return ap_run(h, ptrs, R, G, S, ldr, MI, ldmi, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr);
+}
+
+int mcpg_ap_mi_cells_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,
+                         const int32_t* px, const int32_t* py, int64_t npairs, int32_t max_leaves, int32_t* cells,
+                         int32_t* nleaves, double* mi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || S < 0 || npairs < 0 || max_leaves < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0 || npairs == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (S > kMaxS) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "adaptive-partitioning MI supports S <= 8191");
+  if (!R || !px || !py || !nleaves || !mi || (max_leaves > 0 && !cells))
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null pointer");
+  if (ldr < S) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  for (int64_t n = 0; n < npairs; ++n)
+    if (px[n] < 0 || px[n] >= G || py[n] < 0 || py[n] >= G)
+      return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "pair index out of range");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  return ap_run(h, ptrs, R, G, S, ldr, nullptr, 0, px, py, npairs, max_leaves, cells, nleaves, mi);
+}
+
+}  // extern "C"

@@ -1,0 +1,1109 @@
+// B-spline mutual information: per-gene weight panels, the all-pairs MI contraction on MFMA
+// with a fused p*log2(p) epilogue, and its adjoint.
+//
+// Layout in HBM.  Weights live in "group-tiled" row panels: rows are (gene, bin) pairs, samples
+// run along the row (K of the contraction), zero-padded to Sp = roundup(S, 32).  Genes are
+// packed GW = 80 / bins to an 80-row group (8 genes x 10 bins for the default), so a group is
+// exactly five 16-row MFMA blocks; the remaining 80 - GW*bins rows of a group are zero.  A
+// workgroup tile is two groups (GT = 2*GW genes, 160 rows) on each side.
+//
+// Arithmetic.  The joint table of genes (i, j) is the 10 x 10 block W_i * W_j^T / S of the
+// (rows x rows) Gram matrix; it is never written to HBM.  Three contraction precisions share
+// one tile loop: FP32 (v_mfma_f32_16x16x4_f32), BF16 (v_mfma_f32_16x16x32_bf16) and BF16X3,
+// where each weight is split w = hi + lo into two BF16 values and the product is
+// hi*hi + hi*lo + lo*hi with FP32 accumulation.  Marginal entropies are FP64 and
+// MI = (H_i + H_j) + sum p log2 p is formed in FP64 from the FP32 entropy sum.
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+using mcpg::ceil_div;
+
+constexpr int kMaxBins = 80;
+constexpr int kMaxOrder = 8;
+constexpr int kGroupRows = 80;       // rows of one gene group (5 MFMA blocks of 16)
+constexpr int kTileRows = 160;       // two groups per workgroup tile side
+constexpr int kKStep = 32;           // samples per K step
+constexpr int kThreads = 256;
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct Spline {
+  int bins, order;
+  float knots[kMaxBins + kMaxOrder];
+};
+
+// buildKnotVector<float> (BSplineMI.hpp:35-71): `order` zeros, bins-order internal knots
+// i/(bins-order+1), `order` ones; values are products of an int and a double stored as float.
+Spline make_spline(int bins, int order) {
+  Spline sp;
+  std::memset(&sp, 0, sizeof sp);
+  sp.bins = bins;
+  sp.order = order;
+  const int n_int = bins - order;
+  const double norm = 1.0 / (double)(n_int + 1);
+  for (int i = 0; i < bins + order; ++i) {
+    if (i < order) sp.knots[i] = 0.0f;
+    else if (i < order + n_int) sp.knots[i] = (float)((double)(float)(i - order + 1) * norm);
+    else sp.knots[i] = 1.0f;
+  }
+  return sp;
+}
+
+// Cox-de Boor basis values N_{b,order}(t) for b < bins, evaluated bottom-up with the same
+// float operations, degenerate-span guards (d < 1e-10) and negative truncation as the
+// reference's recursive basisFunction (BSplineMI.hpp:130-164), including the right-end rule
+// N_{bins-1,1}(t) = 1 when |t - knot[bins]| < 1e-10.  `lower` (nullable) receives the order-1
+// level N_{b,order-1}(t) for b < bins+1, used by the derivative.  NB / NO are compile-time
+// capacities (bins <= NB, order <= NO) so that the tables stay in registers.
+template <int NB, int NO>
+__device__ __forceinline__ void basis_values(const Spline& sp, float t, float* out, float* lower) {
+  float N[NB + NO];
+  const int bins = sp.bins, order = sp.order;
+  const float* k = sp.knots;
+  const int n1 = bins + order - 1;
+#pragma unroll
+  for (int i = 0; i < NB + NO - 1; ++i) {
+    if (i < n1) {
+      const bool in = (t >= k[i] && t < k[i + 1] && k[i] < k[i + 1]);
+      const bool endpt = ((double)fabsf(t - k[i + 1]) < 1e-10) && (i + 1 == bins);
+      N[i] = (in || endpt) ? 1.0f : 0.0f;
+    } else {
+      N[i] = 0.0f;
+    }
+  }
+  N[NB + NO - 1] = 0.0f;
+#pragma unroll
+  for (int p = 2; p <= NO; ++p) {
+    if (p <= order) {
+      if (lower && p == order) {
+#pragma unroll
+        for (int i = 0; i < NB + 1; ++i) lower[i] = N[i];
+      }
+      const int n = bins + order - p;
+#pragma unroll
+      for (int i = 0; i < NB + NO - 2; ++i) {
+        if (i < n) {
+          const float d1 = k[i + p - 1] - k[i];
+          const float nn1 = t - k[i];
+          const float d2 = k[i + p] - k[i + 1];
+          const float nn2 = k[i + p] - t;
+          float e1, e2, v;
+          if ((double)d1 < 1e-10 && (double)d2 < 1e-10) {
+            v = 0.0f;
+          } else {
+            if ((double)d1 < 1e-10) {
+              e1 = 0.0f;
+              e2 = nn2 / d2 * N[i + 1];
+            } else if ((double)d2 < 1e-10) {
+              e2 = 0.0f;
+              e1 = nn1 / d1 * N[i];
+            } else {
+              e1 = nn1 / d1 * N[i];
+              e2 = nn2 / d2 * N[i + 1];
+            }
+            v = (e1 + e2 < 0.0f) ? 0.0f : e1 + e2;
+          }
+          N[i] = v;  // in place: N[i+1] of the previous level is read before it is overwritten
+        }
+      }
+    }
+  }
+  if (lower && order == 1) {
+#pragma unroll
+    for (int i = 0; i < NB + 1; ++i) lower[i] = 0.0f;
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) out[b] = N[b];
+}
+
+// d N_{b,order} / dt from the order-1 level (exact derivative of the piecewise polynomial).
+template <int NB>
+__device__ __forceinline__ void basis_derivs(const Spline& sp, const float* lower, float* d) {
+  const int p = sp.order;
+  const float* k = sp.knots;
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    float v = 0.0f;
+    if (b < sp.bins && p > 1) {
+      const float d1 = k[b + p - 1] - k[b];
+      const float d2 = k[b + p] - k[b + 1];
+      if ((double)d1 >= 1e-10) v += lower[b] / d1;
+      if ((double)d2 >= 1e-10) v -= lower[b + 1] / d2;
+      v = (float)(p - 1) * v;
+    }
+    d[b] = v;
+  }
+}
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+struct Layout {
+  int bins, gw, gt;     // bins, genes per group, genes per tile side
+  int64_t G, S, Sp;     // genes, samples, padded samples
+  int64_t ntiles;       // gene tiles per side
+  int64_t rows;         // weight rows allocated = ntiles * 160
+  __host__ __device__ int64_t row_of(int64_t g, int b) const {
+    return (g / gw) * kGroupRows + (g % gw) * bins + b;
+  }
+};
+
+Layout make_layout(int64_t G, int64_t S, int bins) {
+  Layout L;
+  L.bins = bins;
+  L.gw = kGroupRows / bins;
+  L.gt = 2 * L.gw;
+  L.G = G;
+  L.S = S;
+  L.Sp = ceil_div(S, kKStep) * kKStep;
+  L.ntiles = ceil_div(G, L.gt);
+  L.rows = L.ntiles * kTileRows;
+  return L;
+}
+
+// ---------------------------------------------------------------- block reductions
+template <typename T>
+__device__ T block_sum(T v, T* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  T r = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  return r;
+}
+
+// ---------------------------------------------------------------- per-gene preparation
+struct PrepOut {
+  float* w32;          // group-tiled FP32 weights (nullable)
+  uint16_t* whi;       // group-tiled BF16 hi (nullable)
+  uint16_t* wlo;       // group-tiled BF16 lo (nullable)
+  float* wref;         // reference layout G x ldw (nullable)
+  int64_t ldw;
+  double* H64;         // marginal entropy per gene (nullable)
+  double* pmarg;       // marginal probabilities G x bins (nullable)
+  float* xs;           // scaled rows G x S (nullable)
+  float* scl;          // per gene: scale (1/(max-min) as float, or 1) (nullable)
+  int* amin;           // first arg-min (nullable)
+  int* amax;           // first arg-max (nullable)
+  int* flags;          // flags[0] |= 1 on non-finite input
+};
+
+// One workgroup per gene: min-max scaling (MinMaxScale, BSplineMI.hpp:81-115), basis weights
+// (BSplineWeightsKernel::operator(), :248-262) and the FP64 marginal entropy (entropy1d_2, :204-224).
+template <int NB, int NO>
+__global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __restrict__ X, int64_t ldx, Layout L,
+                                                                Spline sp, int do_scale, PrepOut o) {
+  const int64_t g = blockIdx.x;
+  const int tid = threadIdx.x;
+  const float* x = X + g * ldx;
+  const int64_t S = L.S;
+  __shared__ float shf[4];
+  __shared__ int shi[4];
+  __shared__ double shd[4];
+  __shared__ double psum[kMaxBins];
+
+  float mn = S > 0 ? x[0] : 0.0f, mx = mn;
+  int bad = 0;
+  for (int64_t s = tid; s < S; s += kThreads) {
+    const float v = x[s];
+    if (!isfinite(v)) bad = 1;
+    mn = fminf(mn, v);
+    mx = fmaxf(mx, v);
+  }
+  // block min / max
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = fminf(mn, __shfl_xor(mn, off));
+    mx = fmaxf(mx, __shfl_xor(mx, off));
+  }
+  __syncthreads();
+  if ((tid & 63) == 0) { shf[tid >> 6] = mn; }
+  __syncthreads();
+  mn = fminf(fminf(shf[0], shf[1]), fminf(shf[2], shf[3]));
+  __syncthreads();
+  if ((tid & 63) == 0) { shf[tid >> 6] = mx; }
+  __syncthreads();
+  mx = fmaxf(fmaxf(shf[0], shf[1]), fmaxf(shf[2], shf[3]));
+  if (bad) atomicOr(o.flags, 1);
+
+  float scale = 1.0f, offset = 0.0f;
+  if (do_scale && mx > mn) {
+    scale = (float)(1.0 / (double)(mx - mn));
+    offset = mn;
+  }
+  if (o.scl && tid == 0) o.scl[g] = scale;
+  if (o.amin || o.amax) {
+    int imn = INT32_MAX, imx = INT32_MAX;
+    for (int64_t s = tid; s < S; s += kThreads) {
+      if (x[s] == mn && (int)s < imn) imn = (int)s;
+      if (x[s] == mx && (int)s < imx) imx = (int)s;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      imn = min(imn, __shfl_xor(imn, off));
+      imx = min(imx, __shfl_xor(imx, off));
+    }
+    __syncthreads();
+    if ((tid & 63) == 0) shi[tid >> 6] = imn;
+    __syncthreads();
+    imn = min(min(shi[0], shi[1]), min(shi[2], shi[3]));
+    __syncthreads();
+    if ((tid & 63) == 0) shi[tid >> 6] = imx;
+    __syncthreads();
+    imx = min(min(shi[0], shi[1]), min(shi[2], shi[3]));
+    if (tid == 0) {
+      if (o.amin) o.amin[g] = imn;
+      if (o.amax) o.amax[g] = imx;
+    }
+  }
+
+  const int bins = sp.bins;
+  double acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) acc[b] = 0.0;
+  float w[NB];
+  for (int64_t s = tid; s < S; s += kThreads) {
+    const float t = do_scale ? scale * (x[s] - offset) : x[s];
+    if (o.xs) o.xs[g * S + s] = t;
+    basis_values<NB, NO>(sp, t, w, nullptr);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (b >= bins) break;
+      acc[b] += (double)w[b];
+      const int64_t r = L.row_of(g, b);
+      if (o.w32) o.w32[r * L.Sp + s] = w[b];
+      if (o.whi) {
+        const uint16_t hi = f32_to_bf16_rne(w[b]);
+        o.whi[r * L.Sp + s] = hi;
+        if (o.wlo) o.wlo[r * L.Sp + s] = f32_to_bf16_rne(w[b] - bf16_to_f32(hi));
+      }
+      if (o.wref) o.wref[g * o.ldw + (int64_t)b * S + s] = w[b];
+    }
+  }
+  // FP64 marginals and entropy
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (b >= bins) break;
+    const double t = block_sum<double>(acc[b], shd);
+    if (tid == 0) psum[b] = t;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const double norm = 1.0 / (double)S;
+    double H = 0.0;
+    for (int b = 0; b < bins; ++b) {
+      const double p = psum[b] * norm;
+      if (o.pmarg) o.pmarg[g * bins + b] = p;
+      if (p > 0.0) H += p * log2(p);
+    }
+    H = 0.0 - H;
+    if (o.H64) o.H64[g] = H;
+    if (o.wref) o.wref[g * o.ldw + (int64_t)bins * S] = (float)H;
+  }
+}
+
+// Reference-layout weight rows -> group-tiled panels (+ FP64 copy of the stored marginal entropy).
+__global__ void weights_import_kernel(const float* __restrict__ W, int64_t ldw, Layout L, float* w32, uint16_t* whi,
+                                      uint16_t* wlo, double* H64, int* flags) {
+  const int64_t g = blockIdx.y;
+  const int64_t total = (int64_t)L.bins * L.S;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= total; e += (int64_t)gridDim.x * blockDim.x) {
+    const float v = W[g * ldw + e];
+    if (!isfinite(v)) atomicOr(flags, 1);
+    if (e == total) {
+      H64[g] = (double)v;
+      continue;
+    }
+    const int b = (int)(e / L.S);
+    const int64_t s = e % L.S;
+    const int64_t r = L.row_of(g, b);
+    if (w32) w32[r * L.Sp + s] = v;
+    if (whi) {
+      const uint16_t hi = f32_to_bf16_rne(v);
+      whi[r * L.Sp + s] = hi;
+      if (wlo) wlo[r * L.Sp + s] = f32_to_bf16_rne(v - bf16_to_f32(hi));
+    }
+  }
+}
+
+// ---------------------------------------------------------------- all-pairs MI on MFMA
+enum Prec { kFP32 = 0, kBF16 = 1, kBF16X3 = 2 };
+
+struct MIArgs {
+  const float* w32;
+  const uint16_t* whi;
+  const uint16_t* wlo;
+  const double* H64;
+  float* MI;
+  int64_t ldmi;
+  Layout L;
+  int64_t ntri;      // number of (ti <= tj) tile pairs
+  float norm;        // (float)1 / (float)S, the reference's FT norm_factor (BSplineMI.hpp:319)
+};
+
+__device__ __forceinline__ void tri_decode(int64_t Lid, int64_t nb, int64_t* bm, int64_t* bn) {
+  double a = 2.0 * (double)nb + 1.0;
+  int64_t r = (int64_t)((a - sqrt(a * a - 8.0 * (double)Lid)) * 0.5);
+  if (r < 0) r = 0;
+  if (r > nb - 1) r = nb - 1;
+  auto start = [nb](int64_t rr) { return rr * nb - rr * (rr - 1) / 2; };
+  while (r > 0 && start(r) > Lid) --r;
+  while (r + 1 < nb && start(r + 1) <= Lid) ++r;
+  *bm = r;
+  *bn = r + (Lid - start(r));
+}
+
+// LDS row strides (elements): FP32 rows of 32 floats + 4 pad; BF16 rows of 32 halves + 8 pad.
+constexpr int kLdsF32 = 36;
+constexpr int kLdsB16 = 40;
+constexpr int kPanelF32 = kTileRows * kLdsF32;        // floats per panel
+constexpr int kPanelB16 = kTileRows * kLdsB16;        // uint16 per panel
+constexpr int kEpiStride = 81;                        // t-tile row stride (floats)
+constexpr int kEpiWave = kGroupRows * kEpiStride;     // floats per wave t-tile
+
+template <int P>
+struct MISmem;
+template <>
+struct MISmem<kFP32> { static constexpr int kBytes = 2 * kPanelF32 * 4; };
+template <>
+struct MISmem<kBF16> { static constexpr int kBytes = 2 * kPanelB16 * 2; };
+template <>
+struct MISmem<kBF16X3> { static constexpr int kBytes = 4 * kPanelB16 * 2; };
+constexpr int kEpiBytes = 2 * kEpiWave * 4;           // two waves' t-tiles at a time
+template <int P>
+constexpr int mi_smem_bytes() { return MISmem<P>::kBytes > kEpiBytes ? MISmem<P>::kBytes : kEpiBytes; }
+
+// Loads the K-step [k0, k0+32) of one 160-row panel into registers: 16-byte chunks,
+// FP32: 8 chunks per row (1280 per panel, 5 per thread); BF16: 4 chunks per row and array.
+template <int P>
+struct Stage {
+  static constexpr int kPer = (P == kFP32) ? 5 : (P == kBF16 ? 3 : 5);
+  int4 v[kPer];
+};
+
+template <int P>
+__device__ __forceinline__ void stage_load(const MIArgs& a, int64_t row0, int64_t k0, Stage<P>& st) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < Stage<P>::kPer; ++q) {
+    const int idx = tid + kThreads * q;
+    if (P == kFP32) {
+      const int r = idx >> 3, c = idx & 7;
+      st.v[q] = *reinterpret_cast<const int4*>(a.w32 + (row0 + r) * a.L.Sp + k0 + c * 4);
+    } else {
+      const int arr = idx / 640, rem = idx % 640;
+      const int r = rem >> 2, c = rem & 3;
+      if (P == kBF16 && idx >= 640) continue;
+      const uint16_t* src = arr == 0 ? a.whi : a.wlo;
+      st.v[q] = *reinterpret_cast<const int4*>(src + (row0 + r) * a.L.Sp + k0 + c * 8);
+    }
+  }
+}
+
+template <int P>
+__device__ __forceinline__ void stage_store(char* panel, const Stage<P>& st) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < Stage<P>::kPer; ++q) {
+    const int idx = tid + kThreads * q;
+    if (P == kFP32) {
+      const int r = idx >> 3, c = idx & 7;
+      *reinterpret_cast<int4*>(reinterpret_cast<float*>(panel) + r * kLdsF32 + c * 4) = st.v[q];
+    } else {
+      const int arr = idx / 640, rem = idx % 640;
+      const int r = rem >> 2, c = rem & 3;
+      if (P == kBF16 && idx >= 640) continue;
+      uint16_t* dst = reinterpret_cast<uint16_t*>(panel) + arr * kPanelB16;
+      *reinterpret_cast<int4*>(dst + r * kLdsB16 + c * 8) = st.v[q];
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  constexpr int kPanelBytes = (P == kFP32) ? kPanelF32 * 4 : (P == kBF16 ? kPanelB16 * 2 : 2 * kPanelB16 * 2);
+  char* panelA = smem;
+  char* panelB = smem + kPanelBytes;
+
+  int64_t ti, tj;
+  tri_decode(mcpg::xcd_remap(blockIdx.x, a.ntri), a.L.ntiles, &ti, &tj);
+  const bool diag = ti == tj;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int64_t rowA = ti * kTileRows, rowB = tj * kTileRows;
+
+  floatx4 acc[5][5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  Stage<P> sa, sb;
+  const int64_t nk = a.L.Sp / kKStep;
+  stage_load<P>(a, rowA, 0, sa);
+  if (!diag) stage_load<P>(a, rowB, 0, sb);
+  for (int64_t kk = 0; kk < nk; ++kk) {
+    stage_store<P>(panelA, sa);
+    if (!diag) stage_store<P>(panelB, sb);
+    __syncthreads();
+    if (kk + 1 < nk) {
+      stage_load<P>(a, rowA, (kk + 1) * kKStep, sa);
+      if (!diag) stage_load<P>(a, rowB, (kk + 1) * kKStep, sb);
+    }
+    const char* pB = diag ? panelA : panelB;
+    if (P == kFP32) {
+      const float* As = reinterpret_cast<const float*>(panelA);
+      const float* Bs = reinterpret_cast<const float*>(pB);
+      floatx4 af[5][2], bfr[5][2];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        const float* pa = As + (wr * 80 + i * 16 + lr) * kLdsF32 + lq * 8;
+        af[i][0] = *reinterpret_cast<const floatx4*>(pa);
+        af[i][1] = *reinterpret_cast<const floatx4*>(pa + 4);
+        const float* pb = Bs + (wc * 80 + i * 16 + lr) * kLdsF32 + lq * 8;
+        bfr[i][0] = *reinterpret_cast<const floatx4*>(pb);
+        bfr[i][1] = *reinterpret_cast<const floatx4*>(pb + 4);
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+          for (int j = 0; j < 5; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i][u >> 2][u & 3], bfr[j][u >> 2][u & 3], acc[i][j], 0, 0, 0);
+    } else {
+      const uint16_t* Ah = reinterpret_cast<const uint16_t*>(panelA);
+      const uint16_t* Bh = reinterpret_cast<const uint16_t*>(pB);
+      bf16x8 ah[5], bh[5];
+#pragma unroll
+      for (int i = 0; i < 5; ++i) {
+        ah[i] = *reinterpret_cast<const bf16x8*>(Ah + (wr * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
+        bh[i] = *reinterpret_cast<const bf16x8*>(Bh + (wc * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
+      }
+      if (P == kBF16X3) {
+        bf16x8 al[5], bl[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+          al[i] = *reinterpret_cast<const bf16x8*>(Ah + kPanelB16 + (wr * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
+          bl[i] = *reinterpret_cast<const bf16x8*>(Bh + kPanelB16 + (wc * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
+        }
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+          for (int j = 0; j < 5; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          }
+      }
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue: t = p log2 p per joint-table entry, then one gene pair per lane
+  float* E = reinterpret_cast<float*>(smem);
+  const int bins = a.L.bins, gw = a.L.gw;
+  for (int phase = 0; phase < 2; ++phase) {
+    if (wr == phase) {
+      float* Ew = E + wc * kEpiWave;
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float p = acc[i][j][r] * a.norm;
+            const float t = p > 0.0f ? p * log2f(p) : 0.0f;
+            Ew[(i * 16 + lq * 4 + r) * kEpiStride + j * 16 + lr] = t;
+          }
+    }
+    __syncthreads();
+    if (wr == phase) {
+      const float* Ew = E + wc * kEpiWave;
+      for (int pr = lane; pr < gw * gw; pr += 64) {
+        const int pi = pr / gw, pj = pr % gw;
+        const int64_t gi = ti * a.L.gt + wr * gw + pi;
+        const int64_t gj = tj * a.L.gt + wc * gw + pj;
+        if (gi >= a.L.G || gj >= a.L.G) continue;
+        if (diag && gi > gj) continue;
+        float sum = 0.0f;
+        for (int r = 0; r < bins; ++r) {
+          const float* row = Ew + (pi * bins + r) * kEpiStride + pj * bins;
+          for (int c = 0; c < bins; ++c) sum += row[c];
+        }
+        const float mi = (float)((a.H64[gi] + a.H64[gj]) + (double)sum);
+        a.MI[gi * a.ldmi + gj] = mi;
+        if (gi != gj) a.MI[gj * a.ldmi + gi] = mi;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- adjoint
+// Per masked pair: joint table p = W_i W_j^T / S (FP32, one wave per pair), its entropy term and
+// Gm[b][c] = m * phi(p_bc), phi(p) = -(log2 p + 1/ln 2) = dH/dp for p > 0.
+struct PairArgs {
+  const float* w32;
+  Layout L;
+  const int* pi;
+  const int* pj;
+  const float* m;
+  int64_t npairs;
+  const double* H64;
+  float norm;
+  float* Gm;        // npairs x bins*bins
+  double* Lpart;    // per pair m * MI
+};
+
+template <int NB>
+__global__ __launch_bounds__(256) void bspline_pair_adjoint_kernel(PairArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t n = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (n >= a.npairs) return;
+  const int64_t i = a.pi[n], j = a.pj[n];
+  const int bins = a.L.bins;
+  const int64_t Sp = a.L.Sp;
+  // each lane accumulates the full table over its samples, then a wave reduction per entry
+  float part[NB];  // reused per row b
+  double Lsum = 0.0;
+  for (int b = 0; b < bins; ++b) {
+    const float* wi = a.w32 + a.L.row_of(i, b) * Sp;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) part[c] = 0.0f;
+    for (int64_t s = lane; s < a.L.S; s += 64) {
+      const float x = wi[s];
+      if (x == 0.0f) continue;
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < bins) part[c] += x * a.w32[a.L.row_of(j, c) * Sp + s];
+    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+      if (c >= bins) break;
+      float v = part[c];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      const float p = v * a.norm;
+      float gm = 0.0f;
+      if (p > 0.0f) {
+        Lsum += (double)(p * log2f(p));
+        gm = a.m[n] * (-(log2f(p) + 1.4426950408889634f));
+      }
+      if (lane == 0) a.Gm[n * bins * bins + b * bins + c] = gm;
+    }
+  }
+  if (lane == 0) a.Lpart[n] = (double)a.m[n] * ((a.H64[i] + a.H64[j]) + Lsum);
+}
+
+struct GeneArgs {
+  const float* w32;
+  Layout L;
+  Spline sp;
+  const int* off;     // CSR offsets, G+1
+  const int* nbr;     // CSR entries: (j, pair index, orientation) x nnz
+  const float* Gm;
+  const float* m;
+  const double* pmarg;
+  const float* xs;
+  const float* scl;
+  const int* amin;
+  const int* amax;
+  float* dX;
+  int64_t lddx;
+};
+
+constexpr int kMaxSPerThread = 8;   // S <= 256 * 8 = 2048 samples in the adjoint
+
+template <int NB, int NO>
+__global__ __launch_bounds__(256) void bspline_gene_adjoint_kernel(GeneArgs a) {
+  const int64_t i = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int bins = a.L.bins;
+  const int64_t S = a.L.S, Sp = a.L.Sp;
+  __shared__ double shd[4];
+  __shared__ float gsh[kMaxBins * kMaxBins];
+  __shared__ float phi_i[kMaxBins];
+
+  const int e0 = a.off[i], e1 = a.off[i + 1];
+  // c_i * phi(p_i[b]) with c_i = sum of m over the gene's entries
+  if (tid < bins) {
+    double ci = 0.0;
+    for (int e = e0; e < e1; ++e) ci += (double)a.m[a.nbr[3 * e + 1]];
+    const double p = a.pmarg[i * bins + tid];
+    phi_i[tid] = p > 0.0 ? (float)(ci * (-(log2(p) + 1.4426950408889634))) : 0.0f;
+  }
+  float accw[kMaxSPerThread][NB];
+  const int nper = (int)ceil_div(S, 256);
+#pragma unroll
+  for (int q = 0; q < kMaxSPerThread; ++q)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) accw[q][b] = 0.0f;
+
+  for (int e = e0; e < e1; ++e) {
+    const int j = a.nbr[3 * e + 0], n = a.nbr[3 * e + 1], ori = a.nbr[3 * e + 2];
+    __syncthreads();
+    for (int t = tid; t < bins * bins; t += 256) {
+      const int b = t / bins, c = t % bins;
+      // orientation 0: gene i is the row gene of the pair's table; 1: the column gene
+      gsh[t] = ori == 0 ? a.Gm[(int64_t)n * bins * bins + b * bins + c] : a.Gm[(int64_t)n * bins * bins + c * bins + b];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMaxSPerThread; ++q) {
+      const int64_t s = tid + 256 * q;
+      if (q >= nper || s >= S) break;
+      float wj[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) wj[c] = c < bins ? a.w32[a.L.row_of(j, c) * Sp + s] : 0.0f;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (b >= bins) break;
+        float v = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c < bins) v += gsh[b * bins + c] * wj[c];
+        accw[q][b] += v;
+      }
+    }
+  }
+  __syncthreads();
+  // g_s = dL/dxhat_s = sum_b dL/dW[b][s] * N'_b(xhat_s),  dL/dW = (phi_i - acc) / S
+  const float invS = 1.0f / (float)S;
+  float gs[kMaxSPerThread];
+  double s_lo = 0.0, s_hi = 0.0;
+#pragma unroll
+  for (int q = 0; q < kMaxSPerThread; ++q) {
+    const int64_t s = tid + 256 * q;
+    gs[q] = 0.0f;
+    if (q >= nper || s >= S) continue;
+    const float t = a.xs[i * S + s];
+    float w[NB], lower[NB + 1], d[NB];
+    basis_values<NB, NO>(a.sp, t, w, lower);
+    basis_derivs<NB>(a.sp, lower, d);
+    float g = 0.0f;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (b < bins) g += (phi_i[b] - accw[q][b]) * invS * d[b];
+    gs[q] = g;
+    s_lo += (double)g * ((double)t - 1.0);
+    s_hi += (double)g * (double)t;
+  }
+  const double sum_lo = block_sum<double>(s_lo, shd);
+  const double sum_hi = block_sum<double>(s_hi, shd);
+  const float scale = a.scl[i];
+  const int imn = a.amin[i], imx = a.amax[i];
+  const bool constant = !(scale != 1.0f || imn != imx);
+#pragma unroll
+  for (int q = 0; q < kMaxSPerThread; ++q) {
+    const int64_t s = tid + 256 * q;
+    if (q >= nper || s >= S) continue;
+    double v = (double)gs[q] * (double)scale;
+    if (!constant) {
+      if (s == imn) v += sum_lo * (double)scale;
+      if (s == imx) v -= sum_hi * (double)scale;
+    }
+    a.dX[i * a.lddx + s] = (float)v;
+  }
+}
+
+// ---------------------------------------------------------------- host side
+int check_spline_args(mcpg_ctx* h, int bins, int order) {
+  if (order < 1 || order > kMaxOrder) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "spline order must be in [1, 8]");
+  if (bins < order) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bins must be >= spline order");
+  if (bins > kMaxBins) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "bins must be <= 80");
+  return MCPG_OK;
+}
+
+int prep_launch(mcpg_ctx* h, const float* dX, int64_t ldx, const Layout& L, const Spline& sp, int do_scale,
+                const PrepOut& o) {
+  mcpg::LaunchScope ls(h, "bspline_prep_kernel");
+  if (sp.bins <= 16 && sp.order <= 4)
+    hipLaunchKernelGGL((bspline_prep_kernel<16, 4>), dim3((unsigned)L.G), dim3(kThreads), 0, h->stream, dX, ldx, L, sp,
+                       do_scale, o);
+  else
+    hipLaunchKernelGGL((bspline_prep_kernel<kMaxBins, kMaxOrder>), dim3((unsigned)L.G), dim3(kThreads), 0, h->stream, dX,
+                       ldx, L, sp, do_scale, o);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mcpg::hip_error(h, e, "bspline_prep_kernel launch");
+  return MCPG_OK;
+}
+
+int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi, const uint16_t* wlo,
+              const double* H64, const Layout& L, float* MI, int64_t ldmi) {
+  MIArgs a;
+  a.w32 = w32; a.whi = whi; a.wlo = wlo; a.H64 = H64; a.MI = MI; a.ldmi = ldmi; a.L = L;
+  a.ntri = L.ntiles * (L.ntiles + 1) / 2;
+  a.norm = 1.0f / (float)L.S;
+  if (a.ntri > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "MI grid too large");
+  hipError_t e;
+  {
+    const char* name = precision == MCPG_FP32 ? "bspline_mi_kernel<fp32>"
+                       : precision == MCPG_BF16 ? "bspline_mi_kernel<bf16>" : "bspline_mi_kernel<bf16x3>";
+    mcpg::LaunchScope ls(h, name);
+    switch (precision) {
+      case MCPG_FP32:
+        hipLaunchKernelGGL(bspline_mi_kernel<kFP32>, dim3((unsigned)a.ntri), dim3(kThreads), mi_smem_bytes<kFP32>(), h->stream, a);
+        break;
+      case MCPG_BF16:
+        hipLaunchKernelGGL(bspline_mi_kernel<kBF16>, dim3((unsigned)a.ntri), dim3(kThreads), mi_smem_bytes<kBF16>(), h->stream, a);
+        break;
+      default:
+        hipLaunchKernelGGL(bspline_mi_kernel<kBF16X3>, dim3((unsigned)a.ntri), dim3(kThreads), mi_smem_bytes<kBF16X3>(), h->stream, a);
+        break;
+    }
+  }
+  e = hipGetLastError();
+  if (e != hipSuccess) return mcpg::hip_error(h, e, "bspline_mi_kernel launch");
+  return MCPG_OK;
+}
+
+struct WeightBufs {
+  float* w32 = nullptr;
+  uint16_t* whi = nullptr;
+  uint16_t* wlo = nullptr;
+  double* H64 = nullptr;
+};
+
+void size_weight_bufs(mcpg::Sizer& sz, const Layout& L, int precision) {
+  const size_t n = (size_t)L.rows * L.Sp;
+  if (precision == MCPG_FP32) sz.add<float>(n);
+  else { sz.add<uint16_t>(n); if (precision == MCPG_BF16X3) sz.add<uint16_t>(n); }
+  sz.add<double>(L.ntiles * L.gt);
+}
+
+WeightBufs take_weight_bufs(mcpg::Carver& cv, const Layout& L, int precision) {
+  WeightBufs w;
+  const size_t n = (size_t)L.rows * L.Sp;
+  if (precision == MCPG_FP32) w.w32 = cv.take<float>(n);
+  else { w.whi = cv.take<uint16_t>(n); if (precision == MCPG_BF16X3) w.wlo = cv.take<uint16_t>(n); }
+  w.H64 = cv.take<double>(L.ntiles * L.gt);
+  return w;
+}
+
+int clear_weight_bufs(mcpg_ctx* h, const WeightBufs& w, const Layout& L) {
+  const size_t n = (size_t)L.rows * L.Sp;
+  if (w.w32) MCPG_HIP(h, hipMemsetAsync(w.w32, 0, n * sizeof(float), h->stream));
+  if (w.whi) MCPG_HIP(h, hipMemsetAsync(w.whi, 0, n * sizeof(uint16_t), h->stream));
+  if (w.wlo) MCPG_HIP(h, hipMemsetAsync(w.wlo, 0, n * sizeof(uint16_t), h->stream));
+  return MCPG_OK;
+}
+
+int check_flags_nonfinite(mcpg_ctx* h, const char* what) {
+  int flags[1];
+  int st = mcpg::read_flags(h, flags, 1);
+  if (st) return st;
+  if (flags[0] & 1) return mcpg::set_error(h, MCPG_ERR_NONFINITE, std::string(what) + ": non-finite input value");
+  return MCPG_OK;
+}
+
+bool valid_precision(int p) { return p == MCPG_FP32 || p == MCPG_BF16 || p == MCPG_BF16X3; }
+
+}  // namespace
+
+extern "C" {
+
+int mcpg_minmax_scale_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, float* Y,
+                          int64_t ldy) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0 || S == 0) return MCPG_OK;
+  if (!X || !Y) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldx < S || ldy < S) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const Layout L = make_layout(G, S, 10);
+  const Spline sp = make_spline(10, 3);
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) sz.add<float>(G * S);
+  sz.add<float>(G * S);
+  int st;
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const float* dX = X;
+  int64_t lx = ldx;
+  if (ptrs == MCPG_HOST) {
+    float* t = cv.take<float>(G * S);
+    if ((st = mcpg::copy_2d(h, t, S * 4, X, ldx * 4, S * 4, G, hipMemcpyHostToDevice))) return st;
+    dX = t; lx = S;
+  }
+  float* xs = cv.take<float>(G * S);
+  if ((st = mcpg::reset_flags(h))) return st;
+  PrepOut o{};
+  o.xs = xs; o.flags = h->d_flags;
+  if ((st = prep_launch(h, dX, lx, L, sp, 1, o))) return st;
+  if (ptrs == MCPG_HOST) {
+    if ((st = mcpg::copy_2d(h, Y, ldy * 4, xs, S * 4, S * 4, G, hipMemcpyDeviceToHost))) return st;
+  } else {
+    if ((st = mcpg::copy_2d(h, Y, ldy * 4, xs, S * 4, S * 4, G, hipMemcpyDeviceToDevice))) return st;
+  }
+  if ((st = mcpg::finish(h, "mcpg_minmax_scale_f32"))) return st;
+  return check_flags_nonfinite(h, "mcpg_minmax_scale_f32");
+}
+
+int mcpg_bspline_weights_f32(mcpg_handle h, int ptrs, const float* Xs, int64_t G, int64_t S, int64_t ldx, int bins,
+                             int order, float* W, int64_t ldw) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  int st;
+  if ((st = check_spline_args(h, bins, order))) return st;
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (!Xs || !W) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldx < S || ldw < bins * S + 1) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const Layout L = make_layout(G, S, bins);
+  const Spline sp = make_spline(bins, order);
+  const int64_t wrow = (int64_t)bins * S + 1;
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) { sz.add<float>(G * S); sz.add<float>(G * wrow); }
+  int64_t lw = ldw;
+  void* base = sz.off ? mcpg::workspace(h, sz.off, &st) : h->ws;
+  if (sz.off && !base) return st;
+  mcpg::Carver cv(base);
+  const float* dX = Xs;
+  int64_t lx = ldx;
+  float* dW = W;
+  if (ptrs == MCPG_HOST) {
+    float* t = cv.take<float>(G * S);
+    if ((st = mcpg::copy_2d(h, t, S * 4, Xs, ldx * 4, S * 4, G, hipMemcpyHostToDevice))) return st;
+    dX = t; lx = S;
+    dW = cv.take<float>(G * wrow);
+    lw = wrow;
+  }
+  if ((st = mcpg::reset_flags(h))) return st;
+  PrepOut o{};
+  o.wref = dW; o.ldw = lw; o.flags = h->d_flags;
+  if ((st = prep_launch(h, dX, lx, L, sp, 0, o))) return st;
+  if (ptrs == MCPG_HOST &&
+      (st = mcpg::copy_2d(h, W, ldw * 4, dW, wrow * 4, wrow * 4, G, hipMemcpyDeviceToHost)))
+    return st;
+  if ((st = mcpg::finish(h, "mcpg_bspline_weights_f32"))) return st;
+  return check_flags_nonfinite(h, "mcpg_bspline_weights_f32");
+}
+
+int mcpg_bspline_mi_weights_f32(mcpg_handle h, int ptrs, const float* W, int64_t G, int64_t S, int64_t ldw, int bins,
+                                int precision, float* MI, int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (bins < 1 || bins > kMaxBins) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bins must be in [1, 80]");
+  if (!valid_precision(precision)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad precision");
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (!W || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldw < bins * S + 1 || ldmi < G) return mcpg::set_error(h, MCPG_ERR_SHAPE, "weight rows must hold bins*S+1 values");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const Layout L = make_layout(G, S, bins);
+  const int64_t wrow = (int64_t)bins * S + 1;
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) { sz.add<float>(G * wrow); sz.add<float>(G * G); }
+  size_weight_bufs(sz, L, precision);
+  int st;
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const float* dW = W;
+  int64_t lw = ldw;
+  float* dMI = MI;
+  int64_t lm = ldmi;
+  if (ptrs == MCPG_HOST) {
+    float* t = cv.take<float>(G * wrow);
+    if ((st = mcpg::copy_2d(h, t, wrow * 4, W, ldw * 4, wrow * 4, G, hipMemcpyHostToDevice))) return st;
+    dW = t; lw = wrow;
+    dMI = cv.take<float>(G * G); lm = G;
+  }
+  WeightBufs wb = take_weight_bufs(cv, L, precision);
+  if ((st = clear_weight_bufs(h, wb, L))) return st;
+  if ((st = mcpg::reset_flags(h))) return st;
+  {
+    mcpg::LaunchScope ls(h, "weights_import_kernel");
+    hipLaunchKernelGGL(weights_import_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(wrow, 256), 64), (unsigned)G),
+                       dim3(256), 0, h->stream, dW, lw, L, wb.w32, wb.whi, wb.wlo, wb.H64, h->d_flags);
+  }
+  if ((st = mi_launch(h, precision, wb.w32, wb.whi, wb.wlo, wb.H64, L, dMI, lm))) return st;
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, G * 4, G * 4, G, hipMemcpyDeviceToHost))) return st;
+  if ((st = mcpg::finish(h, "mcpg_bspline_mi_weights_f32"))) return st;
+  return check_flags_nonfinite(h, "mcpg_bspline_mi_weights_f32");
+}
+
+int mcpg_bspline_mi_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int bins, int order,
+                        int precision, float* MI, int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  int st;
+  if ((st = check_spline_args(h, bins, order))) return st;
+  if (!valid_precision(precision)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad precision");
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (!X || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldx < S || ldmi < G) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const Layout L = make_layout(G, S, bins);
+  const Spline sp = make_spline(bins, order);
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) { sz.add<float>(G * S); sz.add<float>(G * G); }
+  size_weight_bufs(sz, L, precision);
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const float* dX = X;
+  int64_t lx = ldx;
+  float* dMI = MI;
+  int64_t lm = ldmi;
+  if (ptrs == MCPG_HOST) {
+    float* t = cv.take<float>(G * S);
+    if ((st = mcpg::copy_2d(h, t, S * 4, X, ldx * 4, S * 4, G, hipMemcpyHostToDevice))) return st;
+    dX = t; lx = S;
+    dMI = cv.take<float>(G * G); lm = G;
+  }
+  WeightBufs wb = take_weight_bufs(cv, L, precision);
+  if ((st = clear_weight_bufs(h, wb, L))) return st;
+  if ((st = mcpg::reset_flags(h))) return st;
+  PrepOut o{};
+  o.w32 = wb.w32; o.whi = wb.whi; o.wlo = wb.wlo; o.H64 = wb.H64; o.flags = h->d_flags;
+  if ((st = prep_launch(h, dX, lx, L, sp, 1, o))) return st;
+  if ((st = mi_launch(h, precision, wb.w32, wb.whi, wb.wlo, wb.H64, L, dMI, lm))) return st;
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, G * 4, G * 4, G, hipMemcpyDeviceToHost))) return st;
+  if ((st = mcpg::finish(h, "mcpg_bspline_mi_f32"))) return st;
+  return check_flags_nonfinite(h, "mcpg_bspline_mi_f32");
+}
+
+int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int bins,
+                                 int order, const int32_t* pi, const int32_t* pj, const float* m, int64_t npairs,
+                                 float* dX, int64_t lddx, double* L_out) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  int st;
+  if ((st = check_spline_args(h, bins, order))) return st;
+  if (G < 0 || S < 0 || npairs < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (S > 256 * kMaxSPerThread) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "adjoint supports S <= 2048");
+  if (bins > 16 || order > 4) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "adjoint supports bins <= 16, order <= 4");
+  if (!X || !dX || (npairs > 0 && (!pi || !pj || !m))) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null pointer");
+  if (ldx < S || lddx < S) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  if (npairs > (int64_t)INT32_MAX / 2) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "too many pairs");
+  // CSR of the symmetric pair list: gene -> (other gene, pair index, orientation)
+  std::vector<int> off(G + 1, 0), nbr;
+  for (int64_t n = 0; n < npairs; ++n) {
+    if (pi[n] < 0 || pi[n] >= G || pj[n] < 0 || pj[n] >= G)
+      return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "pair index out of range");
+    off[pi[n] + 1]++;
+    off[pj[n] + 1]++;
+  }
+  for (int64_t g = 0; g < G; ++g) off[g + 1] += off[g];
+  nbr.resize(3 * (size_t)off[G] + 3);
+  {
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int64_t n = 0; n < npairs; ++n) {
+      int e = fill[pi[n]]++;
+      nbr[3 * e] = pj[n]; nbr[3 * e + 1] = (int)n; nbr[3 * e + 2] = 0;
+      e = fill[pj[n]]++;
+      nbr[3 * e] = pi[n]; nbr[3 * e + 1] = (int)n; nbr[3 * e + 2] = 1;
+    }
+  }
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const Layout L = make_layout(G, S, bins);
+  const Spline sp = make_spline(bins, order);
+  const int64_t np1 = npairs > 0 ? npairs : 1;
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) { sz.add<float>(G * S); sz.add<float>(G * S); }
+  size_weight_bufs(sz, L, MCPG_FP32);
+  sz.add<int>(np1); sz.add<int>(np1); sz.add<float>(np1);
+  sz.add<int>(G + 1); sz.add<int>(nbr.size());
+  sz.add<float>(np1 * bins * bins); sz.add<double>(np1);
+  sz.add<double>(G * bins); sz.add<float>(G * S); sz.add<float>(G); sz.add<int>(G); sz.add<int>(G);
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const float* dXin = X;
+  int64_t lx = ldx;
+  float* ddX = dX;
+  int64_t ldd = lddx;
+  if (ptrs == MCPG_HOST) {
+    float* t = cv.take<float>(G * S);
+    if ((st = mcpg::copy_2d(h, t, S * 4, X, ldx * 4, S * 4, G, hipMemcpyHostToDevice))) return st;
+    dXin = t; lx = S;
+    ddX = cv.take<float>(G * S); ldd = S;
+  }
+  WeightBufs wb = take_weight_bufs(cv, L, MCPG_FP32);
+  int* d_pi = cv.take<int>(np1);
+  int* d_pj = cv.take<int>(np1);
+  float* d_m = cv.take<float>(np1);
+  int* d_off = cv.take<int>(G + 1);
+  int* d_nbr = cv.take<int>(nbr.size());
+  float* d_Gm = cv.take<float>(np1 * bins * bins);
+  double* d_Lp = cv.take<double>(np1);
+  double* d_pm = cv.take<double>(G * bins);
+  float* d_xs = cv.take<float>(G * S);
+  float* d_scl = cv.take<float>(G);
+  int* d_amin = cv.take<int>(G);
+  int* d_amax = cv.take<int>(G);
+  if (npairs > 0) {
+    MCPG_HIP(h, hipMemcpyAsync(d_pi, pi, npairs * 4, hipMemcpyHostToDevice, h->stream));
+    MCPG_HIP(h, hipMemcpyAsync(d_pj, pj, npairs * 4, hipMemcpyHostToDevice, h->stream));
+    MCPG_HIP(h, hipMemcpyAsync(d_m, m, npairs * 4, hipMemcpyHostToDevice, h->stream));
+  }
+  MCPG_HIP(h, hipMemcpyAsync(d_off, off.data(), (G + 1) * 4, hipMemcpyHostToDevice, h->stream));
+  MCPG_HIP(h, hipMemcpyAsync(d_nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice, h->stream));
+  if ((st = clear_weight_bufs(h, wb, L))) return st;
+  if ((st = mcpg::reset_flags(h))) return st;
+  PrepOut o{};
+  o.w32 = wb.w32; o.H64 = wb.H64; o.pmarg = d_pm; o.xs = d_xs; o.scl = d_scl; o.amin = d_amin; o.amax = d_amax;
+  o.flags = h->d_flags;
+  if ((st = prep_launch(h, dXin, lx, L, sp, 1, o))) return st;
+  if (npairs > 0) {
+    PairArgs pa;
+    pa.w32 = wb.w32; pa.L = L; pa.pi = d_pi; pa.pj = d_pj; pa.m = d_m; pa.npairs = npairs; pa.H64 = wb.H64;
+    pa.norm = 1.0f / (float)S; pa.Gm = d_Gm; pa.Lpart = d_Lp;
+    mcpg::LaunchScope ls(h, "bspline_pair_adjoint_kernel");
+    hipLaunchKernelGGL(bspline_pair_adjoint_kernel<16>, dim3((unsigned)ceil_div(npairs, 4)), dim3(256), 0, h->stream, pa);
+  }
+  GeneArgs ga;
+  ga.w32 = wb.w32; ga.L = L; ga.sp = sp; ga.off = d_off; ga.nbr = d_nbr; ga.Gm = d_Gm; ga.m = d_m; ga.pmarg = d_pm;
+  ga.xs = d_xs; ga.scl = d_scl; ga.amin = d_amin; ga.amax = d_amax; ga.dX = ddX; ga.lddx = ldd;
+  {
+    mcpg::LaunchScope ls(h, "bspline_gene_adjoint_kernel");
+    hipLaunchKernelGGL((bspline_gene_adjoint_kernel<16, 4>), dim3((unsigned)G), dim3(256), 0, h->stream, ga);
+  }
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, dX, lddx * 4, ddX, S * 4, S * 4, G, hipMemcpyDeviceToHost))) return st;
+  if (L_out) {
+    std::vector<double> lp(np1, 0.0);
+    if (npairs > 0) MCPG_HIP(h, hipMemcpyAsync(lp.data(), d_Lp, npairs * 8, hipMemcpyDeviceToHost, h->stream));
+    if ((st = mcpg::finish(h, "mcpg_bspline_mi_backward_f32"))) return st;
+    double Ls = 0.0;
+    for (int64_t n = 0; n < npairs; ++n) Ls += lp[n];
+    *L_out = Ls;
+  }
+  if ((st = mcpg::finish(h, "mcpg_bspline_mi_backward_f32"))) return st;
+  return check_flags_nonfinite(h, "mcpg_bspline_mi_backward_f32");
+}
+
+}  // extern "C"

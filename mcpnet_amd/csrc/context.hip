@@ -1,0 +1,200 @@
+// Handle lifecycle, error reporting and workspace management of libmcpgpu.
+#include <cstdio>
+#include <new>
+
+#include "internal.h"
+
+namespace mcpg {
+
+int set_error(mcpg_ctx* h, int status, const std::string& msg) {
+  if (h) h->last_error = msg;
+  return status;
+}
+
+int hip_error(mcpg_ctx* h, hipError_t e, const char* where) {
+  std::string msg = std::string(where) + ": " + hipGetErrorString(e);
+  int status = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? MCPG_ERR_OUT_OF_MEMORY : MCPG_ERR_HIP;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) status = MCPG_ERR_NO_DEVICE;
+  (void)hipGetLastError();  // clear the sticky-free error so later calls on the handle start clean
+  return set_error(h, status, msg);
+}
+
+void* workspace(mcpg_ctx* h, size_t bytes, int* status) {
+  *status = MCPG_OK;
+  if (bytes <= h->ws_bytes) return h->ws;
+  if (h->ws) {
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) { *status = hip_error(h, e, "hipStreamSynchronize"); return nullptr; }
+    (void)hipFree(h->ws);
+    h->ws = nullptr;
+    h->ws_bytes = 0;
+  }
+  size_t want = align_up(bytes, size_t(1) << 20);
+  hipError_t e = hipMalloc(&h->ws, want);
+  if (e != hipSuccess) {
+    h->ws = nullptr;
+    *status = hip_error(h, e, "hipMalloc(workspace)");
+    char buf[96];
+    snprintf(buf, sizeof buf, " (requested %zu bytes)", want);
+    h->last_error += buf;
+    return nullptr;
+  }
+  h->ws_bytes = want;
+  return h->ws;
+}
+
+int copy_2d(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
+            size_t rows, hipMemcpyKind kind) {
+  if (rows == 0 || width_bytes == 0) return MCPG_OK;
+  MCPG_HIP(h, hipMemcpy2DAsync(dst, dpitch, src, spitch, width_bytes, rows, kind, h->stream));
+  return MCPG_OK;
+}
+
+int reset_flags(mcpg_ctx* h) {
+  MCPG_HIP(h, hipMemsetAsync(h->d_flags, 0, 16 * sizeof(int), h->stream));
+  return MCPG_OK;
+}
+
+int read_flags(mcpg_ctx* h, int* flags, int n) {
+  MCPG_HIP(h, hipMemcpyAsync(flags, h->d_flags, n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  MCPG_HIP(h, hipStreamSynchronize(h->stream));
+  return MCPG_OK;
+}
+
+int finish(mcpg_ctx* h, const char* where) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_error(h, e, where);
+  e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return hip_error(h, e, where);
+  return MCPG_OK;
+}
+
+void begin_call(mcpg_ctx* h) {
+  h->last_launches = 0;
+  h->last_kernel = "";
+  h->timings.clear();
+  h->events_used = 0;
+}
+
+static hipEvent_t pooled_event(mcpg_ctx* h) {
+  if (h->events_used == h->event_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    h->event_pool.push_back(e);
+  }
+  return h->event_pool[h->events_used++];
+}
+
+LaunchScope::LaunchScope(mcpg_ctx* hh, const char* name) : h(hh) {
+  h->last_launches++;
+  h->last_kernel = name;
+  if (!h->profiling) return;
+  hipEvent_t start = pooled_event(h);
+  stop = pooled_event(h);
+  if (!start || !stop) { stop = nullptr; return; }
+  (void)hipEventRecord(start, h->stream);
+  h->timings.push_back(mcpg_ctx::Timing{name, start, stop});
+}
+
+LaunchScope::~LaunchScope() {
+  if (stop) (void)hipEventRecord(stop, h->stream);
+}
+
+}  // namespace mcpg
+
+extern "C" {
+
+int mcpg_create(mcpg_handle* out, int device) {
+  if (!out) return MCPG_ERR_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MCPG_ERR_NO_DEVICE;
+  if (device < 0 || device >= n) return MCPG_ERR_NO_DEVICE;
+  mcpg_ctx* h = new (std::nothrow) mcpg_ctx();
+  if (!h) return MCPG_ERR_OUT_OF_MEMORY;
+  h->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete h; return MCPG_ERR_NO_DEVICE; }
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return MCPG_ERR_HIP; }
+  if (hipMalloc(&h->d_flags, 16 * sizeof(int)) != hipSuccess) {
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return MCPG_ERR_OUT_OF_MEMORY;
+  }
+  *out = h;
+  return MCPG_OK;
+}
+
+int mcpg_destroy(mcpg_handle h) {
+  if (!h) return MCPG_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
+  if (h->ws) (void)hipFree(h->ws);
+  if (h->d_flags) (void)hipFree(h->d_flags);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return MCPG_OK;
+}
+
+const char* mcpg_status_string(int status) {
+  switch (status) {
+    case MCPG_OK: return "ok";
+    case MCPG_ERR_INVALID_ARG: return "invalid argument";
+    case MCPG_ERR_SHAPE: return "shape mismatch";
+    case MCPG_ERR_NONFINITE: return "non-finite input";
+    case MCPG_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case MCPG_ERR_HIP: return "HIP runtime error";
+    case MCPG_ERR_NO_DEVICE: return "no usable HIP device";
+    case MCPG_ERR_UNSUPPORTED: return "unsupported size";
+    default: return "unknown status";
+  }
+}
+
+const char* mcpg_last_error(mcpg_handle h) { return h ? h->last_error.c_str() : "null handle"; }
+
+int mcpg_synchronize(mcpg_handle h) {
+  MCPG_CHECK_HANDLE(h);
+  MCPG_HIP(h, hipSetDevice(h->device));
+  MCPG_HIP(h, hipStreamSynchronize(h->stream));
+  return MCPG_OK;
+}
+
+int mcpg_get_stream(mcpg_handle h, void** stream) {
+  MCPG_CHECK_HANDLE(h);
+  if (!stream) return MCPG_ERR_INVALID_ARG;
+  *stream = reinterpret_cast<void*>(h->stream);
+  return MCPG_OK;
+}
+
+int mcpg_last_launch_info(mcpg_handle h, int* launches, const char** dominant_kernel) {
+  MCPG_CHECK_HANDLE(h);
+  if (launches) *launches = h->last_launches;
+  if (dominant_kernel) *dominant_kernel = h->last_kernel;
+  return MCPG_OK;
+}
+
+int mcpg_set_profiling(mcpg_handle h, int enable) {
+  MCPG_CHECK_HANDLE(h);
+  h->profiling = enable != 0;
+  return MCPG_OK;
+}
+
+int mcpg_last_timings(mcpg_handle h, int max_entries, const char** names, float* ms, int* count) {
+  MCPG_CHECK_HANDLE(h);
+  if (!count) return MCPG_ERR_INVALID_ARG;
+  MCPG_HIP(h, hipSetDevice(h->device));
+  MCPG_HIP(h, hipStreamSynchronize(h->stream));
+  const int n = (int)h->timings.size();
+  *count = n;
+  for (int i = 0; i < n && i < max_entries; ++i) {
+    if (names) names[i] = h->timings[i].name;
+    if (ms) {
+      float t = 0.0f;
+      MCPG_HIP(h, hipEventElapsedTime(&t, h->timings[i].start, h->timings[i].stop));
+      ms[i] = t;
+    }
+  }
+  return MCPG_OK;
+}
+
+}  // extern "C"

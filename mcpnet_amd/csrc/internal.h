@@ -1,0 +1,108 @@
+// Internal helpers shared by the HIP translation units of libmcpgpu.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "mcpgpu/mcpgpu.h"
+
+struct mcpg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  void* ws = nullptr;        // grow-only device workspace
+  size_t ws_bytes = 0;
+  int* d_flags = nullptr;    // small device array of error flags
+  int last_launches = 0;
+  const char* last_kernel = "";
+  // per-launch timing of the most recent call (only when profiling is on)
+  bool profiling = false;
+  struct Timing {
+    const char* name;
+    hipEvent_t start, stop;
+  };
+  std::vector<Timing> timings;
+  std::vector<hipEvent_t> event_pool;
+  size_t events_used = 0;
+};
+
+namespace mcpg {
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t x, size_t a = kAlign) { return (x + a - 1) / a * a; }
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+int set_error(mcpg_ctx* h, int status, const std::string& msg);
+int hip_error(mcpg_ctx* h, hipError_t e, const char* where);
+
+// Ensures the workspace holds at least `bytes`; returns its base pointer (or nullptr).
+void* workspace(mcpg_ctx* h, size_t bytes, int* status);
+
+// Carves aligned sub-buffers out of a base pointer.
+struct Carver {
+  char* base;
+  size_t off = 0;
+  explicit Carver(void* b) : base(static_cast<char*>(b)) {}
+  template <typename T>
+  T* take(size_t count) {
+    T* p = reinterpret_cast<T*>(base + off);
+    off = align_up(off + count * sizeof(T));
+    return p;
+  }
+};
+// Size counter with the same alignment rule as Carver.
+struct Sizer {
+  size_t off = 0;
+  template <typename T>
+  void add(size_t count) { off = align_up(off + count * sizeof(T)); }
+};
+
+// 2-D strided copies between host/device buffers on the handle's stream.
+int copy_2d(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
+            size_t rows, hipMemcpyKind kind);
+
+// Clears the device error flags and reads them back (synchronising the stream).
+int reset_flags(mcpg_ctx* h);
+int read_flags(mcpg_ctx* h, int* flags, int n);
+
+int finish(mcpg_ctx* h, const char* where);  // stream sync + error check
+
+// Starts an entry point: clears the launch counters and the timing list of the previous call.
+void begin_call(mcpg_ctx* h);
+
+// Brackets one kernel launch with HIP events on the handle's stream when profiling is on.
+struct LaunchScope {
+  mcpg_ctx* h;
+  hipEvent_t stop = nullptr;
+  LaunchScope(mcpg_ctx* hh, const char* name);
+  ~LaunchScope();
+};
+
+// Maps a physical block id onto a logical tile id so that each XCD of the chip (blocks are
+// dispatched round-robin over 8 XCDs) works on one contiguous range of logical tiles, which
+// keeps the operand panels those tiles share inside that XCD's L2.
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t total) {
+  constexpr int64_t kXcd = 8;
+  const int64_t x = b % kXcd;
+  const int64_t local = b / kXcd;
+  const int64_t q = total / kXcd, r = total % kXcd;
+  const int64_t prefix = x * q + (x < r ? x : r);
+  return prefix + local;
+}
+
+}  // namespace mcpg
+
+#define MCPG_CHECK_HANDLE(h)                                   \
+  do {                                                         \
+    if (!(h)) return MCPG_ERR_INVALID_ARG;                     \
+  } while (0)
+
+#define MCPG_HIP(h, call)                                      \
+  do {                                                         \
+    hipError_t e_ = (call);                                    \
+    if (e_ != hipSuccess) return mcpg::hip_error((h), e_, #call); \
+  } while (0)

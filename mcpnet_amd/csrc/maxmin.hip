@@ -1,0 +1,493 @@
+// Max-min ("tropical") products C[i][j] = max_k min(A[i][k], B[j][k]), the guarded ratio and the
+// MCP2/3/4 pipelines built from them.
+//
+// The product is a semiring contraction with no rounding, so it runs on the VALU (MFMA has no
+// max/min form).  FP32 values are mapped to int32 keys whose signed order equals the float
+// order (negative floats get their magnitude bits flipped); the inner loop is then one
+// v_min_i32 per triple plus one v_max3_i32 per two triples, and the result is exact for every
+// tiling and reduction order.  FP64 uses v_min_f64 / v_max_f64.
+//
+// Tiling: a 256-thread workgroup owns a 128 x 128 output tile; each thread keeps an 8 x 8 block
+// of accumulators (rows ty+16m, cols tx+16n).  K advances in chunks of 128 bytes per row, staged
+// through LDS rows padded to 144 bytes so the 16-byte row reads of one wave hit distinct banks;
+// the next chunk is prefetched into registers while the current one is consumed.
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <type_traits>
+
+#include "internal.h"
+
+namespace {
+
+using mcpg::ceil_div;
+
+// ---------------------------------------------------------------- element traits
+struct TrF32 {
+  using T = float;
+  using E = int;                                   // LDS / register element (order key)
+  using V = int4;                                  // 16-byte vector of E
+  static constexpr int kVec = 4;
+  __device__ static __forceinline__ E key(float x) {
+    int b = __float_as_int(x);
+    return b ^ ((b >> 31) & 0x7fffffff);
+  }
+  __device__ static __forceinline__ float val(E k) {
+    int b = k ^ ((k >> 31) & 0x7fffffff);
+    return __int_as_float(b);
+  }
+  __device__ static __forceinline__ E pad() { return INT_MIN; }      // neutral: below every key
+  __device__ static __forceinline__ E init() { return key(-INFINITY); }  // (float)lowest<double>() == -inf
+  __device__ static __forceinline__ E mn(E a, E b) { return a < b ? a : b; }
+  __device__ static __forceinline__ E mx(E a, E b) { return a > b ? a : b; }
+  // acc = max(acc, min(a.i, b.i)) over the 4 lanes of a 16-byte chunk
+  __device__ static __forceinline__ E step(E acc, const V& a, const V& b) {
+    E m0 = mn(a.x, b.x), m1 = mn(a.y, b.y), m2 = mn(a.z, b.z), m3 = mn(a.w, b.w);
+    acc = mx(acc, mx(m0, m1));
+    acc = mx(acc, mx(m2, m3));
+    return acc;
+  }
+  __device__ static __forceinline__ V vmin(const V& a, const V& b) {
+    return V{mn(a.x, b.x), mn(a.y, b.y), mn(a.z, b.z), mn(a.w, b.w)};
+  }
+};
+
+struct TrF64 {
+  using T = double;
+  using E = double;
+  using V = double2;
+  static constexpr int kVec = 2;
+  __device__ static __forceinline__ E key(double x) { return x; }
+  __device__ static __forceinline__ double val(E k) { return k; }
+  __device__ static __forceinline__ E pad() { return -INFINITY; }
+  __device__ static __forceinline__ E init() { return -DBL_MAX; }  // numeric_limits<double>::lowest()
+  __device__ static __forceinline__ E mn(E a, E b) { return __builtin_fmin(a, b); }
+  __device__ static __forceinline__ E mx(E a, E b) { return __builtin_fmax(a, b); }
+  __device__ static __forceinline__ E step(E acc, const V& a, const V& b) {
+    return mx(acc, mx(mn(a.x, b.x), mn(a.y, b.y)));
+  }
+  __device__ static __forceinline__ V vmin(const V& a, const V& b) { return V{mn(a.x, b.x), mn(a.y, b.y)}; }
+};
+
+constexpr int kBM = 128;          // tile rows
+constexpr int kBN = 128;          // tile cols
+constexpr int kThreads = 256;
+constexpr int kRowBytes = 128;    // K chunk per row, bytes
+constexpr int kLdsRowBytes = 144; // padded LDS row
+constexpr int kChunks = kRowBytes / 16;
+
+// ratio_kernel semantics (combine.hpp:214-220)
+template <typename T>
+__device__ __forceinline__ T ratio_of(T a, T m) {
+  const T eps = sizeof(T) == 4 ? (T)FLT_EPSILON : (T)DBL_EPSILON;
+  const T big = sizeof(T) == 4 ? (T)FLT_MAX : (T)DBL_MAX;
+  if (fabs(a) < eps) return (T)0;
+  if (fabs(m) < eps) return big;
+  return (T)((double)a / (double)m);
+}
+
+struct MMArgs {
+  const void* A; int64_t lda;
+  const void* B; int64_t ldb;
+  const void* kmask;       // nullable
+  int64_t M, N, K;
+  void* C; int64_t ldc;    // nullable (ratio-only output)
+  const void* R; int64_t ldr;  // ratio numerator (nullable: no fused ratio)
+  void* O; int64_t ldo;        // ratio output
+  int tri;                 // compute only tiles bm <= bn and mirror them (result known symmetric)
+  int shared;              // A and B are the same matrix (diagonal tiles stage one panel)
+  int aligned;             // rows of A and B are 16-byte aligned and lda/ldb multiples of 16 bytes
+  int64_t nbm, nbn, ntiles;
+};
+
+__device__ __forceinline__ void tri_decode(int64_t L, int64_t nb, int64_t* bm, int64_t* bn) {
+  // row-major enumeration of {(r, c) : 0 <= r <= c < nb}; row r starts at r*nb - r*(r-1)/2
+  double a = 2.0 * (double)nb + 1.0;
+  int64_t r = (int64_t)((a - sqrt(a * a - 8.0 * (double)L)) * 0.5);
+  if (r < 0) r = 0;
+  if (r > nb - 1) r = nb - 1;
+  auto start = [nb](int64_t rr) { return rr * nb - rr * (rr - 1) / 2; };
+  while (r > 0 && start(r) > L) --r;
+  while (r + 1 < nb && start(r + 1) <= L) ++r;
+  *bm = r;
+  *bn = r + (L - start(r));
+}
+
+template <class Tr>
+__device__ __forceinline__ void load_chunk(const typename Tr::T* __restrict__ P, int64_t ld, int64_t rows,
+                                           int64_t K, int64_t row0, int64_t k0, int aligned,
+                                           const typename Tr::T* __restrict__ kmask, int tid,
+                                           typename Tr::V (&st)[4]) {
+  using E = typename Tr::E;
+  using T = typename Tr::T;
+  constexpr int kv = Tr::kVec;
+  const int c = tid & 7;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t r = row0 + (tid >> 3) + 32 * q;
+    const int64_t k = k0 + (int64_t)c * kv;
+    E e[kv];
+    if (aligned && r < rows && k + kv <= K) {
+      typename Tr::V raw = *reinterpret_cast<const typename Tr::V*>(P + r * ld + k);
+      const T* rv = reinterpret_cast<const T*>(&raw);
+#pragma unroll
+      for (int i = 0; i < kv; ++i) e[i] = Tr::key(rv[i]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < kv; ++i) e[i] = (r < rows && k + i < K) ? Tr::key(P[r * ld + k + i]) : Tr::pad();
+    }
+    if (kmask) {
+#pragma unroll
+      for (int i = 0; i < kv; ++i)
+        if (k + i < K) e[i] = Tr::mn(e[i], Tr::key(kmask[k + i]));
+    }
+    st[q] = *reinterpret_cast<typename Tr::V*>(e);
+  }
+}
+
+template <class Tr>
+__global__ __launch_bounds__(kThreads) void maxmin_tile_kernel(MMArgs a) {
+  using E = typename Tr::E;
+  using V = typename Tr::V;
+  using T = typename Tr::T;
+  constexpr int kv = Tr::kVec;
+  constexpr int kLdsRow = kLdsRowBytes / sizeof(E);
+
+  __shared__ __align__(16) E As[kBM * kLdsRow];
+  __shared__ __align__(16) E Bs[kBN * kLdsRow];
+
+  const int64_t L = mcpg::xcd_remap(blockIdx.x, a.ntiles);
+  int64_t bm, bn;
+  if (a.tri) {
+    tri_decode(L, a.nbm, &bm, &bn);
+  } else {
+    const int64_t group = 8;
+    const int64_t per_group = group * a.nbn;
+    const int64_t g = L / per_group;
+    const int64_t first = g * group;
+    const int64_t gsize = (a.nbm - first) < group ? (a.nbm - first) : group;
+    const int64_t in = L % per_group;
+    bm = first + in % gsize;
+    bn = in / gsize;
+  }
+  const int64_t row0 = bm * kBM, col0 = bn * kBN;
+  const bool same = a.shared && bm == bn;
+
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  const T* __restrict__ A = static_cast<const T*>(a.A);
+  const T* __restrict__ B = static_cast<const T*>(a.B);
+  const T* __restrict__ km = static_cast<const T*>(a.kmask);
+
+  E acc[8][8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+#pragma unroll
+    for (int n = 0; n < 8; ++n) acc[m][n] = Tr::init();
+
+  V sa[4], sb[4];
+  constexpr int kChunkElems = kRowBytes / sizeof(E);
+  const int64_t nk = ceil_div(a.K, kChunkElems);
+  if (nk > 0) {
+    load_chunk<Tr>(A, a.lda, a.M, a.K, row0, 0, a.aligned, km, tid, sa);
+    if (!same) load_chunk<Tr>(B, a.ldb, a.N, a.K, col0, 0, a.aligned, nullptr, tid, sb);
+  }
+  for (int64_t kc = 0; kc < nk; ++kc) {
+    // registers -> LDS
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = (tid >> 3) + 32 * q;
+      *reinterpret_cast<V*>(&As[r * kLdsRow + (tid & 7) * kv]) = sa[q];
+      if (!same) *reinterpret_cast<V*>(&Bs[r * kLdsRow + (tid & 7) * kv]) = sb[q];
+    }
+    __syncthreads();
+    if (kc + 1 < nk) {  // prefetch the next chunk while this one is consumed
+      load_chunk<Tr>(A, a.lda, a.M, a.K, row0, (kc + 1) * kChunkElems, a.aligned, km, tid, sa);
+      if (!same) load_chunk<Tr>(B, a.ldb, a.N, a.K, col0, (kc + 1) * kChunkElems, a.aligned, nullptr, tid, sb);
+    }
+    const E* Bsrc = same ? As : Bs;
+#pragma unroll 2
+    for (int c = 0; c < kChunks; ++c) {
+      V av[8], bv[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) av[m] = *reinterpret_cast<const V*>(&As[(ty + 16 * m) * kLdsRow + c * kv]);
+#pragma unroll
+      for (int n = 0; n < 8; ++n) bv[n] = *reinterpret_cast<const V*>(&Bsrc[(tx + 16 * n) * kLdsRow + c * kv]);
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int n = 0; n < 8; ++n) acc[m][n] = Tr::step(acc[m][n], av[m], bv[n]);
+    }
+    __syncthreads();
+  }
+
+  // epilogue: outputs, optional fused ratio, mirror for symmetric off-diagonal tiles
+  T* __restrict__ C = static_cast<T*>(a.C);
+  const T* __restrict__ R = static_cast<const T*>(a.R);
+  T* __restrict__ O = static_cast<T*>(a.O);
+  const bool mirror = a.tri && bm != bn;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int64_t i = row0 + ty + 16 * m;
+    if (i >= a.M) continue;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      const int64_t j = col0 + tx + 16 * n;
+      if (j >= a.N) continue;
+      const T v = Tr::val(acc[m][n]);
+      if (C) {
+        C[i * a.ldc + j] = v;
+        if (mirror) C[j * a.ldc + i] = v;
+      }
+      if (O) {
+        O[i * a.ldo + j] = ratio_of<T>(R[i * a.ldr + j], v);
+        if (mirror) O[j * a.ldo + i] = ratio_of<T>(R[j * a.ldr + i], v);
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void ratio_kernel(const T* __restrict__ A, int64_t lda, const T* __restrict__ Mm, int64_t ldm,
+                             int64_t M, int64_t N, T* __restrict__ O, int64_t ldo) {
+  const int64_t total = M * N;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / N, j = t % N;
+    O[i * ldo + j] = ratio_of<T>(A[i * lda + j], Mm[i * ldm + j]);
+  }
+}
+
+// flags[0] |= 1 if A is not bitwise symmetric
+template <typename T, typename Bits = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type>
+__global__ void symmetry_check_kernel(const T* __restrict__ A, int64_t lda, int64_t N, int* flags) {
+  const int64_t total = N * N;
+  int bad = 0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / N, j = t % N;
+    if (j > i) {
+      if (__builtin_bit_cast(Bits, A[i * lda + j]) != __builtin_bit_cast(Bits, A[j * lda + i])) bad = 1;
+    }
+  }
+  if (bad) atomicOr(flags, 1);
+}
+
+template <class Tr>
+int launch_maxmin(mcpg_ctx* h, const typename Tr::T* A, int64_t lda, const typename Tr::T* B, int64_t ldb,
+                  const typename Tr::T* kmask, int64_t M, int64_t N, int64_t K, int tri,
+                  typename Tr::T* C, int64_t ldc, const typename Tr::T* R, int64_t ldr, typename Tr::T* O,
+                  int64_t ldo) {
+  using T = typename Tr::T;
+  if (M <= 0 || N <= 0) return MCPG_OK;
+  MMArgs a;
+  a.A = A; a.lda = lda; a.B = B; a.ldb = ldb; a.kmask = kmask;
+  a.M = M; a.N = N; a.K = K; a.C = C; a.ldc = ldc; a.R = R; a.ldr = ldr; a.O = O; a.ldo = ldo;
+  a.tri = tri;
+  a.shared = (A == B && lda == ldb && kmask == nullptr) ? 1 : 0;
+  const bool al = (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0) &&
+                  ((lda * (int64_t)sizeof(T)) % 16 == 0) && ((ldb * (int64_t)sizeof(T)) % 16 == 0);
+  a.aligned = al ? 1 : 0;
+  a.nbm = ceil_div(M, kBM);
+  a.nbn = ceil_div(N, kBN);
+  a.ntiles = tri ? a.nbm * (a.nbm + 1) / 2 : a.nbm * a.nbn;
+  if (a.ntiles > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min grid too large");
+  {
+    mcpg::LaunchScope ls(h, sizeof(T) == 4 ? (tri ? "maxmin_tile_kernel<f32,tri>" : "maxmin_tile_kernel<f32>")
+                                           : (tri ? "maxmin_tile_kernel<f64,tri>" : "maxmin_tile_kernel<f64>"));
+    hipLaunchKernelGGL(maxmin_tile_kernel<Tr>, dim3((unsigned)a.ntiles), dim3(kThreads), 0, h->stream, a);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mcpg::hip_error(h, e, "maxmin_tile_kernel launch");
+  return MCPG_OK;
+}
+
+template <typename T>
+struct TraitsOf;
+template <>
+struct TraitsOf<float> { using type = TrF32; };
+template <>
+struct TraitsOf<double> { using type = TrF64; };
+
+template <typename T>
+int maxmin_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* B, int64_t ldb, const T* kmask,
+                 int64_t M, int64_t N, int64_t K, int symmetric, T* C, int64_t ldc) {
+  using Tr = typename TraitsOf<T>::type;
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (M < 0 || N < 0 || K < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (M == 0 || N == 0) return MCPG_OK;
+  if (!A || !B || !C) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (lda < K || ldb < K || ldc < N) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (symmetric && (A != B || M != N || lda != ldb))
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "symmetric max-min needs A == B and M == N");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  if (ptrs == MCPG_DEVICE) {
+    int st = launch_maxmin<Tr>(h, A, lda, B, ldb, kmask, M, N, K, symmetric, C, ldc, nullptr, 0, nullptr, 0);
+    if (st) return st;
+    return mcpg::finish(h, "mcpg_maxmin");
+  }
+  const bool shareB = (A == B && M == N && lda == ldb);
+  mcpg::Sizer sz;
+  sz.add<T>(M * K);
+  if (!shareB) sz.add<T>(N * K);
+  if (kmask) sz.add<T>(K);
+  sz.add<T>(M * N);
+  int st;
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  T* dA = cv.take<T>(M * K);
+  T* dB = shareB ? dA : cv.take<T>(N * K);
+  T* dm = kmask ? cv.take<T>(K) : nullptr;
+  T* dC = cv.take<T>(M * N);
+  if (K > 0) {
+    if ((st = mcpg::copy_2d(h, dA, K * sizeof(T), A, lda * sizeof(T), K * sizeof(T), M, hipMemcpyHostToDevice))) return st;
+    if (!shareB && (st = mcpg::copy_2d(h, dB, K * sizeof(T), B, ldb * sizeof(T), K * sizeof(T), N, hipMemcpyHostToDevice))) return st;
+    if (kmask) MCPG_HIP(h, hipMemcpyAsync(dm, kmask, K * sizeof(T), hipMemcpyHostToDevice, h->stream));
+  }
+  st = launch_maxmin<Tr>(h, dA, K, dB, K, dm, M, N, K, symmetric, dC, N, nullptr, 0, nullptr, 0);
+  if (st) return st;
+  if ((st = mcpg::copy_2d(h, C, ldc * sizeof(T), dC, N * sizeof(T), N * sizeof(T), M, hipMemcpyDeviceToHost))) return st;
+  return mcpg::finish(h, "mcpg_maxmin");
+}
+
+template <typename T>
+int ratio_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* Mm, int64_t ldm, int64_t M, int64_t N,
+                T* O, int64_t ldo) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (M < 0 || N < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (M == 0 || N == 0) return MCPG_OK;
+  if (!A || !Mm || !O) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (lda < N || ldm < N || ldo < N) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const T *dA = A, *dM = Mm;
+  T* dO = O;
+  int64_t la = lda, lm = ldm, lo = ldo;
+  int st;
+  if (ptrs == MCPG_HOST) {
+    mcpg::Sizer sz;
+    sz.add<T>(M * N); sz.add<T>(M * N); sz.add<T>(M * N);
+    void* base = mcpg::workspace(h, sz.off, &st);
+    if (!base) return st;
+    mcpg::Carver cv(base);
+    T* a = cv.take<T>(M * N); T* m = cv.take<T>(M * N); T* o = cv.take<T>(M * N);
+    if ((st = mcpg::copy_2d(h, a, N * sizeof(T), A, lda * sizeof(T), N * sizeof(T), M, hipMemcpyHostToDevice))) return st;
+    if ((st = mcpg::copy_2d(h, m, N * sizeof(T), Mm, ldm * sizeof(T), N * sizeof(T), M, hipMemcpyHostToDevice))) return st;
+    dA = a; dM = m; dO = o; la = lm = lo = N;
+  } else if (ptrs != MCPG_DEVICE) {
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  }
+  const int64_t total = M * N;
+  const int blocks = (int)std::min<int64_t>(ceil_div(total, 256), 65536);
+  {
+    mcpg::LaunchScope ls(h, "ratio_kernel");
+    hipLaunchKernelGGL(ratio_kernel<T>, dim3(blocks), dim3(256), 0, h->stream, dA, la, dM, lm, M, N, dO, lo);
+  }
+  if (ptrs == MCPG_HOST &&
+      (st = mcpg::copy_2d(h, O, ldo * sizeof(T), dO, N * sizeof(T), N * sizeof(T), M, hipMemcpyDeviceToHost)))
+    return st;
+  return mcpg::finish(h, "mcpg_ratio");
+}
+
+template <typename T>
+int mcp_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, int k, T* mm_out, int64_t ldm, T* out,
+              int64_t ldo) {
+  using Tr = typename TraitsOf<T>::type;
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (k < 2 || k > 4) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "MCP order must be 2, 3 or 4");
+  if (N < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (N == 0) return MCPG_OK;
+  if (!A || !out) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (lda < N || ldo < N || (mm_out && ldm < N)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const size_t NN = (size_t)N * N;
+  const bool host = ptrs == MCPG_HOST;
+  mcpg::Sizer sz;
+  if (host) { sz.add<T>(NN); sz.add<T>(NN); }   // A copy, out
+  sz.add<T>(NN);                                 // M1
+  if (k >= 3 && (host || !mm_out)) sz.add<T>(NN);  // final max-min
+  int st;
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const T* dA = A;
+  int64_t la = lda;
+  T* dO = out;
+  int64_t lo = ldo;
+  if (host) {
+    T* a = cv.take<T>(NN);
+    if ((st = mcpg::copy_2d(h, a, N * sizeof(T), A, lda * sizeof(T), N * sizeof(T), N, hipMemcpyHostToDevice))) return st;
+    dA = a; la = N;
+    dO = cv.take<T>(NN); lo = N;
+  }
+  T* M1 = cv.take<T>(NN);
+  T* Mf = nullptr;  // final max-min matrix
+  int64_t lf = N;
+  if (k >= 3) {
+    if (!host && mm_out) { Mf = mm_out; lf = ldm; }
+    else Mf = cv.take<T>(NN);
+  }
+  // the first product A (x) A is symmetric by construction
+  if (k == 2) {
+    T* dst = (!host && mm_out) ? mm_out : M1;
+    int64_t ld = (!host && mm_out) ? ldm : N;
+    if ((st = launch_maxmin<Tr>(h, dA, la, dA, la, nullptr, N, N, N, 1, dst, ld, dA, la, dO, lo))) return st;
+    Mf = dst; lf = ld;
+  } else {
+    if ((st = launch_maxmin<Tr>(h, dA, la, dA, la, nullptr, N, N, N, 1, M1, N, nullptr, 0, nullptr, 0))) return st;
+    if (k == 3) {
+      // A (x) M1 is exactly symmetric when A is (max/min never round); check A bitwise.
+      if ((st = mcpg::reset_flags(h))) return st;
+      {
+        mcpg::LaunchScope ls(h, "symmetry_check_kernel");
+        hipLaunchKernelGGL(symmetry_check_kernel<T>, dim3(1024), dim3(256), 0, h->stream, dA, la, N, h->d_flags);
+      }
+      int flags[1];
+      if ((st = mcpg::read_flags(h, flags, 1))) return st;
+      const int sym = flags[0] == 0;
+      if ((st = launch_maxmin<Tr>(h, dA, la, M1, N, nullptr, N, N, N, sym, Mf, lf, dA, la, dO, lo))) return st;
+    } else {
+      if ((st = launch_maxmin<Tr>(h, M1, N, M1, N, nullptr, N, N, N, 1, Mf, lf, dA, la, dO, lo))) return st;
+    }
+  }
+  if (host) {
+    if ((st = mcpg::copy_2d(h, out, ldo * sizeof(T), dO, N * sizeof(T), N * sizeof(T), N, hipMemcpyDeviceToHost))) return st;
+    if (mm_out && (st = mcpg::copy_2d(h, mm_out, ldm * sizeof(T), Mf, lf * sizeof(T), N * sizeof(T), N, hipMemcpyDeviceToHost)))
+      return st;
+  }
+  return mcpg::finish(h, "mcpg_mcp");
+}
+
+}  // namespace
+
+extern "C" {
+
+int mcpg_maxmin_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, const float* B, int64_t ldb,
+                    const float* kmask, int64_t M, int64_t N, int64_t K, int symmetric, float* C, int64_t ldc) {
+  return maxmin_entry<float>(h, ptrs, A, lda, B, ldb, kmask, M, N, K, symmetric, C, ldc);
+}
+int mcpg_maxmin_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, const double* B, int64_t ldb,
+                    const double* kmask, int64_t M, int64_t N, int64_t K, int symmetric, double* C, int64_t ldc) {
+  return maxmin_entry<double>(h, ptrs, A, lda, B, ldb, kmask, M, N, K, symmetric, C, ldc);
+}
+int mcpg_ratio_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, const float* Mm, int64_t ldm, int64_t M,
+                   int64_t N, float* out, int64_t ldo) {
+  return ratio_entry<float>(h, ptrs, A, lda, Mm, ldm, M, N, out, ldo);
+}
+int mcpg_ratio_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, const double* Mm, int64_t ldm, int64_t M,
+                   int64_t N, double* out, int64_t ldo) {
+  return ratio_entry<double>(h, ptrs, A, lda, Mm, ldm, M, N, out, ldo);
+}
+int mcpg_mcp_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N, int k, float* maxmin_out,
+                 int64_t ldm, float* mcp_out, int64_t ldo) {
+  return mcp_entry<float>(h, ptrs, A, lda, N, k, maxmin_out, ldm, mcp_out, ldo);
+}
+int mcpg_mcp_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, int k, double* maxmin_out,
+                 int64_t ldm, double* mcp_out, int64_t ldo) {
+  return mcp_entry<double>(h, ptrs, A, lda, N, k, maxmin_out, ldm, mcp_out, ldo);
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+"""B-spline MI on the GPU against the CPU oracle.
+
+Bars: min-max scaling and FP32 weights bit-exact (same float operations as the reference);
+marginal entropies within 1e-12 of the FP64 oracle; all-pairs MI max |dMI| vs the FP64 oracle
+<= 1e-5 for FP32 and <= 1e-4 for the BF16 hi+lo split (the BF16 single product is reported,
+bounded loosely); MI matrices exactly symmetric.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from mcpnet_amd import grn
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 1e-5, "bf16x3": 1e-4, "bf16": 5e-2}
+
+
+def gen(G, S, seed=1234):
+    return grn.grn_expression(G, S, seed)[0]
+
+
+def test_minmax_scale_bit_exact(handle):
+    X = gen(50, 333)
+    X[3] = 0.25                 # constant row keeps its values (BSplineMI.hpp:102-103)
+    X[7] = np.linspace(-3, 9, 333, dtype=np.float32)
+    got = handle.minmax_scale(X)
+    assert np.array_equal(got, oracle.minmax_scale_f32(X))
+    assert np.all(got[3] == 0.25)
+
+
+@pytest.mark.parametrize("bins,order", [(10, 3), (7, 2), (16, 4), (5, 1)])
+def test_weights_bit_exact(handle, bins, order):
+    Xs = oracle.minmax_scale_f32(gen(37, 257))
+    Xs[0, :5] = [0.0, 1.0, 0.125, 0.5, 0.875]   # knots and both ends
+    got = handle.bspline_weights(Xs, bins=bins, order=order)
+    ref = oracle.bspline_weights_f32(Xs, bins=bins, order=order)
+    S = Xs.shape[1]
+    assert np.array_equal(got[:, : bins * S], ref[:, : bins * S])
+    np.testing.assert_allclose(got[:, bins * S], ref[:, bins * S], rtol=0, atol=2e-7)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
+@pytest.mark.parametrize("G,S", [(1, 10), (17, 100), (64, 257), (250, 1000)])
+def test_mi_vs_fp64_oracle(handle, precision, G, S):
+    X = gen(G, S, seed=G * 31 + S)
+    got = handle.bspline_mi(X, precision=precision).astype(np.float64)
+    ref = oracle.bspline_mi_f64(X)
+    err = np.abs(got - ref)
+    assert err.max() <= TOL[precision], f"{precision} max |dMI| {err.max():.3e}"
+    assert np.array_equal(got, got.T)
+
+
+def test_mi_bins_and_order_variants(handle):
+    X = gen(40, 300, seed=77)
+    for bins, order in [(7, 2), (16, 4), (20, 3), (4, 4)]:
+        got = handle.bspline_mi(X, bins=bins, order=order, precision="fp32").astype(np.float64)
+        ref = oracle.bspline_mi_f64(X, bins=bins, order=order)
+        assert np.abs(got - ref).max() <= 1e-5, (bins, order)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_mi_from_reference_layout_weights(handle, precision):
+    X = gen(90, 400, seed=5)
+    W = handle.bspline_weights(handle.minmax_scale(X))
+    got = handle.bspline_mi_weights(W, S=400, precision=precision)
+    fused = handle.bspline_mi(X, precision=precision)
+    # the reference layout stores H as float: differences are the float rounding of H
+    assert np.abs(got.astype(np.float64) - fused).max() <= 2e-6
+    assert np.abs(got.astype(np.float64) - oracle.bspline_mi_f64(X)).max() <= TOL[precision]
+
+
+def test_mi_device_pointers_and_invariances(handle, torch_cuda):
+    X = gen(120, 512, seed=9)
+    Xd = torch_cuda.from_numpy(X).cuda()
+    mi = handle.bspline_mi(Xd, precision="fp32").cpu().numpy()
+    # a positive affine map of one gene leaves its scaled row (and MI) unchanged up to rounding
+    X2 = X.copy()
+    X2[5] = 3.0 * X2[5] + 2.0
+    mi2 = handle.bspline_mi(torch_cuda.from_numpy(X2).cuda(), precision="fp32").cpu().numpy()
+    assert np.abs(mi2 - mi).max() <= 2e-5
+    # a joint permutation of the samples leaves every MI unchanged up to summation order
+    perm = np.random.default_rng(0).permutation(512)
+    mi3 = handle.bspline_mi(torch_cuda.from_numpy(np.ascontiguousarray(X[:, perm])).cuda(), precision="fp32").cpu().numpy()
+    assert np.abs(mi3 - mi).max() <= 2e-5
+
+
+def test_mi_rejects_bad_input(handle):
+    import mcpnet_amd
+    X = gen(8, 50)
+    with pytest.raises(mcpnet_amd.MCPGError) as e:
+        handle.bspline_mi(X, bins=2, order=3)
+    assert e.value.status == -1
+    Xn = X.copy()
+    Xn[2, 7] = np.nan
+    with pytest.raises(mcpnet_amd.MCPGError) as e:
+        handle.bspline_mi(Xn)
+    assert e.value.status == -3
+    W = np.zeros((8, 10 * 50), np.float32)     # one column short of bins*S+1
+    with pytest.raises(mcpnet_amd.MCPGError) as e:
+        handle.bspline_mi_weights(W, S=50)
+    assert e.value.status == -2
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
+def test_config2_error_report(handle, torch_cuda, precision):
+    """Config 2 (2000 x 1024): MI error against the FP64 oracle on all pairs."""
+    X = grn.expression(2)
+    got = handle.bspline_mi(torch_cuda.from_numpy(X).cuda(), precision=precision).cpu().numpy().astype(np.float64)
+    ref = oracle.bspline_mi_f64(X)
+    err = np.abs(got - ref)
+    print(f"config2 {precision}: max |dMI| {err.max():.3e} mean {err.mean():.3e}")
+    assert err.max() <= TOL[precision]
+    assert np.array_equal(got, got.T)
+
+
+@pytest.mark.slow
+def test_config3_sampled_pairs(handle, torch_cuda):
+    """Config 3 (20000 x 2000) BF16x3: exact symmetry everywhere; all pairs of 300 sampled genes vs FP64."""
+    import torch
+    X = grn.expression(3)
+    mi = handle.bspline_mi(torch_cuda.from_numpy(X).cuda(), precision="bf16x3")
+    assert bool(torch.equal(mi, mi.t()))
+    rng = np.random.default_rng(42)
+    genes = np.sort(rng.choice(20000, 300, replace=False))
+    li, lj = np.triu_indices(len(genes), k=1)
+    ref = oracle.bspline_mi_pairs_f64(X[genes], li.astype(np.int32), lj.astype(np.int32))
+    gi = torch.from_numpy(genes[li]).long().cuda()
+    gj = torch.from_numpy(genes[lj]).long().cuda()
+    got = mi[gi, gj].cpu().numpy().astype(np.float64)
+    err = np.abs(got - ref)
+    print(f"config3 bf16x3 sampled: max |dMI| {err.max():.3e} mean {err.mean():.3e}")
+    assert err.max() <= 1e-4

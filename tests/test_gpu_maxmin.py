@@ -1,0 +1,150 @@
+"""Max-min products, ratio and MCP2/3/4 on the GPU: bit-exact against the reference's own
+max_of_pairmin (golden vectors from oracle/_ref) and the CPU oracle."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from mcpnet_amd import grn
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(Path(__file__).with_name("golden") / "maxmin_ref.npz")
+
+
+def bits_equal(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_golden_r10c10(handle, dt):
+    X = GOLD[f"r10c10_{dt}"]
+    assert bits_equal(handle.maxmin(X, X), GOLD[f"mm_r10c10_{dt}"])
+    assert bits_equal(handle.maxmin(X, X, symmetric=True), GOLD[f"mm_r10c10_{dt}"])
+    assert bits_equal(handle.maxmin(X, X, kmask=GOLD[f"tf_mask_{dt}"]), GOLD[f"mm_r10c10_masked_{dt}"])
+
+
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_golden_random_with_infinities(handle, dt):
+    R = GOLD[f"rand_{dt}"]
+    got = handle.maxmin(np.ascontiguousarray(R[:96]), np.ascontiguousarray(R[:71]))
+    assert bits_equal(got, GOLD[f"mm_rand_{dt}"])
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 5, 7), (127, 129, 131), (128, 128, 32), (257, 130, 1000),
+                                   (200, 300, 33)])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_random_shapes_vs_oracle(handle, shape, dt):
+    M, N, K = shape
+    rng = np.random.default_rng(M * 1000 + N + K)
+    A = (rng.random((M, K)) * 2 - 0.5).astype(dt)
+    B = (rng.random((N, K)) * 2 - 0.5).astype(dt)
+    got = handle.maxmin(A, B)
+    assert bits_equal(got, oracle.maxmin(A, B))
+    km = np.where(rng.random(K) < 0.4, np.finfo(dt).max, np.finfo(dt).min).astype(dt)
+    assert bits_equal(handle.maxmin(A, B, kmask=km), oracle.maxmin(A, B, km))
+
+
+def test_unaligned_device_views(handle, torch_cuda):
+    rng = np.random.default_rng(5)
+    big = rng.random((150, 203)).astype(np.float32)
+    bd = torch_cuda.from_numpy(big).cuda()
+    A = bd[:, 1:202]           # misaligned start, ld = 203 elements
+    B = bd[3:120, 2:203]
+    ref = oracle.maxmin(np.ascontiguousarray(big[:, 1:202]), np.ascontiguousarray(big[3:120, 2:203]))
+    assert bits_equal(handle.maxmin(A, B).cpu().numpy(), ref)
+
+
+def test_empty_k_follows_reference_init(handle):
+    A32 = np.zeros((4, 0), np.float32)
+    A64 = np.zeros((4, 0), np.float64)
+    assert np.all(handle.maxmin(A32, A32) == -np.inf)
+    assert np.all(handle.maxmin(A64, A64) == -np.finfo(np.float64).max)
+
+
+def test_symmetric_equals_full(handle, torch_cuda):
+    rng = np.random.default_rng(11)
+    A = rng.random((333, 333)).astype(np.float32)
+    A = np.maximum(A, A.T)
+    Ad = torch_cuda.from_numpy(A).cuda()
+    full = handle.maxmin(Ad, Ad).cpu().numpy()
+    sym = handle.maxmin(Ad, Ad, symmetric=True).cpu().numpy()
+    assert bits_equal(full, sym)
+    assert bits_equal(sym, sym.T)
+
+
+def test_ratio_branches(handle):
+    eps32 = np.finfo(np.float32).eps
+    A = np.array([[0.0, eps32 / 2, 0.3, 0.3, -0.2, 1.0]], np.float32)
+    M = np.array([[0.0, 0.5, 0.0, eps32 / 4, 0.7, 3.0]], np.float32)
+    got = handle.ratio(A, M)
+    assert bits_equal(got, oracle.ratio(A, M))
+    assert got[0, 0] == 0 and got[0, 1] == 0 and got[0, 2] == np.finfo(np.float32).max
+    rng = np.random.default_rng(3)
+    A = rng.random((300, 301)).astype(np.float32)
+    M = rng.random((300, 301)).astype(np.float32)
+    assert bits_equal(handle.ratio(A, M), oracle.ratio(A, M))
+    assert bits_equal(handle.ratio(A.astype(np.float64), M.astype(np.float64)),
+                      oracle.ratio(A.astype(np.float64), M.astype(np.float64)))
+
+
+def mi_like(G, seed):
+    X, _, _ = grn.grn_expression(G, 256, seed)
+    A = oracle.bspline_mi_f64(X).astype(np.float32)
+    np.fill_diagonal(A, 0.0)
+    return A
+
+
+@pytest.mark.parametrize("k", [2, 3, 4])
+def test_mcp_vs_oracle(handle, k):
+    A = mi_like(300, 99)
+    out, mm = handle.mcp(A, k, want_maxmin=True)
+    ref_out, ref_mm = oracle.mcp(A, k)
+    assert bits_equal(mm, ref_mm)
+    assert bits_equal(out, ref_out)
+
+
+def test_mcp3_asymmetric_input_takes_full_path(handle):
+    rng = np.random.default_rng(17)
+    A = rng.random((200, 200)).astype(np.float32)   # not symmetric
+    out, mm = handle.mcp(A, 3, want_maxmin=True)
+    ref_out, ref_mm = oracle.mcp(A, 3)
+    assert bits_equal(mm, ref_mm)
+    assert bits_equal(out, ref_out)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_mcp_device_pointers(handle, torch_cuda, dt):
+    A = mi_like(257, 5).astype(dt)
+    Ad = torch_cuda.from_numpy(A).cuda()
+    for k in (2, 3, 4):
+        out, mm = handle.mcp(Ad, k, want_maxmin=True)
+        ref_out, ref_mm = oracle.mcp(A, k)
+        assert bits_equal(mm.cpu().numpy(), ref_mm)
+        assert bits_equal(out.cpu().numpy(), ref_out)
+
+
+@pytest.mark.slow
+def test_mcp3_config1_full(handle, torch_cuda):
+    """MCP3 on a 1000-gene FP32 MI matrix (config 1), bit-exact against the CPU oracle."""
+    X = grn.expression(1)
+    with_gpu = handle.bspline_mi(torch_cuda.from_numpy(X).cuda(), precision="fp32")
+    A = with_gpu.cpu().numpy()
+    np.fill_diagonal(A, 0.0)
+    out, mm = handle.mcp(torch_cuda.from_numpy(A).cuda(), 3, want_maxmin=True)
+    ref_out, ref_mm = oracle.mcp(A, 3)
+    assert bits_equal(mm.cpu().numpy(), ref_mm)
+    assert bits_equal(out.cpu().numpy(), ref_out)
+
+
+def test_maxmin_rejects_bad_arguments(handle):
+    import mcpnet_amd
+    A = np.ones((4, 5), np.float32)
+    B = np.ones((4, 5), np.float32)
+    with pytest.raises(mcpnet_amd.MCPGError):
+        handle.maxmin(A, B, symmetric=True)   # symmetric needs the same operand
+    with pytest.raises(ValueError):
+        handle.maxmin(A, np.ones((4, 6), np.float32))
