@@ -1,0 +1,101 @@
+// Adaptive-partitioning MI: the reference's kernel functor names
+// (include/mcp/correlation/AdaptivePartitioningMI.hpp:40-647 of MCPNet) with the patterns that
+// apply them specialised to the GPU:
+//   add_noise<T>::operator()(in, out)                                   -> mcpg_add_noise_f32
+//   Transform<M<IT>, IntegralCumulativeHistogramKernel<IT,0>, M<IT>>    -> mcpg_cumhist_i32
+//   InnerProduct<M<IT>, AdaptivePartitionRankMIKernel2<IT,OT,0>, M<OT>> -> mcpg_ap_mi_i32
+// The reference's add_noise draws from per-thread std::default_random_engine streams of splash
+// (absent); here the jitter is a seeded counter-based stream, identical for any thread count.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "mcpgpu/convert.hpp"
+#include "splash/ds/aligned_matrix.hpp"
+#include "splash/kernel/rank.hpp"
+#include "splash/pattern/patterns.hpp"
+
+namespace mcp {
+namespace kernel {
+
+// add_noise (AdaptivePartitioningMI.hpp:40-114): out = in + U[mn, mx); only [0, 1e-5) is
+// supported on the GPU, the range the reference's callers use (mi.cpp:130, mcpnet_helper.hpp:113).
+template <typename IT>
+class add_noise {
+  uint64_t seed_;
+
+ public:
+  explicit add_noise(uint64_t seed = 0x4D43504E04ull, IT const& mn = 0.0, IT const& mx = 0.00001) : seed_(seed) {
+    if (!(mn == static_cast<IT>(0.0) && mx == static_cast<IT>(0.00001)))
+      throw mcpgpu::error(MCPG_ERR_UNSUPPORTED, "add_noise: the GPU jitter is U[0, 1e-5)");
+  }
+  void copy_parameters(const add_noise& o) { seed_ = o.seed_; }
+  void operator()(const splash::ds::aligned_matrix<IT>& in, splash::ds::aligned_matrix<IT>& out) const {
+    const size_t G = in.rows(), S = in.columns();
+    std::vector<float> x = mcpgpu::packed<float>(in), y(G * S);
+    mcpgpu::check(mcpg_add_noise_f32(mcpgpu::thread_handle(), MCPG_HOST, x.data(), G, S, S, seed_, y.data(), S),
+                  "add_noise");
+    mcpgpu::unpack(y, G, S, out);
+  }
+};
+
+// IntegralCumulativeHistogramKernel (AdaptivePartitioningMI.hpp:414-442)
+template <typename IT, long firstRank = 0>
+class IntegralCumulativeHistogramKernel {
+  static_assert(firstRank == 0, "ranks start at 0");
+
+ public:
+  void copy_parameters(const IntegralCumulativeHistogramKernel&) {}
+};
+
+// AdaptivePartitionRankMIKernel2 (AdaptivePartitioningMI.hpp:445-647).  The cumulative
+// histograms given to the constructor are not needed: the GPU rebuilds them from the ranks.
+template <typename IT, typename OT, long firstRank = 0>
+class AdaptivePartitionRankMIKernel2 {
+  static_assert(firstRank == 0, "ranks start at 0");
+
+ public:
+  AdaptivePartitionRankMIKernel2(IT const* /*rowPtrs*/, size_t const& /*stride_bytes*/) {}
+  AdaptivePartitionRankMIKernel2() {}
+  void copy_parameters(const AdaptivePartitionRankMIKernel2&) {}
+};
+
+}  // namespace kernel
+}  // namespace mcp
+
+namespace splash {
+namespace pattern {
+
+template <typename IT>
+class Transform<ds::aligned_matrix<IT>, mcp::kernel::IntegralCumulativeHistogramKernel<IT, 0>, ds::aligned_matrix<IT>> {
+ public:
+  void operator()(const ds::aligned_matrix<IT>& in, const mcp::kernel::IntegralCumulativeHistogramKernel<IT, 0>&,
+                  ds::aligned_matrix<IT>& out) const {
+    const size_t G = in.rows(), S = in.columns();
+    std::vector<int32_t> r = mcpgpu::packed<int32_t>(in), hst(G * (S + 1));
+    mcpgpu::check(mcpg_cumhist_i32(mcpgpu::thread_handle(), MCPG_HOST, r.data(), G, S, S, hst.data(), S + 1),
+                  "IntegralCumulativeHistogramKernel");
+    mcpgpu::unpack(hst, G, S + 1, out);
+  }
+};
+
+template <typename IT, typename OT, bool SYM>
+class InnerProduct<ds::aligned_matrix<IT>, mcp::kernel::AdaptivePartitionRankMIKernel2<IT, OT, 0>, ds::aligned_matrix<OT>,
+                   SYM> {
+ public:
+  void operator()(const ds::aligned_matrix<IT>& first, const ds::aligned_matrix<IT>& second,
+                  const mcp::kernel::AdaptivePartitionRankMIKernel2<IT, OT, 0>&, ds::aligned_matrix<OT>& out) const {
+    if (&first != &second && first.data() != second.data())
+      throw mcpgpu::error(MCPG_ERR_INVALID_ARG, "AdaptivePartitionRankMIKernel2: the GPU path pairs rows of one rank matrix");
+    const size_t G = first.rows(), S = first.columns();
+    std::vector<int32_t> r = mcpgpu::packed<int32_t>(first);
+    std::vector<float> mi(G * G);
+    mcpgpu::check(mcpg_ap_mi_i32(mcpgpu::thread_handle(), MCPG_HOST, r.data(), G, S, S, mi.data(), G),
+                  "AdaptivePartitionRankMIKernel2");
+    mcpgpu::unpack(mi, G, G, out);
+  }
+};
+
+}  // namespace pattern
+}  // namespace splash

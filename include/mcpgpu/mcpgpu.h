@@ -166,6 +166,13 @@ int mcpg_ratio_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, const f
 int mcpg_ratio_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, const double* Mm, int64_t ldm,
                    int64_t M, int64_t N, double* out, int64_t ldo);
 
+/* max_kernel<T> (include/mcp/transform/combine.hpp:42-58; used for max(M, M^T) at apps/filter/mcp.cpp:463-474):
+ * out = max(A, B) elementwise. */
+int mcpg_elementwise_max_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, const float* B, int64_t ldb,
+                             int64_t M, int64_t N, float* out, int64_t ldo);
+int mcpg_elementwise_max_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, const double* B, int64_t ldb,
+                             int64_t M, int64_t N, double* out, int64_t ldo);
+
 /* compute_mcp{2,3,4} (apps/pipeline/mcpnet_helper.hpp:198-205, :270-277, :357-364) on an N x N
  * matrix A (MI with its diagonal already set): k = 2: M = A(x)A; k = 3: M = A(x)(A(x)A);
  * k = 4: M = (A(x)A)(x)(A(x)A); out = ratio(A, M).  maxmin_out (nullable) receives M (the

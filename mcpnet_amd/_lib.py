@@ -59,6 +59,8 @@ _SIGS = {
     "mcpg_maxmin_f64": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, c_int, _P, c_int64],
     "mcpg_ratio_f32": [_P, c_int, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_ratio_f64": [_P, c_int, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_elementwise_max_f32": [_P, c_int, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_elementwise_max_f64": [_P, c_int, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_mcp_f32": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64, _P, c_int64],
     "mcpg_mcp_f64": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64, _P, c_int64],
 }
@@ -71,6 +73,13 @@ def load() -> ctypes.CDLL:
     if _lib is not None:
         return _lib
     path = Path(os.environ.get("MCPG_LIBRARY", LIB_PATH))
+    # torch's ROCm build ships its own libamdhip64 and links it by the unversioned name; if this
+    # library were loaded first, torch would load a second HIP runtime into the process and
+    # lose the device.  Loading torch first makes both share its runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not path.exists():
         raise RuntimeError(f"{path} is not built; run `python -m mcpnet_amd.build` (HIP toolchain required)")
     lib = ctypes.CDLL(str(path))
@@ -116,7 +125,10 @@ class _Mat:
             a = np.asarray(x)
             if a.dtype != dtype:
                 raise TypeError(f"{name}: expected {np.dtype(dtype)}, got {a.dtype}")
-            if a.ndim == 2:
+            if a.ndim == 2 and a.size == 0:
+                a = np.ascontiguousarray(a)
+                self.ld = max(a.shape[1], 1)
+            elif a.ndim == 2:
                 if a.strides[1] != a.itemsize:
                     raise ValueError(f"{name}: rows must be contiguous")
                 self.ld = a.strides[0] // a.itemsize if a.shape[0] > 1 else a.shape[1]
@@ -195,6 +207,9 @@ class Handle:
         name = c_char_p()
         self._check(self.lib.mcpg_last_launch_info(self.h, byref(n), byref(name)), "mcpg_last_launch_info")
         return n.value, (name.value or b"").decode()
+
+    def synchronize(self):
+        self._check(self.lib.mcpg_synchronize(self.h), "mcpg_synchronize")
 
     def set_profiling(self, enable: bool = True):
         self._check(self.lib.mcpg_set_profiling(self.h, int(enable)), "mcpg_set_profiling")
@@ -359,6 +374,21 @@ class Handle:
         fn = self.lib.mcpg_ratio_f64 if dt == np.float64 else self.lib.mcpg_ratio_f32
         self._pre(kind)
         self._check(fn(self.h, kind, a.ptr, a.ld, m.ptr, m.ld, a.shape[0], a.shape[1], o.ptr, o.ld), "mcpg_ratio")
+        return out
+
+    def elementwise_max(self, A, B, out=None):
+        dt = np.float64 if str(getattr(A, "dtype", "")).endswith("float64") else np.float32
+        a = _Mat(A, dt, "A")
+        b = _Mat(B, dt, "B")
+        if a.shape != b.shape:
+            raise ValueError("operands must have the same shape")
+        kind = _same_kind(a, b)
+        out = _empty_like_kind(kind, a.shape, dt, A) if out is None else out
+        o = _Mat(out, dt, "out")
+        fn = self.lib.mcpg_elementwise_max_f64 if dt == np.float64 else self.lib.mcpg_elementwise_max_f32
+        self._pre(kind)
+        self._check(fn(self.h, kind, a.ptr, a.ld, b.ptr, b.ld, a.shape[0], a.shape[1], o.ptr, o.ld),
+                    "mcpg_elementwise_max")
         return out
 
     def mcp(self, A, k, want_maxmin=False, out=None, maxmin_out=None):

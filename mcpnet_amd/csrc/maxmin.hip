@@ -257,6 +257,17 @@ __global__ void ratio_kernel(const T* __restrict__ A, int64_t lda, const T* __re
   }
 }
 
+template <typename T>
+__global__ void emax_kernel(const T* __restrict__ A, int64_t lda, const T* __restrict__ B, int64_t ldb, int64_t M,
+                            int64_t N, T* __restrict__ O, int64_t ldo) {
+  const int64_t total = M * N;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / N, j = t % N;
+    const T a = A[i * lda + j], b = B[i * ldb + j];
+    O[i * ldo + j] = a < b ? b : a;   // std::max(a, b) returns a unless a < b
+  }
+}
+
 // flags[0] |= 1 if A is not bitwise symmetric
 template <typename T, typename Bits = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type>
 __global__ void symmetry_check_kernel(const T* __restrict__ A, int64_t lda, int64_t N, int* flags) {
@@ -353,7 +364,7 @@ int maxmin_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* B, int
 
 template <typename T>
 int ratio_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* Mm, int64_t ldm, int64_t M, int64_t N,
-                T* O, int64_t ldo) {
+                T* O, int64_t ldo, bool elementwise_max = false) {
   MCPG_CHECK_HANDLE(h);
   mcpg::begin_call(h);
   if (M < 0 || N < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
@@ -380,7 +391,10 @@ int ratio_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* Mm, int
   }
   const int64_t total = M * N;
   const int blocks = (int)std::min<int64_t>(ceil_div(total, 256), 65536);
-  {
+  if (elementwise_max) {
+    mcpg::LaunchScope ls(h, "emax_kernel");
+    hipLaunchKernelGGL(emax_kernel<T>, dim3(blocks), dim3(256), 0, h->stream, dA, la, dM, lm, M, N, dO, lo);
+  } else {
     mcpg::LaunchScope ls(h, "ratio_kernel");
     hipLaunchKernelGGL(ratio_kernel<T>, dim3(blocks), dim3(256), 0, h->stream, dA, la, dM, lm, M, N, dO, lo);
   }
@@ -480,6 +494,14 @@ int mcpg_ratio_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, const f
 int mcpg_ratio_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, const double* Mm, int64_t ldm, int64_t M,
                    int64_t N, double* out, int64_t ldo) {
   return ratio_entry<double>(h, ptrs, A, lda, Mm, ldm, M, N, out, ldo);
+}
+int mcpg_elementwise_max_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, const float* B, int64_t ldb,
+                             int64_t M, int64_t N, float* out, int64_t ldo) {
+  return ratio_entry<float>(h, ptrs, A, lda, B, ldb, M, N, out, ldo, true);
+}
+int mcpg_elementwise_max_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, const double* B, int64_t ldb,
+                             int64_t M, int64_t N, double* out, int64_t ldo) {
+  return ratio_entry<double>(h, ptrs, A, lda, B, ldb, M, N, out, ldo, true);
 }
 int mcpg_mcp_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N, int k, float* maxmin_out,
                  int64_t ldm, float* mcp_out, int64_t ldo) {
