@@ -351,20 +351,9 @@ struct MIArgs {
   int64_t ldmi;
   Layout L;
   int64_t ntri;      // number of (ti <= tj) tile pairs
+  const uint32_t* order;  // tile coordinates in launch order, (ti << 16) | tj
   float norm;        // (float)1 / (float)S, the reference's FT norm_factor (BSplineMI.hpp:319)
 };
-
-__device__ __forceinline__ void tri_decode(int64_t Lid, int64_t nb, int64_t* bm, int64_t* bn) {
-  double a = 2.0 * (double)nb + 1.0;
-  int64_t r = (int64_t)((a - sqrt(a * a - 8.0 * (double)Lid)) * 0.5);
-  if (r < 0) r = 0;
-  if (r > nb - 1) r = nb - 1;
-  auto start = [nb](int64_t rr) { return rr * nb - rr * (rr - 1) / 2; };
-  while (r > 0 && start(r) > Lid) --r;
-  while (r + 1 < nb && start(r + 1) <= Lid) ++r;
-  *bm = r;
-  *bn = r + (Lid - start(r));
-}
 
 // LDS row strides (elements): FP32 rows of 32 floats + 4 pad; BF16 rows of 32 halves + 8 pad.
 constexpr int kLdsF32 = 36;
@@ -440,7 +429,11 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
   char* panelB = smem + kPanelBytes;
 
   int64_t ti, tj;
-  tri_decode(mcpg::xcd_remap(blockIdx.x, a.ntri), a.L.ntiles, &ti, &tj);
+  {
+    const uint32_t code = a.order[mcpg::xcd_remap(blockIdx.x, a.ntri)];
+    ti = code >> 16;
+    tj = code & 0xffffu;
+  }
   const bool diag = ti == tj;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -754,6 +747,9 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
   a.ntri = L.ntiles * (L.ntiles + 1) / 2;
   a.norm = 1.0f / (float)L.S;
   if (a.ntri > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "MI grid too large");
+  int st;
+  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 8), &st);
+  if (!a.order) return st;
   hipError_t e;
   {
     const char* name = precision == MCPG_FP32 ? "bspline_mi_kernel<fp32>"

@@ -1,6 +1,9 @@
 // Handle lifecycle, error reporting and workspace management of libmcpgpu.
+#include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <new>
+#include <vector>
 
 #include "internal.h"
 
@@ -69,6 +72,44 @@ int finish(mcpg_ctx* h, const char* where) {
   return MCPG_OK;
 }
 
+const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int* status) {
+  *status = MCPG_OK;
+  if (nb > 65535) {
+    *status = set_error(h, MCPG_ERR_UNSUPPORTED, "tile grid too large for the packed tile order");
+    return nullptr;
+  }
+  for (const auto& o : h->orders)
+    if (o.nb == nb && o.group == group) return o.d;
+  std::vector<uint32_t> order;
+  order.reserve((size_t)(nb * (nb + 1) / 2));
+  const int64_t nbb = (nb + group - 1) / group;
+  for (int64_t bi = 0; bi < nbb; ++bi)
+    for (int64_t bj = bi; bj < nbb; ++bj)
+      for (int64_t ti = bi * group; ti < std::min(nb, (bi + 1) * group); ++ti)
+        for (int64_t tj = bj * group; tj < std::min(nb, (bj + 1) * group); ++tj)
+          if (ti <= tj) order.push_back((uint32_t)((ti << 16) | tj));
+  if (h->orders.size() >= 4) {  // keep a few grids (MI and max-min alternate in a pipeline)
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) { *status = hip_error(h, e, "hipStreamSynchronize"); return nullptr; }
+    (void)hipFree(h->orders.front().d);
+    h->orders.erase(h->orders.begin());
+  }
+  uint32_t* d = nullptr;
+  hipError_t e = hipMalloc(&d, std::max<size_t>(order.size(), 1) * sizeof(uint32_t));
+  if (e != hipSuccess) { *status = hip_error(h, e, "hipMalloc(tile order)"); return nullptr; }
+  e = hipMemcpy(d, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) { (void)hipFree(d); *status = hip_error(h, e, "hipMemcpy(tile order)"); return nullptr; }
+  h->orders.push_back(mcpg_ctx::TileOrder{d, nb, group});
+  return d;
+}
+
+int env_group(const char* name, int def) {
+  const char* v = getenv(name);
+  if (!v) return def;
+  int g = atoi(v);
+  return (g >= 1 && g <= 256) ? g : def;
+}
+
 void begin_call(mcpg_ctx* h) {
   h->last_launches = 0;
   h->last_kernel = "";
@@ -130,6 +171,7 @@ int mcpg_destroy(mcpg_handle h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
   if (h->ws) (void)hipFree(h->ws);
+  for (auto& o : h->orders) (void)hipFree(o.d);
   if (h->d_flags) (void)hipFree(h->d_flags);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;

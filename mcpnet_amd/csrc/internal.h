@@ -28,6 +28,13 @@ struct mcpg_ctx {
   std::vector<Timing> timings;
   std::vector<hipEvent_t> event_pool;
   size_t events_used = 0;
+  // cached device lists of upper-triangle tile coordinates (see mcpg::tri_order)
+  struct TileOrder {
+    uint32_t* d = nullptr;
+    int64_t nb = -1;
+    int group = -1;
+  };
+  std::vector<TileOrder> orders;
 };
 
 namespace mcpg {
@@ -70,6 +77,15 @@ int reset_flags(mcpg_ctx* h);
 int read_flags(mcpg_ctx* h, int* flags, int n);
 
 int finish(mcpg_ctx* h, const char* where);  // stream sync + error check
+
+// Device array of the (ti, tj), ti <= tj, tiles of an nb x nb grid, packed as (ti << 16) | tj and
+// ordered in `group` x `group` blocks (blocks row-major over the block triangle, tiles row-major
+// inside a block).  Workgroups that run together then share 2*group row panels instead of one
+// row panel and as many column panels as there are workgroups.  Cached per handle.
+const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int* status);
+
+// Tile-group size: the value of environment variable `name` if set (1..256), else `def`.
+int env_group(const char* name, int def);
 
 // Starts an entry point: clears the launch counters and the timing list of the previous call.
 void begin_call(mcpg_ctx* h);

@@ -98,20 +98,8 @@ struct MMArgs {
   int shared;              // A and B are the same matrix (diagonal tiles stage one panel)
   int aligned;             // rows of A and B are 16-byte aligned and lda/ldb multiples of 16 bytes
   int64_t nbm, nbn, ntiles;
+  const uint32_t* order;   // tri mode: tile coordinates in launch order, (bm << 16) | bn
 };
-
-__device__ __forceinline__ void tri_decode(int64_t L, int64_t nb, int64_t* bm, int64_t* bn) {
-  // row-major enumeration of {(r, c) : 0 <= r <= c < nb}; row r starts at r*nb - r*(r-1)/2
-  double a = 2.0 * (double)nb + 1.0;
-  int64_t r = (int64_t)((a - sqrt(a * a - 8.0 * (double)L)) * 0.5);
-  if (r < 0) r = 0;
-  if (r > nb - 1) r = nb - 1;
-  auto start = [nb](int64_t rr) { return rr * nb - rr * (rr - 1) / 2; };
-  while (r > 0 && start(r) > L) --r;
-  while (r + 1 < nb && start(r + 1) <= L) ++r;
-  *bm = r;
-  *bn = r + (L - start(r));
-}
 
 template <class Tr>
 __device__ __forceinline__ void load_chunk(const typename Tr::T* __restrict__ P, int64_t ld, int64_t rows,
@@ -159,7 +147,9 @@ __global__ __launch_bounds__(kThreads) void maxmin_tile_kernel(MMArgs a) {
   const int64_t L = mcpg::xcd_remap(blockIdx.x, a.ntiles);
   int64_t bm, bn;
   if (a.tri) {
-    tri_decode(L, a.nbm, &bm, &bn);
+    const uint32_t code = a.order[L];
+    bm = code >> 16;
+    bn = code & 0xffffu;
   } else {
     const int64_t group = 8;
     const int64_t per_group = group * a.nbn;
@@ -301,6 +291,12 @@ int launch_maxmin(mcpg_ctx* h, const typename Tr::T* A, int64_t lda, const typen
   a.nbn = ceil_div(N, kBN);
   a.ntiles = tri ? a.nbm * (a.nbm + 1) / 2 : a.nbm * a.nbn;
   if (a.ntiles > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min grid too large");
+  a.order = nullptr;
+  if (tri) {
+    int st;
+    a.order = mcpg::tri_order(h, a.nbm, mcpg::env_group("MCPG_MM_GROUP", 8), &st);
+    if (!a.order) return st;
+  }
   {
     mcpg::LaunchScope ls(h, sizeof(T) == 4 ? (tri ? "maxmin_tile_kernel<f32,tri>" : "maxmin_tile_kernel<f32>")
                                            : (tri ? "maxmin_tile_kernel<f64,tri>" : "maxmin_tile_kernel<f64>"));

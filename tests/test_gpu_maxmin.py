@@ -14,8 +14,8 @@ GOLD = np.load(Path(__file__).with_name("golden") / "maxmin_ref.npz")
 
 
 def bits_equal(a, b):
-    a = np.asarray(a)
-    b = np.asarray(b)
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
     return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
