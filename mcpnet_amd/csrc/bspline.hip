@@ -35,6 +35,7 @@ constexpr int kThreads = 256;
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int v4i __attribute__((ext_vector_type(4)));   // 16-byte staging chunk (a native vector, kept in registers)
 
 struct Spline {
   int bins, order;
@@ -281,23 +282,26 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
     basis_values<NB, NO>(sp, t, w, nullptr);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      if (b >= bins) break;
-      acc[b] += (double)w[b];
-      const int64_t r = L.row_of(g, b);
-      if (o.w32) o.w32[r * L.Sp + s] = w[b];
-      if (o.whi) {
-        const uint16_t hi = f32_to_bf16_rne(w[b]);
-        o.whi[r * L.Sp + s] = hi;
-        if (o.wlo) o.wlo[r * L.Sp + s] = f32_to_bf16_rne(w[b] - bf16_to_f32(hi));
+      if (b < bins) {
+        acc[b] += (double)w[b];
+        const int64_t r = L.row_of(g, b);
+        if (o.w32) o.w32[r * L.Sp + s] = w[b];
+        if (o.whi) {
+          const uint16_t hi = f32_to_bf16_rne(w[b]);
+          o.whi[r * L.Sp + s] = hi;
+          if (o.wlo) o.wlo[r * L.Sp + s] = f32_to_bf16_rne(w[b] - bf16_to_f32(hi));
+        }
+        if (o.wref) o.wref[g * o.ldw + (int64_t)b * S + s] = w[b];
       }
-      if (o.wref) o.wref[g * o.ldw + (int64_t)b * S + s] = w[b];
     }
   }
   // FP64 marginals and entropy
+  for (int b = 0; b < bins; ++b) {
+    double v = 0.0;
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    if (b >= bins) break;
-    const double t = block_sum<double>(acc[b], shd);
+    for (int c = 0; c < NB; ++c)
+      if (c == b) v = acc[c];
+    const double t = block_sum<double>(v, shd);
     if (tid == 0) psum[b] = t;
   }
   __syncthreads();
@@ -376,47 +380,45 @@ template <int P>
 constexpr int mi_smem_bytes() { return MISmem<P>::kBytes > kEpiBytes ? MISmem<P>::kBytes : kEpiBytes; }
 
 // Loads the K-step [k0, k0+32) of one 160-row panel into registers: 16-byte chunks,
-// FP32: 8 chunks per row (1280 per panel, 5 per thread); BF16: 4 chunks per row and array.
+// FP32: 8 chunks per row (1280 per panel, 5 per thread); BF16: 4 chunks per row and array
+// (640 per array; BF16X3 stages hi and lo, 5 per thread; BF16 stages hi only, 3 per thread).
 template <int P>
-struct Stage {
-  static constexpr int kPer = (P == kFP32) ? 5 : (P == kBF16 ? 3 : 5);
-  int4 v[kPer];
+struct StageCount {
+  static constexpr int value = (P == kBF16) ? 3 : 5;
 };
 
 template <int P>
-__device__ __forceinline__ void stage_load(const MIArgs& a, int64_t row0, int64_t k0, Stage<P>& st) {
+__device__ __forceinline__ void stage_load(const MIArgs& a, int64_t row0, int64_t k0, v4i (&st)[StageCount<P>::value]) {
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < Stage<P>::kPer; ++q) {
+  for (int q = 0; q < StageCount<P>::value; ++q) {
     const int idx = tid + kThreads * q;
     if (P == kFP32) {
       const int r = idx >> 3, c = idx & 7;
-      st.v[q] = *reinterpret_cast<const int4*>(a.w32 + (row0 + r) * a.L.Sp + k0 + c * 4);
+      st[q] = *reinterpret_cast<const v4i*>(a.w32 + (row0 + r) * a.L.Sp + k0 + c * 4);
     } else {
-      const int arr = idx / 640, rem = idx % 640;
+      const int arr = idx >= 640 ? 1 : 0, rem = idx - 640 * arr;
       const int r = rem >> 2, c = rem & 3;
-      if (P == kBF16 && idx >= 640) continue;
       const uint16_t* src = arr == 0 ? a.whi : a.wlo;
-      st.v[q] = *reinterpret_cast<const int4*>(src + (row0 + r) * a.L.Sp + k0 + c * 8);
+      if (P == kBF16X3 || idx < 640) st[q] = *reinterpret_cast<const v4i*>(src + (row0 + r) * a.L.Sp + k0 + c * 8);
     }
   }
 }
 
 template <int P>
-__device__ __forceinline__ void stage_store(char* panel, const Stage<P>& st) {
+__device__ __forceinline__ void stage_store(char* panel, const v4i (&st)[StageCount<P>::value]) {
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < Stage<P>::kPer; ++q) {
+  for (int q = 0; q < StageCount<P>::value; ++q) {
     const int idx = tid + kThreads * q;
     if (P == kFP32) {
       const int r = idx >> 3, c = idx & 7;
-      *reinterpret_cast<int4*>(reinterpret_cast<float*>(panel) + r * kLdsF32 + c * 4) = st.v[q];
+      *reinterpret_cast<v4i*>(reinterpret_cast<float*>(panel) + r * kLdsF32 + c * 4) = st[q];
     } else {
-      const int arr = idx / 640, rem = idx % 640;
+      const int arr = idx >= 640 ? 1 : 0, rem = idx - 640 * arr;
       const int r = rem >> 2, c = rem & 3;
-      if (P == kBF16 && idx >= 640) continue;
       uint16_t* dst = reinterpret_cast<uint16_t*>(panel) + arr * kPanelB16;
-      *reinterpret_cast<int4*>(dst + r * kLdsB16 + c * 8) = st.v[q];
+      if (P == kBF16X3 || idx < 640) *reinterpret_cast<v4i*>(dst + r * kLdsB16 + c * 8) = st[q];
     }
   }
 }
@@ -446,7 +448,7 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
 #pragma unroll
     for (int j = 0; j < 5; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  Stage<P> sa, sb;
+  v4i sa[StageCount<P>::value], sb[StageCount<P>::value];
   const int64_t nk = a.L.Sp / kKStep;
   stage_load<P>(a, rowA, 0, sa);
   if (!diag) stage_load<P>(a, rowB, 0, sb);
@@ -459,55 +461,53 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
       if (!diag) stage_load<P>(a, rowB, (kk + 1) * kKStep, sb);
     }
     const char* pB = diag ? panelA : panelB;
+    // B fragments stay in registers for the whole K step; A fragments are read per block row
+    // right before their MFMAs, which keeps the live register set small enough to avoid spills.
     if (P == kFP32) {
       const float* As = reinterpret_cast<const float*>(panelA);
       const float* Bs = reinterpret_cast<const float*>(pB);
-      floatx4 af[5][2], bfr[5][2];
+      floatx4 bfr[5][2];
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        const float* pb = Bs + (wc * 80 + j * 16 + lr) * kLdsF32 + lq * 8;
+        bfr[j][0] = *reinterpret_cast<const floatx4*>(pb);
+        bfr[j][1] = *reinterpret_cast<const floatx4*>(pb + 4);
+      }
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
         const float* pa = As + (wr * 80 + i * 16 + lr) * kLdsF32 + lq * 8;
-        af[i][0] = *reinterpret_cast<const floatx4*>(pa);
-        af[i][1] = *reinterpret_cast<const floatx4*>(pa + 4);
-        const float* pb = Bs + (wc * 80 + i * 16 + lr) * kLdsF32 + lq * 8;
-        bfr[i][0] = *reinterpret_cast<const floatx4*>(pb);
-        bfr[i][1] = *reinterpret_cast<const floatx4*>(pb + 4);
-      }
+        const floatx4 a0 = *reinterpret_cast<const floatx4*>(pa);
+        const floatx4 a1 = *reinterpret_cast<const floatx4*>(pa + 4);
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
-#pragma unroll
-        for (int i = 0; i < 5; ++i)
+        for (int u = 0; u < 8; ++u)
 #pragma unroll
           for (int j = 0; j < 5; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[i][u >> 2][u & 3], bfr[j][u >> 2][u & 3], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(u < 4 ? a0[u & 3] : a1[u & 3], bfr[j][u >> 2][u & 3],
+                                                              acc[i][j], 0, 0, 0);
+      }
     } else {
       const uint16_t* Ah = reinterpret_cast<const uint16_t*>(panelA);
       const uint16_t* Bh = reinterpret_cast<const uint16_t*>(pB);
-      bf16x8 ah[5], bh[5];
+      bf16x8 bh[5], bl[5];
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        bh[j] = *reinterpret_cast<const bf16x8*>(Bh + (wc * 80 + j * 16 + lr) * kLdsB16 + lq * 8);
+        if (P == kBF16X3) bl[j] = *reinterpret_cast<const bf16x8*>(Bh + kPanelB16 + (wc * 80 + j * 16 + lr) * kLdsB16 + lq * 8);
+      }
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
-        ah[i] = *reinterpret_cast<const bf16x8*>(Ah + (wr * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
-        bh[i] = *reinterpret_cast<const bf16x8*>(Bh + (wc * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
-      }
-      if (P == kBF16X3) {
-        bf16x8 al[5], bl[5];
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
-          al[i] = *reinterpret_cast<const bf16x8*>(Ah + kPanelB16 + (wr * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
-          bl[i] = *reinterpret_cast<const bf16x8*>(Bh + kPanelB16 + (wc * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
-        }
-#pragma unroll
-        for (int i = 0; i < 5; ++i)
+        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(Ah + (wr * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
+        if (P == kBF16X3) {
+          const bf16x8 al = *reinterpret_cast<const bf16x8*>(Ah + kPanelB16 + (wr * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
 #pragma unroll
           for (int j = 0; j < 5; ++j) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
           }
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
       }
-#pragma unroll
-      for (int i = 0; i < 5; ++i)
-#pragma unroll
-        for (int j = 0; j < 5; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
@@ -592,7 +592,7 @@ __global__ __launch_bounds__(256) void bspline_pair_adjoint_kernel(PairArgs a) {
     }
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
-      if (c >= bins) break;
+      if (c >= bins) continue;
       float v = part[c];
       for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
       const float p = v * a.norm;
@@ -663,18 +663,19 @@ __global__ __launch_bounds__(256) void bspline_gene_adjoint_kernel(GeneArgs a) {
 #pragma unroll
     for (int q = 0; q < kMaxSPerThread; ++q) {
       const int64_t s = tid + 256 * q;
-      if (q >= nper || s >= S) break;
+      if (q >= nper || s >= S) continue;
       float wj[NB];
 #pragma unroll
       for (int c = 0; c < NB; ++c) wj[c] = c < bins ? a.w32[a.L.row_of(j, c) * Sp + s] : 0.0f;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
-        if (b >= bins) break;
-        float v = 0.0f;
+        if (b < bins) {
+          float v = 0.0f;
 #pragma unroll
-        for (int c = 0; c < NB; ++c)
-          if (c < bins) v += gsh[b * bins + c] * wj[c];
-        accw[q][b] += v;
+          for (int c = 0; c < NB; ++c)
+            if (c < bins) v += gsh[b * bins + c] * wj[c];
+          accw[q][b] += v;
+        }
       }
     }
   }
