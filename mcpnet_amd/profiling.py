@@ -30,76 +30,110 @@ def copy_main(mib: int) -> None:
     print(f"copied {mib} MiB x 3", flush=True)
 
 
-def _find(d: str, name: str):
-    hits = glob.glob(os.path.join(d, "**", f"*{name}"), recursive=True)
-    return hits[0] if hits else None
+def _db(d: str):
+    import sqlite3
+
+    hits = glob.glob(os.path.join(d, "**", "*results.db"), recursive=True) if d else []
+    return sqlite3.connect(hits[0]) if hits else None
 
 
 def _short(name: str) -> str:
     for key in ("bspline_mi_kernel", "maxmin_tile_kernel", "bspline_prep_kernel", "ap_mi_kernel", "ratio_kernel",
                 "symmetry_check_kernel", "rank_kernel", "cumhist_kernel", "noise_kernel", "weights_import_kernel",
-                "bspline_pair_adjoint_kernel", "bspline_gene_adjoint_kernel", "emax_kernel"):
+                "bspline_pair_adjoint_kernel", "bspline_gene_adjoint_kernel", "emax_kernel", "copyBuffer",
+                "fillBuffer"):
         if key in name:
             tmpl = name[name.find(key) + len(key):]
-            return key + (tmpl.split("(")[0] if tmpl.startswith("<") else "")
+            if key == "maxmin_tile_kernel":
+                return key + ("<f64>" if "TrF64" in tmpl else "<f32>")
+            if key == "bspline_mi_kernel":
+                return key + {"<0>": "<fp32>", "<1>": "<bf16>", "<2>": "<bf16x3>"}.get(tmpl[:3], "")
+            return key
     return name[:60]
 
 
-def summarize(trace: str, fetch: str | None, write: str | None, copy_mib: int | None, title: str, command: str) -> str:
-    lines = [f"# {title}", "", f"Command: `{command}`", ""]
-    stats = _find(trace, "kernel_stats.csv") if trace else None
-    if stats:
-        lines += ["| kernel | calls | mean ms | total ms |", "|---|---|---|---|"]
-        with open(stats) as f:
-            for r in csv.DictReader(f):
-                calls = int(r["Calls"])
-                avg_ms = float(r["AverageNs"]) / 1e6
-                lines.append(f"| {_short(r['Name'])} | {calls} | {avg_ms:.3f} | {float(r['TotalDurationNs']) / 1e6:.1f} |")
+def _counter(d: str, counter: str):
+    db = _db(d)
+    if db is None:
+        return {}
+    acc = defaultdict(list)
+    for kn, cn, v in db.execute("select kernel_name, counter_name, value from counters_collection"):
+        if cn == counter:
+            acc[_short(kn)].append(float(v))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def summarize(trace, fetch, write, fetch_copy, write_copy, copy_mib, title, command) -> str:
+    """Markdown summary of rocprofv3 databases (rocpd SQLite output)."""
+    lines = [f"# {title}", "", "Command:", "", "```", command, "```", ""]
+    db = _db(trace)
+    if db is not None:
+        lines += ["## Kernel trace (rocprofv3 --kernel-trace --stats; durations from the rocpd database, in ms)", "",
+                  "| kernel | calls | mean ms | total ms | share |", "|---|---|---|---|---|"]
+        for name, calls, total, avg, pct in db.execute(
+                "select name, total_calls, total_duration, average, percentage from top_kernels"):
+            lines.append(f"| {_short(name)} | {calls} | {avg / 1e3:.3f} | {total / 1e3:.1f} | {pct:.1f}% |")
         lines.append("")
-    for label, d, counter in (("FETCH_SIZE", fetch, "FETCH_SIZE"), ("WRITE_SIZE", write, "WRITE_SIZE")):
+    scale = {}
+    for label, d, dc in (("FETCH_SIZE", fetch, fetch_copy), ("WRITE_SIZE", write, write_copy)):
         if not d:
             continue
-        path = _find(d, "counter_collection.csv")
-        if not path:
-            continue
-        acc = defaultdict(list)
-        with open(path) as f:
-            for r in csv.DictReader(f):
-                if r.get("Counter_Name", counter) != counter:
-                    continue
-                acc[_short(r["Kernel_Name"])].append(float(r["Counter_Value"]))
-        scale = None
-        if copy_mib:
-            copy_vals = [v for k, vs in acc.items() if "copy" in k.lower() or "elementwise" in k.lower() for v in vs]
-            if copy_vals:
-                scale = (copy_mib * 1024.0) / (sum(copy_vals) / len(copy_vals))
-        lines += [f"## {label} (KiB per dispatch" + (f"; copy-kernel scale factor {scale:.3f})" if scale else ")"), "",
-                  "| kernel | dispatches | mean KiB | scaled GB |", "|---|---|---|---|"]
-        for k, vs in sorted(acc.items()):
-            m = sum(vs) / len(vs)
-            gb = m * 1024 * (scale or 1.0) / 1e9
-            lines.append(f"| {k} | {len(vs)} | {m:.0f} | {gb:.3f} |")
+        vals = _counter(d, label)
+        s = 1.0
+        if dc and copy_mib:
+            cv = _counter(dc, label).get("copyBuffer")
+            if cv:
+                s = copy_mib * 1024.0 / cv
+        scale[label] = s
+        lines += [f"## {label} (rocprofv3 --pmc {label}), KiB per dispatch", "",
+                  f"Scale factor from a {copy_mib} MiB device copy (`python -m mcpnet_amd.profiling copy`): "
+                  f"{s:.3f} (bytes moved / counter).", "",
+                  "| kernel | mean KiB | scaled GB per dispatch |", "|---|---|---|"]
+        for k, v in sorted(vals.items()):
+            lines.append(f"| {k} | {v:.0f} | {v * 1024 * s / 1e9:.3f} |")
         lines.append("")
     return "\n".join(lines)
 
 
+def traffic(fetch, write, fetch_copy, write_copy, copy_mib) -> dict:
+    """Scaled HBM bytes per dispatch (FETCH + WRITE) per kernel."""
+    out = defaultdict(float)
+    for label, d, dc in (("FETCH_SIZE", fetch, fetch_copy), ("WRITE_SIZE", write, write_copy)):
+        vals = _counter(d, label)
+        s = 1.0
+        cv = _counter(dc, label).get("copyBuffer") if dc else None
+        if cv and copy_mib:
+            s = copy_mib * 1024.0 / cv
+        for k, v in vals.items():
+            out[k] += v * 1024 * s
+    return dict(out)
+
+
 def main(argv=None) -> None:
+    import json
+
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=1024)
-    s = sub.add_parser("summarize")
-    s.add_argument("--trace")
-    s.add_argument("--pmc-fetch")
-    s.add_argument("--pmc-write")
-    s.add_argument("--copy-mib", type=int)
-    s.add_argument("--title", default="rocprofv3 summary")
-    s.add_argument("--command", default="")
+    for name in ("summarize", "traffic"):
+        s = sub.add_parser(name)
+        s.add_argument("--trace")
+        s.add_argument("--pmc-fetch")
+        s.add_argument("--pmc-write")
+        s.add_argument("--pmc-fetch-copy")
+        s.add_argument("--pmc-write-copy")
+        s.add_argument("--copy-mib", type=int, default=1024)
+        s.add_argument("--title", default="rocprofv3 summary")
+        s.add_argument("--command", default="")
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_main(a.mib)
+    elif a.cmd == "summarize":
+        sys.stdout.write(summarize(a.trace, a.pmc_fetch, a.pmc_write, a.pmc_fetch_copy, a.pmc_write_copy, a.copy_mib,
+                                   a.title, a.command) + "\n")
     else:
-        sys.stdout.write(summarize(a.trace, a.pmc_fetch, a.pmc_write, a.copy_mib, a.title, a.command) + "\n")
+        print(json.dumps(traffic(a.pmc_fetch, a.pmc_write, a.pmc_fetch_copy, a.pmc_write_copy, a.copy_mib), indent=1))
 
 
 if __name__ == "__main__":
