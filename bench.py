@@ -100,14 +100,156 @@ def cpu_baselines(X, S, bins, mi_host, budget_s):
     return out
 
 
+def _timed_steps(args, fn, stream, torch):
+    for i in range(args.warmup):
+        log(f"bench: warmup step {i + 1}/{args.warmup}")
+        fn()
+    torch.cuda.synchronize()
+    ms = []
+    for i in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+        log(f"bench: step {i + 1}/{args.steps} done")
+    return float(np.mean(ms))
+
+
+def bench_ap(args, G, S, seed):
+    """Config 4: jitter + descending rank + adaptive-partitioning MI on the GPU."""
+    import torch
+
+    import mcpnet_amd
+    from mcpnet_amd import grn
+
+    log(f"bench: generating {G} x {S} GRN expression (seed {seed:#x})")
+    X = grn.grn_expression(G, S, seed)[0]
+    h = mcpnet_amd.Handle(0)
+    h.set_profiling(True)
+    stream = torch.cuda.ExternalStream(h.stream)
+    Xd = torch.from_numpy(X).cuda()
+    J = torch.empty_like(Xd)
+    R = torch.empty((G, S), dtype=torch.int32, device="cuda")
+    MI = torch.empty((G, G), dtype=torch.float32, device="cuda")
+    kern = {}
+
+    def step():
+        for call in (lambda: h.add_noise(Xd, seed, out=J), lambda: h.rank(J, out=R), lambda: h.ap_mi(R, out=MI)):
+            call()
+            for name, t in h.last_timings():
+                kern.setdefault(name, []).append(t)
+
+    ms = _timed_steps(args, step, stream, torch)
+    P = G * (G - 1) / 2
+    per = {n: float(np.mean(v)) for n, v in kern.items()}
+    result = {
+        "metric": f"adaptive-partitioning MI gene-pairs/s @{G} genes", "value": P / (ms / 1e3),
+        "unit": "gene-pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32 counts, f64 sums",
+        "data": "synthetic: seeded GRN expression + U[0,1e-5) jitter, descending int32 ranks",
+        "config": {"workload": f"config 4: {G} genes x {S} samples: jitter -> rank -> AP-MI (chi-square 7.815)",
+                   "genes": G, "samples": S, "parallelism": "single GPU"},
+        "kernel_mean_ms": per,
+    }
+    if not args.no_cpu:
+        import oracle
+
+        Rh = R.cpu().numpy()
+        ok = bool(np.array_equal(Rh, grn.rank_desc(grn.jitter(X, seed))))
+        rows = 2
+        t0 = time.perf_counter()
+        ref = oracle.ap_mi_rows(Rh, 0, rows)
+        tp = time.perf_counter() - t0
+        rows = int(min(G, max(2, rows * args.cpu_seconds / max(tp, 1e-3))))
+        t0 = time.perf_counter()
+        ref = oracle.ap_mi_rows(Rh, 0, rows)
+        t = time.perf_counter() - t0
+        got = MI[:rows].cpu().numpy()
+        result["cpu_baseline"] = {"value": rows * G / t, "unit": "gene-pairs/s", "cores": omp_threads(), "kind": "port",
+                                  "sample": f"AdaptivePartitionRankMIKernel2 restatement (OpenMP) on rows 0..{rows - 1} "
+                                            f"x {G} genes, {t:.1f} s; host {cpu_model()}"}
+        result["ranks_bit_exact"] = ok
+        result["mi_max_abs_diff_vs_oracle"] = float(np.abs(got.astype(np.float64) - ref).max())
+        result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+    print(json.dumps(result))
+    h.close()
+
+
+def bench_adjoint(args, G, S, seed):
+    """Config 5: dL/dX of L = sum over a 1% Bernoulli pair mask of MI, with W recomputed."""
+    import torch
+
+    import mcpnet_amd
+    from mcpnet_amd import grn
+
+    log(f"bench: generating {G} x {S} GRN expression (seed {seed:#x})")
+    X = grn.grn_expression(G, S, seed)[0]
+    pi, pj, m = grn.mask_pairs(G, 0.01, seed + 1)
+    h = mcpnet_amd.Handle(0)
+    h.set_profiling(True)
+    stream = torch.cuda.ExternalStream(h.stream)
+    Xd = torch.from_numpy(X).cuda()
+    dX = torch.empty_like(Xd)
+    kern = {}
+    holder = {}
+
+    def step():
+        holder["L"] = h.bspline_mi_backward(Xd, pi, pj, m, out=dX)[1]
+        for name, t in h.last_timings():
+            kern.setdefault(name, []).append(t)
+
+    ms = _timed_steps(args, step, stream, torch)
+    result = {
+        "metric": f"B-spline MI adjoint: masked gene-pairs/s @{G} genes", "value": len(pi) / (ms / 1e3),
+        "unit": "masked gene-pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (f64 marginals)",
+        "data": "synthetic: seeded GRN expression, seeded 1% Bernoulli pair mask",
+        "config": {"workload": f"config 5: {G} genes x {S} samples, {len(pi)} masked pairs: forward recompute + "
+                               f"pair tables + gather GEMM through spline derivatives",
+                   "genes": G, "samples": S, "pairs": int(len(pi)), "parallelism": "single GPU"},
+        "kernel_mean_ms": {n: float(np.mean(v)) for n, v in kern.items()}, "loss": holder["L"],
+        "cpu_baseline": None,
+        "cpu_baseline_note": "the reference has no adjoint",
+    }
+    if not args.no_cpu:
+        import oracle
+
+        g = dX.cpu().numpy()
+        rng = np.random.default_rng(0)
+        errs = []
+        X64 = X.astype(np.float64)
+        scale = float(np.sqrt(np.mean(g.astype(np.float64) ** 2)))
+        for _ in range(50):
+            a, b = int(rng.integers(0, G)), int(rng.integers(0, S))
+            sel = (pi == a) | (pj == a)
+            if not sel.any():
+                continue
+            qi, qj, qm = pi[sel], pj[sel], m[sel]
+            Xp = X64.copy(); Xp[a, b] += 1e-3
+            Xm = X64.copy(); Xm[a, b] -= 1e-3
+            fd = (oracle.bspline_mi_loss_x64(Xp, qi, qj, qm) - oracle.bspline_mi_loss_x64(Xm, qi, qj, qm)) / 2e-3
+            errs.append(abs(fd - g[a, b]) / max(abs(fd), scale))
+        errs = np.array(errs)
+        result["fd_check"] = {"entries": int(len(errs)), "median_rel_err": float(np.median(errs)),
+                              "max_rel_err": float(errs.max()), "h": 1e-3}
+    print(json.dumps(result))
+    h.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--genes", type=int, default=20000)
-    ap.add_argument("--samples", type=int, default=2000)
-    ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16", "bf16x3"])
+    ap.add_argument("--config", type=int, default=3, choices=[1, 2, 3, 4, 5],
+                    help="workload: 3 = 20000 x 2000 B-spline MI + MCP3 (the headline, default); 1 = 1000 x 1000 "
+                         "FP32 MI + MCP3; 2 = 2000 x 1024 MI + MCP2/3/4; 4 = 2000 x 1000 adaptive-partitioning MI; "
+                         "5 = 5000 x 500 B-spline MI adjoint")
+    ap.add_argument("--genes", type=int, default=None)
+    ap.add_argument("--samples", type=int, default=None)
+    ap.add_argument("--precision", default=None, choices=["fp32", "bf16", "bf16x3"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of each bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 MI comparison pass")
@@ -123,9 +265,18 @@ def main():
     import mcpnet_amd
     from mcpnet_amd import grn
 
-    G, S, bins = args.genes, args.samples, 10
-    cid = 3 if (G, S) == (20000, 2000) else 0
-    seed = grn.config_seed(cid if cid else 3)
+    cid = args.config
+    G = args.genes or grn.CONFIGS[cid]["G"]
+    S = args.samples or grn.CONFIGS[cid]["S"]
+    bins = 10
+    if args.precision is None:
+        args.precision = "fp32" if cid == 1 else "bf16x3"
+    seed = grn.config_seed(cid)
+    if cid == 4:
+        return bench_ap(args, G, S, seed)
+    if cid == 5:
+        return bench_adjoint(args, G, S, seed)
+    mcp_orders = [2, 3, 4] if cid == 2 else [3]
     log(f"bench: generating {G} x {S} GRN expression (seed {seed:#x})")
     t0 = time.perf_counter()
     X = grn.grn_expression(G, S, seed)[0]
@@ -152,11 +303,12 @@ def main():
             for name, ms in h.last_timings():
                 kern.setdefault(name, []).append(ms)
         MI.fill_diagonal_(0.0)   # mcpnet.cpp:299-308, default diagonal 0
-        h.mcp(MI, 3, out=MCP, maxmin_out=MM)
+        for k in mcp_orders:
+            h.mcp(MI, k, out=MCP, maxmin_out=MM)
+            if record:
+                for name, ms in h.last_timings():
+                    kern.setdefault(name, []).append(ms)
         e2.record(stream)
-        if record:
-            for name, ms in h.last_timings():
-                kern.setdefault(name, []).append(ms)
         return e0, e1, e2
 
     for i in range(args.warmup):
@@ -178,7 +330,8 @@ def main():
 
     P = G * (G - 1) / 2
     mi_rate = P / t_mi
-    mcp3_ops = 2.0 * G * G * G
+    # algorithmic max-min ops (M*N*K per product): MCP2 one product, MCP3 and MCP4 two
+    mcp3_ops = sum({2: 1.0, 3: 2.0, 4: 2.0}[k] for k in mcp_orders) * G * G * G
     mcp3_gops = mcp3_ops / t_mcp / 1e9
 
     # ---- per-kernel rooflines (algorithmic work per launch / mean launch duration)
@@ -250,12 +403,29 @@ def main():
         err = np.abs(got.astype(np.float64) - ref)
         extra["mi_err_vs_fp64"] = {"max": float(err.max()), "mean": float(err.mean()),
                                    "pairs": int(len(li)), "sample": "all pairs of 200 seeded random genes"}
+        if G <= 2000:   # small configs: every pair, every precision
+            full = oracle.bspline_mi_f64(X)
+            errs = {}
+            for prec in ("fp32", "bf16", "bf16x3"):
+                h.bspline_mi(Xd, bins=bins, precision=prec, out=MI)
+                e = np.abs(MI.cpu().numpy().astype(np.float64) - full)
+                errs[prec] = {"max": float(e.max()), "mean": float(e.mean())}
+            extra["mi_err_vs_fp64_all_pairs"] = errs
+            MI.fill_diagonal_(0.0)
+            A = MI.cpu().numpy()
+            for k in mcp_orders:
+                out, mm = h.mcp(MI, k, want_maxmin=True)
+                ro, rm = oracle.mcp(A, k)
+                extra[f"mcp{k}_bit_exact"] = bool(np.array_equal(out.cpu().numpy(), ro)
+                                                  and np.array_equal(mm.cpu().numpy(), rm))
         log("bench: CPU baselines")
         MI.fill_diagonal_(0.0)
         cpu = cpu_baselines(X, S, bins, MI.cpu().numpy(), args.cpu_seconds)
 
     result = {
-        "metric": "MI gene-pairs/s (split-BF16 B-spline MI) & MCP3 max-min Gop/s @20k genes",
+        "metric": ("MI gene-pairs/s (split-BF16 B-spline MI) & MCP3 max-min Gop/s @20k genes" if cid == 3 else
+                   f"MI gene-pairs/s ({args.precision} B-spline MI) & MCP{'/'.join(map(str, mcp_orders))} "
+                   f"max-min Gop/s @{G} genes"),
         "value": mi_rate,
         "unit": "gene-pairs/s",
         "n_gpus": 1,
@@ -267,8 +437,9 @@ def main():
         "vs_baseline": None,
         "dtype": f"{args.precision} MFMA (fp32 accumulate) MI; f32 max-min",
         "data": "synthetic: seeded linear-Gaussian GRN (3 parents/gene), exp, per-gene min-max",
-        "config": {"workload": f"{G} genes x {S} samples: B-spline MI (10 bins, order 3, {args.precision}) "
-                               f"-> diagonal 0 -> MCP3 (two max-min products + ratio)",
+        "config": {"workload": f"config {cid}: {G} genes x {S} samples: B-spline MI (10 bins, order 3, "
+                               f"{args.precision}) -> diagonal 0 -> "
+                               + ", ".join(f"MCP{k}" for k in mcp_orders) + " (max-min products + ratio)",
                    "genes": G, "samples": S, "bins": bins, "spline_order": 3, "parallelism": "single GPU"},
         "mcp3_maxmin_gops": mcp3_gops,
         "stage_ms": {"mi": t_mi * 1e3, "mcp3": t_mcp * 1e3},

@@ -37,6 +37,7 @@ _SIGS = {
     "orc_cumhist_i32": (None, [_P, c_int64, c_int64, _P]),
     "orc_ap_mi_pair": (c_double, [_P, _P, c_int64, c_int64, c_int64, _P, c_int32, _P]),
     "orc_ap_mi": (None, [_P, c_int64, c_int64, _P]),
+    "orc_ap_mi_rows": (None, [_P, c_int64, c_int64, c_int64, c_int64, _P]),
     "orc_ap_mi_cells": (c_double, [_P, c_int64, c_int64, c_int64, c_int64, _P, c_int32, _P]),
     "orc_maxmin_f32": (None, [_P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64]),
     "orc_maxmin_f64": (None, [_P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64]),
@@ -224,6 +225,15 @@ def ap_mi(R):
     MI = np.empty((G, G), dtype=np.float32)
     lib().orc_ap_mi(R.ctypes.data, G, S, MI.ctypes.data)
     return MI
+
+
+def ap_mi_rows(R, row0, nrows):
+    """Rows row0..row0+nrows-1 of the AP-MI matrix (float32, nrows x G)."""
+    R = _c(R, np.int32)
+    G, S = R.shape
+    out = np.empty((nrows, G), dtype=np.float32)
+    lib().orc_ap_mi_rows(R.ctypes.data, G, S, row0, nrows, out.ctypes.data)
+    return out
 
 
 def ap_mi_cells(R, x, y, max_leaves=1 << 16):

@@ -484,6 +484,20 @@ void orc_ap_mi(const int32_t* R, int64_t G, int64_t S, float* MI) {
   free(Hc);
 }
 
+/* Rows [row0, row0+nrows) of the all-pairs AP-MI matrix (the CPU baseline sample): out is
+ * nrows x G, pairs evaluated as (x = row, y = column) like the symmetric pattern's upper triangle. */
+void orc_ap_mi_rows(const int32_t* R, int64_t G, int64_t S, int64_t row0, int64_t nrows, float* out) {
+  int32_t* Hc = (int32_t*)malloc(sizeof(int32_t) * G * (S + 1));
+  orc_cumhist_i32(R, G, S, Hc);
+#pragma omp parallel for schedule(dynamic, 8) collapse(2)
+  for (int64_t r = 0; r < nrows; ++r)
+    for (int64_t j = 0; j < G; ++j) {
+      const int64_t i = row0 + r;
+      out[r * G + j] = (float)(i <= j ? orc_ap_mi_pair(R, Hc, S, i, j, NULL, 0, NULL) : orc_ap_mi_pair(R, Hc, S, j, i, NULL, 0, NULL));
+    }
+  free(Hc);
+}
+
 /* Single pair evaluation with leaf recording; Hc built for the two rows. */
 double orc_ap_mi_cells(const int32_t* R, int64_t G, int64_t S, int64_t gx, int64_t gy, int32_t* cells,
                        int32_t max_leaves, int32_t* nleaves) {

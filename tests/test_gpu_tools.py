@@ -1,0 +1,124 @@
+"""The mi / mcp / mcpnet command-line tools (C++, through the splash pattern shim and the C ABI),
+run on the reference's data/r10c10.exp (copied to tests/golden/) and on seeded GRN data, checked
+against the CPU oracle."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from mcpnet_amd import grn
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+BIN = ROOT / "mcpnet_amd" / "bin"
+R10 = Path(__file__).with_name("golden") / "r10c10.exp"
+
+
+def read_exp(path):
+    rows = [ln.rstrip("\n").split("\t") for ln in Path(path).read_text().splitlines() if ln.strip()]
+    return [r[0] for r in rows[1:]], np.array([[float(v) for v in r[2:]] for r in rows[1:]])
+
+
+def write_exp(path, X, names=None):
+    G, S = X.shape
+    names = names or [f"g{i}" for i in range(G)]
+    with open(path, "w") as f:
+        f.write("Id\tAlias\t" + "\t".join(f"s{j}" for j in range(S)) + "\n")
+        for i in range(G):
+            f.write(names[i] + "\t---\t" + "\t".join(repr(float(v)) for v in X[i]) + "\n")
+
+
+def run(tool, *args):
+    exe = BIN / tool
+    assert exe.exists(), f"{exe} is not built (python -m mcpnet_amd.build)"
+    res = subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    return res.stdout
+
+
+def test_mi_bspline_on_reference_data(tmp_path):
+    out = tmp_path / "r10c10.mi.exp"
+    run("mi", "-i", R10, "-o", out, "-m", "0", "--single")
+    _, mi = read_exp(out)
+    _, X = read_exp(R10)
+    ref = oracle.bspline_mi_f64(X.astype(np.float32))
+    assert np.abs(mi - ref).max() <= 1e-5
+
+
+def test_mi_adaptive_partitioning_double(tmp_path):
+    X = grn.grn_expression(24, 300, 9)[0]
+    src = tmp_path / "x.exp"
+    write_exp(src, X.astype(np.float64))
+    out = tmp_path / "ap.exp"
+    run("mi", "-i", src, "-o", out, "-m", "1", "--noise-seed", "77")
+    _, mi = read_exp(out)
+    Xin = read_exp(src)[1].astype(np.float32)
+    R = grn.rank_desc(grn.jitter(Xin, 77))
+    assert np.abs(mi - oracle.ap_mi(R)).max() <= 1e-6
+
+
+@pytest.mark.parametrize("method,k", [(1, 2), (2, 3), (3, 4)])
+def test_mcp_matches_oracle_bit_exact(tmp_path, method, k):
+    X = grn.grn_expression(40, 200, 4)[0]
+    A = oracle.bspline_mi_f64(X)           # double MI, as the reference's default DataType
+    names = [f"g{i}" for i in range(40)]
+    src = tmp_path / "mi.exp"
+    write_exp(src, A, names)
+    out = tmp_path / f"mcp{k}.exp"
+    mm = tmp_path / f"mm{k}.exp"
+    run("mcp", "-i", src, "-o", out, "-m", method, "--diagonal", "0", "-x", mm)
+    Ain = read_exp(src)[1]
+    np.fill_diagonal(Ain, 0.0)
+    ref_out, ref_mm = oracle.mcp(Ain, k)
+    assert np.array_equal(read_exp(out)[1], ref_out)
+    assert np.array_equal(read_exp(mm)[1], ref_mm)
+
+
+def test_mcp_tf_masked(tmp_path):
+    X = grn.grn_expression(30, 200, 6)[0]
+    A = oracle.bspline_mi_f64(X)
+    np.fill_diagonal(A, 0.0)
+    names = [f"g{i}" for i in range(30)]
+    src = tmp_path / "mi.exp"
+    write_exp(src, A, names)
+    tfs = [0, 4, 7, 19]
+    tf_file = tmp_path / "tfs.txt"
+    tf_file.write_text("\n".join(names[t] for t in tfs) + "\nNOT_A_GENE\n")
+    out = tmp_path / "mcp2tf.exp"
+    run("mcp", "-i", src, "-o", out, "-m", "1", "-f", tf_file, "--tf-gene-transition", "1")
+    got_names, got = read_exp(out)
+    assert got_names == [names[t] for t in tfs]
+    mask = np.full(30, np.finfo(np.float64).min)
+    mask[tfs] = np.finfo(np.float64).max
+    Ain = read_exp(src)[1]
+    mm = oracle.maxmin(np.ascontiguousarray(Ain[tfs]), Ain, mask)      # tfs-tfs-genes (masked)
+    assert np.array_equal(got, oracle.ratio(np.ascontiguousarray(Ain[tfs]), mm))
+
+
+def test_mcpnet_pipeline(tmp_path):
+    X = grn.grn_expression(32, 256, 12)[0]
+    src = tmp_path / "x.exp"
+    write_exp(src, X.astype(np.float64))
+    prefix = tmp_path / "net"
+    run("mcpnet", "-i", src, "-o", prefix, "--mi-method", "0", "-m", "1", "2", "--single")
+    _, mi = read_exp(str(prefix) + ".mi.exp")
+    Xin = read_exp(src)[1].astype(np.float32)
+    assert np.abs(mi - oracle.bspline_mi_f64(Xin)).max() <= 1e-5
+    A = mi.astype(np.float32)
+    np.fill_diagonal(A, 0.0)
+    for k in (2, 3):
+        got = read_exp(str(prefix) + f".mcp{k}.exp")[1].astype(np.float32)
+        assert np.array_equal(got, oracle.mcp(A, k)[0])
+
+
+def test_tools_report_errors(tmp_path):
+    exe = BIN / "mcp"
+    res = subprocess.run([str(exe), "-i", str(tmp_path / "missing.exp"), "-o", str(tmp_path / "o.exp")],
+                         capture_output=True, text=True, timeout=60)
+    assert res.returncode != 0 and "cannot open" in res.stderr
+    res = subprocess.run([str(BIN / "mi"), "-i", str(R10), "-o", str(tmp_path / "o.h5")],
+                         capture_output=True, text=True, timeout=60)
+    assert res.returncode != 0 and "HDF5" in res.stderr
