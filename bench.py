@@ -67,14 +67,14 @@ def cpu_baselines(X, S, bins, mi_host, budget_s):
     # ---- B-spline MI: rows [0, R) against genes [0, C) of the same workload (same S, bins)
     C = min(G, 2000)
     W = oracle.bspline_weights_f32(oracle.minmax_scale_f32(X[:C]), bins=bins, order=3)
-    rows = 8
-    t0 = time.perf_counter()
-    oracle.port_bspline_mi_rows_f32(W, S, bins, 0, rows)
-    t_probe = time.perf_counter() - t0
-    rows = int(min(C, max(8, rows * (0.5 * budget_s) / max(t_probe, 1e-3))))
-    t0 = time.perf_counter()
-    oracle.port_bspline_mi_rows_f32(W, S, bins, 0, rows)
-    t = time.perf_counter() - t0
+    rows, t = 4, 0.0
+    while True:   # grow the sample until it takes about the budget
+        t0 = time.perf_counter()
+        oracle.port_bspline_mi_rows_f32(W, S, bins, 0, rows)
+        t = time.perf_counter() - t0
+        if t >= 0.5 * budget_s or rows >= C:
+            break
+        rows = int(min(C, rows * max(2.0, 0.9 * budget_s / max(t, 1e-3))))
     out["mi"] = {
         "value": rows * C / t, "unit": "gene-pairs/s", "cores": omp_threads(), "kind": "port",
         "sample": f"FP32 B-spline MI inner product (100 dots of S={S} + log2 per pair, OpenMP) on "
@@ -85,13 +85,13 @@ def cpu_baselines(X, S, bins, mi_host, budget_s):
         A = mi_host
         N = A.shape[0]
         r = 4
-        t0 = time.perf_counter()
-        oracle.ref_maxmin(A[:r], A)
-        t_probe = time.perf_counter() - t0
-        r = int(min(N, max(4, r * (0.5 * budget_s) / max(t_probe, 1e-3))))
-        t0 = time.perf_counter()
-        oracle.ref_maxmin(A[:r], A)
-        t = time.perf_counter() - t0
+        while True:
+            t0 = time.perf_counter()
+            oracle.ref_maxmin(A[:r], A)
+            t = time.perf_counter() - t0
+            if t >= 0.5 * budget_s or r >= N:
+                break
+            r = int(min(N, r * max(2.0, 0.9 * budget_s / max(t, 1e-3))))
         out["maxmin"] = {
             "value": r * N * N / t / 1e9, "unit": "Gop/s", "cores": omp_threads(), "kind": "reference",
             "sample": f"reference max_of_pairmin (simd_maxmin.hpp, float = scalar path) over rows 0..{r - 1} "
@@ -365,7 +365,7 @@ def main():
     traffic_file = ROOT / "profiles" / "traffic.json"
     if traffic_file.exists():
         try:
-            tr = json.loads(traffic_file.read_text())
+            tr = json.loads(traffic_file.read_text()).get(f"config{cid}", {})
             for n in roof:
                 if n in tr:
                     roof[n]["traffic"] = tr[n]
