@@ -4,8 +4,9 @@
 // Layout in HBM.  Weights live in "group-tiled" row panels: rows are (gene, bin) pairs, samples
 // run along the row (K of the contraction), zero-padded to Sp = roundup(S, 32).  Genes are
 // packed GW = 80 / bins to an 80-row group (8 genes x 10 bins for the default), so a group is
-// exactly five 16-row MFMA blocks; the remaining 80 - GW*bins rows of a group are zero.  A
-// workgroup tile is two groups (GT = 2*GW genes, 160 rows) on each side.
+// exactly five 16-row MFMA blocks; the remaining 80 - GW*bins rows of a group are zero.  Within
+// a group the samples are blocked by 32 (Layout::at), so the slice a K step reads is one
+// contiguous 80 x 32 run.  A workgroup tile is two groups (GT = 2*GW genes, 160 rows) per side.
 //
 // Arithmetic.  The joint table of genes (i, j) is the 10 x 10 block W_i * W_j^T / S of the
 // (rows x rows) Gram matrix; it is never written to HBM.  Three contraction precisions share
@@ -160,6 +161,11 @@ struct Layout {
   __host__ __device__ int64_t row_of(int64_t g, int b) const {
     return (g / gw) * kGroupRows + (g % gw) * bins + b;
   }
+  // Element (row, sample) of a K-blocked panel plane: each 80-row group stores its samples in
+  // blocks of 32, so one group's K step is one contiguous 80 x 32 slice.
+  __host__ __device__ int64_t at(int64_t row, int64_t s) const {
+    return (((row / kGroupRows) * (Sp / kKStep) + s / kKStep) * kGroupRows + row % kGroupRows) * kKStep + s % kKStep;
+  }
 };
 
 Layout make_layout(int64_t G, int64_t S, int bins) {
@@ -284,12 +290,12 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
     for (int b = 0; b < NB; ++b) {
       if (b < bins) {
         acc[b] += (double)w[b];
-        const int64_t r = L.row_of(g, b);
-        if (o.w32) o.w32[r * L.Sp + s] = w[b];
+        const int64_t e = L.at(L.row_of(g, b), s);
+        if (o.w32) o.w32[e] = w[b];
         if (o.whi) {
           const uint16_t hi = f32_to_bf16_rne(w[b]);
-          o.whi[r * L.Sp + s] = hi;
-          if (o.wlo) o.wlo[r * L.Sp + s] = f32_to_bf16_rne(w[b] - bf16_to_f32(hi));
+          o.whi[e] = hi;
+          if (o.wlo) o.wlo[e] = f32_to_bf16_rne(w[b] - bf16_to_f32(hi));
         }
         if (o.wref) o.wref[g * o.ldw + (int64_t)b * S + s] = w[b];
       }
@@ -333,12 +339,12 @@ __global__ void weights_import_kernel(const float* __restrict__ W, int64_t ldw, 
     }
     const int b = (int)(e / L.S);
     const int64_t s = e % L.S;
-    const int64_t r = L.row_of(g, b);
-    if (w32) w32[r * L.Sp + s] = v;
+    const int64_t ei = L.at(L.row_of(g, b), s);
+    if (w32) w32[ei] = v;
     if (whi) {
       const uint16_t hi = f32_to_bf16_rne(v);
-      whi[r * L.Sp + s] = hi;
-      if (wlo) wlo[r * L.Sp + s] = f32_to_bf16_rne(v - bf16_to_f32(hi));
+      whi[ei] = hi;
+      if (wlo) wlo[ei] = f32_to_bf16_rne(v - bf16_to_f32(hi));
     }
   }
 }
@@ -395,12 +401,12 @@ __device__ __forceinline__ void stage_load(const MIArgs& a, int64_t row0, int64_
     const int idx = tid + kThreads * q;
     if (P == kFP32) {
       const int r = idx >> 3, c = idx & 7;
-      st[q] = *reinterpret_cast<const v4i*>(a.w32 + (row0 + r) * a.L.Sp + k0 + c * 4);
+      st[q] = *reinterpret_cast<const v4i*>(a.w32 + a.L.at(row0 + r, k0) + c * 4);
     } else {
       const int arr = idx >= 640 ? 1 : 0, rem = idx - 640 * arr;
       const int r = rem >> 2, c = rem & 3;
       const uint16_t* src = arr == 0 ? a.whi : a.wlo;
-      if (P == kBF16X3 || idx < 640) st[q] = *reinterpret_cast<const v4i*>(src + (row0 + r) * a.L.Sp + k0 + c * 8);
+      if (P == kBF16X3 || idx < 640) st[q] = *reinterpret_cast<const v4i*>(src + a.L.at(row0 + r, k0) + c * 8);
     }
   }
 }
@@ -432,7 +438,7 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
 
   int64_t ti, tj;
   {
-    const uint32_t code = a.order[mcpg::xcd_remap(blockIdx.x, a.ntri)];
+    const uint32_t code = a.order[blockIdx.x];
     ti = code >> 16;
     tj = code & 0xffffu;
   }
@@ -580,15 +586,15 @@ __global__ __launch_bounds__(256) void bspline_pair_adjoint_kernel(PairArgs a) {
   float part[NB];  // reused per row b
   double Lsum = 0.0;
   for (int b = 0; b < bins; ++b) {
-    const float* wi = a.w32 + a.L.row_of(i, b) * Sp;
+    const int64_t ri = a.L.row_of(i, b);
 #pragma unroll
     for (int c = 0; c < NB; ++c) part[c] = 0.0f;
     for (int64_t s = lane; s < a.L.S; s += 64) {
-      const float x = wi[s];
+      const float x = a.w32[a.L.at(ri, s)];
       if (x == 0.0f) continue;
 #pragma unroll
       for (int c = 0; c < NB; ++c)
-        if (c < bins) part[c] += x * a.w32[a.L.row_of(j, c) * Sp + s];
+        if (c < bins) part[c] += x * a.w32[a.L.at(a.L.row_of(j, c), s)];
     }
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
@@ -666,7 +672,7 @@ __global__ __launch_bounds__(256) void bspline_gene_adjoint_kernel(GeneArgs a) {
       if (q >= nper || s >= S) continue;
       float wj[NB];
 #pragma unroll
-      for (int c = 0; c < NB; ++c) wj[c] = c < bins ? a.w32[a.L.row_of(j, c) * Sp + s] : 0.0f;
+      for (int c = 0; c < NB; ++c) wj[c] = c < bins ? a.w32[a.L.at(a.L.row_of(j, c), s)] : 0.0f;
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
         if (b < bins) {
@@ -749,7 +755,7 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
   a.norm = 1.0f / (float)L.S;
   if (a.ntri > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "MI grid too large");
   int st;
-  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 8), &st);
+  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 8), mcpg::env_group("MCPG_MI_XCD", 1), &st);
   if (!a.order) return st;
   hipError_t e;
   {

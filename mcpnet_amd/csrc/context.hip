@@ -72,22 +72,36 @@ int finish(mcpg_ctx* h, const char* where) {
   return MCPG_OK;
 }
 
-const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int* status) {
+const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int xcd_deal, int* status) {
   *status = MCPG_OK;
   if (nb > 65535) {
     *status = set_error(h, MCPG_ERR_UNSUPPORTED, "tile grid too large for the packed tile order");
     return nullptr;
   }
+  // layout of the table in launch order: by default tiles are dealt so that each XCD (blocks go to
+  // the 8 XCDs round-robin by id) receives one contiguous run of the blocked order
+  const int xcd = xcd_deal != 0;
+  const int key = group * 2 + xcd;
   for (const auto& o : h->orders)
-    if (o.nb == nb && o.group == group) return o.d;
-  std::vector<uint32_t> order;
-  order.reserve((size_t)(nb * (nb + 1) / 2));
+    if (o.nb == nb && o.group == key) return o.d;
+  std::vector<uint32_t> blocked;
+  blocked.reserve((size_t)(nb * (nb + 1) / 2));
   const int64_t nbb = (nb + group - 1) / group;
   for (int64_t bi = 0; bi < nbb; ++bi)
     for (int64_t bj = bi; bj < nbb; ++bj)
       for (int64_t ti = bi * group; ti < std::min(nb, (bi + 1) * group); ++ti)
         for (int64_t tj = bj * group; tj < std::min(nb, (bj + 1) * group); ++tj)
-          if (ti <= tj) order.push_back((uint32_t)((ti << 16) | tj));
+          if (ti <= tj) blocked.push_back((uint32_t)((ti << 16) | tj));
+  std::vector<uint32_t> order(blocked.size());
+  const int64_t total = (int64_t)blocked.size();
+  for (int64_t b = 0; b < total; ++b) {
+    int64_t l = b;
+    if (xcd) {
+      const int64_t x = b % 8, local = b / 8, q = total / 8, r = total % 8;
+      l = x * q + (x < r ? x : r) + local;
+    }
+    order[b] = blocked[l];
+  }
   if (h->orders.size() >= 4) {  // keep a few grids (MI and max-min alternate in a pipeline)
     hipError_t e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) { *status = hip_error(h, e, "hipStreamSynchronize"); return nullptr; }
@@ -99,7 +113,7 @@ const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int* status) {
   if (e != hipSuccess) { *status = hip_error(h, e, "hipMalloc(tile order)"); return nullptr; }
   e = hipMemcpy(d, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e != hipSuccess) { (void)hipFree(d); *status = hip_error(h, e, "hipMemcpy(tile order)"); return nullptr; }
-  h->orders.push_back(mcpg_ctx::TileOrder{d, nb, group});
+  h->orders.push_back(mcpg_ctx::TileOrder{d, nb, key});
   return d;
 }
 
@@ -107,7 +121,7 @@ int env_group(const char* name, int def) {
   const char* v = getenv(name);
   if (!v) return def;
   int g = atoi(v);
-  return (g >= 1 && g <= 256) ? g : def;
+  return (g >= 0 && g <= 256) ? g : def;
 }
 
 void begin_call(mcpg_ctx* h) {

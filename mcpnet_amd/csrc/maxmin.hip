@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kThreads) void maxmin_tile_kernel(MMArgs a) {
   const int64_t L = mcpg::xcd_remap(blockIdx.x, a.ntiles);
   int64_t bm, bn;
   if (a.tri) {
-    const uint32_t code = a.order[L];
+    const uint32_t code = a.order[blockIdx.x];
     bm = code >> 16;
     bn = code & 0xffffu;
   } else {
@@ -302,7 +302,7 @@ int launch_maxmin(mcpg_ctx* h, const typename Tr::T* A, int64_t lda, const typen
   a.order = nullptr;
   if (tri) {
     int st;
-    a.order = mcpg::tri_order(h, a.nbm, mcpg::env_group("MCPG_MM_GROUP", 8), &st);
+    a.order = mcpg::tri_order(h, a.nbm, mcpg::env_group("MCPG_MM_GROUP", 8), mcpg::env_group("MCPG_MM_XCD", 0), &st);
     if (!a.order) return st;
   }
   {
