@@ -755,7 +755,7 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
   a.norm = 1.0f / (float)L.S;
   if (a.ntri > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "MI grid too large");
   int st;
-  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 8), mcpg::env_group("MCPG_MI_XCD", 1), &st);
+  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 8), mcpg::env_group("MCPG_MI_XCD", 0), &st);
   if (!a.order) return st;
   hipError_t e;
   {

@@ -80,10 +80,11 @@ int finish(mcpg_ctx* h, const char* where);  // stream sync + error check
 
 // Device array of the (ti, tj), ti <= tj, tiles of an nb x nb grid, packed as (ti << 16) | tj and
 // indexed by block id.  Tiles are ordered in `group` x `group` blocks (blocks row-major over the
-// block triangle, tiles row-major inside a block), then dealt so that each of the 8 XCDs
-// (workgroups go to XCDs round-robin by id) runs one contiguous stretch of that order.
-// Workgroups that run together on an XCD then share 2*group row panels through its L2 instead
-// of one row panel and as many column panels as there are workgroups.  Cached per handle.
+// block triangle, tiles row-major inside a block), so that workgroups running together share
+// 2*group row panels through L2 instead of one row panel and as many column panels as there are
+// workgroups.  With xcd_deal set the order is further dealt so that each of the 8 XCDs
+// (workgroups go to XCDs round-robin by id) runs one contiguous stretch of it.  Both kernels
+// that use the table measured faster without the deal at 20000 genes.  Cached per handle.
 const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int xcd_deal, int* status);
 
 // Tile-group size: the value of environment variable `name` if set (1..256), else `def`.
