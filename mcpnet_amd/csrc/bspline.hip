@@ -157,7 +157,8 @@ struct Layout {
   int bins, gw, gt;     // bins, genes per group, genes per tile side
   int64_t G, S, Sp;     // genes, samples, padded samples
   int64_t ntiles;       // gene tiles per side
-  int64_t rows;         // weight rows allocated = ntiles * 160
+  int64_t rows;         // weight rows allocated = ntiles * tr
+  int tr;               // rows per workgroup tile side (160: FP32 kernel, 320: BF16 kernels)
   __host__ __device__ int64_t row_of(int64_t g, int b) const {
     return (g / gw) * kGroupRows + (g % gw) * bins + b;
   }
@@ -168,16 +169,18 @@ struct Layout {
   }
 };
 
-Layout make_layout(int64_t G, int64_t S, int bins) {
+// groups_per_tile: 80-row gene groups per workgroup tile side (2 for the FP32 kernel, 4 for BF16)
+Layout make_layout(int64_t G, int64_t S, int bins, int groups_per_tile = 2) {
   Layout L;
   L.bins = bins;
   L.gw = kGroupRows / bins;
-  L.gt = 2 * L.gw;
+  L.gt = groups_per_tile * L.gw;
+  L.tr = groups_per_tile * kGroupRows;
   L.G = G;
   L.S = S;
   L.Sp = ceil_div(S, kKStep) * kKStep;
   L.ntiles = ceil_div(G, L.gt);
-  L.rows = L.ntiles * kTileRows;
+  L.rows = L.ntiles * L.tr;
   return L;
 }
 
@@ -365,76 +368,90 @@ struct MIArgs {
   float norm;        // (float)1 / (float)S, the reference's FT norm_factor (BSplineMI.hpp:319)
 };
 
-// LDS row strides (elements): FP32 rows of 32 floats + 4 pad; BF16 rows of 32 halves + 8 pad.
+// LDS layouts.
+// * FP32 panels: rows of 32 floats + 4 pad (144 B), staged through registers.
+// * BF16 planes: rows of 32 bf16 (64 B, no padding) whose four 16-byte chunks are stored
+//   XOR-swizzled, chunk' = chunk ^ (row & 8 ? 3 : 0).  A gfx950 ds_read_b128 serves 16 lanes per
+//   pass from 64 four-byte banks; the lanes of one pass are {0-3, 12-15, 20-23, 24-27} and its
+//   translates by 4 and by 32 (measured on MI355X).  The MFMA operand read (lane -> row lane & 15,
+//   chunk lane >> 4) then covers all 64 banks in every pass.  The planes are filled by
+//   global_load_lds_dwordx4, which writes 64 lanes x 16 B contiguously into LDS, so the swizzle is
+//   applied to the global source addresses.  Two stages (K steps) are in flight: the loads of step
+//   k + 1 run under the MFMAs of step k.
 constexpr int kLdsF32 = 36;
-constexpr int kLdsB16 = 40;
-constexpr int kPanelF32 = kTileRows * kLdsF32;        // floats per panel
-constexpr int kPanelB16 = kTileRows * kLdsB16;        // uint16 per panel
-constexpr int kEpiStride = 81;                        // t-tile row stride (floats)
+constexpr int kPanelF32 = kTileRows * kLdsF32;        // floats per FP32 panel
+constexpr int kEpiStride = 81;                        // FP32 kernel t-tile row stride (floats)
 constexpr int kEpiWave = kGroupRows * kEpiStride;     // floats per wave t-tile
-
-template <int P>
-struct MISmem;
-template <>
-struct MISmem<kFP32> { static constexpr int kBytes = 2 * kPanelF32 * 4; };
-template <>
-struct MISmem<kBF16> { static constexpr int kBytes = 2 * kPanelB16 * 2; };
-template <>
-struct MISmem<kBF16X3> { static constexpr int kBytes = 4 * kPanelB16 * 2; };
 constexpr int kEpiBytes = 2 * kEpiWave * 4;           // two waves' t-tiles at a time
-template <int P>
-constexpr int mi_smem_bytes() { return MISmem<P>::kBytes > kEpiBytes ? MISmem<P>::kBytes : kEpiBytes; }
 
-// Loads the K-step [k0, k0+32) of one 160-row panel into registers: 16-byte chunks,
-// FP32: 8 chunks per row (1280 per panel, 5 per thread); BF16: 4 chunks per row and array
-// (640 per array; BF16X3 stages hi and lo, 5 per thread; BF16 stages hi only, 3 per thread).
-template <int P>
-struct StageCount {
-  static constexpr int value = (P == kBF16) ? 3 : 5;
-};
+constexpr int kRowB16 = 64;                           // bytes per BF16 plane row (32 samples)
+constexpr int kPlaneB16 = kTileRows * kRowB16;        // bytes per 160-row plane K slice (10 KB)
+constexpr int kDmaRows = 16;                          // rows covered by one 1 KB direct load
+constexpr int kDmaPerPlane = kTileRows / kDmaRows;    // 10
+// buffer resource word 3 for raw (untyped, unswizzled) gfx9-family buffer accesses
+constexpr int kRsrcWord3 = 0x00020000;
 
 template <int P>
-__device__ __forceinline__ void stage_load(const MIArgs& a, int64_t row0, int64_t k0, v4i (&st)[StageCount<P>::value]) {
+constexpr int mi_planes() { return P == kBF16X3 ? 2 : 1; }
+template <int P>
+constexpr int mi_stage_bytes() { return 2 * mi_planes<P>() * kPlaneB16; }   // panels A and B
+template <int P>
+constexpr int mi_smem_bytes() {
+  constexpr int body = (P == kFP32) ? 2 * kPanelF32 * 4 : 2 * mi_stage_bytes<P>();
+  return body > kEpiBytes ? body : kEpiBytes;
+}
+
+// FP32: loads the K-step [k0, k0+32) of one 160-row panel into registers (16-byte chunks, 8 per
+// row, 5 per thread) and stores them to the padded LDS panel.
+constexpr int kStageF32 = 5;
+
+__device__ __forceinline__ void stage_load_f32(const MIArgs& a, int64_t row0, int64_t k0, v4i (&st)[kStageF32]) {
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int q = 0; q < StageCount<P>::value; ++q) {
+  for (int q = 0; q < kStageF32; ++q) {
     const int idx = tid + kThreads * q;
-    if (P == kFP32) {
-      const int r = idx >> 3, c = idx & 7;
-      st[q] = *reinterpret_cast<const v4i*>(a.w32 + a.L.at(row0 + r, k0) + c * 4);
-    } else {
-      const int arr = idx >= 640 ? 1 : 0, rem = idx - 640 * arr;
-      const int r = rem >> 2, c = rem & 3;
-      const uint16_t* src = arr == 0 ? a.whi : a.wlo;
-      if (P == kBF16X3 || idx < 640) st[q] = *reinterpret_cast<const v4i*>(src + a.L.at(row0 + r, k0) + c * 8);
+    const int r = idx >> 3, c = idx & 7;
+    st[q] = *reinterpret_cast<const v4i*>(a.w32 + a.L.at(row0 + r, k0) + c * 4);
+  }
+}
+
+__device__ __forceinline__ void stage_store_f32(char* panel, const v4i (&st)[kStageF32]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < kStageF32; ++q) {
+    const int idx = tid + kThreads * q;
+    const int r = idx >> 3, c = idx & 7;
+    *reinterpret_cast<v4i*>(reinterpret_cast<float*>(panel) + r * kLdsF32 + c * 4) = st[q];
+  }
+}
+
+// BF16: the direct loads of one K step.  Each 1 KB load fills 16 rows x 64 B of one plane slice:
+// LDS row 16c + lane / 4, physical chunk lane % 4, from logical chunk (lane % 4) ^ swizzle of the
+// K-blocked panel.  A 16-row run never crosses an 80-row group, so the source is one contiguous
+// 1 KB.  The loads go through a buffer resource (scalar base and range, hardware bounds check);
+// the per-load part of the source offset is scalar, `lbyte` is the lane's part.
+typedef __attribute__((address_space(3))) void* lds_ptr;
+
+__device__ __forceinline__ void dma_issue(__amdgpu_buffer_rsrc_t rsrc, uint32_t lbyte, uint32_t sbase, uint32_t grp_stride,
+                                          int count, int c0, char* dst) {
+  // chunks c0 .. c0 + count - 1 of one plane slice; chunk c covers rows 16c .. 16c + 15, i.e.
+  // row block c % 5 of 80-row group c / 5
+#pragma unroll
+  for (int j = 0; j < kDmaPerPlane; ++j) {
+    if (j < count) {
+      const int c = c0 + j;
+      const uint32_t soff = sbase + (uint32_t)(c / 5) * grp_stride + (uint32_t)(c % 5) * (kDmaRows * kKStep * 2);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr)(dst + j * 1024), 16, lbyte, soff, 0, 0);
     }
   }
 }
 
-template <int P>
-__device__ __forceinline__ void stage_store(char* panel, const v4i (&st)[StageCount<P>::value]) {
-  const int tid = threadIdx.x;
-#pragma unroll
-  for (int q = 0; q < StageCount<P>::value; ++q) {
-    const int idx = tid + kThreads * q;
-    if (P == kFP32) {
-      const int r = idx >> 3, c = idx & 7;
-      *reinterpret_cast<v4i*>(reinterpret_cast<float*>(panel) + r * kLdsF32 + c * 4) = st[q];
-    } else {
-      const int arr = idx >= 640 ? 1 : 0, rem = idx - 640 * arr;
-      const int r = rem >> 2, c = rem & 3;
-      uint16_t* dst = reinterpret_cast<uint16_t*>(panel) + arr * kPanelB16;
-      if (P == kBF16X3 || idx < 640) *reinterpret_cast<v4i*>(dst + r * kLdsB16 + c * 8) = st[q];
-    }
-  }
-}
-
+// All-pairs MI: 160 x 160 workgroup tile (16 x 16 genes for 10 bins), four waves of 80 x 80
+// (5 x 5 blocks of 16 x 16 MFMA outputs), two workgroups per CU.  FP32 operands are staged through
+// registers into padded LDS panels; BF16 planes arrive through the two-stage direct-load pipeline.
 template <int P>
 __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
   extern __shared__ __align__(16) char smem[];
-  constexpr int kPanelBytes = (P == kFP32) ? kPanelF32 * 4 : (P == kBF16 ? kPanelB16 * 2 : 2 * kPanelB16 * 2);
-  char* panelA = smem;
-  char* panelB = smem + kPanelBytes;
 
   int64_t ti, tj;
   {
@@ -454,57 +471,96 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
 #pragma unroll
     for (int j = 0; j < 5; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  v4i sa[StageCount<P>::value], sb[StageCount<P>::value];
   const int64_t nk = a.L.Sp / kKStep;
-  stage_load<P>(a, rowA, 0, sa);
-  if (!diag) stage_load<P>(a, rowB, 0, sb);
-  for (int64_t kk = 0; kk < nk; ++kk) {
-    stage_store<P>(panelA, sa);
-    if (!diag) stage_store<P>(panelB, sb);
-    __syncthreads();
-    if (kk + 1 < nk) {
-      stage_load<P>(a, rowA, (kk + 1) * kKStep, sa);
-      if (!diag) stage_load<P>(a, rowB, (kk + 1) * kKStep, sb);
-    }
-    const char* pB = diag ? panelA : panelB;
-    // B fragments stay in registers for the whole K step; A fragments are read per block row
-    // right before their MFMAs, which keeps the live register set small enough to avoid spills.
-    if (P == kFP32) {
+  if constexpr (P == kFP32) {
+    char* panelA = smem;
+    char* panelB = smem + kPanelF32 * 4;
+    v4i sa[kStageF32], sb[kStageF32];
+    stage_load_f32(a, rowA, 0, sa);
+    if (!diag) stage_load_f32(a, rowB, 0, sb);
+    for (int64_t kk = 0; kk < nk; ++kk) {
+      stage_store_f32(panelA, sa);
+      if (!diag) stage_store_f32(panelB, sb);
+      __syncthreads();
+      if (kk + 1 < nk) {
+        stage_load_f32(a, rowA, (kk + 1) * kKStep, sa);
+        if (!diag) stage_load_f32(a, rowB, (kk + 1) * kKStep, sb);
+      }
       const float* As = reinterpret_cast<const float*>(panelA);
-      const float* Bs = reinterpret_cast<const float*>(pB);
+      const float* Bs = reinterpret_cast<const float*>(diag ? panelA : panelB);
+      // B fragments stay in registers for the whole K step; A fragments are read per block row.
       floatx4 bfr[5][2];
-#pragma unroll
+  #pragma unroll
       for (int j = 0; j < 5; ++j) {
         const float* pb = Bs + (wc * 80 + j * 16 + lr) * kLdsF32 + lq * 8;
         bfr[j][0] = *reinterpret_cast<const floatx4*>(pb);
         bfr[j][1] = *reinterpret_cast<const floatx4*>(pb + 4);
       }
-#pragma unroll
+  #pragma unroll
       for (int i = 0; i < 5; ++i) {
         const float* pa = As + (wr * 80 + i * 16 + lr) * kLdsF32 + lq * 8;
         const floatx4 a0 = *reinterpret_cast<const floatx4*>(pa);
         const floatx4 a1 = *reinterpret_cast<const floatx4*>(pa + 4);
-#pragma unroll
+  #pragma unroll
         for (int u = 0; u < 8; ++u)
-#pragma unroll
+  #pragma unroll
           for (int j = 0; j < 5; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(u < 4 ? a0[u & 3] : a1[u & 3], bfr[j][u >> 2][u & 3],
                                                               acc[i][j], 0, 0, 0);
       }
-    } else {
-      const uint16_t* Ah = reinterpret_cast<const uint16_t*>(panelA);
-      const uint16_t* Bh = reinterpret_cast<const uint16_t*>(pB);
+      __syncthreads();
+    }
+  } else {
+    constexpr int np = mi_planes<P>();
+    constexpr int kStage = mi_stage_bytes<P>();
+    // Load assignment: the stage's plane slices (A.hi, [A.lo,] B.hi, [B.lo]; A only on diagonal
+    // tiles) are split evenly over the four waves; each wave takes a contiguous run of one
+    // slice's 16-row chunks.
+    const int swave = __builtin_amdgcn_readfirstlane(wave);
+    const int nslices = (diag ? 1 : 2) * np;                   // 1, 2 or 4
+    const int wps = 4 / nslices;                               // waves per slice
+    const int slice = swave / wps;
+    const int nper = (kDmaPerPlane + wps - 1) / wps;
+    const int c0 = (swave % wps) * nper;
+    const int count = kDmaPerPlane - c0 < nper ? kDmaPerPlane - c0 : nper;
+    const int lrow = lane >> 2;
+    const uint32_t lbyte = (lrow * kKStep + ((((lane & 3) ^ ((lrow & 8) ? 3 : 0))) << 3)) * 2;
+    const uint32_t plane_bytes = (uint32_t)(a.L.rows * a.L.Sp * 2);
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)((P == kBF16X3 && (slice % np) == 1) ? a.wlo : a.whi), (short)0, plane_bytes, kRsrcWord3);
+    const uint32_t grp_stride = (uint32_t)(nk * kGroupRows * kKStep * 2);      // bytes between groups
+    const uint32_t kstep_bytes = (uint32_t)(kGroupRows * kKStep * 2);
+    const uint32_t sbase0 = (uint32_t)(((slice / np) == 0 ? rowA : rowB) / kGroupRows) * grp_stride;
+    const int dst_off = slice * kPlaneB16 + c0 * 1024;
+    // byte offset of this lane's MFMA fragment within a 16-row block of a swizzled plane
+    const int frag = lr * kRowB16 + ((lq ^ ((lr & 8) ? 3 : 0)) << 4);
+
+    dma_issue(rsrc, lbyte, sbase0, grp_stride, count, c0, smem + dst_off);
+    for (int64_t kk = 0; kk < nk; ++kk) {
+      // this wave's loads of step kk have landed; after the barrier, everyone's have, and every
+      // wave is done reading the other stage (step kk - 1), which the next loads overwrite
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      const char* st = smem + (kk & 1) * kStage;
+      if (kk + 1 < nk)
+        dma_issue(rsrc, lbyte, sbase0 + (uint32_t)(kk + 1) * kstep_bytes, grp_stride, count, c0,
+                  smem + ((kk + 1) & 1) * kStage + dst_off);
+      const char* sA = st;
+      const char* sB = diag ? st : st + np * kPlaneB16;
+      // B fragments stay in registers for the whole K step; A fragments are read per block row.
       bf16x8 bh[5], bl[5];
 #pragma unroll
       for (int j = 0; j < 5; ++j) {
-        bh[j] = *reinterpret_cast<const bf16x8*>(Bh + (wc * 80 + j * 16 + lr) * kLdsB16 + lq * 8);
-        if (P == kBF16X3) bl[j] = *reinterpret_cast<const bf16x8*>(Bh + kPanelB16 + (wc * 80 + j * 16 + lr) * kLdsB16 + lq * 8);
+        const int o = (wc * 80 + j * 16) * kRowB16 + frag;
+        bh[j] = *reinterpret_cast<const bf16x8*>(sB + o);
+        if (P == kBF16X3) bl[j] = *reinterpret_cast<const bf16x8*>(sB + kPlaneB16 + o);
       }
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
-        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(Ah + (wr * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
+        const int o = (wr * 80 + i * 16) * kRowB16 + frag;
+        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(sA + o);
         if (P == kBF16X3) {
-          const bf16x8 al = *reinterpret_cast<const bf16x8*>(Ah + kPanelB16 + (wr * 80 + i * 16 + lr) * kLdsB16 + lq * 8);
+          const bf16x8 al = *reinterpret_cast<const bf16x8*>(sA + kPlaneB16 + o);
 #pragma unroll
           for (int j = 0; j < 5; ++j) {
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
@@ -754,23 +810,39 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
   a.ntri = L.ntiles * (L.ntiles + 1) / 2;
   a.norm = 1.0f / (float)L.S;
   if (a.ntri > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "MI grid too large");
+  // the BF16 kernels address a weight plane through a buffer resource with a 32-bit byte range
+  if (precision != MCPG_FP32 && (uint64_t)L.rows * (uint64_t)L.Sp * 2 >= (1ull << 32))
+    return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "weight panels exceed 4 GiB");
   int st;
-  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 8), mcpg::env_group("MCPG_MI_XCD", 0), &st);
-  if (!a.order) return st;
   hipError_t e;
+  if (L.tr != kTileRows) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: weight layout does not match the MI kernel");
+  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 16), mcpg::env_group("MCPG_MI_XCD", 0), &st);
+  if (!a.order) return st;
+  const unsigned grid = (unsigned)a.ntri;
+  static bool lds_attr_set = false;
+  if (!lds_attr_set) {
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_kernel<kFP32>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mi_smem_bytes<kFP32>()));
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_kernel<kBF16>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mi_smem_bytes<kBF16>()));
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_kernel<kBF16X3>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, mi_smem_bytes<kBF16X3>()));
+    lds_attr_set = true;
+  }
   {
     const char* name = precision == MCPG_FP32 ? "bspline_mi_kernel<fp32>"
                        : precision == MCPG_BF16 ? "bspline_mi_kernel<bf16>" : "bspline_mi_kernel<bf16x3>";
     mcpg::LaunchScope ls(h, name);
     switch (precision) {
       case MCPG_FP32:
-        hipLaunchKernelGGL(bspline_mi_kernel<kFP32>, dim3((unsigned)a.ntri), dim3(kThreads), mi_smem_bytes<kFP32>(), h->stream, a);
+        hipLaunchKernelGGL(bspline_mi_kernel<kFP32>, dim3(grid), dim3(kThreads), mi_smem_bytes<kFP32>(), h->stream, a);
         break;
       case MCPG_BF16:
-        hipLaunchKernelGGL(bspline_mi_kernel<kBF16>, dim3((unsigned)a.ntri), dim3(kThreads), mi_smem_bytes<kBF16>(), h->stream, a);
+        hipLaunchKernelGGL(bspline_mi_kernel<kBF16>, dim3(grid), dim3(kThreads), mi_smem_bytes<kBF16>(), h->stream, a);
         break;
       default:
-        hipLaunchKernelGGL(bspline_mi_kernel<kBF16X3>, dim3((unsigned)a.ntri), dim3(kThreads), mi_smem_bytes<kBF16X3>(), h->stream, a);
+        hipLaunchKernelGGL(bspline_mi_kernel<kBF16X3>, dim3(grid), dim3(kThreads), mi_smem_bytes<kBF16X3>(), h->stream,
+                           a);
         break;
     }
   }
@@ -819,6 +891,9 @@ int check_flags_nonfinite(mcpg_ctx* h, const char* what) {
 }
 
 bool valid_precision(int p) { return p == MCPG_FP32 || p == MCPG_BF16 || p == MCPG_BF16X3; }
+
+// gene groups per workgroup tile side of the MI kernel that runs at this precision
+int mi_groups_per_tile(int) { return kTileRows / kGroupRows; }
 
 }  // namespace
 
@@ -920,7 +995,7 @@ int mcpg_bspline_mi_weights_f32(mcpg_handle h, int ptrs, const float* W, int64_t
   if (ldw < bins * S + 1 || ldmi < G) return mcpg::set_error(h, MCPG_ERR_SHAPE, "weight rows must hold bins*S+1 values");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
-  const Layout L = make_layout(G, S, bins);
+  const Layout L = make_layout(G, S, bins, mi_groups_per_tile(precision));
   const int64_t wrow = (int64_t)bins * S + 1;
   mcpg::Sizer sz;
   if (ptrs == MCPG_HOST) { sz.add<float>(G * wrow); sz.add<float>(G * G); }
@@ -967,7 +1042,7 @@ int mcpg_bspline_mi_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int6
   if (ldx < S || ldmi < G) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
-  const Layout L = make_layout(G, S, bins);
+  const Layout L = make_layout(G, S, bins, mi_groups_per_tile(precision));
   const Spline sp = make_spline(bins, order);
   mcpg::Sizer sz;
   if (ptrs == MCPG_HOST) { sz.add<float>(G * S); sz.add<float>(G * G); }

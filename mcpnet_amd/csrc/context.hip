@@ -72,26 +72,26 @@ int finish(mcpg_ctx* h, const char* where) {
   return MCPG_OK;
 }
 
-const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int xcd_deal, int* status) {
+const uint32_t* tile_order(mcpg_ctx* h, int64_t ni, int64_t nj, int ratio, int gi, int gj, int xcd_deal,
+                           int* status) {
   *status = MCPG_OK;
-  if (nb > 65535) {
+  if (ni > 65535 || nj > 65535) {
     *status = set_error(h, MCPG_ERR_UNSUPPORTED, "tile grid too large for the packed tile order");
     return nullptr;
   }
-  // layout of the table in launch order: by default tiles are dealt so that each XCD (blocks go to
-  // the 8 XCDs round-robin by id) receives one contiguous run of the blocked order
+  gi = std::max(gi, 1);
+  gj = std::max(gj, 1);
+  // layout of the table in launch order: with xcd_deal, tiles are dealt so that each XCD (blocks
+  // go to the 8 XCDs round-robin by id) receives one contiguous run of the blocked order
   const int xcd = xcd_deal != 0;
-  const int key = group * 2 + xcd;
   for (const auto& o : h->orders)
-    if (o.nb == nb && o.group == key) return o.d;
+    if (o.ni == ni && o.nj == nj && o.ratio == ratio && o.gi == gi && o.gj == gj && o.xcd == xcd) return o.d;
   std::vector<uint32_t> blocked;
-  blocked.reserve((size_t)(nb * (nb + 1) / 2));
-  const int64_t nbb = (nb + group - 1) / group;
-  for (int64_t bi = 0; bi < nbb; ++bi)
-    for (int64_t bj = bi; bj < nbb; ++bj)
-      for (int64_t ti = bi * group; ti < std::min(nb, (bi + 1) * group); ++ti)
-        for (int64_t tj = bj * group; tj < std::min(nb, (bj + 1) * group); ++tj)
-          if (ti <= tj) blocked.push_back((uint32_t)((ti << 16) | tj));
+  for (int64_t bi = 0; bi < ni; bi += gi)
+    for (int64_t bj = 0; bj < nj; bj += gj)
+      for (int64_t ti = bi; ti < std::min(ni, bi + gi); ++ti)
+        for (int64_t tj = std::max(bj, ratio * ti); tj < std::min(nj, bj + gj); ++tj)
+          blocked.push_back((uint32_t)((ti << 16) | tj));
   std::vector<uint32_t> order(blocked.size());
   const int64_t total = (int64_t)blocked.size();
   for (int64_t b = 0; b < total; ++b) {
@@ -113,8 +113,14 @@ const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int xcd_deal, int*
   if (e != hipSuccess) { *status = hip_error(h, e, "hipMalloc(tile order)"); return nullptr; }
   e = hipMemcpy(d, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e != hipSuccess) { (void)hipFree(d); *status = hip_error(h, e, "hipMemcpy(tile order)"); return nullptr; }
-  h->orders.push_back(mcpg_ctx::TileOrder{d, nb, key});
+  mcpg_ctx::TileOrder o;
+  o.d = d; o.ni = ni; o.nj = nj; o.ratio = ratio; o.gi = gi; o.gj = gj; o.xcd = xcd; o.count = total;
+  h->orders.push_back(o);
   return d;
+}
+
+const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int xcd_deal, int* status) {
+  return tile_order(h, nb, nb, 1, group, group, xcd_deal, status);
 }
 
 int env_group(const char* name, int def) {

@@ -28,11 +28,11 @@ struct mcpg_ctx {
   std::vector<Timing> timings;
   std::vector<hipEvent_t> event_pool;
   size_t events_used = 0;
-  // cached device lists of upper-triangle tile coordinates (see mcpg::tri_order)
+  // cached device lists of upper-triangle tile coordinates (see mcpg::tile_order)
   struct TileOrder {
     uint32_t* d = nullptr;
-    int64_t nb = -1;
-    int group = -1;
+    int64_t ni = -1, nj = -1, count = 0;
+    int ratio = 0, gi = 0, gj = 0, xcd = 0;
   };
   std::vector<TileOrder> orders;
 };
@@ -78,13 +78,16 @@ int read_flags(mcpg_ctx* h, int* flags, int n);
 
 int finish(mcpg_ctx* h, const char* where);  // stream sync + error check
 
-// Device array of the (ti, tj), ti <= tj, tiles of an nb x nb grid, packed as (ti << 16) | tj and
-// indexed by block id.  Tiles are ordered in `group` x `group` blocks (blocks row-major over the
-// block triangle, tiles row-major inside a block), so that workgroups running together share
-// 2*group row panels through L2 instead of one row panel and as many column panels as there are
+// Device array of the tiles (ti, tj) of an ni x nj grid with tj >= ratio * ti, packed as
+// (ti << 16) | tj and indexed by block id.  Tiles are ordered in gi x gj blocks (blocks row-major,
+// tiles row-major inside a block), so that workgroups running together share gi row panels and
+// gj column panels through L2 instead of one row panel and as many column panels as there are
 // workgroups.  With xcd_deal set the order is further dealt so that each of the 8 XCDs
-// (workgroups go to XCDs round-robin by id) runs one contiguous stretch of it.  Both kernels
-// that use the table measured faster without the deal at 20000 genes.  Cached per handle.
+// (workgroups go to XCDs round-robin by id) runs one contiguous stretch of it.  Cached per handle.
+const uint32_t* tile_order(mcpg_ctx* h, int64_t ni, int64_t nj, int ratio, int gi, int gj, int xcd_deal, int* status);
+
+// The upper triangle (ti <= tj) of an nb x nb grid in group x group blocks.  Both kernels that use
+// it measured faster without the XCD deal at 20000 genes.
 const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int xcd_deal, int* status);
 
 // Tile-group size: the value of environment variable `name` if set (1..256), else `def`.
