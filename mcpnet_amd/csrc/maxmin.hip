@@ -8,9 +8,13 @@
 // tiling and reduction order.  FP64 uses v_min_f64 / v_max_f64.
 //
 // Tiling: a 256-thread workgroup owns a 128 x 128 output tile; each thread keeps an 8 x 8 block
-// of accumulators (rows ty+16m, cols tx+16n).  K advances in chunks of 128 bytes per row, staged
-// through LDS rows padded to 144 bytes so the 16-byte row reads of one wave hit distinct banks;
-// the next chunk is prefetched into registers while the current one is consumed.
+// of accumulators (rows ty+16m, cols tx+16n).
+// * FP32: the operands are converted once to padded int32 key matrices (key_prep_kernel; the K
+//   mask of the masked kernel is folded in there).  K advances in chunks of 16 keys per row that
+//   arrive in LDS by direct buffer loads, two chunks in flight, into XOR-swizzled rows whose
+//   operand reads are bank-conflict free.  An MCP pipeline keeps M1 in key form on the device.
+// * FP64: K advances in chunks of 128 bytes per row, staged through registers into LDS rows
+//   padded to 144 bytes; the next chunk is prefetched while the current one is consumed.
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -245,6 +249,212 @@ __global__ __launch_bounds__(kThreads) void maxmin_tile_kernel(MMArgs a) {
   }
 }
 
+// ---------------------------------------------------------------- FP32 key path
+// The FP32 products run on int32 order keys prepared once per operand: rows padded to a multiple
+// of the tile, row length ldk a multiple of the K chunk, padding INT_MIN (below every key, so it
+// never wins a max of mins).  The K mask of the masked kernel is folded in at preparation.
+__device__ __forceinline__ int f32_key(float x) {
+  const int b = __float_as_int(x);
+  return b ^ ((b >> 31) & 0x7fffffff);
+}
+__device__ __forceinline__ float key_f32(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
+
+__global__ void key_prep_kernel(const float* __restrict__ X, int64_t ldx, int64_t rows, int64_t K,
+                                const float* __restrict__ kmask, int* __restrict__ out, int64_t ldk, int64_t rows_pad) {
+  const int64_t total = rows_pad * ldk;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / ldk, k = t % ldk;
+    int v = INT_MIN;
+    if (r < rows && k < K) {
+      v = f32_key(X[r * ldx + k]);
+      if (kmask) v = min(v, f32_key(kmask[k]));
+    }
+    out[t] = v;
+  }
+}
+
+// INT_MIN into the padding of a key matrix whose [0, rows) x [0, cols) part a product writes
+__global__ void key_pad_kernel(int* __restrict__ out, int64_t ldk, int64_t rows, int64_t cols, int64_t rows_pad) {
+  const int64_t pad_cols = ldk - cols;
+  const int64_t total = rows * pad_cols + (rows_pad - rows) * ldk;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    if (t < rows * pad_cols) out[(t / pad_cols) * ldk + cols + t % pad_cols] = INT_MIN;
+    else out[rows * ldk + (t - rows * pad_cols)] = INT_MIN;
+  }
+}
+
+struct MKArgs {
+  const int* A; int64_t lda;   // keys, rows padded to kBM, lda multiple of the K chunk
+  const int* B; int64_t ldb;
+  uint32_t a_bytes, b_bytes;   // buffer ranges
+  int64_t M, N, Kp;            // Kp: padded K (multiple of the K chunk)
+  float* C; int64_t ldc;       // float output (nullable)
+  int* Ck; int64_t ldck;       // key output (nullable)
+  const float* R; int64_t ldr; // ratio numerator (nullable: no fused ratio)
+  float* O; int64_t ldo;       // ratio output
+  int tri, shared;
+  int64_t nbm, nbn, ntiles;
+  const uint32_t* order;       // tri mode: tile coordinates in launch order, (bm << 16) | bn
+};
+
+typedef __attribute__((address_space(3))) void* lds_ptr;
+constexpr int kRsrcWord3 = 0x00020000;   // raw (untyped, unswizzled) gfx9-family buffer access
+// LDS operand reads use a native vector type: with it the compiler leaves the ordering of the
+// direct loads and the reads to the explicit waits and barriers of the pipeline (a struct type
+// makes it wait for every pending load before each read, serialising the stages).
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int step4(int acc, const v4i& a, const v4i& b) {
+  const int m0 = min(a.x, b.x), m1 = min(a.y, b.y), m2 = min(a.z, b.z), m3 = min(a.w, b.w);
+  acc = TrF32::max3(acc, m0, m1);
+  return TrF32::max3(acc, m2, m3);
+}
+
+// K chunk of KC keys per stage.  LDS panel rows hold KC keys (KC / 4 chunks of 16 B) stored
+// XOR-swizzled, chunk' = chunk ^ swz(row), so that the 16 distinct rows of a B-operand read
+// (lane -> row lane & 15) and the two rows of an A-operand read fall in distinct bank quads of
+// every 16-lane ds_read_b128 pass (passes: lanes {0-3, 12-15, 20-23, 24-27} and translates).
+template <int KC>
+struct KeyChunk {
+  static constexpr int kRowBytes = KC * 4;
+  static constexpr int kChunks = KC / 4;
+  static constexpr int kRowsPerLoad = 1024 / kRowBytes;         // rows per 1 KB direct load
+  static constexpr int kLoadsPerPanel = kBM / kRowsPerLoad;
+  static constexpr int kPanelBytes = kBM * kRowBytes;
+  static constexpr int kStageBytes = 2 * kPanelBytes;
+  __device__ static __forceinline__ int swz(int row) {
+    return KC == 32 ? (row >> 1) & 7 : (row >> 2) & 3;
+  }
+};
+
+// this wave's direct loads of one K chunk: loads q0 .. q0 + nload - 1 of its panel
+template <int KC>
+__device__ __forceinline__ void key_dma_issue(__amdgpu_buffer_rsrc_t rsrc, const uint32_t (&lbyte)[2], uint32_t sbase,
+                                              uint32_t rbytes, int nload, int q0, char* dst) {
+#pragma unroll
+  for (int j = 0; j < KeyChunk<KC>::kLoadsPerPanel / 2; ++j) {
+    if (j < nload) {
+      const int q = q0 + j;
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr)(dst + q * 1024), 16, lbyte[q & 1], sbase + (uint32_t)q * rbytes,
+                                               0, 0);
+    }
+  }
+}
+
+template <int KC>
+__global__ __launch_bounds__(kThreads) void maxmin_key_kernel(MKArgs a) {
+  using KCh = KeyChunk<KC>;
+  extern __shared__ __align__(16) char smem[];
+
+  int64_t bm, bn;
+  if (a.tri) {
+    const uint32_t code = a.order[blockIdx.x];
+    bm = code >> 16;
+    bn = code & 0xffffu;
+  } else {
+    const int64_t L = blockIdx.x;
+    const int64_t group = 8;
+    const int64_t per_group = group * a.nbn;
+    const int64_t g = L / per_group;
+    const int64_t first = g * group;
+    const int64_t gsize = (a.nbm - first) < group ? (a.nbm - first) : group;
+    const int64_t in = L % per_group;
+    bm = first + in % gsize;
+    bn = in / gsize;
+  }
+  const int64_t row0 = bm * kBM, col0 = bn * kBN;
+  const bool same = a.shared && bm == bn;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tx = tid & 15, ty = tid >> 4;
+
+  int acc[8][8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+#pragma unroll
+    for (int n = 0; n < 8; ++n) acc[m][n] = f32_key(-INFINITY);   // (float)lowest<double>() == -inf
+
+  // Direct loads: waves 0-1 fill the A panel, waves 2-3 the B panel (on a diagonal tile of a
+  // shared operand all four fill A), each a contiguous run of 1 KB loads.  A load covers
+  // kRowsPerLoad rows; lane -> row lane / kChunks of the load, physical chunk lane % kChunks.
+  const int swave = __builtin_amdgcn_readfirstlane(wave);
+  const int panel = same ? 0 : (swave >> 1);
+  const int nload = same ? KCh::kLoadsPerPanel / 4 : KCh::kLoadsPerPanel / 2;
+  const int q0 = same ? swave * nload : (swave & 1) * nload;
+  const int64_t prow0 = panel == 0 ? row0 : col0;
+  const int64_t ld = panel == 0 ? a.lda : a.ldb;
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(panel == 0 ? a.A : a.B), (short)0, panel == 0 ? a.a_bytes : a.b_bytes, kRsrcWord3);
+  const int lrow = lane / KCh::kChunks, pc = lane % KCh::kChunks;
+  // the swizzle of a load row depends on the load index only through (q * kRowsPerLoad) bits
+  // above the swizzle's: KC = 16 never, KC = 32 through q's parity
+  uint32_t lbyte[2];
+#pragma unroll
+  for (int par = 0; par < 2; ++par) {
+    const int row = par * KCh::kRowsPerLoad + lrow;
+    lbyte[par] = (uint32_t)(lrow * ld * 4 + ((pc ^ KCh::swz(row)) << 4));
+  }
+  const uint32_t rbytes = (uint32_t)(KCh::kRowsPerLoad * ld * 4);
+  const uint32_t sbase = (uint32_t)(prow0 * ld * 4);
+  char* const dst0 = smem + panel * KCh::kPanelBytes;
+
+
+  const int sa = KCh::swz(ty), sb = KCh::swz(tx);
+  const int64_t nk = a.Kp / KC;
+  if (nk > 0) key_dma_issue<KC>(rsrc, lbyte, sbase, rbytes, nload, q0, dst0);
+  for (int64_t kc = 0; kc < nk; ++kc) {
+    // this wave's loads of chunk kc have landed; after the barrier, everyone's have, and every
+    // wave is done reading the other stage (chunk kc - 1), which the next loads overwrite
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (kc + 1 < nk)
+      key_dma_issue<KC>(rsrc, lbyte, sbase + (uint32_t)((kc + 1) * KCh::kRowBytes), rbytes, nload, q0,
+                        dst0 + ((kc + 1) & 1) * KCh::kStageBytes);
+    const char* As = smem + (kc & 1) * KCh::kStageBytes;
+    const char* Bs = same ? As : As + KCh::kPanelBytes;
+#pragma unroll
+    for (int c = 0; c < KCh::kChunks; ++c) {
+      v4i av[8], bv[8];
+      const int oa = ty * KCh::kRowBytes + ((c ^ sa) << 4);
+      const int ob = tx * KCh::kRowBytes + ((c ^ sb) << 4);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) av[m] = *reinterpret_cast<const v4i*>(As + oa + 16 * m * KCh::kRowBytes);
+#pragma unroll
+      for (int n = 0; n < 8; ++n) bv[n] = *reinterpret_cast<const v4i*>(Bs + ob + 16 * n * KCh::kRowBytes);
+#pragma unroll
+      for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int n = 0; n < 8; ++n) acc[m][n] = step4(acc[m][n], av[m], bv[n]);
+    }
+  }
+
+  // epilogue: float and/or key outputs, optional fused ratio, mirror for symmetric off-diagonal tiles
+  const bool mirror = a.tri && bm != bn;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int64_t i = row0 + ty + 16 * m;
+    if (i >= a.M) continue;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      const int64_t j = col0 + tx + 16 * n;
+      if (j >= a.N) continue;
+      const float v = key_f32(acc[m][n]);
+      if (a.C) {
+        a.C[i * a.ldc + j] = v;
+        if (mirror) a.C[j * a.ldc + i] = v;
+      }
+      if (a.Ck) {
+        a.Ck[i * a.ldck + j] = acc[m][n];
+        if (mirror) a.Ck[j * a.ldck + i] = acc[m][n];
+      }
+      if (a.O) {
+        a.O[i * a.ldo + j] = ratio_of<float>(a.R[i * a.ldr + j], v);
+        if (mirror) a.O[j * a.ldo + i] = ratio_of<float>(a.R[j * a.ldr + i], v);
+      }
+    }
+  }
+}
+
 template <typename T>
 __global__ void ratio_kernel(const T* __restrict__ A, int64_t lda, const T* __restrict__ Mm, int64_t ldm,
                              int64_t M, int64_t N, T* __restrict__ O, int64_t ldo) {
@@ -315,6 +525,125 @@ int launch_maxmin(mcpg_ctx* h, const typename Tr::T* A, int64_t lda, const typen
   return MCPG_OK;
 }
 
+// ---------------------------------------------------------------- FP32 key path, host side
+constexpr int kKeyAlign = 32;   // row length of key matrices: a multiple of the largest K chunk
+
+struct KeyMat {
+  int* d = nullptr;
+  int64_t rows = 0, rows_pad = 0, ldk = 0;
+  uint32_t bytes() const { return (uint32_t)(rows_pad * ldk * 4); }
+};
+
+KeyMat key_geometry(int64_t rows, int64_t K) {
+  KeyMat km;
+  km.rows = rows;
+  km.rows_pad = ceil_div(rows, kBM) * kBM;
+  km.ldk = std::max<int64_t>(ceil_div(K, kKeyAlign) * kKeyAlign, kKeyAlign);
+  return km;
+}
+
+int key_check(mcpg_ctx* h, const KeyMat& km) {
+  if ((uint64_t)km.rows_pad * (uint64_t)km.ldk * 4 >= (1ull << 32))
+    return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min operand exceeds 4 GiB of keys");
+  return MCPG_OK;
+}
+
+int grid_of(int64_t total) { return (int)std::min<int64_t>(std::max<int64_t>(ceil_div(total, 256), 1), 65536); }
+
+int prep_keys(mcpg_ctx* h, const float* X, int64_t ldx, int64_t K, const float* kmask, const KeyMat& km) {
+  mcpg::LaunchScope ls(h, "key_prep_kernel");
+  hipLaunchKernelGGL(key_prep_kernel, dim3(grid_of(km.rows_pad * km.ldk)), dim3(256), 0, h->stream, X, ldx, km.rows, K,
+                     kmask, km.d, km.ldk, km.rows_pad);
+  return MCPG_OK;
+}
+
+// padding of a key matrix whose [0, rows) x [0, cols) part a product will write
+int pad_keys(mcpg_ctx* h, const KeyMat& km, int64_t cols) {
+  const int64_t total = km.rows * (km.ldk - cols) + (km.rows_pad - km.rows) * km.ldk;
+  if (total == 0) return MCPG_OK;
+  mcpg::LaunchScope ls(h, "key_pad_kernel");
+  hipLaunchKernelGGL(key_pad_kernel, dim3(grid_of(total)), dim3(256), 0, h->stream, km.d, km.ldk, km.rows, cols,
+                     km.rows_pad);
+  return MCPG_OK;
+}
+
+// C / Ck / O over [0, M) x [0, N) of A (x) B, with K = A.ldk (== B.ldk)
+int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M, int64_t N, int tri, float* C,
+                       int64_t ldc, int* Ck, int64_t ldck, const float* R, int64_t ldr, float* O, int64_t ldo) {
+  if (M <= 0 || N <= 0) return MCPG_OK;
+  MKArgs a;
+  a.A = A.d; a.lda = A.ldk; a.B = B.d; a.ldb = B.ldk; a.a_bytes = A.bytes(); a.b_bytes = B.bytes();
+  a.M = M; a.N = N; a.Kp = A.ldk;
+  a.C = C; a.ldc = ldc; a.Ck = Ck; a.ldck = ldck; a.R = R; a.ldr = ldr; a.O = O; a.ldo = ldo;
+  a.tri = tri;
+  a.shared = A.d == B.d ? 1 : 0;
+  a.nbm = ceil_div(M, kBM);
+  a.nbn = ceil_div(N, kBN);
+  a.ntiles = tri ? a.nbm * (a.nbm + 1) / 2 : a.nbm * a.nbn;
+  if (a.ntiles > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min grid too large");
+  a.order = nullptr;
+  if (tri) {
+    int st;
+    a.order = mcpg::tri_order(h, a.nbm, mcpg::env_group("MCPG_MM_GROUP", 8), mcpg::env_group("MCPG_MM_XCD", 0), &st);
+    if (!a.order) return st;
+  }
+  const int kc = mcpg::env_group("MCPG_MM_KC", 16) == 32 ? 32 : 16;
+  static bool attr_set = false;
+  if (!attr_set) {
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&maxmin_key_kernel<16>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 2 * KeyChunk<16>::kStageBytes));
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&maxmin_key_kernel<32>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 2 * KeyChunk<32>::kStageBytes));
+    attr_set = true;
+  }
+  {
+    mcpg::LaunchScope ls(h, tri ? "maxmin_tile_kernel<f32,tri>" : "maxmin_tile_kernel<f32>");
+    if (kc == 16)
+      hipLaunchKernelGGL(maxmin_key_kernel<16>, dim3((unsigned)a.ntiles), dim3(kThreads), 2 * KeyChunk<16>::kStageBytes,
+                         h->stream, a);
+    else
+      hipLaunchKernelGGL(maxmin_key_kernel<32>, dim3((unsigned)a.ntiles), dim3(kThreads), 2 * KeyChunk<32>::kStageBytes,
+                         h->stream, a);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mcpg::hip_error(h, e, "maxmin_key_kernel launch");
+  return MCPG_OK;
+}
+
+// mcpg_maxmin_f32: keys of A (mask folded in) and of B (unless shared), then one product.  The key
+// buffers are sized and carved by the caller together with its other workspace.
+struct KeyPlan {
+  KeyMat ka, kb;
+  bool shareB = false;
+};
+
+// same_operand: A and B are the same matrix; a K mask applies to A only, so B then needs its own keys
+KeyPlan plan_keys(bool same_operand, bool masked, int64_t M, int64_t N, int64_t K) {
+  KeyPlan p;
+  p.ka = key_geometry(M, K);
+  p.shareB = same_operand && !masked;
+  p.kb = p.shareB ? p.ka : key_geometry(N, K);
+  return p;
+}
+
+void size_keys(mcpg::Sizer& sz, const KeyPlan& p) {
+  sz.add<int>(p.ka.rows_pad * p.ka.ldk);
+  if (!p.shareB) sz.add<int>(p.kb.rows_pad * p.kb.ldk);
+}
+
+void take_keys(mcpg::Carver& cv, KeyPlan& p) {
+  p.ka.d = cv.take<int>(p.ka.rows_pad * p.ka.ldk);
+  p.kb.d = p.shareB ? p.ka.d : cv.take<int>(p.kb.rows_pad * p.kb.ldk);
+}
+
+int run_keys(mcpg_ctx* h, const KeyPlan& p, const float* A, int64_t lda, const float* B, int64_t ldb, const float* kmask,
+             int64_t K, int symmetric, float* C, int64_t ldc) {
+  int st;
+  if ((st = prep_keys(h, A, lda, K, kmask, p.ka))) return st;
+  if (!p.shareB && (st = prep_keys(h, B, ldb, K, nullptr, p.kb))) return st;
+  return launch_maxmin_keys(h, p.ka, p.kb, p.ka.rows, p.kb.rows, symmetric, C, ldc, nullptr, 0, nullptr, 0, nullptr, 0);
+}
+
 template <typename T>
 struct TraitsOf;
 template <>
@@ -336,31 +665,50 @@ int maxmin_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* B, int
     return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "symmetric max-min needs A == B and M == N");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
-  if (ptrs == MCPG_DEVICE) {
-    int st = launch_maxmin<Tr>(h, A, lda, B, ldb, kmask, M, N, K, symmetric, C, ldc, nullptr, 0, nullptr, 0);
+  constexpr bool f32 = std::is_same<T, float>::value;
+  const bool host = ptrs == MCPG_HOST;
+  const bool shareB = (A == B && M == N && lda == ldb);
+  // FP32 runs on order keys (sized here with the host staging); FP64 reads the values directly
+  KeyPlan kp;
+  if constexpr (f32) {
+    kp = plan_keys(shareB, kmask != nullptr, M, N, K);
+    int st;
+    if ((st = key_check(h, kp.ka)) || (st = key_check(h, kp.kb))) return st;
+  }
+  mcpg::Sizer sz;
+  if (host) {
+    sz.add<T>(M * K);
+    if (!shareB) sz.add<T>(N * K);
+    if (kmask) sz.add<T>(K);
+    sz.add<T>(M * N);
+  }
+  if constexpr (f32) size_keys(sz, kp);
+  int st;
+  void* base = sz.off ? mcpg::workspace(h, sz.off, &st) : h->ws;
+  if (sz.off && !base) return st;
+  mcpg::Carver cv(base);
+  if (!host) {
+    if constexpr (f32) {
+      take_keys(cv, kp);
+      st = run_keys(h, kp, A, lda, B, ldb, kmask, K, symmetric, C, ldc);
+    } else {
+      st = launch_maxmin<Tr>(h, A, lda, B, ldb, kmask, M, N, K, symmetric, C, ldc, nullptr, 0, nullptr, 0);
+    }
     if (st) return st;
     return mcpg::finish(h, "mcpg_maxmin");
   }
-  const bool shareB = (A == B && M == N && lda == ldb);
-  mcpg::Sizer sz;
-  sz.add<T>(M * K);
-  if (!shareB) sz.add<T>(N * K);
-  if (kmask) sz.add<T>(K);
-  sz.add<T>(M * N);
-  int st;
-  void* base = mcpg::workspace(h, sz.off, &st);
-  if (!base) return st;
-  mcpg::Carver cv(base);
   T* dA = cv.take<T>(M * K);
   T* dB = shareB ? dA : cv.take<T>(N * K);
   T* dm = kmask ? cv.take<T>(K) : nullptr;
   T* dC = cv.take<T>(M * N);
+  if constexpr (f32) take_keys(cv, kp);
   if (K > 0) {
     if ((st = mcpg::copy_2d(h, dA, K * sizeof(T), A, lda * sizeof(T), K * sizeof(T), M, hipMemcpyHostToDevice))) return st;
     if (!shareB && (st = mcpg::copy_2d(h, dB, K * sizeof(T), B, ldb * sizeof(T), K * sizeof(T), N, hipMemcpyHostToDevice))) return st;
     if (kmask) MCPG_HIP(h, hipMemcpyAsync(dm, kmask, K * sizeof(T), hipMemcpyHostToDevice, h->stream));
   }
-  st = launch_maxmin<Tr>(h, dA, K, dB, K, dm, M, N, K, symmetric, dC, N, nullptr, 0, nullptr, 0);
+  if constexpr (f32) st = run_keys(h, kp, dA, K, dB, K, dm, K, symmetric, dC, N);
+  else st = launch_maxmin<Tr>(h, dA, K, dB, K, dm, M, N, K, symmetric, dC, N, nullptr, 0, nullptr, 0);
   if (st) return st;
   if ((st = mcpg::copy_2d(h, C, ldc * sizeof(T), dC, N * sizeof(T), N * sizeof(T), M, hipMemcpyDeviceToHost))) return st;
   return mcpg::finish(h, "mcpg_maxmin");
@@ -408,6 +756,65 @@ int ratio_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* Mm, int
   return mcpg::finish(h, "mcpg_ratio");
 }
 
+// MCP2/3/4 on FP32 order keys: keys of A once; M1 = A (x) A stays in key form on the device; the
+// last product writes the max-min matrix only when asked for and fuses the ratio.
+int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N, int k, float* mm_out, int64_t ldm,
+                 float* out, int64_t ldo) {
+  const size_t NN = (size_t)N * N;
+  KeyMat ka = key_geometry(N, N), k1 = key_geometry(N, N);
+  int st;
+  if ((st = key_check(h, ka))) return st;
+  mcpg::Sizer sz;
+  if (host) { sz.add<float>(NN); sz.add<float>(NN); if (mm_out) sz.add<float>(NN); }
+  sz.add<int>(ka.rows_pad * ka.ldk);
+  if (k >= 3) sz.add<int>(k1.rows_pad * k1.ldk);
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const float* dA = A;
+  int64_t la = lda;
+  float* dO = out;
+  int64_t lo = ldo;
+  float* dM = mm_out;
+  int64_t lm = ldm;
+  if (host) {
+    float* a = cv.take<float>(NN);
+    if ((st = mcpg::copy_2d(h, a, N * 4, A, lda * 4, N * 4, N, hipMemcpyHostToDevice))) return st;
+    dA = a; la = N;
+    dO = cv.take<float>(NN); lo = N;
+    if (mm_out) { dM = cv.take<float>(NN); lm = N; }
+  }
+  ka.d = cv.take<int>(ka.rows_pad * ka.ldk);
+  if ((st = prep_keys(h, dA, la, N, nullptr, ka))) return st;
+  if (k == 2) {
+    if ((st = launch_maxmin_keys(h, ka, ka, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo))) return st;
+  } else {
+    k1.d = cv.take<int>(k1.rows_pad * k1.ldk);
+    if ((st = pad_keys(h, k1, N))) return st;
+    // the first product A (x) A is symmetric by construction
+    if ((st = launch_maxmin_keys(h, ka, ka, N, N, 1, nullptr, 0, k1.d, k1.ldk, nullptr, 0, nullptr, 0))) return st;
+    if (k == 3) {
+      // A (x) M1 is exactly symmetric when A is (max/min never round); check A bitwise.
+      if ((st = mcpg::reset_flags(h))) return st;
+      {
+        mcpg::LaunchScope ls(h, "symmetry_check_kernel");
+        hipLaunchKernelGGL(symmetry_check_kernel<float>, dim3(1024), dim3(256), 0, h->stream, dA, la, N, h->d_flags);
+      }
+      int flags[1];
+      if ((st = mcpg::read_flags(h, flags, 1))) return st;
+      const int sym = flags[0] == 0;
+      if ((st = launch_maxmin_keys(h, ka, k1, N, N, sym, dM, lm, nullptr, 0, dA, la, dO, lo))) return st;
+    } else {
+      if ((st = launch_maxmin_keys(h, k1, k1, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo))) return st;
+    }
+  }
+  if (host) {
+    if ((st = mcpg::copy_2d(h, out, ldo * 4, dO, N * 4, N * 4, N, hipMemcpyDeviceToHost))) return st;
+    if (mm_out && (st = mcpg::copy_2d(h, mm_out, ldm * 4, dM, N * 4, N * 4, N, hipMemcpyDeviceToHost))) return st;
+  }
+  return mcpg::finish(h, "mcpg_mcp");
+}
+
 template <typename T>
 int mcp_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, int k, T* mm_out, int64_t ldm, T* out,
               int64_t ldo) {
@@ -421,6 +828,7 @@ int mcp_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, int k, 
   if (lda < N || ldo < N || (mm_out && ldm < N)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
+  if constexpr (std::is_same<T, float>::value) return mcp_f32_keys(h, ptrs == MCPG_HOST, A, lda, N, k, mm_out, ldm, out, ldo);
   const size_t NN = (size_t)N * N;
   const bool host = ptrs == MCPG_HOST;
   mcpg::Sizer sz;
