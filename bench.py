@@ -349,7 +349,7 @@ def main():
         roof[mi_name] = {"bound": "mfma", "achieved": ach, "peak": peak_mi, "unit": "TFLOP/s", "frac": ach / peak_mi,
                          "traffic": None, "mean_ms": mi_ms, "work_per_launch": mi_flops,
                          "work_unit": "algorithmic flops 2*B^2*S per unordered gene pair (split products not credited)"}
-    for name, tri in (("maxmin_tile_kernel<f32,tri>", True), ("maxmin_tile_kernel<f32>", False)):
+    for name, tri in (("maxmin_key_kernel<f32,tri>", True), ("maxmin_key_kernel<f32>", False)):
         ms = mean_ms(name)
         if not ms:
             continue
@@ -366,9 +366,10 @@ def main():
     if traffic_file.exists():
         try:
             tr = json.loads(traffic_file.read_text()).get(f"config{cid}", {})
-            for n in roof:
-                if n in tr:
-                    roof[n]["traffic"] = tr[n]
+            for n in roof:   # the rocprof names carry no ",tri" launch mode
+                t = tr.get(n, tr.get(n.replace(",tri", "")))
+                if t is not None:
+                    roof[n]["traffic"] = t
         except (OSError, ValueError):
             pass
 

@@ -516,8 +516,7 @@ int launch_maxmin(mcpg_ctx* h, const typename Tr::T* A, int64_t lda, const typen
     if (!a.order) return st;
   }
   {
-    mcpg::LaunchScope ls(h, sizeof(T) == 4 ? (tri ? "maxmin_tile_kernel<f32,tri>" : "maxmin_tile_kernel<f32>")
-                                           : (tri ? "maxmin_tile_kernel<f64,tri>" : "maxmin_tile_kernel<f64>"));
+    mcpg::LaunchScope ls(h, tri ? "maxmin_tile_kernel<f64,tri>" : "maxmin_tile_kernel<f64>");
     hipLaunchKernelGGL(maxmin_tile_kernel<Tr>, dim3((unsigned)a.ntiles), dim3(kThreads), 0, h->stream, a);
   }
   hipError_t e = hipGetLastError();
@@ -597,7 +596,7 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
     attr_set = true;
   }
   {
-    mcpg::LaunchScope ls(h, tri ? "maxmin_tile_kernel<f32,tri>" : "maxmin_tile_kernel<f32>");
+    mcpg::LaunchScope ls(h, tri ? "maxmin_key_kernel<f32,tri>" : "maxmin_key_kernel<f32>");
     if (kc == 16)
       hipLaunchKernelGGL(maxmin_key_kernel<16>, dim3((unsigned)a.ntiles), dim3(kThreads), 2 * KeyChunk<16>::kStageBytes,
                          h->stream, a);

@@ -38,14 +38,17 @@ def _db(d: str):
 
 
 def _short(name: str) -> str:
-    for key in ("bspline_mi_kernel", "maxmin_tile_kernel", "bspline_prep_kernel", "ap_mi_kernel", "ratio_kernel",
-                "symmetry_check_kernel", "rank_kernel", "cumhist_kernel", "noise_kernel", "weights_import_kernel",
-                "bspline_pair_adjoint_kernel", "bspline_gene_adjoint_kernel", "emax_kernel", "copyBuffer",
-                "fillBuffer"):
+    for key in ("bspline_mi_kernel", "maxmin_key_kernel", "maxmin_tile_kernel", "key_prep_kernel", "key_pad_kernel",
+                "bspline_prep_kernel", "ap_mi_kernel", "ratio_kernel", "symmetry_check_kernel", "rank_kernel",
+                "cumhist_kernel", "noise_kernel", "weights_import_kernel", "bspline_pair_adjoint_kernel",
+                "bspline_gene_adjoint_kernel", "emax_kernel", "combo_kernel", "stouffer_kernel", "row_stats_kernel",
+                "dpi_kernel", "copyBuffer", "fillBuffer"):
         if key in name:
             tmpl = name[name.find(key) + len(key):]
             if key == "maxmin_tile_kernel":
                 return key + ("<f64>" if "TrF64" in tmpl else "<f32>")
+            if key == "maxmin_key_kernel":
+                return key + "<f32>"
             if key == "bspline_mi_kernel":
                 return key + {"<0>": "<fp32>", "<1>": "<bf16>", "<2>": "<bf16x3>"}.get(tmpl[:3], "")
             return key
