@@ -183,6 +183,51 @@ int mcpg_mcp_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N
 int mcpg_mcp_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, int k,
                  double* maxmin_out, int64_t ldm, double* mcp_out, int64_t ldo);
 
+/* compute_maxmins (apps/pipeline/mcpnet_helper.hpp:436-460), the three products the ensemble
+ * MCP combines (apps/pipeline/mcpnet.cpp:546-553): M1 = A(x)A, M2 = A(x)M1, M3 = M1(x)M1 of the
+ * N x N matrix A.  Each output is written only when its pointer is non-null; M1 stays on the
+ * device between the products.  Exact, like mcpg_maxmin. */
+int mcpg_maxmins_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N, float* M1, int64_t ld1,
+                     float* M2, int64_t ld2, float* M3, int64_t ld3);
+int mcpg_maxmins_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, double* M1, int64_t ld1,
+                     double* M2, int64_t ld2, double* M3, int64_t ld3);
+
+/* ------------------------------------------------------------------ ensemble and filters */
+
+/* compute_combo (apps/pipeline/mcpnet_helper.hpp:509-571; called per coefficient row at
+ * apps/pipeline/mcpnet.cpp:589-592): with (a, b, c, d) = coeffs and s = a*MI + b*M1 + c*M2 + d*M3
+ * accumulated in double (a coefficient of 1 adds the value unscaled, 0 skips the term and its
+ * matrix, which may then be NULL), out = |MI| < eps ? 0 : |s| < eps ? T_MAX : (T)(MI / s).
+ * Bit-identical to an IEEE evaluation of that expression (no contraction). */
+int mcpg_combo_f32(mcpg_handle h, int ptrs, const float* MI, int64_t ldmi, const float* M1, int64_t ld1,
+                   const float* M2, int64_t ld2, const float* M3, int64_t ld3, int64_t M, int64_t N,
+                   const float* coeffs, float* out, int64_t ldo);
+int mcpg_combo_f64(mcpg_handle h, int ptrs, const double* MI, int64_t ldmi, const double* M1, int64_t ld1,
+                   const double* M2, int64_t ld2, const double* M3, int64_t ld3, int64_t M, int64_t N,
+                   const double* coeffs, double* out, int64_t ldo);
+
+/* compute_stouffer (apps/pipeline/mcpnet_helper.hpp:574-584): per-row mean and unbiased standard
+ * deviation (splash::kernel::GaussianParamsExclude1<T,T,true>; exclude_ones != 0 leaves entries
+ * equal to 1 out of both moments), then stouffer_vector_kernel (include/mcp/transform/stouffer.hpp:56-80):
+ * out[i][j] = (z_i + z_j) / sqrt(2), z_r = (X[i][j] - mean_r) / sd_r (0 when sd_r < eps).
+ * The moments are FP64; the splash reduction they replace is not in the reference tree. */
+int mcpg_stouffer_f32(mcpg_handle h, int ptrs, const float* X, int64_t ldx, int64_t N, int exclude_ones,
+                      float* out, int64_t ldo);
+int mcpg_stouffer_f64(mcpg_handle h, int ptrs, const double* X, int64_t ldx, int64_t N, int exclude_ones,
+                      double* out, int64_t ldo);
+
+/* DPI filter (apps/filter/dpi.cpp:236-300): the mask of InnerProduct<..., dpi_kernel<T>> or, with
+ * a TF list, dpi_tf_kernel<T> (include/mcp/filter/dpi.hpp:42-110, EQUALITY_EXCLUSION), applied with
+ * mask<T, true> (include/mcp/filter/threshold.hpp:34-54): out[x][z] = 0 when x == z or some gene
+ * y != x, z (a TF when exactly one of x, z is a TF) has A[x][y] >= i_xz and A[z][y] >= i_xz,
+ * i_xz = A[x][z] / (1 - tolerance); else A[x][z].  tfs (nullable, length N): > 0 marks a TF.
+ * removed (nullable) receives the number of entries that changed.  The max over y is a max-min
+ * product with the excluded entries at -inf, so the decision is exact. */
+int mcpg_dpi_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N, float tolerance, const float* tfs,
+                 float* out, int64_t ldo, int64_t* removed);
+int mcpg_dpi_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, double tolerance,
+                 const double* tfs, double* out, int64_t ldo, int64_t* removed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -63,6 +63,14 @@ _SIGS = {
     "mcpg_elementwise_max_f64": [_P, c_int, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_mcp_f32": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64, _P, c_int64],
     "mcpg_mcp_f64": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64, _P, c_int64],
+    "mcpg_maxmins_f32": [_P, c_int, _P, c_int64, c_int64, _P, c_int64, _P, c_int64, _P, c_int64],
+    "mcpg_maxmins_f64": [_P, c_int, _P, c_int64, c_int64, _P, c_int64, _P, c_int64, _P, c_int64],
+    "mcpg_combo_f32": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, _P, c_int64],
+    "mcpg_combo_f64": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, _P, c_int64],
+    "mcpg_stouffer_f32": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64],
+    "mcpg_stouffer_f64": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64],
+    "mcpg_dpi_f32": [_P, c_int, _P, c_int64, c_int64, c_float, _P, _P, c_int64, POINTER(c_int64)],
+    "mcpg_dpi_f64": [_P, c_int, _P, c_int64, c_int64, c_double, _P, _P, c_int64, POINTER(c_int64)],
 }
 EXPORTS = ["mcpg_status_string", "mcpg_last_error", *_SIGS.keys()]
 
@@ -409,3 +417,81 @@ class Handle:
         self._check(fn(self.h, a.kind, a.ptr, a.ld, N, int(k), mm.ptr if mm else None, mm.ld if mm else 0, o.ptr, o.ld),
                     "mcpg_mcp")
         return out, maxmin_out
+
+    # ------------------------------------------------------------------ ensemble and filters
+    def maxmins(self, A, want=(True, True, True)):
+        """compute_maxmins: (M1, M2, M3) = (A(x)A, A(x)M1, M1(x)M1); entries not wanted are None."""
+        dt = np.float64 if str(getattr(A, "dtype", "")).endswith("float64") else np.float32
+        a = _Mat(A, dt, "A")
+        N = a.shape[0]
+        if a.shape[1] != N:
+            raise ValueError("maxmins needs a square matrix")
+        outs = [_empty_like_kind(a.kind, (N, N), dt, A) if w else None for w in want]
+        ms = [_Mat(o, dt, "out") if o is not None else None for o in outs]
+        args = []
+        for m in ms:
+            args += [m.ptr if m else None, m.ld if m else 0]
+        fn = self.lib.mcpg_maxmins_f64 if dt == np.float64 else self.lib.mcpg_maxmins_f32
+        self._pre(a.kind)
+        self._check(fn(self.h, a.kind, a.ptr, a.ld, N, *args), "mcpg_maxmins")
+        return tuple(outs)
+
+    def combo(self, MI, M1, M2, M3, coeffs, out=None):
+        """compute_combo: MI / (a MI + b M1 + c M2 + d M3) with the reference's guards."""
+        dt = np.float64 if str(getattr(MI, "dtype", "")).endswith("float64") else np.float32
+        mi = _Mat(MI, dt, "MI")
+        M, N = mi.shape
+        cf = np.ascontiguousarray(coeffs, dtype=dt)
+        if cf.shape != (4,):
+            raise ValueError("coeffs must hold 4 values")
+        mats = [None if (x is None or cf[q + 1] == 0) else _Mat(x, dt, f"M{q + 1}") for q, x in enumerate((M1, M2, M3))]
+        for m in mats:
+            if m is not None and m.shape != (M, N):
+                raise ValueError("operands must have the same shape")
+        kind = _same_kind(mi, *mats)
+        out = _empty_like_kind(kind, (M, N), dt, MI) if out is None else out
+        o = _Mat(out, dt, "out")
+        args = []
+        for m in mats:
+            args += [m.ptr if m else None, m.ld if m else 0]
+        fn = self.lib.mcpg_combo_f64 if dt == np.float64 else self.lib.mcpg_combo_f32
+        self._pre(kind)
+        self._check(fn(self.h, kind, mi.ptr, mi.ld, *args, M, N, cf.ctypes.data, o.ptr, o.ld), "mcpg_combo")
+        return out
+
+    def stouffer(self, X, exclude_ones=True, out=None):
+        """compute_stouffer on a square score matrix."""
+        dt = np.float64 if str(getattr(X, "dtype", "")).endswith("float64") else np.float32
+        x = _Mat(X, dt, "X")
+        N = x.shape[0]
+        if x.shape[1] != N:
+            raise ValueError("Stouffer needs a square matrix")
+        out = _empty_like_kind(x.kind, (N, N), dt, X) if out is None else out
+        o = _Mat(out, dt, "out")
+        fn = self.lib.mcpg_stouffer_f64 if dt == np.float64 else self.lib.mcpg_stouffer_f32
+        self._pre(x.kind)
+        self._check(fn(self.h, x.kind, x.ptr, x.ld, N, int(exclude_ones), o.ptr, o.ld), "mcpg_stouffer")
+        return out
+
+    def dpi(self, A, tolerance=0.1, tfs=None, out=None):
+        """DPI-filtered copy of the square MI matrix A; returns (out, removed count)."""
+        dt = np.float64 if str(getattr(A, "dtype", "")).endswith("float64") else np.float32
+        a = _Mat(A, dt, "A")
+        N = a.shape[0]
+        if a.shape[1] != N:
+            raise ValueError("DPI needs a square matrix")
+        tf = None if tfs is None else _Mat(tfs, dt, "tfs")
+        if tf is not None and tf.shape != (N,):
+            raise ValueError("tfs must have one entry per gene")
+        kind = _same_kind(a, tf)
+        out = _empty_like_kind(kind, (N, N), dt, A) if out is None else out
+        o = _Mat(out, dt, "out")
+        removed = c_int64(0)
+        if dt == np.float64:
+            fn, tol = self.lib.mcpg_dpi_f64, c_double(tolerance)
+        else:
+            fn, tol = self.lib.mcpg_dpi_f32, c_float(tolerance)
+        self._pre(kind)
+        self._check(fn(self.h, kind, a.ptr, a.ld, N, tol, tf.ptr if tf else None, o.ptr, o.ld, byref(removed)),
+                    "mcpg_dpi")
+        return out, removed.value

@@ -18,7 +18,9 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <limits>
 #include <type_traits>
+#include <vector>
 
 #include "internal.h"
 
@@ -283,6 +285,20 @@ __global__ void key_pad_kernel(int* __restrict__ out, int64_t ldk, int64_t rows,
   }
 }
 
+// DPI operand keys (include/mcp/filter/dpi.hpp:56-66): the diagonal is excluded (a triangle
+// x-y-z needs y != x, z), and with a TF list the columns of non-TF genes too (for the TF
+// variant's pairs with exactly one TF endpoint, dpi.hpp:90-101).  Excluded entries are INT_MIN.
+__global__ void dpi_key_kernel(const float* __restrict__ A, int64_t lda, int64_t N, const float* __restrict__ tfs,
+                               int* __restrict__ out, int64_t ldk, int64_t rows_pad) {
+  const int64_t total = rows_pad * ldk;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / ldk, k = t % ldk;
+    int v = INT_MIN;
+    if (r < N && k < N && r != k && !(tfs && !(tfs[k] > 0.0f))) v = f32_key(A[r * lda + k]);
+    out[t] = v;
+  }
+}
+
 struct MKArgs {
   const int* A; int64_t lda;   // keys, rows padded to kBM, lda multiple of the K chunk
   const int* B; int64_t ldb;
@@ -490,6 +506,52 @@ __global__ void symmetry_check_kernel(const T* __restrict__ A, int64_t lda, int6
   if (bad) atomicOr(flags, 1);
 }
 
+// DPI decision and mask application (include/mcp/filter/dpi.hpp:56-66, :86-105;
+// include/mcp/filter/threshold.hpp:34-54 with negate = true; apps/filter/dpi.cpp:236-285).  An edge
+// x-z is removed when x == z or some allowed third gene y has min(A[x][y], A[z][y]) >= i_xz
+// (EQUALITY_EXCLUSION), i.e. when W[x][z] = max over allowed y of that minimum reaches i_xz.
+// i_xz = A[x][z] * tol_multiplier: a double product stored in T for dpi_kernel, a T product for
+// dpi_tf_kernel (whose multiplier is stored in T).  With a TF list, pairs with exactly one TF
+// endpoint use WT (y restricted to TFs).  Removed edges become 0; `removed` counts the entries
+// that changed.
+template <typename T>
+__global__ void dpi_apply_kernel(const T* __restrict__ A, int64_t lda, int64_t N, const T* __restrict__ W, int64_t ldw,
+                                 const T* __restrict__ WT, int64_t ldwt, const T* __restrict__ tfs, double tm_d, T tm_t,
+                                 T* __restrict__ O, int64_t ldo, unsigned long long* removed) {
+  const int64_t total = N * N;
+  unsigned long long cnt = 0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t x = t / N, z = t % N;
+    const T a = A[x * lda + z];
+    bool del = true;
+    if (x != z) {
+      if (tfs) {
+        const T ixz = a * tm_t;
+        const bool tf_xz = (tfs[x] > (T)0) != (tfs[z] > (T)0);
+        del = (tf_xz ? WT[x * ldwt + z] : W[x * ldw + z]) >= ixz;
+      } else {
+        const T ixz = (T)((double)a * tm_d);
+        del = W[x * ldw + z] >= ixz;
+      }
+    }
+    const T o = del ? (T)0 : a;
+    O[x * ldo + z] = o;
+    cnt += (o != a) ? 1 : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(removed, cnt);
+}
+
+// copy of A with the diagonal set to `diag` (FP64 DPI operand: y = x and y = z drop out of the max)
+template <typename T>
+__global__ void diag_copy_kernel(const T* __restrict__ A, int64_t lda, int64_t N, T diag, T* __restrict__ O) {
+  const int64_t total = N * N;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / N, j = t % N;
+    O[t] = i == j ? diag : A[i * lda + j];
+  }
+}
+
 template <class Tr>
 int launch_maxmin(mcpg_ctx* h, const typename Tr::T* A, int64_t lda, const typename Tr::T* B, int64_t ldb,
                   const typename Tr::T* kmask, int64_t M, int64_t N, int64_t K, int tri,
@@ -641,6 +703,229 @@ int run_keys(mcpg_ctx* h, const KeyPlan& p, const float* A, int64_t lda, const f
   if ((st = prep_keys(h, A, lda, K, kmask, p.ka))) return st;
   if (!p.shareB && (st = prep_keys(h, B, ldb, K, nullptr, p.kb))) return st;
   return launch_maxmin_keys(h, p.ka, p.kb, p.ka.rows, p.kb.rows, symmetric, C, ldc, nullptr, 0, nullptr, 0, nullptr, 0);
+}
+
+// compute_maxmins (apps/pipeline/mcpnet_helper.hpp:436-460) on FP32 keys: M1 = A (x) A,
+// M2 = A (x) M1, M3 = M1 (x) M1, each written only when its output pointer is set.
+int maxmins_f32(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N, float* M1, int64_t ld1, float* M2,
+                int64_t ld2, float* M3, int64_t ld3) {
+  const size_t NN = (size_t)N * N;
+  KeyMat ka = key_geometry(N, N), k1 = key_geometry(N, N);
+  int st;
+  if ((st = key_check(h, ka))) return st;
+  mcpg::Sizer sz;
+  if (host) { sz.add<float>(NN); for (float* p : {M1, M2, M3}) if (p) sz.add<float>(NN); }
+  sz.add<int>(ka.rows_pad * ka.ldk);
+  sz.add<int>(k1.rows_pad * k1.ldk);
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const float* dA = A;
+  int64_t la = lda;
+  float* dM[3] = {M1, M2, M3};
+  int64_t ldM[3] = {ld1, ld2, ld3};
+  if (host) {
+    float* a = cv.take<float>(NN);
+    if ((st = mcpg::copy_2d(h, a, N * 4, A, lda * 4, N * 4, N, hipMemcpyHostToDevice))) return st;
+    dA = a; la = N;
+    for (int q = 0; q < 3; ++q)
+      if (dM[q]) { dM[q] = cv.take<float>(NN); ldM[q] = N; }
+  }
+  ka.d = cv.take<int>(ka.rows_pad * ka.ldk);
+  k1.d = cv.take<int>(k1.rows_pad * k1.ldk);
+  if ((st = prep_keys(h, dA, la, N, nullptr, ka))) return st;
+  if ((st = pad_keys(h, k1, N))) return st;
+  if ((st = launch_maxmin_keys(h, ka, ka, N, N, 1, dM[0], ldM[0], k1.d, k1.ldk, nullptr, 0, nullptr, 0))) return st;
+  if (dM[1]) {
+    if ((st = mcpg::reset_flags(h))) return st;
+    {
+      mcpg::LaunchScope ls(h, "symmetry_check_kernel");
+      hipLaunchKernelGGL(symmetry_check_kernel<float>, dim3(1024), dim3(256), 0, h->stream, dA, la, N, h->d_flags);
+    }
+    int flags[1];
+    if ((st = mcpg::read_flags(h, flags, 1))) return st;
+    if ((st = launch_maxmin_keys(h, ka, k1, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0, nullptr, 0, nullptr, 0)))
+      return st;
+  }
+  if (dM[2] && (st = launch_maxmin_keys(h, k1, k1, N, N, 1, dM[2], ldM[2], nullptr, 0, nullptr, 0, nullptr, 0)))
+    return st;
+  if (host) {
+    float* out[3] = {M1, M2, M3};
+    const int64_t ldo[3] = {ld1, ld2, ld3};
+    for (int q = 0; q < 3; ++q)
+      if (out[q] && (st = mcpg::copy_2d(h, out[q], ldo[q] * 4, dM[q], N * 4, N * 4, N, hipMemcpyDeviceToHost))) return st;
+  }
+  return mcpg::finish(h, "mcpg_maxmins");
+}
+
+// DPI filter of an N x N MI matrix.  FP32: the max-min products run on keys with the excluded
+// entries at INT_MIN; FP64: on a copy with a -inf diagonal and a -inf / +max TF column mask.
+template <typename T>
+int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T tolerance, const T* tfs, T* out, int64_t ldo,
+              int64_t* removed) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (N < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (removed) *removed = 0;
+  if (N == 0) return MCPG_OK;
+  if (!A || !out) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (lda < N || ldo < N) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (!(tolerance >= (T)0 && tolerance < (T)1)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "tolerance must be in [0, 1)");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  constexpr bool f32 = std::is_same<T, float>::value;
+  const bool host = ptrs == MCPG_HOST;
+  const size_t NN = (size_t)N * N;
+  KeyMat kx = key_geometry(N, N), kt = key_geometry(N, N);
+  int st;
+  if (f32 && (st = key_check(h, kx))) return st;
+  mcpg::Sizer sz;
+  if (host) { sz.add<T>(NN); sz.add<T>(NN); }
+  if (tfs) sz.add<T>(N);
+  sz.add<T>(NN);                                  // W
+  if (tfs) sz.add<T>(NN);                         // WT
+  if (f32) { sz.add<int>(kx.rows_pad * kx.ldk); if (tfs) sz.add<int>(kt.rows_pad * kt.ldk); }
+  else { sz.add<T>(NN); if (tfs) sz.add<T>(N); }  // diagonal-excluded copy, TF column mask
+  sz.add<unsigned long long>(1);
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const T* dA = A;
+  int64_t la = lda;
+  T* dO = out;
+  int64_t lo = ldo;
+  if (host) {
+    T* a = cv.take<T>(NN);
+    if ((st = mcpg::copy_2d(h, a, N * sizeof(T), A, lda * sizeof(T), N * sizeof(T), N, hipMemcpyHostToDevice))) return st;
+    dA = a; la = N;
+    dO = cv.take<T>(NN); lo = N;
+  }
+  T* dtf = nullptr;
+  if (tfs) {
+    dtf = cv.take<T>(N);
+    MCPG_HIP(h, hipMemcpyAsync(dtf, tfs, N * sizeof(T), host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, h->stream));
+  }
+  T* W = cv.take<T>(NN);
+  T* WT = tfs ? cv.take<T>(NN) : nullptr;
+  if constexpr (f32) {
+    kx.d = cv.take<int>(kx.rows_pad * kx.ldk);
+    {
+      mcpg::LaunchScope ls(h, "dpi_key_kernel");
+      hipLaunchKernelGGL(dpi_key_kernel, dim3(grid_of(kx.rows_pad * kx.ldk)), dim3(256), 0, h->stream, dA, la, N,
+                         (const float*)nullptr, kx.d, kx.ldk, kx.rows_pad);
+    }
+    if ((st = launch_maxmin_keys(h, kx, kx, N, N, 1, W, N, nullptr, 0, nullptr, 0, nullptr, 0))) return st;
+    if (tfs) {
+      kt.d = cv.take<int>(kt.rows_pad * kt.ldk);
+      {
+        mcpg::LaunchScope ls(h, "dpi_key_kernel");
+        hipLaunchKernelGGL(dpi_key_kernel, dim3(grid_of(kt.rows_pad * kt.ldk)), dim3(256), 0, h->stream, dA, la, N, dtf,
+                           kt.d, kt.ldk, kt.rows_pad);
+      }
+      // max over TF y of min(A[x][y], A[z][y]): the mask on one operand suffices (min is associative)
+      if ((st = launch_maxmin_keys(h, kt, kx, N, N, 1, WT, N, nullptr, 0, nullptr, 0, nullptr, 0))) return st;
+    }
+  } else {
+    T* Ad = cv.take<T>(NN);
+    {
+      mcpg::LaunchScope ls(h, "diag_copy_kernel");
+      hipLaunchKernelGGL(diag_copy_kernel<T>, dim3(grid_of(NN)), dim3(256), 0, h->stream, dA, la, N, (T)-INFINITY, Ad);
+    }
+    if ((st = launch_maxmin<TrF64>(h, (const double*)Ad, N, (const double*)Ad, N, nullptr, N, N, N, 1, (double*)W, N,
+                                   nullptr, 0, nullptr, 0)))
+      return st;
+    if (tfs) {
+      std::vector<T> mask(N), htf(N);
+      if (host) std::copy(tfs, tfs + N, htf.begin());
+      else MCPG_HIP(h, hipMemcpy(htf.data(), tfs, N * sizeof(T), hipMemcpyDeviceToHost));
+      for (int64_t k = 0; k < N; ++k) mask[k] = htf[k] > (T)0 ? std::numeric_limits<T>::max() : (T)-INFINITY;
+      T* dmask = cv.take<T>(N);
+      MCPG_HIP(h, hipMemcpy(dmask, mask.data(), N * sizeof(T), hipMemcpyHostToDevice));
+      if ((st = launch_maxmin<TrF64>(h, (const double*)Ad, N, (const double*)Ad, N, (const double*)dmask, N, N, N, 1,
+                                     (double*)WT, N, nullptr, 0, nullptr, 0)))
+        return st;
+    }
+  }
+  unsigned long long* dcount = cv.take<unsigned long long>(1);
+  MCPG_HIP(h, hipMemsetAsync(dcount, 0, sizeof(unsigned long long), h->stream));
+  const double tm_d = 1.0 / (1.0 - (double)tolerance);          // dpi_kernel: double multiplier
+  const T tm_t = (T)(1.0 / (1.0 - (double)tolerance));          // dpi_tf_kernel: multiplier stored in T
+  {
+    mcpg::LaunchScope ls(h, "dpi_apply_kernel");
+    hipLaunchKernelGGL(dpi_apply_kernel<T>, dim3(grid_of(NN)), dim3(256), 0, h->stream, dA, la, N, W, N, WT, N, dtf, tm_d,
+                       tm_t, dO, lo, dcount);
+  }
+  unsigned long long cnt = 0;
+  MCPG_HIP(h, hipMemcpyAsync(&cnt, dcount, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
+  if (host && (st = mcpg::copy_2d(h, out, ldo * sizeof(T), dO, N * sizeof(T), N * sizeof(T), N, hipMemcpyDeviceToHost)))
+    return st;
+  if ((st = mcpg::finish(h, "mcpg_dpi"))) return st;
+  if (removed) *removed = (int64_t)cnt;
+  return MCPG_OK;
+}
+
+// compute_maxmins on FP64 values
+int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N, double* M1, int64_t ld1, double* M2,
+                int64_t ld2, double* M3, int64_t ld3) {
+  const size_t NN = (size_t)N * N;
+  mcpg::Sizer sz;
+  if (host) { sz.add<double>(NN); for (double* p : {M2, M3}) if (p) sz.add<double>(NN); }
+  if (host || !M1) sz.add<double>(NN);   // M1 is always needed on the device
+  int st;
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const double* dA = A;
+  int64_t la = lda;
+  double* dM[3] = {M1, M2, M3};
+  int64_t ldM[3] = {ld1, ld2, ld3};
+  if (host) {
+    double* a = cv.take<double>(NN);
+    if ((st = mcpg::copy_2d(h, a, N * 8, A, lda * 8, N * 8, N, hipMemcpyHostToDevice))) return st;
+    dA = a; la = N;
+    for (int q = 1; q < 3; ++q)
+      if (dM[q]) { dM[q] = cv.take<double>(NN); ldM[q] = N; }
+  }
+  if (host || !M1) { dM[0] = cv.take<double>(NN); ldM[0] = N; }
+  if ((st = launch_maxmin<TrF64>(h, dA, la, dA, la, nullptr, N, N, N, 1, dM[0], ldM[0], nullptr, 0, nullptr, 0))) return st;
+  if (dM[1]) {
+    if ((st = mcpg::reset_flags(h))) return st;
+    {
+      mcpg::LaunchScope ls(h, "symmetry_check_kernel");
+      hipLaunchKernelGGL(symmetry_check_kernel<double>, dim3(1024), dim3(256), 0, h->stream, dA, la, N, h->d_flags);
+    }
+    int flags[1];
+    if ((st = mcpg::read_flags(h, flags, 1))) return st;
+    if ((st = launch_maxmin<TrF64>(h, dA, la, dM[0], ldM[0], nullptr, N, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0,
+                                   nullptr, 0)))
+      return st;
+  }
+  if (dM[2] && (st = launch_maxmin<TrF64>(h, dM[0], ldM[0], dM[0], ldM[0], nullptr, N, N, N, 1, dM[2], ldM[2], nullptr, 0,
+                                          nullptr, 0)))
+    return st;
+  if (host) {
+    double* out[3] = {M1, M2, M3};
+    const int64_t ldo[3] = {ld1, ld2, ld3};
+    for (int q = 0; q < 3; ++q)
+      if (out[q] && (st = mcpg::copy_2d(h, out[q], ldo[q] * 8, dM[q], N * 8, N * 8, N, hipMemcpyDeviceToHost))) return st;
+  }
+  return mcpg::finish(h, "mcpg_maxmins");
+}
+
+template <typename T>
+int maxmins_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T* M1, int64_t ld1, T* M2, int64_t ld2,
+                  T* M3, int64_t ld3) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (N < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (N == 0) return MCPG_OK;
+  if (!A) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (lda < N || (M1 && ld1 < N) || (M2 && ld2 < N) || (M3 && ld3 < N))
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  if constexpr (std::is_same<T, float>::value) return maxmins_f32(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
+  else return maxmins_f64(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
 }
 
 template <typename T>
@@ -921,6 +1206,22 @@ int mcpg_mcp_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N
 int mcpg_mcp_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, int k, double* maxmin_out,
                  int64_t ldm, double* mcp_out, int64_t ldo) {
   return mcp_entry<double>(h, ptrs, A, lda, N, k, maxmin_out, ldm, mcp_out, ldo);
+}
+int mcpg_maxmins_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N, float* M1, int64_t ld1, float* M2,
+                     int64_t ld2, float* M3, int64_t ld3) {
+  return maxmins_entry<float>(h, ptrs, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
+}
+int mcpg_maxmins_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, double* M1, int64_t ld1,
+                     double* M2, int64_t ld2, double* M3, int64_t ld3) {
+  return maxmins_entry<double>(h, ptrs, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
+}
+int mcpg_dpi_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N, float tolerance, const float* tfs,
+                 float* out, int64_t ldo, int64_t* removed) {
+  return dpi_entry<float>(h, ptrs, A, lda, N, tolerance, tfs, out, ldo, removed);
+}
+int mcpg_dpi_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, double tolerance, const double* tfs,
+                 double* out, int64_t ldo, int64_t* removed) {
+  return dpi_entry<double>(h, ptrs, A, lda, N, tolerance, tfs, out, ldo, removed);
 }
 
 }  // extern "C"

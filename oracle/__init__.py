@@ -44,6 +44,12 @@ _SIGS = {
     "orc_ratio_f32": (None, [_P, _P, c_int64, _P]),
     "orc_ratio_f64": (None, [_P, _P, c_int64, _P]),
     "port_bspline_mi_rows_f32": (None, [_P, c_int64, c_int64, c_int, c_int64, c_int64, _P]),
+    "orc_combo_f32": (None, [_P, _P, _P, _P, c_int64, _P, _P]),
+    "orc_combo_f64": (None, [_P, _P, _P, _P, c_int64, _P, _P]),
+    "orc_stouffer_f32": (None, [_P, c_int64, c_int, _P]),
+    "orc_stouffer_f64": (None, [_P, c_int64, c_int, _P]),
+    "orc_dpi_f32": (c_int64, [_P, c_int64, c_float, _P, _P]),
+    "orc_dpi_f64": (c_int64, [_P, c_int64, c_double, _P, _P]),
 }
 
 _lib = None
@@ -300,3 +306,41 @@ def mcp(A, k):
     else:
         raise ValueError("k must be 2, 3 or 4")
     return ratio(A, Mk), Mk
+
+
+# ------------------------------------------------------------------ ensemble and filters
+def maxmins(A):
+    """compute_maxmins (mcpnet_helper.hpp:436-460): (A(x)A, A(x)M1, M1(x)M1)."""
+    M1 = maxmin(A, A)
+    return M1, maxmin(A, M1), maxmin(M1, M1)
+
+
+def combo(MI, M1, M2, M3, coeffs):
+    dt = np.float64 if np.asarray(MI).dtype == np.float64 else np.float32
+    mats = [_c(x if x is not None else np.zeros_like(MI), dt) for x in (MI, M1, M2, M3)]
+    cf = _c(coeffs, dt)
+    out = np.empty_like(mats[0])
+    fn = lib().orc_combo_f64 if dt == np.float64 else lib().orc_combo_f32
+    fn(mats[0].ctypes.data, mats[1].ctypes.data, mats[2].ctypes.data, mats[3].ctypes.data, mats[0].size, cf.ctypes.data,
+       out.ctypes.data)
+    return out
+
+
+def stouffer(X, exclude_ones=True):
+    dt = np.float64 if np.asarray(X).dtype == np.float64 else np.float32
+    X = _c(X, dt)
+    out = np.empty_like(X)
+    fn = lib().orc_stouffer_f64 if dt == np.float64 else lib().orc_stouffer_f32
+    fn(X.ctypes.data, X.shape[0], int(exclude_ones), out.ctypes.data)
+    return out
+
+
+def dpi(A, tolerance=0.1, tfs=None):
+    """(DPI-filtered A, number of changed entries) following dpi.hpp's per-pair scan."""
+    dt = np.float64 if np.asarray(A).dtype == np.float64 else np.float32
+    A = _c(A, dt)
+    tf = None if tfs is None else _c(tfs, dt)
+    out = np.empty_like(A)
+    fn = lib().orc_dpi_f64 if dt == np.float64 else lib().orc_dpi_f32
+    n = fn(A.ctypes.data, A.shape[0], tolerance, tf.ctypes.data if tf is not None else None, out.ctypes.data)
+    return out, int(n)

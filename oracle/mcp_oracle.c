@@ -19,6 +19,10 @@
  *                        brute force
  *   ratio                closed-form branches of combine.hpp:214-220
  *   MI gradient (f64)    central finite differences of orc_bspline_mi_loss_f64
+ *   combo                unit coefficient rows reduce to the MCP2/3/4 ratios (orc_ratio) and to 1
+ *   Stouffer             numpy mean / std (ddof 1) z-scores
+ *   DPI                  an independent vectorised numpy statement; the weakest edge of a
+ *                        triangle is removed at tolerance 0 (ARACNE's data processing inequality)
  */
 #include <float.h>
 #include <math.h>
@@ -559,3 +563,100 @@ void orc_ratio_f64(const double* A, const double* Mm, int64_t n, double* out) {
   for (int64_t i = 0; i < n; ++i)
     out[i] = (fabs(A[i]) < DBL_EPSILON) ? 0.0 : (fabs(Mm[i]) < DBL_EPSILON) ? DBL_MAX : A[i] / Mm[i];
 }
+
+/* ------------------------------------------------------------------ ensemble and filters */
+
+/* compute_combo (apps/pipeline/mcpnet_helper.hpp:509-571): s accumulated in double, a coefficient
+ * of 1 adds the value unscaled and 0 skips the term; the quotient is a double division stored in T. */
+#define ORC_COMBO(NAME, T, EPS, BIG)                                                              \
+  void NAME(const T* MI, const T* M1, const T* M2, const T* M3, int64_t n, const T* cs, T* out) { \
+    const double a = cs[0], b = cs[1], c = cs[2], d = cs[3];                                      \
+    for (int64_t i = 0; i < n; ++i) {                                                             \
+      double s = 0.0;                                                                             \
+      if (a == 1.0) s += MI[i]; else if (a != 0.0) s += a * MI[i];                                \
+      if (b == 1.0) s += M1[i]; else if (b != 0.0) s += b * M1[i];                                \
+      if (c == 1.0) s += M2[i]; else if (c != 0.0) s += c * M2[i];                                \
+      if (d == 1.0) s += M3[i]; else if (d != 0.0) s += d * M3[i];                                \
+      out[i] = (fabs((double)MI[i]) < (double)EPS) ? (T)0                                         \
+               : (fabs(s) < (double)EPS) ? BIG : (T)((double)MI[i] / s);                          \
+    }                                                                                             \
+  }
+ORC_COMBO(orc_combo_f32, float, FLT_EPSILON, FLT_MAX)
+ORC_COMBO(orc_combo_f64, double, DBL_EPSILON, DBL_MAX)
+
+/* compute_stouffer (apps/pipeline/mcpnet_helper.hpp:574-584): row mean and unbiased standard
+ * deviation (sequential FP64 sums; entries equal to 1 skipped when exclude_ones), then
+ * stouffer_vector_kernel (include/mcp/transform/stouffer.hpp:56-80) in T. */
+#define ORC_STOUFFER(NAME, T, EPS)                                                                \
+  void NAME(const T* X, int64_t N, int exclude_ones, T* out) {                                    \
+    T* mean = (T*)malloc(sizeof(T) * (size_t)N);                                                  \
+    T* sd = (T*)malloc(sizeof(T) * (size_t)N);                                                    \
+    for (int64_t r = 0; r < N; ++r) {                                                             \
+      const T* x = X + r * N;                                                                     \
+      double s = 0.0, c = 0.0, q = 0.0;                                                           \
+      for (int64_t j = 0; j < N; ++j) {                                                           \
+        if (exclude_ones && x[j] == (T)1) continue;                                               \
+        s += (double)x[j];                                                                        \
+        c += 1.0;                                                                                 \
+      }                                                                                           \
+      const double mu = c > 0.0 ? s / c : 0.0;                                                    \
+      for (int64_t j = 0; j < N; ++j) {                                                           \
+        if (exclude_ones && x[j] == (T)1) continue;                                               \
+        q += ((double)x[j] - mu) * ((double)x[j] - mu);                                           \
+      }                                                                                           \
+      mean[r] = (T)mu;                                                                            \
+      sd[r] = (T)sqrt(c > 1.0 ? q / (c - 1.0) : 0.0);                                             \
+    }                                                                                             \
+    const T isqrt2 = (T)0.70710678118654752440084436210485L;                                      \
+    for (int64_t i = 0; i < N; ++i) {                                                             \
+      const T denom = (fabs((double)sd[i]) < (double)EPS) ? (T)0 : (T)(1.0 / (double)sd[i]);      \
+      for (int64_t j = 0; j < N; ++j) {                                                           \
+        const T x = X[i * N + j];                                                                 \
+        const T zi = (T)((x - mean[i]) * denom);                                                  \
+        const T zj = (fabs((double)sd[j]) < (double)EPS) ? (T)0 : (T)((x - mean[j]) / sd[j]);     \
+        out[i * N + j] = (zi + zj) * isqrt2;                                                      \
+      }                                                                                           \
+    }                                                                                             \
+    free(mean);                                                                                   \
+    free(sd);                                                                                     \
+  }
+ORC_STOUFFER(orc_stouffer_f32, float, FLT_EPSILON)
+ORC_STOUFFER(orc_stouffer_f64, double, DBL_EPSILON)
+
+/* DPI mask (include/mcp/filter/dpi.hpp:56-66 dpi_kernel and :86-105 dpi_tf_kernel, both with
+ * EQUALITY_EXCLUSION) applied as mask<T, true> (include/mcp/filter/threshold.hpp:46-53): the
+ * reference's early-exit scan over y, pair by pair.  dpi_kernel keeps its multiplier in double,
+ * dpi_tf_kernel in T.  Returns the number of entries that changed. */
+#define ORC_DPI(NAME, T)                                                                          \
+  int64_t NAME(const T* A, int64_t N, T tolerance, const T* tfs, T* out) {                        \
+    const double tm_d = 1.0 / (1.0 - (double)tolerance);                                          \
+    const T tm_t = (T)(1.0 / (1.0 - (double)tolerance));                                          \
+    int64_t removed = 0;                                                                          \
+    _Pragma("omp parallel for reduction(+ : removed) schedule(static)")                           \
+    for (int64_t x = 0; x < N; ++x) {                                                             \
+      const T* rx = A + x * N;                                                                    \
+      for (int64_t z = 0; z < N; ++z) {                                                           \
+        const T* rz = A + z * N;                                                                  \
+        int del = 1;                                                                              \
+        if (x != z) {                                                                             \
+          int64_t y = 0;                                                                          \
+          if (tfs) {                                                                              \
+            const T ixz = rx[z] * tm_t;                                                           \
+            const int tf_xz = (tfs[x] > 0) ^ (tfs[z] > 0);                                        \
+            for (; y < N && (y == x || y == z || rx[y] < ixz || rz[y] < ixz ||                    \
+                             (tfs[y] <= 0 && tf_xz)); ++y) {}                                     \
+          } else {                                                                                \
+            const T ixz = (T)((double)rx[z] * tm_d);                                              \
+            for (; y < N && (y == x || y == z || rx[y] < ixz || rz[y] < ixz); ++y) {}             \
+          }                                                                                       \
+          del = y < N;                                                                            \
+        }                                                                                         \
+        const T o = del ? (T)0 : rx[z];                                                           \
+        out[x * N + z] = o;                                                                       \
+        removed += (o != rx[z]);                                                                  \
+      }                                                                                           \
+    }                                                                                             \
+    return removed;                                                                               \
+  }
+ORC_DPI(orc_dpi_f32, float)
+ORC_DPI(orc_dpi_f64, double)

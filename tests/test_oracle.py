@@ -273,3 +273,98 @@ def test_gradient_matches_finite_differences():
         Xm = X64.copy(); Xm[a, b] -= h
         fd = (oracle.bspline_mi_loss_x64(Xp, pi, pj, m) - oracle.bspline_mi_loss_x64(Xm, pi, pj, m)) / (2 * h)
         assert abs(fd - g[a, b]) <= 1e-5 * max(1.0, abs(fd)), (a, b, fd, g[a, b])
+
+
+# ------------------------------------------------------------------ ensemble and filters
+def _sym(rng, n, dt):
+    A = rng.random((n, n)).astype(dt)
+    A = np.maximum(A, A.T)
+    np.fill_diagonal(A, 0)
+    return A
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_combo_unit_coefficients_reduce_to_ratios(dt):
+    """A coefficient row (0,1,0,0) is MCP2 = ratio(MI, M1), (0,0,1,0) MCP3, (0,0,0,1) MCP4 -- each
+    through orc_ratio, an independent function -- and (1,0,0,0) gives 1 wherever |MI| >= eps."""
+    rng = np.random.default_rng(11)
+    A = _sym(rng, 40, dt)
+    A[3, 5] = A[5, 3] = 0
+    M1, M2, M3 = oracle.maxmins(A)
+    for q, M in enumerate((M1, M2, M3)):
+        cf = np.zeros(4, dt)
+        cf[q + 1] = 1
+        assert np.array_equal(oracle.combo(A, M1, M2, M3, cf), oracle.ratio(A, M))
+    ones = oracle.combo(A, M1, M2, M3, np.array([1, 0, 0, 0], dt))
+    assert np.array_equal(ones, np.where(np.abs(A) < np.finfo(dt).eps, 0, 1).astype(dt))
+    # a general row equals the double-precision expression of compute_combo
+    cf = np.array([0.25, 0.25, 0.25, 0.25], dt)
+    s = sum(float(c) * M.astype(np.float64) for c, M in zip(cf, (A, M1, M2, M3)))
+    want = np.where(np.abs(A) < np.finfo(dt).eps, 0, np.where(np.abs(s) < np.finfo(dt).eps, np.finfo(dt).max,
+                                                                A.astype(np.float64) / s)).astype(dt)
+    assert np.array_equal(oracle.combo(A, M1, M2, M3, cf), want)
+
+
+@pytest.mark.parametrize("exclude_ones", [False, True])
+def test_stouffer_matches_numpy_zscores(exclude_ones):
+    rng = np.random.default_rng(12)
+    X = rng.random((50, 50))
+    X[rng.random((50, 50)) < 0.1] = 1.0
+    X[7] = 0.25                                      # a constant row: sd 0 -> its z-scores are 0
+    keep = (X != 1.0) if exclude_ones else np.ones_like(X, bool)
+    mean = np.array([X[r][keep[r]].mean() for r in range(50)])
+    sd = np.array([X[r][keep[r]].std(ddof=1) for r in range(50)])
+    zi = np.where(sd[:, None] < np.finfo(np.float64).eps, 0, (X - mean[:, None]) / np.where(sd[:, None] == 0, 1, sd[:, None]))
+    zj = np.where(sd[None, :] < np.finfo(np.float64).eps, 0, (X - mean[None, :]) / np.where(sd[None, :] == 0, 1, sd[None, :]))
+    want = (zi + zj) * np.sqrt(0.5)
+    got = oracle.stouffer(X, exclude_ones)
+    assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+    got32 = oracle.stouffer(X.astype(np.float32), exclude_ones)
+    assert np.allclose(got32, want, rtol=1e-5, atol=1e-5)
+
+
+def _dpi_numpy(A, tol, tfs=None):
+    """Vectorised statement of the DPI rule: edge x-z goes when some y != x, z has
+    A[x][y] >= i_xz and A[z][y] >= i_xz (y restricted to TFs when exactly one endpoint is a TF)."""
+    n = A.shape[0]
+    dt = A.dtype
+    out = A.copy()
+    for x in range(n):
+        for z in range(n):
+            if x == z:
+                out[x, z] = 0
+                continue
+            t = np.float64(dt.type(tol))                # the tolerance arrives as T
+            if tfs is None:
+                ixz = dt.type(np.float64(A[x, z]) * (1.0 / (1.0 - t)))
+                ok = np.ones(n, bool)
+            else:
+                ixz = A[x, z] * dt.type(1.0 / (1.0 - t))
+                ok = np.ones(n, bool) if (tfs[x] > 0) == (tfs[z] > 0) else (tfs > 0)
+            ok[[x, z]] = False
+            if np.any(ok & (A[x] >= ixz) & (A[z] >= ixz)):
+                out[x, z] = 0
+    return out
+
+
+def test_dpi_removes_the_weakest_triangle_edge():
+    A = np.array([[0, .9, .5], [.9, 0, .8], [.5, .8, 0]], np.float32)
+    out, n = oracle.dpi(A, 0.0)
+    assert np.array_equal(out, np.array([[0, .9, 0], [.9, 0, .8], [0, .8, 0]], np.float32)) and n == 2
+    # a tolerance above the gap keeps it: 0.5 / (1 - 0.45) > 0.8
+    out, n = oracle.dpi(A, 0.45)
+    assert np.array_equal(out, A) and n == 0
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("with_tfs", [False, True])
+def test_dpi_matches_vectorised_statement(dt, with_tfs):
+    rng = np.random.default_rng(13)
+    n = 45
+    A = (rng.random((n, n)) ** 2).astype(dt)        # not symmetric: i_xz uses row x
+    tfs = np.where(rng.random(n) < 0.3, np.finfo(dt).max, np.finfo(dt).min).astype(dt) if with_tfs else None
+    for tol in (0.0, 0.1, 0.3):
+        got, cnt = oracle.dpi(A, tol, tfs)
+        want = _dpi_numpy(A, tol, tfs)
+        assert np.array_equal(got, want)
+        assert cnt == int(np.count_nonzero(got != A))
