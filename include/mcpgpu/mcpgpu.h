@@ -221,12 +221,14 @@ int mcpg_stouffer_f64(mcpg_handle h, int ptrs, const double* X, int64_t ldx, int
  * mask<T, true> (include/mcp/filter/threshold.hpp:34-54): out[x][z] = 0 when x == z or some gene
  * y != x, z (a TF when exactly one of x, z is a TF) has A[x][y] >= i_xz and A[z][y] >= i_xz,
  * i_xz = A[x][z] / (1 - tolerance); else A[x][z].  tfs (nullable, length N): > 0 marks a TF.
- * removed (nullable) receives the number of entries that changed.  The max over y is a max-min
- * product with the excluded entries at -inf, so the decision is exact. */
+ * mask (nullable, one byte per entry) receives the decision itself (1 = remove), the boolean
+ * matrix the reference's InnerProduct produces; out and mask may each be NULL, not both.
+ * removed (nullable) receives the number of entries of out that changed.  The max over y is a
+ * max-min product with the excluded entries at -inf, so the decision is exact. */
 int mcpg_dpi_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N, float tolerance, const float* tfs,
-                 float* out, int64_t ldo, int64_t* removed);
+                 float* out, int64_t ldo, uint8_t* mask, int64_t ldmask, int64_t* removed);
 int mcpg_dpi_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, double tolerance,
-                 const double* tfs, double* out, int64_t ldo, int64_t* removed);
+                 const double* tfs, double* out, int64_t ldo, uint8_t* mask, int64_t ldmask, int64_t* removed);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,15 @@ class BinaryOp;
 template <typename InMat1, typename InMat2, typename Kernel, typename OutMat>
 class GlobalBinaryOp;
 
+// params = reduc(in.row(r)) for every row r, then out.row(i) = kernel(in.row(i), &params[i], params)
+// (ReduceTransform / GlobalReduceTransform at apps/pipeline/mcpnet_helper.hpp:574-584 and
+// apps/transform/transform.cpp:218-235).
+template <typename InMat, typename Reduc, typename Kernel, typename OutMat>
+class ReduceTransform;
+
+template <typename InMat, typename Reduc, typename Kernel, typename OutMat>
+class GlobalReduceTransform;
+
 }  // namespace pattern
 
 namespace utils {

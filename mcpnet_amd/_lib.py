@@ -69,8 +69,8 @@ _SIGS = {
     "mcpg_combo_f64": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, _P, c_int64],
     "mcpg_stouffer_f32": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64],
     "mcpg_stouffer_f64": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64],
-    "mcpg_dpi_f32": [_P, c_int, _P, c_int64, c_int64, c_float, _P, _P, c_int64, POINTER(c_int64)],
-    "mcpg_dpi_f64": [_P, c_int, _P, c_int64, c_int64, c_double, _P, _P, c_int64, POINTER(c_int64)],
+    "mcpg_dpi_f32": [_P, c_int, _P, c_int64, c_int64, c_float, _P, _P, c_int64, _P, c_int64, POINTER(c_int64)],
+    "mcpg_dpi_f64": [_P, c_int, _P, c_int64, c_int64, c_double, _P, _P, c_int64, _P, c_int64, POINTER(c_int64)],
 }
 EXPORTS = ["mcpg_status_string", "mcpg_last_error", *_SIGS.keys()]
 
@@ -473,8 +473,9 @@ class Handle:
         self._check(fn(self.h, x.kind, x.ptr, x.ld, N, int(exclude_ones), o.ptr, o.ld), "mcpg_stouffer")
         return out
 
-    def dpi(self, A, tolerance=0.1, tfs=None, out=None):
-        """DPI-filtered copy of the square MI matrix A; returns (out, removed count)."""
+    def dpi(self, A, tolerance=0.1, tfs=None, out=None, want_mask=False):
+        """DPI-filtered copy of the square MI matrix A; returns (out, removed count) or, with
+        want_mask, (out, removed count, uint8 mask of removed edges)."""
         dt = np.float64 if str(getattr(A, "dtype", "")).endswith("float64") else np.float32
         a = _Mat(A, dt, "A")
         N = a.shape[0]
@@ -486,12 +487,23 @@ class Handle:
         kind = _same_kind(a, tf)
         out = _empty_like_kind(kind, (N, N), dt, A) if out is None else out
         o = _Mat(out, dt, "out")
+        mask = None
+        mk = None
+        if want_mask:
+            if kind == MCPG_DEVICE:
+                import torch
+                mask = torch.empty((N, N), dtype=torch.uint8, device=A.device)
+                mk_ptr, mk_ld = mask.data_ptr(), N
+            else:
+                mask = np.empty((N, N), dtype=np.uint8)
+                mk_ptr, mk_ld = mask.ctypes.data, N
+            mk = (mk_ptr, mk_ld)
         removed = c_int64(0)
         if dt == np.float64:
             fn, tol = self.lib.mcpg_dpi_f64, c_double(tolerance)
         else:
             fn, tol = self.lib.mcpg_dpi_f32, c_float(tolerance)
         self._pre(kind)
-        self._check(fn(self.h, kind, a.ptr, a.ld, N, tol, tf.ptr if tf else None, o.ptr, o.ld, byref(removed)),
-                    "mcpg_dpi")
-        return out, removed.value
+        self._check(fn(self.h, kind, a.ptr, a.ld, N, tol, tf.ptr if tf else None, o.ptr, o.ld,
+                       mk[0] if mk else None, mk[1] if mk else 0, byref(removed)), "mcpg_dpi")
+        return (out, removed.value, mask) if want_mask else (out, removed.value)

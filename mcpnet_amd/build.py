@@ -86,7 +86,7 @@ def build_library(verbose: bool = False, jobs: int = 4) -> Path:
     return lib
 
 
-TOOLS = {"mi": "mi.cpp", "mcp": "mcp.cpp", "mcpnet": "mcpnet.cpp"}
+TOOLS = {"mi": "mi.cpp", "mcp": "mcp.cpp", "mcpnet": "mcpnet.cpp", "dpi": "dpi.cpp"}
 
 
 def build_tools(verbose: bool = False) -> list[Path]:

@@ -512,12 +512,14 @@ __global__ void symmetry_check_kernel(const T* __restrict__ A, int64_t lda, int6
 // (EQUALITY_EXCLUSION), i.e. when W[x][z] = max over allowed y of that minimum reaches i_xz.
 // i_xz = A[x][z] * tol_multiplier: a double product stored in T for dpi_kernel, a T product for
 // dpi_tf_kernel (whose multiplier is stored in T).  With a TF list, pairs with exactly one TF
-// endpoint use WT (y restricted to TFs).  Removed edges become 0; `removed` counts the entries
-// that changed.
+// endpoint use WT (y restricted to TFs).  Removed edges become 0 in O; `mask` (nullable) receives
+// the decision itself (1 = remove), the boolean matrix InnerProduct<..., dpi_kernel> produces;
+// `removed` counts the entries of O that changed.
 template <typename T>
 __global__ void dpi_apply_kernel(const T* __restrict__ A, int64_t lda, int64_t N, const T* __restrict__ W, int64_t ldw,
                                  const T* __restrict__ WT, int64_t ldwt, const T* __restrict__ tfs, double tm_d, T tm_t,
-                                 T* __restrict__ O, int64_t ldo, unsigned long long* removed) {
+                                 T* __restrict__ O, int64_t ldo, uint8_t* __restrict__ mask, int64_t ldmask,
+                                 unsigned long long* removed) {
   const int64_t total = N * N;
   unsigned long long cnt = 0;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
@@ -535,7 +537,8 @@ __global__ void dpi_apply_kernel(const T* __restrict__ A, int64_t lda, int64_t N
       }
     }
     const T o = del ? (T)0 : a;
-    O[x * ldo + z] = o;
+    if (O) O[x * ldo + z] = o;
+    if (mask) mask[x * ldmask + z] = del ? 1 : 0;
     cnt += (o != a) ? 1 : 0;
   }
   for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
@@ -762,14 +765,15 @@ int maxmins_f32(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N, 
 // entries at INT_MIN; FP64: on a copy with a -inf diagonal and a -inf / +max TF column mask.
 template <typename T>
 int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T tolerance, const T* tfs, T* out, int64_t ldo,
-              int64_t* removed) {
+              uint8_t* mask, int64_t ldmask, int64_t* removed) {
   MCPG_CHECK_HANDLE(h);
   mcpg::begin_call(h);
   if (N < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
   if (removed) *removed = 0;
   if (N == 0) return MCPG_OK;
-  if (!A || !out) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
-  if (lda < N || ldo < N) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (!A || (!out && !mask)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (lda < N || (out && ldo < N) || (mask && ldmask < N))
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (!(tolerance >= (T)0 && tolerance < (T)1)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "tolerance must be in [0, 1)");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
@@ -780,7 +784,7 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
   int st;
   if (f32 && (st = key_check(h, kx))) return st;
   mcpg::Sizer sz;
-  if (host) { sz.add<T>(NN); sz.add<T>(NN); }
+  if (host) { sz.add<T>(NN); if (out) sz.add<T>(NN); if (mask) sz.add<uint8_t>(NN); }
   if (tfs) sz.add<T>(N);
   sz.add<T>(NN);                                  // W
   if (tfs) sz.add<T>(NN);                         // WT
@@ -794,11 +798,14 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
   int64_t la = lda;
   T* dO = out;
   int64_t lo = ldo;
+  uint8_t* dmk = mask;
+  int64_t lmk = ldmask;
   if (host) {
     T* a = cv.take<T>(NN);
     if ((st = mcpg::copy_2d(h, a, N * sizeof(T), A, lda * sizeof(T), N * sizeof(T), N, hipMemcpyHostToDevice))) return st;
     dA = a; la = N;
-    dO = cv.take<T>(NN); lo = N;
+    if (out) { dO = cv.take<T>(NN); lo = N; }
+    if (mask) { dmk = cv.take<uint8_t>(NN); lmk = N; }
   }
   T* dtf = nullptr;
   if (tfs) {
@@ -853,12 +860,14 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
   {
     mcpg::LaunchScope ls(h, "dpi_apply_kernel");
     hipLaunchKernelGGL(dpi_apply_kernel<T>, dim3(grid_of(NN)), dim3(256), 0, h->stream, dA, la, N, W, N, WT, N, dtf, tm_d,
-                       tm_t, dO, lo, dcount);
+                       tm_t, dO, lo, dmk, lmk, dcount);
   }
   unsigned long long cnt = 0;
   MCPG_HIP(h, hipMemcpyAsync(&cnt, dcount, sizeof cnt, hipMemcpyDeviceToHost, h->stream));
-  if (host && (st = mcpg::copy_2d(h, out, ldo * sizeof(T), dO, N * sizeof(T), N * sizeof(T), N, hipMemcpyDeviceToHost)))
+  if (host && out &&
+      (st = mcpg::copy_2d(h, out, ldo * sizeof(T), dO, N * sizeof(T), N * sizeof(T), N, hipMemcpyDeviceToHost)))
     return st;
+  if (host && mask && (st = mcpg::copy_2d(h, mask, ldmask, dmk, N, N, N, hipMemcpyDeviceToHost))) return st;
   if ((st = mcpg::finish(h, "mcpg_dpi"))) return st;
   if (removed) *removed = (int64_t)cnt;
   return MCPG_OK;
@@ -1216,12 +1225,12 @@ int mcpg_maxmins_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int6
   return maxmins_entry<double>(h, ptrs, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
 }
 int mcpg_dpi_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N, float tolerance, const float* tfs,
-                 float* out, int64_t ldo, int64_t* removed) {
-  return dpi_entry<float>(h, ptrs, A, lda, N, tolerance, tfs, out, ldo, removed);
+                 float* out, int64_t ldo, uint8_t* mask, int64_t ldmask, int64_t* removed) {
+  return dpi_entry<float>(h, ptrs, A, lda, N, tolerance, tfs, out, ldo, mask, ldmask, removed);
 }
 int mcpg_dpi_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, double tolerance, const double* tfs,
-                 double* out, int64_t ldo, int64_t* removed) {
-  return dpi_entry<double>(h, ptrs, A, lda, N, tolerance, tfs, out, ldo, removed);
+                 double* out, int64_t ldo, uint8_t* mask, int64_t ldmask, int64_t* removed) {
+  return dpi_entry<double>(h, ptrs, A, lda, N, tolerance, tfs, out, ldo, mask, ldmask, removed);
 }
 
 }  // extern "C"

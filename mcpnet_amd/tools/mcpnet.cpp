@@ -1,16 +1,21 @@
-// mcpnet: expression -> MI (B-spline or adaptive partitioning) -> diagonal -> MCP2/3/4 scores.
-// Follows the MI and MCP stages of apps/pipeline/mcpnet.cpp:159-515 and the compute_* helpers of
-// apps/pipeline/mcpnet_helper.hpp:56-364, on the GPU.  Outputs are <prefix>.mi.<fmt> and
-// <prefix>.mcp{2,3,4}.<fmt>.  The ensemble (-f coefficients), Stouffer and evaluation (-g, -x,
-// --eval) stages of the reference are not part of this build.
+// mcpnet: expression -> MI (B-spline or adaptive partitioning) -> diagonal -> MCP2/3/4 scores and
+// the ensemble MCP.  Follows the MI, MCP and ensemble stages of apps/pipeline/mcpnet.cpp:159-700
+// and the compute_* helpers of apps/pipeline/mcpnet_helper.hpp:56-584, on the GPU.  Outputs are
+// <prefix>.mi.<fmt>, <prefix>.mcp{2,3,4}.<fmt> and, for -m 4, <prefix>.mcp_<a>_<b>_<c>_<d>.<fmt>
+// (combination ratio) and <prefix>.mcpst_<a>_<b>_<c>_<d>.<fmt> (its Stouffer combination) per
+// coefficient row.  The evaluation stage (-g, -x, --eval: AUPR / AUROC against a ground truth)
+// is not part of this build.
 #include <cstdio>
+#include <iomanip>
 #include <set>
+#include <sstream>
 
 #include "common.hpp"
 #include "mcp/correlation/AdaptivePartitioningMI.hpp"
 #include "mcp/correlation/BSplineMI.hpp"
 #include "mcp/filter/mcp.hpp"
 #include "mcp/transform/combine.hpp"
+#include "mcp/transform/stouffer.hpp"
 #include "splash/kernel/rank.hpp"
 
 namespace {
@@ -63,6 +68,58 @@ void compute_mcp(const splash::ds::aligned_matrix<T>& g2g, int k, splash::ds::al
   gen(g2g, part, dmaxmin, part, mcp::kernel::ratio_kernel<T, T>(), dmcp, part);
 }
 
+// compute_maxmins (mcpnet_helper.hpp:436-460): M1 = A(x)A, M2 = A(x)M1, M3 = M1(x)M1
+inline int maxmins_call(mcpg_handle h, const float* a, int64_t la, int64_t N, float* m1, int64_t l1, float* m2, int64_t l2,
+                        float* m3, int64_t l3) {
+  return mcpg_maxmins_f32(h, MCPG_HOST, a, la, N, m1, l1, m2, l2, m3, l3);
+}
+inline int maxmins_call(mcpg_handle h, const double* a, int64_t la, int64_t N, double* m1, int64_t l1, double* m2,
+                        int64_t l2, double* m3, int64_t l3) {
+  return mcpg_maxmins_f64(h, MCPG_HOST, a, la, N, m1, l1, m2, l2, m3, l3);
+}
+template <typename T>
+void compute_maxmins(const splash::ds::aligned_matrix<T>& g2g, splash::ds::aligned_matrix<T>& m1,
+                     splash::ds::aligned_matrix<T>& m2, splash::ds::aligned_matrix<T>& m3) {
+  const size_t N = g2g.rows();
+  m1.resize(N, N);
+  m2.resize(N, N);
+  m3.resize(N, N);
+  mcpgpu::check(maxmins_call(mcpgpu::thread_handle(), g2g.data(), g2g.ld(), N, m1.data(), m1.ld(), m2.data(), m2.ld(),
+                             m3.data(), m3.ld()),
+                "compute_maxmins");
+}
+
+// compute_combo (mcpnet_helper.hpp:509-571)
+inline int combo_call(mcpg_handle h, const float* mi, int64_t lmi, const float* m1, int64_t l1, const float* m2,
+                      int64_t l2, const float* m3, int64_t l3, int64_t M, int64_t N, const float* c, float* o,
+                      int64_t lo) {
+  return mcpg_combo_f32(h, MCPG_HOST, mi, lmi, m1, l1, m2, l2, m3, l3, M, N, c, o, lo);
+}
+inline int combo_call(mcpg_handle h, const double* mi, int64_t lmi, const double* m1, int64_t l1, const double* m2,
+                      int64_t l2, const double* m3, int64_t l3, int64_t M, int64_t N, const double* c, double* o,
+                      int64_t lo) {
+  return mcpg_combo_f64(h, MCPG_HOST, mi, lmi, m1, l1, m2, l2, m3, l3, M, N, c, o, lo);
+}
+template <typename T>
+void compute_combo(const splash::ds::aligned_matrix<T>& mi, const splash::ds::aligned_matrix<T>& m1,
+                   const splash::ds::aligned_matrix<T>& m2, const splash::ds::aligned_matrix<T>& m3, const T* cs,
+                   splash::ds::aligned_matrix<T>& out) {
+  out.resize(m1.rows(), m1.columns());
+  mcpgpu::check(combo_call(mcpgpu::thread_handle(), mi.data(), mi.ld(), m1.data(), m1.ld(), m2.data(), m2.ld(),
+                           m3.data(), m3.ld(), m1.rows(), m1.columns(), cs, out.data(), out.ld()),
+                "compute_combo");
+}
+
+// compute_stouffer (mcpnet_helper.hpp:574-584)
+template <typename T>
+void compute_stouffer(const splash::ds::aligned_matrix<T>& in, splash::ds::aligned_matrix<T>& out) {
+  using Mat = splash::ds::aligned_matrix<T>;
+  splash::pattern::ReduceTransform<Mat, splash::kernel::GaussianParamsExclude1<T, T, true>,
+                                   mcp::kernel::stouffer_vector_kernel<T>, Mat>
+      transformer;
+  transformer(in, splash::kernel::GaussianParamsExclude1<T, T, true>(), mcp::kernel::stouffer_vector_kernel<T>(), out);
+}
+
 template <typename T>
 int run(const cli::Parser& p) {
   using Mat = splash::ds::aligned_matrix<T>;
@@ -89,12 +146,45 @@ int run(const cli::Parser& p) {
   for (const auto& m : p.all("--method")) comps.insert(std::stoi(m));
   if (comps.empty()) comps.insert(2);
   for (int m : comps) {
-    if (m < 1 || m > 3) throw std::runtime_error("-m accepts 1 (MCP2), 2 (MCP3), 3 (MCP4)");
+    if (m < 1 || m > 4) throw std::runtime_error("-m accepts 1 (MCP2), 2 (MCP3), 3 (MCP4), 4 (ensemble MCP)");
+    if (m == 4) continue;
     t = tool::Timer();
     Mat dratio;
     compute_mcp(mi, m + 1, dratio);
     printf("Computed,MCP%d,,%f,sec\n", m + 1, t.seconds());
     mio::write_matrix(prefix + ".mcp" + std::to_string(m + 1) + fmt, genes, genes, dratio);
+  }
+  if (comps.count(4)) {   // ensemble MCP (mcpnet.cpp:518-700, the branch without evaluation)
+    t = tool::Timer();
+    Mat coeffs;
+    if (!p.str("--coefficients").empty()) {
+      std::vector<std::string> rdummy, cdummy;
+      coeffs = mio::read_matrix<T>(p.str("--coefficients"), rdummy, cdummy);
+      if (coeffs.columns() < 4) throw std::runtime_error("the coefficient file needs 4 columns");
+    } else {
+      coeffs.resize(1, 4);
+      for (size_t j = 0; j < 4; ++j) coeffs.at(0, j) = static_cast<T>(0.25);
+    }
+    printf("Loaded,%zux%zu coefficients,,%f,sec\n", coeffs.rows(), coeffs.columns(), t.seconds());
+    t = tool::Timer();
+    Mat dmaxmin1, dmaxmin2, dmaxmin3;
+    compute_maxmins(mi, dmaxmin1, dmaxmin2, dmaxmin3);
+    printf("Computed,MAXMIN 1 2 3,,%f,sec\n", t.seconds());
+    for (size_t i = 0; i < coeffs.rows(); ++i) {
+      const T* c = coeffs.data(i);
+      if (c[0] == 1 && c[1] == 0 && c[2] == 0 && c[3] == 0) continue;
+      std::ostringstream tag;
+      tag << std::fixed << std::setprecision(3) << c[0] << "_" << c[1] << "_" << c[2] << "_" << c[3];
+      t = tool::Timer();
+      Mat dratio, dratiost;
+      compute_combo(mi, dmaxmin1, dmaxmin2, dmaxmin3, c, dratio);
+      printf("Computed,Combo,(%s),%f,sec\n", tag.str().c_str(), t.seconds());
+      mio::write_matrix(prefix + ".mcp_" + tag.str() + fmt, genes, genes, dratio);
+      t = tool::Timer();
+      compute_stouffer(dratio, dratiost);
+      printf("Computed,Combo Stouffer,(%s),%f,sec\n", tag.str().c_str(), t.seconds());
+      mio::write_matrix(prefix + ".mcpst_" + tag.str() + fmt, genes, genes, dratiost);
+    }
   }
   return 0;
 }
@@ -106,7 +196,8 @@ int main(int argc, char** argv) {
   tool::common_options(p);
   p.option("--mi-method", "MI Algorithm: 0 = Bspline, 1 (default) = Adaptive Partitioning", true, "1");
   p.option("--mi-input", "input file holds a precomputed MI matrix", false);
-  p.option("-m,--method", "MCP2=1, MCP3=2, MCP4=3 (one or more)", true, "");
+  p.option("-m,--method", "Algorithm: MCP2=1, MCP3=2, MCP4=3, EnsembleMCP=4 (one or more)", true, "");
+  p.option("-f,--coefficients", "file with combo coefficients (rows of 4).  For ensemble MCP only.", true, "");
   p.option("--clamped", "output is clamped (no effect, as in the reference)", false);
   p.option("--diagonal", "MI diagonal value; negative keeps the MI diagonal", true, "0");
   p.option("--format", "output format: exp, csv or tsv", true, "exp");

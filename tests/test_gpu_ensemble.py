@@ -98,3 +98,13 @@ def test_dpi_asymmetric_and_device(handle, torch_cuda):
     got, n = handle.dpi(torch_cuda.from_numpy(A).cuda(), 0.05)
     want, m = oracle.dpi(A, 0.05)
     assert bits_equal(got.cpu().numpy(), want) and n == m
+
+
+def test_dpi_mask_output(handle):
+    A = mi_matrix(90, 200, 10)
+    A[2, 7] = A[7, 2] = 0                                    # removed edge that is already 0: only the mask shows it
+    got, n, mask = handle.dpi(A, 0.1, want_mask=True)
+    want, m = oracle.dpi(A, 0.1)
+    assert bits_equal(got, want) and n == m
+    assert np.array_equal(mask.astype(bool) & (A != 0), want != A)
+    assert mask.diagonal().all()

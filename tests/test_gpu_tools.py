@@ -122,3 +122,57 @@ def test_tools_report_errors(tmp_path):
     res = subprocess.run([str(BIN / "mi"), "-i", str(R10), "-o", str(tmp_path / "o.h5")],
                          capture_output=True, text=True, timeout=60)
     assert res.returncode != 0 and "HDF5" in res.stderr
+
+
+@pytest.mark.parametrize("with_tfs", [False, True])
+def test_dpi_tool(tmp_path, with_tfs):
+    X = grn.grn_expression(36, 200, 15)[0]
+    A = oracle.bspline_mi_f64(X)
+    names = [f"g{i}" for i in range(36)]
+    src = tmp_path / "mi.exp"
+    write_exp(src, A, names)
+    out = tmp_path / "dpi.exp"
+    args = ["-i", src, "-o", out, "-l", "0.1"]
+    tf_idx = [1, 5, 9, 30]
+    if with_tfs:
+        tf_file = tmp_path / "tfs.txt"
+        tf_file.write_text("\n".join(names[t] for t in tf_idx) + "\n")
+        args += ["-f", tf_file]
+    stdout = run("dpi", *args)
+    Ain = read_exp(src)[1]
+    np.fill_diagonal(Ain, 0.0)                    # --diagonal defaults to 0
+    tfs = None
+    if with_tfs:
+        tfs = np.full(36, np.finfo(np.float64).min)
+        tfs[tf_idx] = np.finfo(np.float64).max
+    want, removed = oracle.dpi(Ain, 0.1, tfs)
+    got_names, got = read_exp(out)
+    if with_tfs:
+        assert got_names == [names[t] for t in tf_idx]
+        want = want[tf_idx]
+    assert np.array_equal(got, want)
+    assert f"Mask removing {removed} elements" in stdout
+
+
+def test_mcpnet_ensemble(tmp_path):
+    X = grn.grn_expression(48, 256, 16)[0]
+    A = oracle.bspline_mi_f64(X)
+    names = [f"g{i}" for i in range(48)]
+    src = tmp_path / "mi.exp"
+    write_exp(src, A, names)
+    cf = np.array([[0.25, 0.25, 0.25, 0.25], [1, 0, 0, 0], [0.5, 0, 0.5, 0]])
+    cfile = tmp_path / "coeffs.exp"
+    write_exp(cfile, cf, ["r0", "r1", "r2"])
+    prefix = tmp_path / "ens"
+    run("mcpnet", "-i", src, "-o", prefix, "--mi-input", "-m", "4", "-f", cfile)
+    Ain = read_exp(src)[1]
+    np.fill_diagonal(Ain, 0.0)
+    M1, M2, M3 = oracle.maxmins(Ain)
+    for row in (cf[0], cf[2]):
+        tag = "_".join(f"{v:.3f}" for v in row)
+        got = read_exp(f"{prefix}.mcp_{tag}.exp")[1]
+        want = oracle.combo(Ain, M1, M2, M3, row)
+        assert np.array_equal(got, want)
+        gst = read_exp(f"{prefix}.mcpst_{tag}.exp")[1]
+        assert np.allclose(gst, oracle.stouffer(want, True), rtol=1e-12, atol=1e-12)
+    assert not Path(f"{prefix}.mcp_1.000_0.000_0.000_0.000.exp").exists()   # (1,0,0,0) is skipped
