@@ -238,6 +238,59 @@ def bench_adjoint(args, G, S, seed):
     h.close()
 
 
+def bench_ensemble_dpi(args, h, MI, stream, torch):
+    """The ensemble MCP (compute_maxmins, compute_combo at the default coefficients, compute_stouffer)
+    and the DPI filter (tolerance 0.1) on the MI matrix of the workload (diagonal 0), timed with
+    HIP events on the library's stream; checked against the oracle unless --no-cpu."""
+    G = MI.shape[0]
+    cf = np.full(4, 0.25, np.float32)
+    out = {}
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            r = fn()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        return float(np.mean(ms)), r
+
+    t_mm, (M1, M2, M3) = timed(lambda: h.maxmins(MI))
+    t_cb, R = timed(lambda: h.combo(MI, M1, M2, M3, cf))
+    t_st, Z = timed(lambda: h.stouffer(R))
+    t_dpi, (D, removed) = timed(lambda: h.dpi(MI, 0.1))
+    ops = 3.0 * G ** 3
+    out["ensemble"] = {"maxmins_ms": t_mm, "combo_ms": t_cb, "stouffer_ms": t_st,
+                       "maxmins_gops": ops / (t_mm / 1e3) / 1e9,
+                       "work": "M1 = A(x)A, M2 = A(x)M1, M3 = M1(x)M1 (3 G^3 algorithmic max-min ops), combination "
+                               "ratio at coefficients 0.25 x 4, Stouffer of the ratio"}
+    out["dpi"] = {"ms": t_dpi, "removed": int(removed), "tolerance": 0.1,
+                  "work": "one max-min triangle product with the diagonal excluded + mask application"}
+    if not args.no_cpu:
+        import oracle
+
+        A = MI.cpu().numpy()
+        r1, r2, r3 = oracle.maxmins(A)
+        out["ensemble"]["maxmins_bit_exact"] = bool(all(np.array_equal(g.cpu().numpy(), w)
+                                                        for g, w in zip((M1, M2, M3), (r1, r2, r3))))
+        rw = oracle.combo(A, r1, r2, r3, cf)
+        out["ensemble"]["combo_bit_exact"] = bool(np.array_equal(R.cpu().numpy(), rw))
+        out["ensemble"]["stouffer_max_abs_diff"] = float(np.abs(Z.cpu().numpy().astype(np.float64)
+                                                                - oracle.stouffer(rw)).max())
+        t0 = time.perf_counter()
+        dw, rm = oracle.dpi(A, 0.1)
+        tc = time.perf_counter() - t0
+        out["dpi"]["bit_exact"] = bool(np.array_equal(D.cpu().numpy(), dw) and rm == removed)
+        out["dpi"]["cpu_baseline"] = {"value_ms": tc * 1e3, "cores": omp_threads(), "kind": "port",
+                                      "sample": f"the reference's per-pair early-exit DPI scan (oracle restatement, "
+                                                f"OpenMP) on the same {G} x {G} matrix; host {cpu_model()}"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,6 +306,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of each bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 MI comparison pass")
+    ap.add_argument("--extras", action="store_true", help="also time the ensemble MCP and the DPI filter "
+                                                          "(always on for config 2)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -422,6 +477,10 @@ def main():
         log("bench: CPU baselines")
         MI.fill_diagonal_(0.0)
         cpu = cpu_baselines(X, S, bins, MI.cpu().numpy(), args.cpu_seconds)
+    if cid == 2 or args.extras:
+        log("bench: ensemble MCP and DPI")
+        MI.fill_diagonal_(0.0)
+        extra.update(bench_ensemble_dpi(args, h, MI, stream, torch))
 
     result = {
         "metric": ("MI gene-pairs/s (split-BF16 B-spline MI) & MCP3 max-min Gop/s @20k genes" if cid == 3 else
