@@ -86,7 +86,20 @@ def build_library(verbose: bool = False, jobs: int = 4) -> Path:
     return lib
 
 
-TOOLS = {"mi": "mi.cpp", "mcp": "mcp.cpp", "mcpnet": "mcpnet.cpp", "dpi": "dpi.cpp"}
+TOOLS = {"mi": "mi.cpp", "mcp": "mcp.cpp", "mcpnet": "mcpnet.cpp", "dpi": "dpi.cpp", "convert": "convert.cpp"}
+HOST_ONLY_TOOLS = {"convert"}   # no GPU work: not linked against libmcpgpu
+
+# HDF5 header for the tools' .h5 reader/writer; libhdf5 itself is opened at run time (dlopen)
+HDF5_INCLUDE_DIRS = [os.environ.get("MCPG_HDF5_INCLUDE", ""), "/usr/include/hdf5/serial", "/usr/include",
+                     "/opt/conda/include"]
+
+
+def _hdf5_flags() -> list[str]:
+    for d in HDF5_INCLUDE_DIRS:
+        if d and (Path(d) / "hdf5.h").exists():
+            # only the HDF5 headers are wanted from a prefix that holds other libraries' headers too
+            return ["-idirafter", d]
+    return ["-DMIO_NO_HDF5"]
 
 
 def build_tools(verbose: bool = False) -> list[Path]:
@@ -97,13 +110,17 @@ def build_tools(verbose: bool = False) -> list[Path]:
     BINDIR.mkdir(exist_ok=True)
     cxx = shutil.which("g++") or "g++"
     rocm_inc = "/opt/rocm/include"
+    h5 = _hdf5_flags()
     out = []
     for name, src in TOOLS.items():
         if not (tooldir / src).exists():
             continue
         exe = BINDIR / name
-        cmd = [cxx, "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{tooldir}", f"-I{rocm_inc}",
-               str(tooldir / src), "-o", str(exe), f"-L{LIBDIR}", "-lmcpgpu", f"-Wl,-rpath,$ORIGIN/../lib"]
+        cmd = [cxx, "-O2", "-std=c++17", "-Wall", f"-I{INCLUDE}", f"-I{tooldir}", f"-I{rocm_inc}", *h5,
+               str(tooldir / src), "-o", str(exe)]
+        if name not in HOST_ONLY_TOOLS:
+            cmd += [f"-L{LIBDIR}", "-lmcpgpu", "-Wl,-rpath,$ORIGIN/../lib"]
+        cmd += ["-ldl"]
         _run(cmd, verbose)
         out.append(exe)
     return out

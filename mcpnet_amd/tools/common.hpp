@@ -17,7 +17,7 @@ namespace tool {
 // at apps/correlation/mi.cpp:83-93, :300); spellings of the options beyond -i/-o/-t/-m are this
 // build's, the reference's own spellings live in splash, which its tree does not include.
 inline void common_options(cli::Parser& p) {
-  p.option("-i,--input", "input matrix (.exp, .csv or .tsv; rows = genes)");
+  p.option("-i,--input", "input matrix (.exp, .csv, .tsv or .h5; rows = genes)");
   p.option("-o,--output", "output file");
   p.option("-t,--threads", "host threads (accepted for compatibility; the work runs on the GPU)", true, "1");
   p.option("--single", "use float (default: double, as the reference)", false);
@@ -27,6 +27,19 @@ inline void common_options(cli::Parser& p) {
   p.option("--rmax", "random maximum", true, "1.0");
   p.option("--num-vectors", "rows of the random matrix", true, "100");
   p.option("--vector-size", "columns of the random matrix", true, "1000");
+  p.option("--h5-group", "HDF5 group of the input matrix", true, "array");
+  p.option("--h5-gene-key", "HDF5 dataset of the row (gene) names", true, "axis1");
+  p.option("--h5-samples-key", "HDF5 dataset of the column (sample) names", true, "axis0");
+  p.option("--h5-matrix-key", "HDF5 dataset of the values", true, "block0_values");
+}
+
+inline mio::H5Keys h5_keys(const cli::Parser& p) {
+  mio::H5Keys k;
+  k.group = p.str("--h5-group");
+  k.genes = p.str("--h5-gene-key");
+  k.samples = p.str("--h5-samples-key");
+  k.matrix = p.str("--h5-matrix-key");
+  return k;
 }
 
 inline uint64_t splitmix64(uint64_t seed, uint64_t ctr) {
@@ -59,7 +72,7 @@ splash::ds::aligned_matrix<T> load_input(const cli::Parser& p, std::vector<std::
     return random_matrix<T>(p.integer("--rseed"), p.real("--rmin"), p.real("--rmax"), p.integer("--num-vectors"),
                             p.integer("--vector-size"), genes, samples);
   if (p.str("--input").empty()) throw std::runtime_error("no input: give -i FILE or --random");
-  return mio::read_matrix<T>(p.str("--input"), genes, samples);
+  return mio::read_matrix<T>(p.str("--input"), genes, samples, h5_keys(p));
 }
 
 struct Timer {

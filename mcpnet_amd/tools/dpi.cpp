@@ -19,8 +19,8 @@ int run(const cli::Parser& p) {
   std::vector<std::string> genes, samples;
   tool::Timer t;
   MatrixType input = tool::load_input<DataType>(p, genes, samples);
-  if (input.rows() == samples.size() && input.columns() == genes.size() && input.rows() != input.columns())
-    input = input.local_transpose();   // dpi.cpp:123-125
+  // dpi.cpp:123-125: transposed when the shape matches (samples x genes), square inputs included
+  if (input.rows() == samples.size() && input.columns() == genes.size()) input = input.local_transpose();
   printf("Load data in %f sec\n", t.seconds());
   const double diagonal = p.real("--diagonal");
   if (diagonal >= 0.0)   // dpi.cpp:130-135

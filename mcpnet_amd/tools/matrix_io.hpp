@@ -3,8 +3,8 @@
 //         then one row per gene "name<TAB>alias<TAB>values..."
 //   .csv  comma separated, header ",col..." then "name,values..."
 //   .tsv  tab separated, same layout as .csv
-// The reference also reads and writes HDF5 (pandas fixed format, /array/block0_values); the
-// HDF5 library is not part of this build, so .h5 paths are rejected with a clear message.
+//   .h5   HDF5, pandas fixed layout (/array/axis0, axis1, block0_values), through h5io.hpp; a
+//         build without the HDF5 header (MIO_NO_HDF5) rejects .h5 paths with a clear message
 #pragma once
 
 #include <cstdio>
@@ -16,8 +16,19 @@
 #include <vector>
 
 #include "splash/ds/aligned_matrix.hpp"
+#ifndef MIO_NO_HDF5
+#include "h5io.hpp"
+#endif
 
 namespace mio {
+
+#ifdef MIO_NO_HDF5
+struct H5Keys {
+  std::string group = "array", genes = "axis1", samples = "axis0", matrix = "block0_values";
+};
+#endif
+
+inline bool is_h5(const std::string& e) { return e == ".h5" || e == ".hdf5"; }
 
 inline std::string ext(const std::string& p) {
   auto d = p.find_last_of('.');
@@ -44,9 +55,13 @@ inline std::string trim(const std::string& s) {
 // rows = genes (first column names), columns = samples (header names)
 template <typename T>
 splash::ds::aligned_matrix<T> read_matrix(const std::string& path, std::vector<std::string>& rows,
-                                          std::vector<std::string>& cols) {
+                                          std::vector<std::string>& cols, const H5Keys& keys = H5Keys()) {
   const std::string e = ext(path);
-  if (e == ".h5" || e == ".hdf5") throw std::runtime_error(path + ": HDF5 files are not supported in this build; use .exp, .csv or .tsv");
+#ifdef MIO_NO_HDF5
+  if (is_h5(e)) throw std::runtime_error(path + ": this build has no HDF5 support; use .exp, .csv or .tsv");
+#else
+  if (is_h5(e)) return read_h5<T>(path, keys, rows, cols);
+#endif
   const bool exp = e == ".exp";
   const char sep = (e == ".csv") ? ',' : '\t';
   const size_t skip = exp ? 2 : 1;
@@ -78,7 +93,11 @@ template <typename T>
 void write_matrix(const std::string& path, const std::vector<std::string>& rows, const std::vector<std::string>& cols,
                   const splash::ds::aligned_matrix<T>& m) {
   const std::string e = ext(path);
-  if (e == ".h5" || e == ".hdf5") throw std::runtime_error(path + ": HDF5 files are not supported in this build; use .exp, .csv or .tsv");
+#ifdef MIO_NO_HDF5
+  if (is_h5(e)) throw std::runtime_error(path + ": this build has no HDF5 support; use .exp, .csv or .tsv");
+#else
+  if (is_h5(e)) return write_h5<T>(path, "array", rows, cols, m);   // convert.cpp:72 writes group "array"
+#endif
   const bool exp = e == ".exp";
   const char sep = (e == ".csv") ? ',' : '\t';
   std::ofstream out(path);

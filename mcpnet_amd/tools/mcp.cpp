@@ -34,8 +34,8 @@ int run(const cli::Parser& p) {
   MatrixType input = tool::load_input<DataType>(p, genes, samples);
   MatrixType input1, input2;
   if (compute == BIN_MCP) {
-    input1 = mio::read_matrix<DataType>(p.str("--first"), g1, s1);
-    input2 = mio::read_matrix<DataType>(p.str("--second"), g2, s2);
+    input1 = mio::read_matrix<DataType>(p.str("--first"), g1, s1, tool::h5_keys(p));
+    input2 = mio::read_matrix<DataType>(p.str("--second"), g2, s2, tool::h5_keys(p));
   }
   printf("Load data in %f sec\n", t.seconds());
 

@@ -127,21 +127,28 @@ int run(const cli::Parser& p) {
   tool::Timer t;
   Mat input = tool::load_input<T>(p, genes, samples);
   printf("Load data in %f sec\n", t.seconds());
-  const std::string prefix = p.str("--output"), fmt = "." + p.str("--format");
+  const bool mi_input = p.seen("--mi-input");
+  const int mim = (int)p.integer("--mi-method");
+  // output names: <output>[_tf](_bs|_ap|_inmi).<stage>.h5 (mcpnet.cpp:265-278)
+  const std::string prefix = p.str("--output") + (mi_input ? "_inmi" : mim == 0 ? "_bs" : "_ap");
+  const std::string fmt = "." + p.str("--format");
   Mat mi;
-  if (p.seen("--mi-input")) {
+  if (mi_input) {
     mi = input;
   } else {
     t = tool::Timer();
-    const int mim = (int)p.integer("--mi-method");
     if (mim == 0) compute_mi(input, 10, (int)p.integer("--precision-code"), mi);
     else compute_ap_mi(input, (uint64_t)p.integer("--noise-seed"), mi);
     printf("Computed,MI,,%f,sec\n", t.seconds());
-    mio::write_matrix(prefix + ".mi" + fmt, genes, genes, mi);
   }
   const double diagonal = p.real("--diagonal");
   if (diagonal >= 0.0)   // mcpnet.cpp:299-308
     for (size_t i = 0; i < std::min(mi.rows(), mi.columns()); ++i) mi.at(i, i) = static_cast<T>(diagonal);
+  if (!mi_input) {   // written after the diagonal is set (mcpnet.cpp:353-356)
+    t = tool::Timer();
+    mio::write_matrix(prefix + ".mi" + fmt, genes, genes, mi);
+    printf("Wrote,MI,,%f,sec\n", t.seconds());
+  }
   std::set<int> comps;
   for (const auto& m : p.all("--method")) comps.insert(std::stoi(m));
   if (comps.empty()) comps.insert(2);
@@ -200,7 +207,7 @@ int main(int argc, char** argv) {
   p.option("-f,--coefficients", "file with combo coefficients (rows of 4).  For ensemble MCP only.", true, "");
   p.option("--clamped", "output is clamped (no effect, as in the reference)", false);
   p.option("--diagonal", "MI diagonal value; negative keeps the MI diagonal", true, "0");
-  p.option("--format", "output format: exp, csv or tsv", true, "exp");
+  p.option("--format", "output format: h5 (as the reference), exp, csv or tsv", true, "h5");
   p.option("--precision-code", "B-spline contraction: 0 fp32, 1 bf16, 2 bf16x3", true, "0");
   p.option("--noise-seed", "seed of the U[0,1e-5) rank jitter", true, "1296388676612");
   try {

@@ -25,6 +25,9 @@ int run(const cli::Parser& p) {
   std::vector<std::string> genes, samples;
   tool::Timer t;
   Mat input = tool::load_input<DataType>(p, genes, samples);
+  // mi.cpp:95-96: a matrix whose shape matches (samples x genes) is transposed; a square input,
+  // whose gene and sample counts agree, is transposed as well
+  if (input.rows() == samples.size() && input.columns() == genes.size()) input = input.local_transpose();
   printf("Load data in %f sec\n", t.seconds());
   printf("No. GENES    : %zu\nNo. SAMPLES  : %zu\n", input.rows(), input.columns());
   const int method = (int)p.integer("--method");

@@ -15,6 +15,8 @@ pytestmark = pytest.mark.gpu
 ROOT = Path(__file__).resolve().parents[1]
 BIN = ROOT / "mcpnet_amd" / "bin"
 R10 = Path(__file__).with_name("golden") / "r10c10.exp"
+R10_H5 = Path(__file__).with_name("golden") / "r10c10.h5"
+R128_H5 = Path(__file__).with_name("golden") / "r128c1000.h5"
 
 
 def read_exp(path):
@@ -39,13 +41,36 @@ def run(tool, *args):
     return res.stdout
 
 
-def test_mi_bspline_on_reference_data(tmp_path):
+def h5_to_exp(path):
+    out = Path(str(path) + ".exp")
+    run("convert", "-i", path, "-o", out)
+    return read_exp(out)
+
+
+@pytest.mark.parametrize("src", [R10, R10_H5])
+def test_mi_bspline_on_reference_data(tmp_path, src):
     out = tmp_path / "r10c10.mi.exp"
-    run("mi", "-i", R10, "-o", out, "-m", "0", "--single")
+    run("mi", "-i", src, "-o", out, "-m", "0", "--single")
     _, mi = read_exp(out)
     _, X = read_exp(R10)
-    ref = oracle.bspline_mi_f64(X.astype(np.float32))
+    # 10 genes x 10 samples: the shape also matches (samples x genes), so mi.cpp:95-96 transposes
+    ref = oracle.bspline_mi_f64(np.ascontiguousarray(X.T).astype(np.float32))
     assert np.abs(mi - ref).max() <= 1e-5
+
+
+def test_mcpnet_on_reference_h5(tmp_path):
+    """mcpnet -i data/r128c1000.h5 --mi-method 0 -m 2: the .h5 MI and MCP3 outputs against the oracle."""
+    prefix = tmp_path / "r128"
+    run("mcpnet", "-i", R128_H5, "-o", prefix, "--mi-method", "0", "-m", "2", "--single")
+    X = h5_to_exp(R128_H5)[1].astype(np.float32)
+    names, mi = h5_to_exp(str(prefix) + "_bs.mi.h5")
+    assert names == [str(i) for i in range(128)]
+    ref = oracle.bspline_mi_f64(X)
+    np.fill_diagonal(ref, 0.0)                        # written after --diagonal 0 (mcpnet.cpp:299-356)
+    assert np.abs(mi - ref).max() <= 1e-5
+    A = mi.astype(np.float32)
+    got = h5_to_exp(str(prefix) + "_bs.mcp3.h5")[1].astype(np.float32)
+    assert np.array_equal(got, oracle.mcp(A, 3)[0])
 
 
 def test_mi_adaptive_partitioning_double(tmp_path):
@@ -103,14 +128,15 @@ def test_mcpnet_pipeline(tmp_path):
     src = tmp_path / "x.exp"
     write_exp(src, X.astype(np.float64))
     prefix = tmp_path / "net"
-    run("mcpnet", "-i", src, "-o", prefix, "--mi-method", "0", "-m", "1", "2", "--single")
-    _, mi = read_exp(str(prefix) + ".mi.exp")
+    run("mcpnet", "-i", src, "-o", prefix, "--mi-method", "0", "-m", "1", "2", "--single", "--format", "exp")
+    _, mi = read_exp(str(prefix) + "_bs.mi.exp")
     Xin = read_exp(src)[1].astype(np.float32)
-    assert np.abs(mi - oracle.bspline_mi_f64(Xin)).max() <= 1e-5
+    ref = oracle.bspline_mi_f64(Xin)
+    np.fill_diagonal(ref, 0.0)
+    assert np.abs(mi - ref).max() <= 1e-5
     A = mi.astype(np.float32)
-    np.fill_diagonal(A, 0.0)
     for k in (2, 3):
-        got = read_exp(str(prefix) + f".mcp{k}.exp")[1].astype(np.float32)
+        got = read_exp(str(prefix) + f"_bs.mcp{k}.exp")[1].astype(np.float32)
         assert np.array_equal(got, oracle.mcp(A, k)[0])
 
 
@@ -119,9 +145,9 @@ def test_tools_report_errors(tmp_path):
     res = subprocess.run([str(exe), "-i", str(tmp_path / "missing.exp"), "-o", str(tmp_path / "o.exp")],
                          capture_output=True, text=True, timeout=60)
     assert res.returncode != 0 and "cannot open" in res.stderr
-    res = subprocess.run([str(BIN / "mi"), "-i", str(R10), "-o", str(tmp_path / "o.h5")],
+    res = subprocess.run([str(BIN / "mi"), "-i", str(R10_H5), "-o", str(tmp_path / "o.h5"), "--h5-matrix-key", "x"],
                          capture_output=True, text=True, timeout=60)
-    assert res.returncode != 0 and "HDF5" in res.stderr
+    assert res.returncode != 0 and "no dataset /array/x" in res.stderr
 
 
 @pytest.mark.parametrize("with_tfs", [False, True])
@@ -139,7 +165,7 @@ def test_dpi_tool(tmp_path, with_tfs):
         tf_file.write_text("\n".join(names[t] for t in tf_idx) + "\n")
         args += ["-f", tf_file]
     stdout = run("dpi", *args)
-    Ain = read_exp(src)[1]
+    Ain = np.ascontiguousarray(read_exp(src)[1].T)   # square input: dpi.cpp:123-125 transposes it
     np.fill_diagonal(Ain, 0.0)                    # --diagonal defaults to 0
     tfs = None
     if with_tfs:
@@ -164,7 +190,8 @@ def test_mcpnet_ensemble(tmp_path):
     cfile = tmp_path / "coeffs.exp"
     write_exp(cfile, cf, ["r0", "r1", "r2"])
     prefix = tmp_path / "ens"
-    run("mcpnet", "-i", src, "-o", prefix, "--mi-input", "-m", "4", "-f", cfile)
+    run("mcpnet", "-i", src, "-o", prefix, "--mi-input", "-m", "4", "-f", cfile, "--format", "exp")
+    prefix = f"{prefix}_inmi"
     Ain = read_exp(src)[1]
     np.fill_diagonal(Ain, 0.0)
     M1, M2, M3 = oracle.maxmins(Ain)
