@@ -33,6 +33,15 @@ inline int stouffer_call(mcpg_handle h, const float* x, int64_t lx, int64_t N, i
 inline int stouffer_call(mcpg_handle h, const double* x, int64_t lx, int64_t N, int excl, double* o, int64_t lo) {
   return mcpg_stouffer_f64(h, MCPG_HOST, x, lx, N, excl, o, lo);
 }
+// compute_stouffer_tf (apps/pipeline/mcpnet_helper.hpp:587-638): the StandardScore<T, T, false>
+// normalisations of the TF rows and gene columns and the zscored_stouffer_kernel combination
+// (stouffer.hpp:84-99) in one call.
+inline int stouffer_tf_call(mcpg_handle h, const float* x, int64_t lx, int64_t M, int64_t N, float* o, int64_t lo) {
+  return mcpg_stouffer_tf_f32(h, MCPG_HOST, x, lx, M, N, o, lo);
+}
+inline int stouffer_tf_call(mcpg_handle h, const double* x, int64_t lx, int64_t M, int64_t N, double* o, int64_t lo) {
+  return mcpg_stouffer_tf_f64(h, MCPG_HOST, x, lx, M, N, o, lo);
+}
 }  // namespace mcpgpu
 
 namespace splash {

@@ -216,6 +216,18 @@ int mcpg_stouffer_f32(mcpg_handle h, int ptrs, const float* X, int64_t ldx, int6
 int mcpg_stouffer_f64(mcpg_handle h, int ptrs, const double* X, int64_t ldx, int64_t N, int exclude_ones,
                       double* out, int64_t ldo);
 
+/* compute_stouffer_tf (apps/pipeline/mcpnet_helper.hpp:587-638; apps/transform/transform.cpp:160-200)
+ * on an M x N TFs-to-genes score matrix: Transform<M, StandardScore<T, T, false>, M> z-scores of
+ * every row (a TF over its genes) and of every column (a gene over the TFs, through the
+ * transpose), combined by BinaryOp<M, M, zscored_stouffer_kernel<T>, M>
+ * (include/mcp/transform/stouffer.hpp:84-99): out = (zr + zc) * sqrt(1/2).  z = (x - mean) / sd
+ * with the population standard deviation, moments in FP64; a row or column with sd < eps(T)
+ * scores 0. */
+int mcpg_stouffer_tf_f32(mcpg_handle h, int ptrs, const float* X, int64_t ldx, int64_t M, int64_t N, float* out,
+                         int64_t ldo);
+int mcpg_stouffer_tf_f64(mcpg_handle h, int ptrs, const double* X, int64_t ldx, int64_t M, int64_t N, double* out,
+                         int64_t ldo);
+
 /* DPI filter (apps/filter/dpi.cpp:236-300): the mask of InnerProduct<..., dpi_kernel<T>> or, with
  * a TF list, dpi_tf_kernel<T> (include/mcp/filter/dpi.hpp:42-110, EQUALITY_EXCLUSION), applied with
  * mask<T, true> (include/mcp/filter/threshold.hpp:34-54): out[x][z] = 0 when x == z or some gene

@@ -69,6 +69,8 @@ _SIGS = {
     "mcpg_combo_f64": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, _P, c_int64],
     "mcpg_stouffer_f32": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64],
     "mcpg_stouffer_f64": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64],
+    "mcpg_stouffer_tf_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_stouffer_tf_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_dpi_f32": [_P, c_int, _P, c_int64, c_int64, c_float, _P, _P, c_int64, _P, c_int64, POINTER(c_int64)],
     "mcpg_dpi_f64": [_P, c_int, _P, c_int64, c_int64, c_double, _P, _P, c_int64, _P, c_int64, POINTER(c_int64)],
 }
@@ -471,6 +473,18 @@ class Handle:
         fn = self.lib.mcpg_stouffer_f64 if dt == np.float64 else self.lib.mcpg_stouffer_f32
         self._pre(x.kind)
         self._check(fn(self.h, x.kind, x.ptr, x.ld, N, int(exclude_ones), o.ptr, o.ld), "mcpg_stouffer")
+        return out
+
+    def stouffer_tf(self, X, out=None):
+        """compute_stouffer_tf on a TFs x genes score matrix (row and column z-scores)."""
+        dt = np.float64 if str(getattr(X, "dtype", "")).endswith("float64") else np.float32
+        x = _Mat(X, dt, "X")
+        M, N = x.shape
+        out = _empty_like_kind(x.kind, (M, N), dt, X) if out is None else out
+        o = _Mat(out, dt, "out")
+        fn = self.lib.mcpg_stouffer_tf_f64 if dt == np.float64 else self.lib.mcpg_stouffer_tf_f32
+        self._pre(x.kind)
+        self._check(fn(self.h, x.kind, x.ptr, x.ld, M, N, o.ptr, o.ld), "mcpg_stouffer_tf")
         return out
 
     def dpi(self, A, tolerance=0.1, tfs=None, out=None, want_mask=False):

@@ -1,5 +1,5 @@
-// Ensemble MCP epilogues on the N x N score matrices: the linear-combination ratio of
-// compute_combo and the Stouffer combination of compute_stouffer.
+// Ensemble MCP epilogues on the score matrices: the linear-combination ratio of compute_combo,
+// the Stouffer combination of compute_stouffer (N x N) and of compute_stouffer_tf (TFs x genes).
 //
 // Both are HBM-bound elementwise / row-reduction passes.  Arithmetic follows the reference
 // expression by expression (compiled without contraction, so no FMA rewrites the products).
@@ -104,6 +104,87 @@ __global__ void stouffer_kernel(const T* __restrict__ X, int64_t ldx, int64_t M,
     const T zi = (T)((x - mean[i]) * denom);
     const T zj = (fabs(sd[j]) < Lim<T>::eps) ? (T)0 : (T)((x - mean[j]) / sd[j]);
     O[i * ldo + j] = (zi + zj) * isqrt2;
+  }
+}
+
+// StandardScore<T, T, false> moments (splash; used by compute_stouffer_tf,
+// apps/pipeline/mcpnet_helper.hpp:591-613): mean and population standard deviation (n in the
+// variance; with them the reference's Pearson correlation, a dot product of z-scores divided by
+// the count at include/mcp/correlation/Pearson.hpp:34-36, is exact) of each row, in FP64, one
+// block per row, two passes.
+template <typename T>
+__global__ __launch_bounds__(256) void zrow_moments_kernel(const T* __restrict__ X, int64_t ldx, int64_t N,
+                                                           double* __restrict__ mean, double* __restrict__ sd) {
+  const int64_t r = blockIdx.x;
+  const T* x = X + r * ldx;
+  __shared__ double sh[4];
+  auto block_sum = [&](double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return sh[0] + sh[1] + sh[2] + sh[3];
+  };
+  double s = 0.0;
+  for (int64_t j = threadIdx.x; j < N; j += 256) s += (double)x[j];
+  const double mu = block_sum(s) / (double)N;
+  double q = 0.0;
+  for (int64_t j = threadIdx.x; j < N; j += 256) {
+    const double dv = (double)x[j] - mu;
+    q += dv * dv;
+  }
+  const double var = block_sum(q) / (double)N;
+  if (threadIdx.x == 0) {
+    mean[r] = mu;
+    sd[r] = sqrt(var);
+  }
+}
+
+// The same moments down each column of an M x N matrix: 64 columns per block (one per lane, so
+// each wave reads 64 consecutive elements of a row), the 4 waves split the rows.
+template <typename T>
+__global__ __launch_bounds__(256) void zcol_moments_kernel(const T* __restrict__ X, int64_t ldx, int64_t M, int64_t N,
+                                                           double* __restrict__ mean, double* __restrict__ sd) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t c = (int64_t)blockIdx.x * 64 + lane;
+  __shared__ double sh[4][64];
+  double s = 0.0;
+  if (c < N)
+    for (int64_t r = w; r < M; r += 4) s += (double)X[r * ldx + c];
+  sh[w][lane] = s;
+  __syncthreads();
+  const double mu = (sh[0][lane] + sh[1][lane] + sh[2][lane] + sh[3][lane]) / (double)M;
+  __syncthreads();
+  double q = 0.0;
+  if (c < N)
+    for (int64_t r = w; r < M; r += 4) {
+      const double dv = (double)X[r * ldx + c] - mu;
+      q += dv * dv;
+    }
+  sh[w][lane] = q;
+  __syncthreads();
+  if (w == 0 && c < N) {
+    mean[c] = mu;
+    sd[c] = sqrt((sh[0][lane] + sh[1][lane] + sh[2][lane] + sh[3][lane]) / (double)M);
+  }
+}
+
+// compute_stouffer_tf's combination (zscored_stouffer_kernel, include/mcp/transform/stouffer.hpp:84-99)
+// of the row z-score (TF normalised over its genes) and the column z-score (gene normalised over
+// the TFs): z = (T)((x - mean) / sd) in double, 0 when sd < eps(T); out = (zr + zc) * (T)sqrt(1/2).
+template <typename T>
+__global__ void stouffer_tf_kernel(const T* __restrict__ X, int64_t ldx, int64_t M, int64_t N,
+                                   const double* __restrict__ rmean, const double* __restrict__ rsd,
+                                   const double* __restrict__ cmean, const double* __restrict__ csd, T* __restrict__ O,
+                                   int64_t ldo) {
+  const T isqrt2 = (T)0.70710678118654752440084436210485L;
+  const int64_t total = M * N;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / N, j = t % N;
+    const double x = (double)X[i * ldx + j];
+    const T zr = rsd[i] < (double)Lim<T>::eps ? (T)0 : (T)((x - rmean[i]) / rsd[i]);
+    const T zc = csd[j] < (double)Lim<T>::eps ? (T)0 : (T)((x - cmean[j]) / csd[j]);
+    O[i * ldo + j] = (zr + zc) * isqrt2;
   }
 }
 
@@ -218,6 +299,60 @@ int stouffer_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t ldx, int64_t N, in
   return mcpg::finish(h, "mcpg_stouffer");
 }
 
+template <typename T>
+int stouffer_tf_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t ldx, int64_t M, int64_t N, T* out, int64_t ldo) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (M < 0 || N < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (M == 0 || N == 0) return MCPG_OK;
+  if (!X || !out) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldx < N || ldo < N) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  if (M > (int64_t)INT32_MAX || ceil_div(N, 64) > (int64_t)INT32_MAX)
+    return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "matrix too large");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const bool host = ptrs == MCPG_HOST;
+  mcpg::Sizer sz;
+  if (host) { sz.add<T>(M * N); sz.add<T>(M * N); }
+  sz.add<double>(2 * M);
+  sz.add<double>(2 * N);
+  int st;
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const T* dX = X;
+  int64_t lx = ldx;
+  T* dO = out;
+  int64_t lo = ldo;
+  if (host) {
+    T* x = cv.take<T>(M * N);
+    if ((st = mcpg::copy_2d(h, x, N * sizeof(T), X, ldx * sizeof(T), N * sizeof(T), M, hipMemcpyHostToDevice))) return st;
+    dX = x; lx = N;
+    dO = cv.take<T>(M * N); lo = N;
+  }
+  double* rm = cv.take<double>(2 * M);
+  double* rs = rm + M;
+  double* cm = cv.take<double>(2 * N);
+  double* cs = cm + N;
+  {
+    mcpg::LaunchScope ls(h, "zrow_moments_kernel");
+    hipLaunchKernelGGL(zrow_moments_kernel<T>, dim3((unsigned)M), dim3(256), 0, h->stream, dX, lx, N, rm, rs);
+  }
+  {
+    mcpg::LaunchScope ls(h, "zcol_moments_kernel");
+    hipLaunchKernelGGL(zcol_moments_kernel<T>, dim3((unsigned)ceil_div(N, 64)), dim3(256), 0, h->stream, dX, lx, M, N,
+                       cm, cs);
+  }
+  {
+    mcpg::LaunchScope ls(h, "stouffer_tf_kernel");
+    hipLaunchKernelGGL(stouffer_tf_kernel<T>, dim3(grid_of(M * N)), dim3(256), 0, h->stream, dX, lx, M, N, rm, rs, cm, cs,
+                       dO, lo);
+  }
+  if (host && (st = mcpg::copy_2d(h, out, ldo * sizeof(T), dO, N * sizeof(T), N * sizeof(T), M, hipMemcpyDeviceToHost)))
+    return st;
+  return mcpg::finish(h, "mcpg_stouffer_tf");
+}
+
 }  // namespace
 
 extern "C" {
@@ -239,6 +374,15 @@ int mcpg_stouffer_f32(mcpg_handle h, int ptrs, const float* X, int64_t ldx, int6
 int mcpg_stouffer_f64(mcpg_handle h, int ptrs, const double* X, int64_t ldx, int64_t N, int exclude_ones, double* out,
                       int64_t ldo) {
   return stouffer_entry<double>(h, ptrs, X, ldx, N, exclude_ones, out, ldo);
+}
+
+int mcpg_stouffer_tf_f32(mcpg_handle h, int ptrs, const float* X, int64_t ldx, int64_t M, int64_t N, float* out,
+                         int64_t ldo) {
+  return stouffer_tf_entry<float>(h, ptrs, X, ldx, M, N, out, ldo);
+}
+int mcpg_stouffer_tf_f64(mcpg_handle h, int ptrs, const double* X, int64_t ldx, int64_t M, int64_t N, double* out,
+                         int64_t ldo) {
+  return stouffer_tf_entry<double>(h, ptrs, X, ldx, M, N, out, ldo);
 }
 
 }  // extern "C"

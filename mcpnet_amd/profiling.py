@@ -42,6 +42,7 @@ def _short(name: str) -> str:
                 "bspline_prep_kernel", "ap_mi_kernel", "ratio_kernel", "symmetry_check_kernel", "rank_kernel",
                 "cumhist_kernel", "noise_kernel", "weights_import_kernel", "bspline_pair_adjoint_kernel",
                 "bspline_gene_adjoint_kernel", "emax_kernel", "combo_kernel", "stouffer_kernel", "row_stats_kernel",
+                "zrow_moments_kernel", "zcol_moments_kernel", "stouffer_tf_kernel",
                 "dpi_kernel", "copyBuffer", "fillBuffer"):
         if key in name:
             tmpl = name[name.find(key) + len(key):]

@@ -4,7 +4,10 @@
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <fstream>
+#include <limits>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "cli.hpp"
@@ -73,6 +76,43 @@ splash::ds::aligned_matrix<T> load_input(const cli::Parser& p, std::vector<std::
                             p.integer("--vector-size"), genes, samples);
   if (p.str("--input").empty()) throw std::runtime_error("no input: give -i FILE or --random");
   return mio::read_matrix<T>(p.str("--input"), genes, samples, h5_keys(p));
+}
+
+// Transcription factors: tfs[i] = max for a gene named in the file (one name per line, trimmed),
+// lowest otherwise; tf_names lists the TF genes in gene order.  The limits are those of L:
+// apps/filter/mcp.cpp:250-257 and dpi.cpp:180-187 take them from DataType, mcpnet_helper.hpp's
+// load_tfs (:1074-1120) from double (so +-inf in a float vector).
+template <typename T, typename L = T>
+void load_tfs(const std::string& path, const std::vector<std::string>& genes, std::vector<T>& tfs,
+              std::vector<std::string>& tf_names) {
+  std::unordered_set<std::string> TFs;
+  std::ifstream ifs(path);
+  if (!ifs) throw std::runtime_error("cannot open " + path);
+  std::string line;
+  while (std::getline(ifs, line)) TFs.insert(mio::trim(line));
+  tfs.clear();
+  tf_names.clear();
+  for (const auto& g : genes) {
+    const bool is_tf = TFs.count(mio::trim(g)) > 0;
+    tfs.push_back(static_cast<T>(is_tf ? std::numeric_limits<L>::max() : std::numeric_limits<L>::lowest()));
+    if (is_tf) tf_names.push_back(g);
+  }
+  printf("TFs specified %zu, found %zu\n", TFs.size(), tf_names.size());
+}
+
+// The TF rows of a genes x columns matrix, in gene order (filter_mat_rows_by_tfs,
+// mcpnet_helper.hpp:1126-1145).
+template <typename T>
+splash::ds::aligned_matrix<T> filter_rows_by_tfs(const splash::ds::aligned_matrix<T>& in, const std::vector<T>& tfs) {
+  size_t n = 0;
+  for (T v : tfs) n += v > (T)0;
+  splash::ds::aligned_matrix<T> out(n, in.columns());
+  for (size_t i = 0, j = 0; i < in.rows(); ++i)
+    if (tfs[i] > (T)0) {
+      std::copy(in.data(i), in.data(i) + in.columns(), out.data(j));
+      ++j;
+    }
+  return out;
 }
 
 struct Timer {

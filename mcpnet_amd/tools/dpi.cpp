@@ -35,17 +35,7 @@ int run(const cli::Parser& p) {
   std::vector<std::string> tf_names;
   const std::string tf_input = p.str("--tf-input");
   if (!tf_input.empty()) {
-    std::unordered_set<std::string> TFs;
-    std::ifstream ifs(tf_input);
-    if (!ifs) throw std::runtime_error("cannot open " + tf_input);
-    std::string line;
-    while (std::getline(ifs, line)) TFs.insert(mio::trim(line));
-    for (const auto& g : genes) {
-      const bool is_tf = TFs.count(mio::trim(g)) > 0;
-      tfs.push_back(is_tf ? std::numeric_limits<DataType>::max() : std::numeric_limits<DataType>::lowest());
-      if (is_tf) tf_names.push_back(g);
-    }
-    printf("TFs specified %zu, found %zu\n", TFs.size(), tf_names.size());
+    tool::load_tfs(tf_input, genes, tfs, tf_names);
     if (tf_names.empty()) return 0;
   }
 

@@ -52,30 +52,13 @@ int run(const cli::Parser& p) {
   std::vector<DataType> tfs;
   std::vector<std::string> tf_names;
   if (!tf_input.empty()) {
-    std::unordered_set<std::string> TFs;
-    std::ifstream ifs(tf_input);
-    if (!ifs) throw std::runtime_error("cannot open " + tf_input);
-    std::string line;
-    while (std::getline(ifs, line)) TFs.insert(mio::trim(line));
-    for (const auto& g : genes) {
-      const bool is_tf = TFs.count(mio::trim(g)) > 0;
-      tfs.push_back(is_tf ? std::numeric_limits<DataType>::max() : std::numeric_limits<DataType>::lowest());
-      if (is_tf) tf_names.push_back(g);
-    }
-    printf("TFs specified %zu, found %zu\n", TFs.size(), tf_names.size());
+    tool::load_tfs(tf_input, genes, tfs, tf_names);
     if (tf_names.empty()) return 0;
   }
   const bool tfonly = !tf_input.empty();
   MatrixType tfs_to_genes(1, input.columns());
   MatrixType& genes_to_genes = input;
-  if (tfonly) {
-    tfs_to_genes.resize(tf_names.size(), input.columns());
-    for (size_t i = 0, j = 0; i < genes.size(); ++i)
-      if (tfs[i] > 0) {
-        std::copy(input.data(i), input.data(i) + input.columns(), tfs_to_genes.data(j));
-        ++j;
-      }
-  }
+  if (tfonly) tfs_to_genes = tool::filter_rows_by_tfs(input, tfs);
 
   MXKernelType mxkernel;
   MaskedMXKernelType maskedmxkernel(tfs);

@@ -120,6 +120,82 @@ void compute_stouffer(const splash::ds::aligned_matrix<T>& in, splash::ds::align
   transformer(in, splash::kernel::GaussianParamsExclude1<T, T, true>(), mcp::kernel::stouffer_vector_kernel<T>(), out);
 }
 
+// compute_maxmin{1,2,3}_tfs (mcpnet_helper.hpp:208-232, :280-321, :366-424): the max-min product
+// of a path of k legs starting at a TF (rows of t2g), with tf_gene_transition choosing the legs
+// whose intermediate genes must be TFs (the masked kernel).
+template <typename T>
+void compute_maxmin_tfs(const splash::ds::aligned_matrix<T>& t2g, const splash::ds::aligned_matrix<T>& g2g, int k,
+                        int transition, const std::vector<T>& tfs, splash::ds::aligned_matrix<T>& dmaxmin) {
+  using Mat = splash::ds::aligned_matrix<T>;
+  using MX = mcp::kernel::mcp2_maxmin_kernel<T, false, true>;
+  using MMX = mcp::kernel::mcp2_maxmin_kernel<T, true, true>;
+  if (transition < 0 || transition > k - 1) throw std::runtime_error("tf_gene_transition out of range");
+  const MX mx;
+  const MMX mmx(tfs);
+  splash::pattern::InnerProduct<Mat, MX, Mat, true> mxgen_s;
+  splash::pattern::InnerProduct<Mat, MX, Mat, false> mxgen;
+  splash::pattern::InnerProduct<Mat, MMX, Mat, true> mmxgen_s;
+  splash::pattern::InnerProduct<Mat, MMX, Mat, false> mmxgen;
+  if (k == 2) {
+    if (transition == 1) mmxgen(t2g, g2g, mmx, dmaxmin);   // tfs-tfs-genes
+    else mxgen(t2g, g2g, mx, dmaxmin);                     // tfs-genes-genes
+  } else if (k == 3) {
+    Mat maxmin1;
+    if (transition == 2) mmxgen_s(g2g, g2g, mmx, maxmin1);
+    else mxgen_s(g2g, g2g, mx, maxmin1);
+    if (transition == 2 || transition == 1) mmxgen(t2g, maxmin1, mmx, dmaxmin);
+    else mxgen(t2g, maxmin1, mx, dmaxmin);
+  } else {
+    Mat maxmin1, maxmin2;
+    if (transition == 3) mmxgen_s(g2g, g2g, mmx, maxmin1);
+    else mxgen_s(g2g, g2g, mx, maxmin1);
+    if (transition == 0) mxgen(t2g, g2g, mx, maxmin2);
+    else mmxgen(t2g, g2g, mmx, maxmin2);
+    if (transition == 2 || transition == 3) mmxgen(maxmin2, maxmin1, mmx, dmaxmin);
+    else mxgen(maxmin2, maxmin1, mx, dmaxmin);
+  }
+}
+
+// compute_mcp{2,3,4}_tfs (mcpnet_helper.hpp:235-244, :325-333, :427-435), called by mcpnet with
+// tf_gene_transition = k - 1 (mcpnet.cpp:372-376, :421-425, :468-472)
+template <typename T>
+void compute_mcp_tfs(const splash::ds::aligned_matrix<T>& t2g, const splash::ds::aligned_matrix<T>& g2g, int k,
+                     const std::vector<T>& tfs, splash::ds::aligned_matrix<T>& dmcp) {
+  using Mat = splash::ds::aligned_matrix<T>;
+  Mat dmaxmin;
+  compute_maxmin_tfs(t2g, g2g, k, k - 1, tfs, dmaxmin);
+  splash::pattern::BinaryOp<Mat, Mat, mcp::kernel::ratio_kernel<T, T>, Mat> gen;
+  const auto part = splash::utils::partition<size_t>::make_partition(dmaxmin.rows());
+  gen(t2g, part, dmaxmin, part, mcp::kernel::ratio_kernel<T, T>(), dmcp, part);
+}
+
+// compute_maxmins_tfs (mcpnet_helper.hpp:469-505), all products masked to TF intermediates:
+// M1 = tg (x) gg (ttg), M2 = M1 (x) gg (tttg), M3 = M1 (x) (gg (x) gg) (ttttg)
+template <typename T>
+void compute_maxmins_tfs(const splash::ds::aligned_matrix<T>& t2g, const splash::ds::aligned_matrix<T>& g2g,
+                         const std::vector<T>& tfs, splash::ds::aligned_matrix<T>& m1, splash::ds::aligned_matrix<T>& m2,
+                         splash::ds::aligned_matrix<T>& m3) {
+  using Mat = splash::ds::aligned_matrix<T>;
+  using MMX = mcp::kernel::mcp2_maxmin_kernel<T, true, true>;
+  const MMX mmx(tfs);
+  splash::pattern::InnerProduct<Mat, MMX, Mat, false> mxgen;
+  splash::pattern::InnerProduct<Mat, MMX, Mat, true> mxgen_s;
+  mxgen(t2g, g2g, mmx, m1);
+  mxgen(m1, g2g, mmx, m2);
+  Mat m3a;
+  mxgen_s(g2g, g2g, mmx, m3a);
+  mxgen(m1, m3a, mmx, m3);
+}
+
+// compute_stouffer_tf (mcpnet_helper.hpp:587-638)
+template <typename T>
+void compute_stouffer_tf(const splash::ds::aligned_matrix<T>& in, splash::ds::aligned_matrix<T>& out) {
+  out.resize(in.rows(), in.columns());
+  mcpgpu::check(mcpgpu::stouffer_tf_call(mcpgpu::thread_handle(), in.data(), in.ld(), in.rows(), in.columns(),
+                                         out.data(), out.ld()),
+                "compute_stouffer_tf");
+}
+
 template <typename T>
 int run(const cli::Parser& p) {
   using Mat = splash::ds::aligned_matrix<T>;
@@ -127,10 +203,24 @@ int run(const cli::Parser& p) {
   tool::Timer t;
   Mat input = tool::load_input<T>(p, genes, samples);
   printf("Load data in %f sec\n", t.seconds());
+  // transcription factors (mcpnet.cpp:201-213): the outputs keep only the TF rows
+  const std::string tf_input = p.str("--tf-input");
+  const bool tfonly = !tf_input.empty();
+  std::vector<T> tfs;
+  std::vector<std::string> tf_names;
+  if (tfonly) {
+    tool::load_tfs<T, double>(tf_input, genes, tfs, tf_names);
+    if (tf_names.empty()) {
+      printf("ERROR: no transcription factors in the specified file");
+      return 0;
+    }
+  }
+  const std::vector<std::string>& row_names = tfonly ? tf_names : genes;
   const bool mi_input = p.seen("--mi-input");
   const int mim = (int)p.integer("--mi-method");
   // output names: <output>[_tf](_bs|_ap|_inmi).<stage>.h5 (mcpnet.cpp:265-278)
-  const std::string prefix = p.str("--output") + (mi_input ? "_inmi" : mim == 0 ? "_bs" : "_ap");
+  const std::string prefix =
+      p.str("--output") + (tfonly ? "_tf" : "") + (mi_input ? "_inmi" : mim == 0 ? "_bs" : "_ap");
   const std::string fmt = "." + p.str("--format");
   Mat mi;
   if (mi_input) {
@@ -144,9 +234,13 @@ int run(const cli::Parser& p) {
   const double diagonal = p.real("--diagonal");
   if (diagonal >= 0.0)   // mcpnet.cpp:299-308
     for (size_t i = 0; i < std::min(mi.rows(), mi.columns()); ++i) mi.at(i, i) = static_cast<T>(diagonal);
+  if (tfonly && mi.rows() != genes.size()) throw std::runtime_error("the MI matrix must have one row per gene");
+  Mat mi_tf;
+  if (tfonly) mi_tf = tool::filter_rows_by_tfs(mi, tfs);   // mcpnet.cpp:316-324
+  const Mat& mi_rows = tfonly ? mi_tf : mi;
   if (!mi_input) {   // written after the diagonal is set (mcpnet.cpp:353-356)
     t = tool::Timer();
-    mio::write_matrix(prefix + ".mi" + fmt, genes, genes, mi);
+    mio::write_matrix(prefix + ".mi" + fmt, row_names, genes, mi_rows);
     printf("Wrote,MI,,%f,sec\n", t.seconds());
   }
   std::set<int> comps;
@@ -157,9 +251,10 @@ int run(const cli::Parser& p) {
     if (m == 4) continue;
     t = tool::Timer();
     Mat dratio;
-    compute_mcp(mi, m + 1, dratio);
+    if (tfonly) compute_mcp_tfs(mi_tf, mi, m + 1, tfs, dratio);
+    else compute_mcp(mi, m + 1, dratio);
     printf("Computed,MCP%d,,%f,sec\n", m + 1, t.seconds());
-    mio::write_matrix(prefix + ".mcp" + std::to_string(m + 1) + fmt, genes, genes, dratio);
+    mio::write_matrix(prefix + ".mcp" + std::to_string(m + 1) + fmt, row_names, genes, dratio);
   }
   if (comps.count(4)) {   // ensemble MCP (mcpnet.cpp:518-700, the branch without evaluation)
     t = tool::Timer();
@@ -175,7 +270,8 @@ int run(const cli::Parser& p) {
     printf("Loaded,%zux%zu coefficients,,%f,sec\n", coeffs.rows(), coeffs.columns(), t.seconds());
     t = tool::Timer();
     Mat dmaxmin1, dmaxmin2, dmaxmin3;
-    compute_maxmins(mi, dmaxmin1, dmaxmin2, dmaxmin3);
+    if (tfonly) compute_maxmins_tfs(mi_tf, mi, tfs, dmaxmin1, dmaxmin2, dmaxmin3);
+    else compute_maxmins(mi, dmaxmin1, dmaxmin2, dmaxmin3);
     printf("Computed,MAXMIN 1 2 3,,%f,sec\n", t.seconds());
     for (size_t i = 0; i < coeffs.rows(); ++i) {
       const T* c = coeffs.data(i);
@@ -184,13 +280,14 @@ int run(const cli::Parser& p) {
       tag << std::fixed << std::setprecision(3) << c[0] << "_" << c[1] << "_" << c[2] << "_" << c[3];
       t = tool::Timer();
       Mat dratio, dratiost;
-      compute_combo(mi, dmaxmin1, dmaxmin2, dmaxmin3, c, dratio);
+      compute_combo(mi_rows, dmaxmin1, dmaxmin2, dmaxmin3, c, dratio);
       printf("Computed,Combo,(%s),%f,sec\n", tag.str().c_str(), t.seconds());
-      mio::write_matrix(prefix + ".mcp_" + tag.str() + fmt, genes, genes, dratio);
+      mio::write_matrix(prefix + ".mcp_" + tag.str() + fmt, row_names, genes, dratio);
       t = tool::Timer();
-      compute_stouffer(dratio, dratiost);
+      if (tfonly) compute_stouffer_tf(dratio, dratiost);
+      else compute_stouffer(dratio, dratiost);
       printf("Computed,Combo Stouffer,(%s),%f,sec\n", tag.str().c_str(), t.seconds());
-      mio::write_matrix(prefix + ".mcpst_" + tag.str() + fmt, genes, genes, dratiost);
+      mio::write_matrix(prefix + ".mcpst_" + tag.str() + fmt, row_names, genes, dratiost);
     }
   }
   return 0;
@@ -210,6 +307,7 @@ int main(int argc, char** argv) {
   p.option("--format", "output format: h5 (as the reference), exp, csv or tsv", true, "h5");
   p.option("--precision-code", "B-spline contraction: 0 fp32, 1 bf16, 2 bf16x3", true, "0");
   p.option("--noise-seed", "seed of the U[0,1e-5) rank jitter", true, "1296388676612");
+  p.option("--tf-input", "transcription factor file, 1 per line.", true, "");
   try {
     p.parse(argc, argv);
     if (p.str("--output").empty()) throw std::runtime_error("no output prefix: give -o PREFIX");

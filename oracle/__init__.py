@@ -48,6 +48,8 @@ _SIGS = {
     "orc_combo_f64": (None, [_P, _P, _P, _P, c_int64, _P, _P]),
     "orc_stouffer_f32": (None, [_P, c_int64, c_int, _P]),
     "orc_stouffer_f64": (None, [_P, c_int64, c_int, _P]),
+    "orc_stouffer_tf_f32": (None, [_P, c_int64, c_int64, _P]),
+    "orc_stouffer_tf_f64": (None, [_P, c_int64, c_int64, _P]),
     "orc_dpi_f32": (c_int64, [_P, c_int64, c_float, _P, _P]),
     "orc_dpi_f64": (c_int64, [_P, c_int64, c_double, _P, _P]),
 }
@@ -332,6 +334,16 @@ def stouffer(X, exclude_ones=True):
     out = np.empty_like(X)
     fn = lib().orc_stouffer_f64 if dt == np.float64 else lib().orc_stouffer_f32
     fn(X.ctypes.data, X.shape[0], int(exclude_ones), out.ctypes.data)
+    return out
+
+
+def stouffer_tf(X):
+    """compute_stouffer_tf on a TFs x genes matrix: (row z + column z) / sqrt(2)."""
+    dt = np.float64 if np.asarray(X).dtype == np.float64 else np.float32
+    X = _c(X, dt)
+    out = np.empty_like(X)
+    fn = lib().orc_stouffer_tf_f64 if dt == np.float64 else lib().orc_stouffer_tf_f32
+    fn(X.ctypes.data, X.shape[0], X.shape[1], out.ctypes.data)
     return out
 
 

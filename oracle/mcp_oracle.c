@@ -21,6 +21,9 @@
  *   MI gradient (f64)    central finite differences of orc_bspline_mi_loss_f64
  *   combo                unit coefficient rows reduce to the MCP2/3/4 ratios (orc_ratio) and to 1
  *   Stouffer             numpy mean / std (ddof 1) z-scores
+ *   Stouffer (TF)        numpy mean / std (ddof 0) z-scores of rows and columns; the ddof-0 z-scores
+ *                        give numpy's Pearson correlation as dot / count, as the reference's
+ *                        PearsonKernel computes it from the same StandardScore<T, T, false>
  *   DPI                  an independent vectorised numpy statement; the weakest edge of a
  *                        triangle is removed at tolerance 0 (ARACNE's data processing inequality)
  */
@@ -622,6 +625,48 @@ ORC_COMBO(orc_combo_f64, double, DBL_EPSILON, DBL_MAX)
   }
 ORC_STOUFFER(orc_stouffer_f32, float, FLT_EPSILON)
 ORC_STOUFFER(orc_stouffer_f64, double, DBL_EPSILON)
+
+/* compute_stouffer_tf (apps/pipeline/mcpnet_helper.hpp:587-638): StandardScore<T, T, false> of every
+ * row and every column of the M x N TFs-to-genes matrix (mean and population standard deviation in
+ * double; z = (T)((x - mean) / sd), 0 when sd < eps(T)), then zscored_stouffer_kernel
+ * (include/mcp/transform/stouffer.hpp:84-99): out = (zr + zc) * isqrt2 in T. */
+#define ORC_STOUFFER_TF(NAME, T, EPS)                                                             \
+  void NAME(const T* X, int64_t M, int64_t N, T* out) {                                           \
+    double* rm = (double*)malloc(sizeof(double) * (size_t)M);                                     \
+    double* rs = (double*)malloc(sizeof(double) * (size_t)M);                                     \
+    double* cm = (double*)malloc(sizeof(double) * (size_t)N);                                     \
+    double* cs = (double*)malloc(sizeof(double) * (size_t)N);                                     \
+    for (int64_t i = 0; i < M; ++i) {                                                             \
+      double s = 0.0, q = 0.0;                                                                    \
+      for (int64_t j = 0; j < N; ++j) s += (double)X[i * N + j];                                  \
+      const double mu = s / (double)N;                                                            \
+      for (int64_t j = 0; j < N; ++j) q += ((double)X[i * N + j] - mu) * ((double)X[i * N + j] - mu); \
+      rm[i] = mu;                                                                                 \
+      rs[i] = sqrt(q / (double)N);                                                                \
+    }                                                                                             \
+    for (int64_t j = 0; j < N; ++j) {                                                             \
+      double s = 0.0, q = 0.0;                                                                    \
+      for (int64_t i = 0; i < M; ++i) s += (double)X[i * N + j];                                  \
+      const double mu = s / (double)M;                                                            \
+      for (int64_t i = 0; i < M; ++i) q += ((double)X[i * N + j] - mu) * ((double)X[i * N + j] - mu); \
+      cm[j] = mu;                                                                                 \
+      cs[j] = sqrt(q / (double)M);                                                                \
+    }                                                                                             \
+    const T isqrt2 = (T)0.70710678118654752440084436210485L;                                      \
+    for (int64_t i = 0; i < M; ++i)                                                               \
+      for (int64_t j = 0; j < N; ++j) {                                                           \
+        const double x = (double)X[i * N + j];                                                    \
+        const T zr = rs[i] < (double)EPS ? (T)0 : (T)((x - rm[i]) / rs[i]);                       \
+        const T zc = cs[j] < (double)EPS ? (T)0 : (T)((x - cm[j]) / cs[j]);                       \
+        out[i * N + j] = (zr + zc) * isqrt2;                                                      \
+      }                                                                                           \
+    free(rm);                                                                                     \
+    free(rs);                                                                                     \
+    free(cm);                                                                                     \
+    free(cs);                                                                                     \
+  }
+ORC_STOUFFER_TF(orc_stouffer_tf_f32, float, FLT_EPSILON)
+ORC_STOUFFER_TF(orc_stouffer_tf_f64, double, DBL_EPSILON)
 
 /* DPI mask (include/mcp/filter/dpi.hpp:56-66 dpi_kernel and :86-105 dpi_tf_kernel, both with
  * EQUALITY_EXCLUSION) applied as mask<T, true> (include/mcp/filter/threshold.hpp:46-53): the

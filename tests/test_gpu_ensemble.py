@@ -1,4 +1,5 @@
-"""Ensemble MCP (compute_maxmins, compute_combo, compute_stouffer) and the DPI filter on the GPU,
+"""Ensemble MCP (compute_maxmins, compute_combo, compute_stouffer, compute_stouffer_tf) and the DPI
+filter on the GPU,
 against the CPU oracle: bit-exact for the max-min products, the combination ratio and the DPI
 decisions; the Stouffer z-scores within a few float ulps (FP64 row moments, reduction order free)."""
 import numpy as np
@@ -77,6 +78,31 @@ def test_stouffer(handle, dt, exclude_ones):
     tol = 2e-6 if dt == np.float32 else 1e-12
     assert np.allclose(got, want, rtol=tol, atol=tol)
     assert np.all(np.isfinite(got))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", [(1, 5), (23, 140), (64, 1001), (130, 64)])
+def test_stouffer_tf(handle, dt, shape):
+    M, N = shape
+    rng = np.random.default_rng(M * 7 + N)
+    R = (rng.random((M, N)) * 3).astype(dt)
+    if M > 3:
+        R[3] = 0.75                                   # a constant TF row
+    if N > 17:
+        R[:, 17] = 1.25                               # a constant gene column
+    got = handle.stouffer_tf(R)
+    want = oracle.stouffer_tf(R)
+    tol = 2e-6 if dt == np.float32 else 1e-12
+    assert np.allclose(got, want, rtol=tol, atol=tol)
+    assert np.all(np.isfinite(got))
+
+
+def test_stouffer_tf_device(handle, torch_cuda):
+    torch = torch_cuda
+    R = (np.random.default_rng(5).random((40, 333)) * 2).astype(np.float32)
+    Rd = torch.from_numpy(R).cuda()
+    got = handle.stouffer_tf(Rd).cpu().numpy()
+    assert np.allclose(got, oracle.stouffer_tf(R), rtol=2e-6, atol=2e-6)
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])

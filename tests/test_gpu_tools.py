@@ -203,3 +203,51 @@ def test_mcpnet_ensemble(tmp_path):
         gst = read_exp(f"{prefix}.mcpst_{tag}.exp")[1]
         assert np.allclose(gst, oracle.stouffer(want, True), rtol=1e-12, atol=1e-12)
     assert not Path(f"{prefix}.mcp_1.000_0.000_0.000_0.000.exp").exists()   # (1,0,0,0) is skipped
+
+
+def test_mcpnet_tf_mode(tmp_path):
+    """mcpnet --tf-input: MCP2/3/4 from the TF rows (compute_mcp{2,3,4}_tfs with transitions 1, 2, 3)
+    and the TF ensemble (compute_maxmins_tfs, combo on the TF rows, compute_stouffer_tf), in
+    double as the reference runs by default."""
+    X = grn.grn_expression(40, 256, 21)[0]
+    A = oracle.bspline_mi_f64(X)
+    names = [f"g{i}" for i in range(40)]
+    src = tmp_path / "mi.exp"
+    write_exp(src, A, names)
+    tf_idx = [2, 7, 11, 19, 33]
+    tf_file = tmp_path / "tfs.txt"
+    tf_file.write_text("\n".join(names[t] for t in tf_idx) + "\nnot_a_gene\n")
+    cf = np.array([[0.25, 0.25, 0.25, 0.25], [0.5, 0, 0.5, 0]])
+    cfile = tmp_path / "coeffs.exp"
+    write_exp(cfile, cf, ["r0", "r1"])
+    prefix = tmp_path / "net"
+    out = run("mcpnet", "-i", src, "-o", prefix, "--mi-input", "--tf-input", tf_file, "-m", "1", "2", "3", "4",
+              "-f", cfile, "--format", "exp")
+    assert "TFs specified 6, found 5" in out
+    Ain = read_exp(src)[1]
+    np.fill_diagonal(Ain, 0.0)
+    mask = np.full(40, -np.finfo(np.float64).max)
+    mask[tf_idx] = np.finfo(np.float64).max
+    T = np.ascontiguousarray(Ain[tf_idx])
+    mm = oracle.maxmin
+    G1 = mm(Ain, Ain, mask)                                  # gene-tf-gene
+    want = {
+        2: mm(T, Ain, mask),                                 # tf-tf-gene
+        3: mm(T, G1, mask),                                  # tf-tf-tf-gene
+        4: mm(mm(T, Ain, mask), G1, mask),                   # (tf-tf-gene)-tf-gene
+    }
+    base = f"{prefix}_tf_inmi"
+    for k in (2, 3, 4):
+        got_names, got = read_exp(f"{base}.mcp{k}.exp")
+        assert got_names == [names[t] for t in tf_idx]
+        assert np.array_equal(got, oracle.ratio(T, want[k])), k
+    M1 = mm(T, Ain, mask)
+    M2 = mm(M1, Ain, mask)
+    M3 = mm(M1, G1, mask)
+    for row in cf:
+        tag = "_".join(f"{v:.3f}" for v in row)
+        got = read_exp(f"{base}.mcp_{tag}.exp")[1]
+        w = oracle.combo(T, M1, M2, M3, row)
+        assert np.array_equal(got, w)
+        gst = read_exp(f"{base}.mcpst_{tag}.exp")[1]
+        assert np.allclose(gst, oracle.stouffer_tf(w), rtol=1e-12, atol=1e-12)

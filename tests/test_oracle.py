@@ -323,6 +323,29 @@ def test_stouffer_matches_numpy_zscores(exclude_ones):
     assert np.allclose(got32, want, rtol=1e-5, atol=1e-5)
 
 
+def test_stouffer_tf_matches_scipy_zscores():
+    """compute_stouffer_tf: StandardScore<T, T, false> of the TF rows and of the gene columns.
+    The `false` is the population (ddof 0) standard deviation: only with it do z-scores give the
+    Pearson correlation as dot / count, which is how the reference's PearsonKernel
+    (include/mcp/correlation/Pearson.hpp:34-36) uses the same StandardScore<T, T, false>."""
+    from scipy import stats
+    rng = np.random.default_rng(21)
+    X = rng.random((12, 40))
+    X[3] = 0.5                                       # constant TF row: its z-scores are 0
+    X[:, 17] = 0.25                                  # constant gene column: its z-scores are 0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        zr = np.nan_to_num(stats.zscore(X, axis=1, ddof=0))
+        zc = np.nan_to_num(stats.zscore(X, axis=0, ddof=0))
+    want = (zr + zc) * np.sqrt(0.5)
+    assert np.allclose(oracle.stouffer_tf(X), want, rtol=1e-12, atol=1e-12)
+    assert np.allclose(oracle.stouffer_tf(X.astype(np.float32)), want, rtol=1e-5, atol=1e-5)
+    Y = rng.random((6, 30))
+    zy = stats.zscore(Y, axis=1, ddof=0)
+    assert np.allclose(zy @ zy.T / Y.shape[1], np.corrcoef(Y), rtol=1e-12, atol=1e-12)
+    zy1 = stats.zscore(Y, axis=1, ddof=1)
+    assert not np.allclose(zy1 @ zy1.T / Y.shape[1], np.corrcoef(Y), rtol=1e-6, atol=1e-6)
+
+
 def _dpi_numpy(A, tol, tfs=None):
     """Vectorised statement of the DPI rule: edge x-z goes when some y != x, z has
     A[x][y] >= i_xz and A[z][y] >= i_xz (y restricted to TFs when exactly one endpoint is a TF)."""
