@@ -183,6 +183,30 @@ int mcpg_mcp_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N
 int mcpg_mcp_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, int k,
                  double* maxmin_out, int64_t ldm, double* mcp_out, int64_t ldo);
 
+/* The mcpnet stage chain (apps/pipeline/mcpnet.cpp:286-515 with compute_mi / compute_ap_mi, the
+ * diagonal, filter_local_mat_rows_by_tfs and compute_mcp{2,3,4}[_tfs], mcpnet_helper.hpp:56-435)
+ * in one call, every intermediate in device memory: with MCPG_HOST pointers the only transfers
+ * are X in and the requested outputs out.
+ *   mi_method: 0 = B-spline MI (bins, order, precision as mcpg_bspline_mi_f32), 1 = adaptive
+ *     partitioning (noise_seed, as mcpg_add_noise_f32 -> mcpg_rank_f32 -> mcpg_ap_mi_i32),
+ *     -1 = X already is the G x G MI matrix (S ignored, ldx >= G).
+ *   diagonal >= 0 sets the MI diagonal (mcpnet.cpp:299-308); a negative value keeps it.
+ *   tfs (nullable, length G, memory per `ptrs`): > 0 marks a TF.  The MCP outputs then hold the
+ *     T TF rows (T = number of TFs, in gene order) and order k follows compute_mcp{k}_tfs with
+ *     tf_gene_transition k - 1; the tfs values themselves are the masked kernel's kmask.
+ *   mi (nullable): the whole G x G MI after the diagonal is set (the ensemble stage needs all of
+ *     it; its TF rows are what mcpnet writes in TF mode).
+ *   mcp2 / mcp3 / mcp4 (nullable): the MCP scores of that order, G or T rows by G columns.
+ * The f64 form computes the MI in FP32-class arithmetic from FP32 values of X and widens it
+ * before the diagonal; its max-min products and ratios run in FP64, as the stage entry points'. */
+int mcpg_mcpnet_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int mi_method, int bins,
+                    int order, int precision, uint64_t noise_seed, double diagonal, const float* tfs, float* mi,
+                    int64_t ldmi, float* mcp2, int64_t ld2, float* mcp3, int64_t ld3, float* mcp4, int64_t ld4);
+int mcpg_mcpnet_f64(mcpg_handle h, int ptrs, const double* X, int64_t G, int64_t S, int64_t ldx, int mi_method,
+                    int bins, int order, int precision, uint64_t noise_seed, double diagonal, const double* tfs,
+                    double* mi, int64_t ldmi, double* mcp2, int64_t ld2, double* mcp3, int64_t ld3, double* mcp4,
+                    int64_t ld4);
+
 /* compute_maxmins (apps/pipeline/mcpnet_helper.hpp:436-460), the three products the ensemble
  * MCP combines (apps/pipeline/mcpnet.cpp:546-553): M1 = A(x)A, M2 = A(x)M1, M3 = M1(x)M1 of the
  * N x N matrix A.  Each output is written only when its pointer is non-null; M1 stays on the

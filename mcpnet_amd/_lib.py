@@ -69,6 +69,10 @@ _SIGS = {
     "mcpg_combo_f64": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, _P, c_int64],
     "mcpg_stouffer_f32": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64],
     "mcpg_stouffer_f64": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64],
+    "mcpg_mcpnet_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, c_int, c_int, c_uint64, c_double, _P,
+                        _P, c_int64, _P, c_int64, _P, c_int64, _P, c_int64],
+    "mcpg_mcpnet_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, c_int, c_int, c_uint64, c_double, _P,
+                        _P, c_int64, _P, c_int64, _P, c_int64, _P, c_int64],
     "mcpg_stouffer_tf_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_stouffer_tf_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_dpi_f32": [_P, c_int, _P, c_int64, c_int64, c_float, _P, _P, c_int64, _P, c_int64, POINTER(c_int64)],
@@ -419,6 +423,41 @@ class Handle:
         self._check(fn(self.h, a.kind, a.ptr, a.ld, N, int(k), mm.ptr if mm else None, mm.ld if mm else 0, o.ptr, o.ld),
                     "mcpg_mcp")
         return out, maxmin_out
+
+    def mcpnet(self, X, mi_method=0, orders=(2, 3, 4), want_mi=True, bins=10, order=3, precision="fp32",
+               noise_seed=1296388676612, diagonal=0.0, tfs=None):
+        """The mcpnet stage chain in one device-resident call: MI (mi_method 0 B-spline, 1 adaptive
+        partitioning, -1 X is the MI) -> diagonal -> TF rows (tfs > 0) -> MCP of each order.
+        Returns {"mi": G x G MI, 2: ..., 3: ..., 4: ...} for what was asked."""
+        dt = np.float64 if str(getattr(X, "dtype", "")).endswith("float64") else np.float32
+        x = _Mat(X, dt, "X")
+        G, S = x.shape
+        if mi_method < 0 and S != G:
+            raise ValueError("an MI input must be square")
+        tf = None
+        R = G
+        if tfs is not None:
+            tf = _Mat(tfs, dt, "tfs")
+            if tf.shape[0] != G:
+                raise ValueError("tfs needs one value per gene")
+            vals = tfs.detach().cpu().numpy() if hasattr(tfs, "detach") else np.asarray(tfs)
+            R = int(np.count_nonzero(vals > 0))
+        res = {}
+        args = []
+        for key in ("mi", 2, 3, 4):
+            want = want_mi if key == "mi" else key in orders
+            if want:
+                res[key] = _empty_like_kind(x.kind, (G if key == "mi" else R, G), dt, X)
+                m = _Mat(res[key], dt, str(key))
+                args += [m.ptr, m.ld]
+            else:
+                args += [None, 0]
+        fn = self.lib.mcpg_mcpnet_f64 if dt == np.float64 else self.lib.mcpg_mcpnet_f32
+        self._pre(x.kind)
+        self._check(fn(self.h, x.kind, x.ptr, G, S if mi_method >= 0 else 0, x.ld, int(mi_method), int(bins),
+                       int(order), PRECISIONS[precision], c_uint64(noise_seed), c_double(diagonal),
+                       tf.ptr if tf else None, *args), "mcpg_mcpnet")
+        return res
 
     # ------------------------------------------------------------------ ensemble and filters
     def maxmins(self, A, want=(True, True, True)):

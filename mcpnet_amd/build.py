@@ -19,7 +19,7 @@ LIBDIR = PKG / "lib"
 BINDIR = PKG / "bin"
 INCLUDE = ROOT / "include"
 
-HIP_SOURCES = ["context.hip", "maxmin.hip", "bspline.hip", "apmi.hip", "ensemble.hip"]
+HIP_SOURCES = ["context.hip", "maxmin.hip", "bspline.hip", "apmi.hip", "ensemble.hip", "pipeline.hip"]
 ARCH = os.environ.get("MCPG_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -131,6 +131,7 @@ int env_group(const char* name, int def) {
 }
 
 void begin_call(mcpg_ctx* h) {
+  if (h->nest > 0) return;
   h->last_launches = 0;
   h->last_kernel = "";
   h->timings.clear();

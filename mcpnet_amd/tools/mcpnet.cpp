@@ -1,71 +1,43 @@
 // mcpnet: expression -> MI (B-spline or adaptive partitioning) -> diagonal -> MCP2/3/4 scores and
 // the ensemble MCP.  Follows the MI, MCP and ensemble stages of apps/pipeline/mcpnet.cpp:159-700
-// and the compute_* helpers of apps/pipeline/mcpnet_helper.hpp:56-584, on the GPU.  Outputs are
+// and the compute_* helpers of apps/pipeline/mcpnet_helper.hpp:56-638, on the GPU: MI, diagonal
+// and MCP2/3/4 run as one device-resident call (mcpg_mcpnet_*), the ensemble through the
+// maxmins / combo / Stouffer entry points.
+// With <prefix> = <output>[_tf](_bs|_ap|_inmi) (mcpnet.cpp:265-278) the outputs are
 // <prefix>.mi.<fmt>, <prefix>.mcp{2,3,4}.<fmt> and, for -m 4, <prefix>.mcp_<a>_<b>_<c>_<d>.<fmt>
 // (combination ratio) and <prefix>.mcpst_<a>_<b>_<c>_<d>.<fmt> (its Stouffer combination) per
-// coefficient row.  The evaluation stage (-g, -x, --eval: AUPR / AUROC against a ground truth)
-// is not part of this build.
+// coefficient row; <fmt> is h5 unless --format says otherwise.  With --tf-input every output
+// keeps only the TF rows: the MCP products start at a TF and pass through TFs
+// (compute_mcp{2,3,4}_tfs, compute_maxmins_tfs) and the ensemble's Stouffer step z-scores TF rows
+// and gene columns (compute_stouffer_tf).  The evaluation stage (-g, -x, --eval: AUPR / AUROC
+// against a ground truth) is not part of this build.
 #include <cstdio>
 #include <iomanip>
 #include <set>
 #include <sstream>
 
 #include "common.hpp"
-#include "mcp/correlation/AdaptivePartitioningMI.hpp"
-#include "mcp/correlation/BSplineMI.hpp"
 #include "mcp/filter/mcp.hpp"
 #include "mcp/transform/combine.hpp"
 #include "mcp/transform/stouffer.hpp"
-#include "splash/kernel/rank.hpp"
 
 namespace {
 
-// compute_mi (mcpnet_helper.hpp:56-99)
-template <typename T>
-void compute_mi(const splash::ds::aligned_matrix<T>& input, int bins, int precision, splash::ds::aligned_matrix<T>& dmi) {
-  using Mat = splash::ds::aligned_matrix<T>;
-  splash::pattern::GlobalTransform<Mat, mcp::kernel::MinMaxScale<T>, Mat> scaler;
-  Mat dscaled(input.rows(), input.columns());
-  scaler(input, mcp::kernel::MinMaxScale<T>(), dscaled);
-  const int S = (int)input.columns();
-  splash::pattern::Transform<Mat, mcp::kernel::BSplineWeightsKernel<T>, Mat> bspline_weights;
-  Mat weighted(dscaled.rows(), (size_t)S * bins + 1);
-  bspline_weights(dscaled, mcp::kernel::BSplineWeightsKernel<T>(bins, 3, S), weighted);
-  splash::pattern::InnerProduct<Mat, mcp::kernel::BSplineMIKernel<T>, Mat, true> bspline_mi;
-  bspline_mi(weighted, weighted, mcp::kernel::BSplineMIKernel<T>(bins, S, precision), dmi);
+// MI -> diagonal -> TF rows -> MCP2/3/4 in one device-resident call, in place of compute_mi /
+// compute_ap_mi (mcpnet_helper.hpp:56-167), the diagonal loop (mcpnet.cpp:299-308),
+// filter_local_mat_rows_by_tfs and compute_mcp{2,3,4}[_tfs] (mcpnet_helper.hpp:188-435): the
+// MI and the max-min intermediates never cross the bus.
+inline int mcpnet_call(mcpg_handle h, const float* x, int64_t G, int64_t S, int64_t ldx, int mim, int precision,
+                       uint64_t seed, double diagonal, const float* tfs, float* mi, int64_t ldmi, float* const* o,
+                       const int64_t* ldo) {
+  return mcpg_mcpnet_f32(h, MCPG_HOST, x, G, S, ldx, mim, 10, 3, precision, seed, diagonal, tfs, mi, ldmi, o[0], ldo[0],
+                         o[1], ldo[1], o[2], ldo[2]);
 }
-
-// compute_ap_mi (mcpnet_helper.hpp:102-167), with noise
-template <typename T>
-void compute_ap_mi(const splash::ds::aligned_matrix<T>& input, uint64_t seed, splash::ds::aligned_matrix<T>& dmi) {
-  using Mat = splash::ds::aligned_matrix<T>;
-  Mat noisy(input.rows(), input.columns());
-  mcp::kernel::add_noise<T> noise_adder(seed);
-  noise_adder(input, noisy);
-  using RankKernelType = splash::kernel::Rank<T, size_t, 0, false>;
-  splash::pattern::Transform<Mat, RankKernelType, splash::ds::aligned_matrix<size_t>> normalizer;
-  splash::ds::aligned_matrix<size_t> ranked(noisy.rows(), noisy.columns());
-  normalizer(noisy, RankKernelType(), ranked);
-  using MIKernelType = mcp::kernel::AdaptivePartitionRankMIKernel2<size_t, T, 0>;
-  splash::pattern::InnerProduct<splash::ds::aligned_matrix<size_t>, MIKernelType, Mat, true> adaptive_mi;
-  adaptive_mi(ranked, ranked, MIKernelType(), dmi);
-}
-
-// compute_mcp{2,3,4} (mcpnet_helper.hpp:188-205, :247-277, :335-364)
-template <typename T>
-void compute_mcp(const splash::ds::aligned_matrix<T>& g2g, int k, splash::ds::aligned_matrix<T>& dmcp) {
-  using Mat = splash::ds::aligned_matrix<T>;
-  using MXKernelType = mcp::kernel::mcp2_maxmin_kernel<T, false, true>;
-  splash::pattern::InnerProduct<Mat, MXKernelType, Mat, true> mxgen_s;
-  splash::pattern::InnerProduct<Mat, MXKernelType, Mat, false> mxgen;
-  Mat maxmin1, dmaxmin;
-  mxgen_s(g2g, g2g, MXKernelType(), maxmin1);
-  if (k == 2) dmaxmin = maxmin1;
-  else if (k == 3) mxgen(g2g, maxmin1, MXKernelType(), dmaxmin);
-  else mxgen_s(maxmin1, maxmin1, MXKernelType(), dmaxmin);
-  splash::pattern::BinaryOp<Mat, Mat, mcp::kernel::ratio_kernel<T, T>, Mat> gen;
-  const auto part = splash::utils::partition<size_t>::make_partition(dmaxmin.rows());
-  gen(g2g, part, dmaxmin, part, mcp::kernel::ratio_kernel<T, T>(), dmcp, part);
+inline int mcpnet_call(mcpg_handle h, const double* x, int64_t G, int64_t S, int64_t ldx, int mim, int precision,
+                       uint64_t seed, double diagonal, const double* tfs, double* mi, int64_t ldmi, double* const* o,
+                       const int64_t* ldo) {
+  return mcpg_mcpnet_f64(h, MCPG_HOST, x, G, S, ldx, mim, 10, 3, precision, seed, diagonal, tfs, mi, ldmi, o[0], ldo[0],
+                         o[1], ldo[1], o[2], ldo[2]);
 }
 
 // compute_maxmins (mcpnet_helper.hpp:436-460): M1 = A(x)A, M2 = A(x)M1, M3 = M1(x)M1
@@ -120,55 +92,6 @@ void compute_stouffer(const splash::ds::aligned_matrix<T>& in, splash::ds::align
   transformer(in, splash::kernel::GaussianParamsExclude1<T, T, true>(), mcp::kernel::stouffer_vector_kernel<T>(), out);
 }
 
-// compute_maxmin{1,2,3}_tfs (mcpnet_helper.hpp:208-232, :280-321, :366-424): the max-min product
-// of a path of k legs starting at a TF (rows of t2g), with tf_gene_transition choosing the legs
-// whose intermediate genes must be TFs (the masked kernel).
-template <typename T>
-void compute_maxmin_tfs(const splash::ds::aligned_matrix<T>& t2g, const splash::ds::aligned_matrix<T>& g2g, int k,
-                        int transition, const std::vector<T>& tfs, splash::ds::aligned_matrix<T>& dmaxmin) {
-  using Mat = splash::ds::aligned_matrix<T>;
-  using MX = mcp::kernel::mcp2_maxmin_kernel<T, false, true>;
-  using MMX = mcp::kernel::mcp2_maxmin_kernel<T, true, true>;
-  if (transition < 0 || transition > k - 1) throw std::runtime_error("tf_gene_transition out of range");
-  const MX mx;
-  const MMX mmx(tfs);
-  splash::pattern::InnerProduct<Mat, MX, Mat, true> mxgen_s;
-  splash::pattern::InnerProduct<Mat, MX, Mat, false> mxgen;
-  splash::pattern::InnerProduct<Mat, MMX, Mat, true> mmxgen_s;
-  splash::pattern::InnerProduct<Mat, MMX, Mat, false> mmxgen;
-  if (k == 2) {
-    if (transition == 1) mmxgen(t2g, g2g, mmx, dmaxmin);   // tfs-tfs-genes
-    else mxgen(t2g, g2g, mx, dmaxmin);                     // tfs-genes-genes
-  } else if (k == 3) {
-    Mat maxmin1;
-    if (transition == 2) mmxgen_s(g2g, g2g, mmx, maxmin1);
-    else mxgen_s(g2g, g2g, mx, maxmin1);
-    if (transition == 2 || transition == 1) mmxgen(t2g, maxmin1, mmx, dmaxmin);
-    else mxgen(t2g, maxmin1, mx, dmaxmin);
-  } else {
-    Mat maxmin1, maxmin2;
-    if (transition == 3) mmxgen_s(g2g, g2g, mmx, maxmin1);
-    else mxgen_s(g2g, g2g, mx, maxmin1);
-    if (transition == 0) mxgen(t2g, g2g, mx, maxmin2);
-    else mmxgen(t2g, g2g, mmx, maxmin2);
-    if (transition == 2 || transition == 3) mmxgen(maxmin2, maxmin1, mmx, dmaxmin);
-    else mxgen(maxmin2, maxmin1, mx, dmaxmin);
-  }
-}
-
-// compute_mcp{2,3,4}_tfs (mcpnet_helper.hpp:235-244, :325-333, :427-435), called by mcpnet with
-// tf_gene_transition = k - 1 (mcpnet.cpp:372-376, :421-425, :468-472)
-template <typename T>
-void compute_mcp_tfs(const splash::ds::aligned_matrix<T>& t2g, const splash::ds::aligned_matrix<T>& g2g, int k,
-                     const std::vector<T>& tfs, splash::ds::aligned_matrix<T>& dmcp) {
-  using Mat = splash::ds::aligned_matrix<T>;
-  Mat dmaxmin;
-  compute_maxmin_tfs(t2g, g2g, k, k - 1, tfs, dmaxmin);
-  splash::pattern::BinaryOp<Mat, Mat, mcp::kernel::ratio_kernel<T, T>, Mat> gen;
-  const auto part = splash::utils::partition<size_t>::make_partition(dmaxmin.rows());
-  gen(t2g, part, dmaxmin, part, mcp::kernel::ratio_kernel<T, T>(), dmcp, part);
-}
-
 // compute_maxmins_tfs (mcpnet_helper.hpp:469-505), all products masked to TF intermediates:
 // M1 = tg (x) gg (ttg), M2 = M1 (x) gg (tttg), M3 = M1 (x) (gg (x) gg) (ttttg)
 template <typename T>
@@ -218,44 +141,44 @@ int run(const cli::Parser& p) {
   const std::vector<std::string>& row_names = tfonly ? tf_names : genes;
   const bool mi_input = p.seen("--mi-input");
   const int mim = (int)p.integer("--mi-method");
+  if (mim < 0 || mim > 1) throw std::runtime_error("--mi-method must be 0 (B-spline) or 1 (adaptive partitioning)");
   // output names: <output>[_tf](_bs|_ap|_inmi).<stage>.h5 (mcpnet.cpp:265-278)
   const std::string prefix =
       p.str("--output") + (tfonly ? "_tf" : "") + (mi_input ? "_inmi" : mim == 0 ? "_bs" : "_ap");
   const std::string fmt = "." + p.str("--format");
-  Mat mi;
-  if (mi_input) {
-    mi = input;
-  } else {
-    t = tool::Timer();
-    if (mim == 0) compute_mi(input, 10, (int)p.integer("--precision-code"), mi);
-    else compute_ap_mi(input, (uint64_t)p.integer("--noise-seed"), mi);
-    printf("Computed,MI,,%f,sec\n", t.seconds());
-  }
-  const double diagonal = p.real("--diagonal");
-  if (diagonal >= 0.0)   // mcpnet.cpp:299-308
-    for (size_t i = 0; i < std::min(mi.rows(), mi.columns()); ++i) mi.at(i, i) = static_cast<T>(diagonal);
-  if (tfonly && mi.rows() != genes.size()) throw std::runtime_error("the MI matrix must have one row per gene");
-  Mat mi_tf;
-  if (tfonly) mi_tf = tool::filter_rows_by_tfs(mi, tfs);   // mcpnet.cpp:316-324
-  const Mat& mi_rows = tfonly ? mi_tf : mi;
-  if (!mi_input) {   // written after the diagonal is set (mcpnet.cpp:353-356)
-    t = tool::Timer();
-    mio::write_matrix(prefix + ".mi" + fmt, row_names, genes, mi_rows);
-    printf("Wrote,MI,,%f,sec\n", t.seconds());
-  }
+  const size_t G = input.rows();
+  if (mi_input && input.columns() != G) throw std::runtime_error("an MI input must be square");
+  if (tfonly && G != genes.size()) throw std::runtime_error("the input must have one row per gene");
   std::set<int> comps;
   for (const auto& m : p.all("--method")) comps.insert(std::stoi(m));
   if (comps.empty()) comps.insert(2);
-  for (int m : comps) {
+  for (int m : comps)
     if (m < 1 || m > 4) throw std::runtime_error("-m accepts 1 (MCP2), 2 (MCP3), 3 (MCP4), 4 (ensemble MCP)");
-    if (m == 4) continue;
-    t = tool::Timer();
-    Mat dratio;
-    if (tfonly) compute_mcp_tfs(mi_tf, mi, m + 1, tfs, dratio);
-    else compute_mcp(mi, m + 1, dratio);
-    printf("Computed,MCP%d,,%f,sec\n", m + 1, t.seconds());
-    mio::write_matrix(prefix + ".mcp" + std::to_string(m + 1) + fmt, row_names, genes, dratio);
-  }
+
+  t = tool::Timer();
+  Mat mi(G, G);
+  Mat dratio[3];
+  T* outs[3] = {nullptr, nullptr, nullptr};
+  int64_t ldo[3] = {0, 0, 0};
+  for (int m = 1; m <= 3; ++m)
+    if (comps.count(m)) {
+      dratio[m - 1].resize(tfonly ? tf_names.size() : G, G);
+      outs[m - 1] = dratio[m - 1].data();
+      ldo[m - 1] = dratio[m - 1].ld();
+    }
+  mcpgpu::check(mcpnet_call(mcpgpu::thread_handle(), input.data(), G, mi_input ? 0 : input.columns(), input.ld(),
+                            mi_input ? -1 : mim, (int)p.integer("--precision-code"), (uint64_t)p.integer("--noise-seed"),
+                            p.real("--diagonal"), tfonly ? tfs.data() : nullptr, mi.data(), mi.ld(), outs, ldo),
+                "mcpnet pipeline");
+  printf("Computed,MI and MCP,,%f,sec\n", t.seconds());
+  Mat mi_tf;
+  if (tfonly) mi_tf = tool::filter_rows_by_tfs(mi, tfs);   // mcpnet.cpp:316-324
+  const Mat& mi_rows = tfonly ? mi_tf : mi;
+  t = tool::Timer();
+  if (!mi_input) mio::write_matrix(prefix + ".mi" + fmt, row_names, genes, mi_rows);   // mcpnet.cpp:353-356
+  for (int m = 1; m <= 3; ++m)
+    if (comps.count(m)) mio::write_matrix(prefix + ".mcp" + std::to_string(m + 1) + fmt, row_names, genes, dratio[m - 1]);
+  printf("Wrote,MI and MCP,,%f,sec\n", t.seconds());
   if (comps.count(4)) {   // ensemble MCP (mcpnet.cpp:518-700, the branch without evaluation)
     t = tool::Timer();
     Mat coeffs;

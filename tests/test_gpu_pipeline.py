@@ -1,0 +1,85 @@
+"""mcpg_mcpnet_*: the device-resident mcpnet stage chain (MI -> diagonal -> TF rows -> MCP2/3/4)
+must give exactly what the stage entry points give one by one, and, for the TF orders, what the
+oracle's masked max-min compositions give (compute_mcp{2,3,4}_tfs, mcpnet_helper.hpp:208-435)."""
+import numpy as np
+import pytest
+
+import oracle
+from mcpnet_amd import MCPGError, grn
+
+pytestmark = pytest.mark.gpu
+
+
+def bits_equal(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_bspline_chain_matches_stages(handle, precision):
+    X = grn.grn_expression(150, 320, 31)[0]
+    res = handle.mcpnet(X, mi_method=0, precision=precision)
+    MI = handle.bspline_mi(X, precision=precision)
+    np.fill_diagonal(MI, 0.0)
+    assert bits_equal(res["mi"], MI)
+    for k in (2, 3, 4):
+        assert bits_equal(res[k], handle.mcp(MI, k)[0]), k
+
+
+def test_ap_chain_matches_stages(handle):
+    X = grn.grn_expression(70, 256, 32)[0]
+    res = handle.mcpnet(X, mi_method=1, noise_seed=99, orders=(3,), diagonal=-1.0)
+    MI = handle.ap_mi(handle.rank(handle.add_noise(X, 99)))   # diagonal kept (negative --diagonal)
+    assert bits_equal(res["mi"], MI)
+    assert bits_equal(res[3], handle.mcp(MI, 3)[0])
+    assert 2 not in res and 4 not in res
+
+
+def test_double_chain_widens_the_mi(handle):
+    X = grn.grn_expression(60, 200, 33)[0].astype(np.float64)
+    res = handle.mcpnet(X, mi_method=0, diagonal=0.5)
+    MI = handle.bspline_mi(X.astype(np.float32)).astype(np.float64)
+    np.fill_diagonal(MI, 0.5)
+    assert res["mi"].dtype == np.float64 and bits_equal(res["mi"], MI)
+    for k in (2, 3, 4):
+        assert bits_equal(res[k], oracle.mcp(MI, k)[0]), k
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_tf_orders_match_oracle(handle, dt):
+    X = grn.grn_expression(48, 256, 34)[0]
+    A = oracle.bspline_mi_f64(X).astype(dt)
+    tf_idx = [1, 4, 9, 22, 40]
+    with np.errstate(over="ignore"):                          # as mcpnet's load_tfs: double limits in T
+        tfs = np.full(48, np.finfo(np.float64).min).astype(dt)
+        tfs[tf_idx] = np.finfo(np.float64).max
+    res = handle.mcpnet(A, mi_method=-1, tfs=tfs)
+    Ad = A.copy()
+    np.fill_diagonal(Ad, 0)
+    T = np.ascontiguousarray(Ad[tf_idx])
+    assert bits_equal(res["mi"], Ad)
+    mm = oracle.maxmin
+    G1 = mm(Ad, Ad, tfs)
+    want = {2: mm(T, Ad, tfs), 3: mm(T, G1, tfs), 4: mm(mm(T, Ad, tfs), G1, tfs)}
+    for k in (2, 3, 4):
+        assert bits_equal(res[k], oracle.ratio(T, want[k])), k
+
+
+def test_device_tensors_match_host(handle, torch_cuda):
+    torch = torch_cuda
+    X = grn.grn_expression(90, 300, 35)[0]
+    tfs = np.full(90, -np.inf, np.float32)
+    tfs[[0, 5, 17, 60]] = np.inf
+    host = handle.mcpnet(X, mi_method=0, tfs=tfs)
+    dev = handle.mcpnet(torch.from_numpy(X).cuda(), mi_method=0, tfs=torch.from_numpy(tfs).cuda())
+    for key in ("mi", 2, 3, 4):
+        assert bits_equal(dev[key].cpu().numpy(), host[key]), key
+
+
+def test_errors(handle):
+    A = np.random.default_rng(1).random((8, 8)).astype(np.float32)
+    with pytest.raises(MCPGError):
+        handle.mcpnet(A, mi_method=-1, tfs=np.full(8, -1.0, np.float32))   # no TF selected
+    with pytest.raises(MCPGError):
+        handle.mcpnet(A, mi_method=2)
