@@ -376,8 +376,10 @@ struct MIArgs {
 //   translates by 4 and by 32 (measured on MI355X).  The MFMA operand read (lane -> row lane & 15,
 //   chunk lane >> 4) then covers all 64 banks in every pass.  The planes are filled by
 //   global_load_lds_dwordx4, which writes 64 lanes x 16 B contiguously into LDS, so the swizzle is
-//   applied to the global source addresses.  Two stages (K steps) are in flight: the loads of step
-//   k + 1 run under the MFMAs of step k.
+//   applied to the global source addresses.  The planes of NST K steps sit in a ring of stage
+//   buffers; with early release every wave copies its fragments of step k into registers and a
+//   barrier hands the buffer back, so the loads of steps k + 1 .. k + NST run under the MFMAs of
+//   step k.
 constexpr int kLdsF32 = 36;
 constexpr int kPanelF32 = kTileRows * kLdsF32;        // floats per FP32 panel
 constexpr int kEpiStride = 81;                        // FP32 kernel t-tile row stride (floats)
@@ -395,10 +397,26 @@ template <int P>
 constexpr int mi_planes() { return P == kBF16X3 ? 2 : 1; }
 template <int P>
 constexpr int mi_stage_bytes() { return 2 * mi_planes<P>() * kPlaneB16; }   // panels A and B
-template <int P>
+// NST: K steps held in LDS (one being computed, NST - 1 in flight)
+template <int P, int NST>
 constexpr int mi_smem_bytes() {
-  constexpr int body = (P == kFP32) ? 2 * kPanelF32 * 4 : 2 * mi_stage_bytes<P>();
+  constexpr int body = (P == kFP32) ? 2 * kPanelF32 * 4 : NST * mi_stage_bytes<P>();
   return body > kEpiBytes ? body : kEpiBytes;
+}
+
+// Waits until at most n of this wave's vector-memory loads are outstanding (n wave-uniform;
+// s_waitcnt takes an immediate, so n is rounded down to one of the counts the MI kernel uses).
+__device__ __forceinline__ void wait_vm_at_most(int n) {
+  if (n >= 20) asm volatile("s_waitcnt vmcnt(20)" ::: "memory");
+  else if (n >= 15) asm volatile("s_waitcnt vmcnt(15)" ::: "memory");
+  else if (n >= 10) asm volatile("s_waitcnt vmcnt(10)" ::: "memory");
+  else if (n >= 9) asm volatile("s_waitcnt vmcnt(9)" ::: "memory");
+  else if (n >= 6) asm volatile("s_waitcnt vmcnt(6)" ::: "memory");
+  else if (n >= 5) asm volatile("s_waitcnt vmcnt(5)" ::: "memory");
+  else if (n >= 3) asm volatile("s_waitcnt vmcnt(3)" ::: "memory");
+  else if (n >= 2) asm volatile("s_waitcnt vmcnt(2)" ::: "memory");
+  else if (n >= 1) asm volatile("s_waitcnt vmcnt(1)" ::: "memory");
+  else asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 }
 
 // FP32: loads the K-step [k0, k0+32) of one 160-row panel into registers (16-byte chunks, 8 per
@@ -446,10 +464,25 @@ __device__ __forceinline__ void dma_issue(__amdgpu_buffer_rsrc_t rsrc, uint32_t 
   }
 }
 
+// Chunks j0 and j0 + 1 of the `count` chunks dma_issue would issue.
+__device__ __forceinline__ void dma_issue_part(__amdgpu_buffer_rsrc_t rsrc, uint32_t lbyte, uint32_t sbase,
+                                               uint32_t grp_stride, int count, int c0, int j0, char* dst) {
+#pragma unroll
+  for (int jj = 0; jj < 2; ++jj) {
+    const int j = j0 + jj;
+    if (j < count) {
+      const int c = c0 + j;
+      const uint32_t soff = sbase + (uint32_t)(c / 5) * grp_stride + (uint32_t)(c % 5) * (kDmaRows * kKStep * 2);
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr)(dst + j * 1024), 16, lbyte, soff, 0, 0);
+    }
+  }
+}
+
 // All-pairs MI: 160 x 160 workgroup tile (16 x 16 genes for 10 bins), four waves of 80 x 80
-// (5 x 5 blocks of 16 x 16 MFMA outputs), two workgroups per CU.  FP32 operands are staged through
-// registers into padded LDS panels; BF16 planes arrive through the two-stage direct-load pipeline.
-template <int P>
+// (5 x 5 blocks of 16 x 16 MFMA outputs).  FP32 operands are staged through registers into padded
+// LDS panels (two workgroups per CU); BF16 planes arrive through an NST-stage ring of direct loads
+// (NST - 1 K steps in flight while one is computed).
+template <int P, int NST, bool ER>
 __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
   extern __shared__ __align__(16) char smem[];
 
@@ -535,32 +568,58 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
     // byte offset of this lane's MFMA fragment within a 16-row block of a swizzled plane
     const int frag = lr * kRowB16 + ((lq ^ ((lr & 8) ? 3 : 0)) << 4);
 
-    dma_issue(rsrc, lbyte, sbase0, grp_stride, count, c0, smem + dst_off);
+    // A ring of NST stage buffers.  Without early release (ER) a buffer is read fragment by
+    // fragment during its step's MFMAs, so NST - 1 steps are in flight.  With ER every wave copies
+    // all of its step's fragments into registers first and a second barrier frees the buffer for
+    // the loads of step kk + NST, so all NST buffers are in flight during the MFMAs.
+    constexpr int kAhead = ER ? NST : NST - 1;   // steps issued before step 0 is computed
+#pragma unroll
+    for (int s = 0; s < kAhead; ++s)
+      if (s < nk) dma_issue(rsrc, lbyte, sbase0 + (uint32_t)s * kstep_bytes, grp_stride, count, c0, smem + s * kStage + dst_off);
     for (int64_t kk = 0; kk < nk; ++kk) {
-      // this wave's loads of step kk have landed; after the barrier, everyone's have, and every
-      // wave is done reading the other stage (step kk - 1), which the next loads overwrite
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-      __syncthreads();
-      const char* st = smem + (kk & 1) * kStage;
-      if (kk + 1 < nk)
-        dma_issue(rsrc, lbyte, sbase0 + (uint32_t)(kk + 1) * kstep_bytes, grp_stride, count, c0,
-                  smem + ((kk + 1) & 1) * kStage + dst_off);
+      // this wave's loads of step kk have landed once only the loads of the later steps already
+      // issued (at most kAhead - 1) are outstanding; after the barrier everyone's have
+      const int64_t later = nk - 1 - kk < kAhead - 1 ? nk - 1 - kk : kAhead - 1;
+      wait_vm_at_most((int)later * count);
+      // a bare barrier: __syncthreads()'s release fence would wait for every load in flight
+      asm volatile("s_barrier" ::: "memory");
+      const char* st = smem + (kk % NST) * kStage;
+      // The loads of a later step are issued two chunks at a time between the MFMA block rows
+      // below, so that a wave never waits long on a full load queue with MFMA work at hand.
+      // Without ER they fill the buffer of step kk - 1, which every wave finished before the
+      // barrier; with ER the buffer of step kk, once the second barrier has freed it.
+      const int64_t next = ER ? kk + NST : kk + NST - 1;
+      const uint32_t next_src = sbase0 + (uint32_t)next * kstep_bytes;
+      char* const next_dst = smem + (ER ? kk % NST : (kk + NST - 1) % NST) * kStage + dst_off;
+      const int next_count = next < nk ? count : 0;
       const char* sA = st;
       const char* sB = diag ? st : st + np * kPlaneB16;
-      // B fragments stay in registers for the whole K step; A fragments are read per block row.
-      bf16x8 bh[5], bl[5];
+      // B fragments stay in registers for the whole K step; A fragments are read per block row
+      // (with ER: all of them up front).
+      bf16x8 bh[5], bl[5], ahs[5], als[5];
 #pragma unroll
       for (int j = 0; j < 5; ++j) {
         const int o = (wc * 80 + j * 16) * kRowB16 + frag;
         bh[j] = *reinterpret_cast<const bf16x8*>(sB + o);
         if (P == kBF16X3) bl[j] = *reinterpret_cast<const bf16x8*>(sB + kPlaneB16 + o);
       }
+      if (ER) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+          const int o = (wr * 80 + i * 16) * kRowB16 + frag;
+          ahs[i] = *reinterpret_cast<const bf16x8*>(sA + o);
+          if (P == kBF16X3) als[i] = *reinterpret_cast<const bf16x8*>(sA + kPlaneB16 + o);
+        }
+        // every wave's reads of this buffer are complete after the barrier
+        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+      }
 #pragma unroll
       for (int i = 0; i < 5; ++i) {
+        dma_issue_part(rsrc, lbyte, next_src, grp_stride, next_count, c0, 2 * i, next_dst);
         const int o = (wr * 80 + i * 16) * kRowB16 + frag;
-        const bf16x8 ah = *reinterpret_cast<const bf16x8*>(sA + o);
+        const bf16x8 ah = ER ? ahs[i] : *reinterpret_cast<const bf16x8*>(sA + o);
         if (P == kBF16X3) {
-          const bf16x8 al = *reinterpret_cast<const bf16x8*>(sA + kPlaneB16 + o);
+          const bf16x8 al = ER ? als[i] : *reinterpret_cast<const bf16x8*>(sA + kPlaneB16 + o);
 #pragma unroll
           for (int j = 0; j < 5; ++j) {
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
@@ -816,35 +875,37 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
   int st;
   hipError_t e;
   if (L.tr != kTileRows) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: weight layout does not match the MI kernel");
-  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 16), mcpg::env_group("MCPG_MI_XCD", 0), &st);
+  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 32), mcpg::env_group("MCPG_MI_XCD", 0), &st);
   if (!a.order) return st;
   const unsigned grid = (unsigned)a.ntri;
+  // BF16 planes: a ring of 2 (BF16X3, 80 KB of LDS) or 4 (BF16, 80 KB) stage buffers with early
+  // release, two workgroups per CU; MCPG_MI_ER=0 selects the 2-stage ring without it
+  const bool er = mcpg::env_group("MCPG_MI_ER", 1) != 0;
   static bool lds_attr_set = false;
   if (!lds_attr_set) {
-    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_kernel<kFP32>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, mi_smem_bytes<kFP32>()));
-    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_kernel<kBF16>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, mi_smem_bytes<kBF16>()));
-    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_kernel<kBF16X3>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, mi_smem_bytes<kBF16X3>()));
+#define MI_ATTR(P, N, E)                                                                                  \
+  MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_kernel<P, N, E>),            \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, mi_smem_bytes<P, N>()))
+    MI_ATTR(kFP32, 2, false);
+    MI_ATTR(kBF16, 2, false);
+    MI_ATTR(kBF16, 4, true);
+    MI_ATTR(kBF16X3, 2, false);
+    MI_ATTR(kBF16X3, 2, true);
+#undef MI_ATTR
     lds_attr_set = true;
   }
   {
     const char* name = precision == MCPG_FP32 ? "bspline_mi_kernel<fp32>"
                        : precision == MCPG_BF16 ? "bspline_mi_kernel<bf16>" : "bspline_mi_kernel<bf16x3>";
     mcpg::LaunchScope ls(h, name);
-    switch (precision) {
-      case MCPG_FP32:
-        hipLaunchKernelGGL(bspline_mi_kernel<kFP32>, dim3(grid), dim3(kThreads), mi_smem_bytes<kFP32>(), h->stream, a);
-        break;
-      case MCPG_BF16:
-        hipLaunchKernelGGL(bspline_mi_kernel<kBF16>, dim3(grid), dim3(kThreads), mi_smem_bytes<kBF16>(), h->stream, a);
-        break;
-      default:
-        hipLaunchKernelGGL(bspline_mi_kernel<kBF16X3>, dim3(grid), dim3(kThreads), mi_smem_bytes<kBF16X3>(), h->stream,
-                           a);
-        break;
-    }
+#define MI_GO(P, N, E) \
+  hipLaunchKernelGGL((bspline_mi_kernel<P, N, E>), dim3(grid), dim3(kThreads), (mi_smem_bytes<P, N>()), h->stream, a)
+    if (precision == MCPG_FP32) MI_GO(kFP32, 2, false);
+    else if (precision == MCPG_BF16 && er) MI_GO(kBF16, 4, true);
+    else if (precision == MCPG_BF16) MI_GO(kBF16, 2, false);
+    else if (er) MI_GO(kBF16X3, 2, true);
+    else MI_GO(kBF16X3, 2, false);
+#undef MI_GO
   }
   e = hipGetLastError();
   if (e != hipSuccess) return mcpg::hip_error(h, e, "bspline_mi_kernel launch");

@@ -51,7 +51,7 @@ def _short(name: str) -> str:
             if key == "maxmin_key_kernel":
                 return key + "<f32>"
             if key == "bspline_mi_kernel":
-                return key + {"<0>": "<fp32>", "<1>": "<bf16>", "<2>": "<bf16x3>"}.get(tmpl[:3], "")
+                return key + {"0": "<fp32>", "1": "<bf16>", "2": "<bf16x3>"}.get(tmpl[1:2], "")
             return key
     return name[:60]
 
