@@ -2,7 +2,9 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -46,10 +48,84 @@ void* workspace(mcpg_ctx* h, size_t bytes, int* status) {
   return h->ws;
 }
 
+// Pageable host memory moves far below the link rate when the runtime stages it itself.  Large
+// copies therefore go through two pinned buffers: the CPU packs (or unpacks) one chunk of rows,
+// split over a few threads, while the DMA engine moves the other.
+constexpr size_t kPinBytes = size_t(32) << 20;
+constexpr size_t kPinMin = size_t(8) << 20;
+
+static void copy_rows_parallel(char* dst, size_t dpitch, const char* src, size_t spitch, size_t width, size_t rows) {
+  const size_t bytes = width * rows;
+  const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, bytes >> 22));   // one thread per 4 MB, at most 8
+  auto part = [&](int t) {
+    const size_t r0 = rows * t / nt, r1 = rows * (t + 1) / nt;
+    if (dpitch == width && spitch == width) {
+      std::memcpy(dst + r0 * width, src + r0 * width, (r1 - r0) * width);
+    } else {
+      for (size_t r = r0; r < r1; ++r) std::memcpy(dst + r * dpitch, src + r * spitch, width);
+    }
+  };
+  if (nt == 1) { part(0); return; }
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(part, t);
+  part(0);
+  for (auto& x : th) x.join();
+}
+
+static int ensure_pinned(mcpg_ctx* h) {
+  for (int b = 0; b < 2; ++b) {
+    if (!h->pin[b]) MCPG_HIP(h, hipHostMalloc(&h->pin[b], kPinBytes, hipHostMallocDefault));
+    if (!h->pin_ev[b]) MCPG_HIP(h, hipEventCreateWithFlags(&h->pin_ev[b], hipEventDisableTiming));
+  }
+  return MCPG_OK;
+}
+
 int copy_2d(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
             size_t rows, hipMemcpyKind kind) {
   if (rows == 0 || width_bytes == 0) return MCPG_OK;
-  MCPG_HIP(h, hipMemcpy2DAsync(dst, dpitch, src, spitch, width_bytes, rows, kind, h->stream));
+  const bool h2d = kind == hipMemcpyHostToDevice, d2h = kind == hipMemcpyDeviceToHost;
+  if ((!h2d && !d2h) || width_bytes * rows < kPinMin || width_bytes > kPinBytes) {
+    MCPG_HIP(h, hipMemcpy2DAsync(dst, dpitch, src, spitch, width_bytes, rows, kind, h->stream));
+    return MCPG_OK;
+  }
+  int st;
+  if ((st = ensure_pinned(h))) return st;
+  const size_t per = kPinBytes / width_bytes;          // rows per chunk
+  const size_t nchunks = (rows + per - 1) / per;
+  char* d = static_cast<char*>(dst);
+  const char* s = static_cast<const char*>(src);
+  if (h2d) {
+    for (size_t c = 0; c < nchunks; ++c) {
+      const int b = (int)(c & 1);
+      const size_t r0 = c * per, n = std::min(per, rows - r0);
+      if (h->pin_busy[b]) MCPG_HIP(h, hipEventSynchronize(h->pin_ev[b]));
+      copy_rows_parallel(static_cast<char*>(h->pin[b]), width_bytes, s + r0 * spitch, spitch, width_bytes, n);
+      MCPG_HIP(h, hipMemcpy2DAsync(d + r0 * dpitch, dpitch, h->pin[b], width_bytes, width_bytes, n,
+                                   hipMemcpyHostToDevice, h->stream));
+      MCPG_HIP(h, hipEventRecord(h->pin_ev[b], h->stream));
+      h->pin_busy[b] = true;
+    }
+    return MCPG_OK;
+  }
+  // device -> host: chunk c + 1 is in flight while the CPU unpacks chunk c
+  auto enqueue = [&](size_t c) -> int {
+    const int b = (int)(c & 1);
+    const size_t r0 = c * per, n = std::min(per, rows - r0);
+    MCPG_HIP(h, hipMemcpy2DAsync(h->pin[b], width_bytes, s + r0 * spitch, spitch, width_bytes, n,
+                                 hipMemcpyDeviceToHost, h->stream));
+    MCPG_HIP(h, hipEventRecord(h->pin_ev[b], h->stream));
+    h->pin_busy[b] = true;
+    return MCPG_OK;
+  };
+  if ((st = enqueue(0))) return st;
+  for (size_t c = 0; c < nchunks; ++c) {
+    if (c + 1 < nchunks && (st = enqueue(c + 1))) return st;
+    const int b = (int)(c & 1);
+    const size_t r0 = c * per, n = std::min(per, rows - r0);
+    MCPG_HIP(h, hipEventSynchronize(h->pin_ev[b]));
+    copy_rows_parallel(d + r0 * dpitch, dpitch, static_cast<const char*>(h->pin[b]), width_bytes, width_bytes, n);
+    h->pin_busy[b] = false;
+  }
   return MCPG_OK;
 }
 
@@ -191,6 +267,10 @@ int mcpg_destroy(mcpg_handle h) {
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
+  for (int b = 0; b < 2; ++b) {
+    if (h->pin_ev[b]) (void)hipEventDestroy(h->pin_ev[b]);
+    if (h->pin[b]) (void)hipHostFree(h->pin[b]);
+  }
   if (h->ws) (void)hipFree(h->ws);
   for (auto& o : h->orders) (void)hipFree(o.d);
   if (h->d_flags) (void)hipFree(h->d_flags);

@@ -38,6 +38,10 @@ struct mcpg_ctx {
   // > 0 while a composite entry point (mcpg_mcpnet_*) runs its stages through other entry
   // points: the stages then keep the composite call's launch log and timings
   int nest = 0;
+  // two pinned staging buffers for large host <-> device copies (see mcpg::copy_2d)
+  void* pin[2] = {nullptr, nullptr};
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  bool pin_busy[2] = {false, false};
 };
 
 namespace mcpg {
@@ -71,7 +75,9 @@ struct Sizer {
   void add(size_t count) { off = align_up(off + count * sizeof(T)); }
 };
 
-// 2-D strided copies between host/device buffers on the handle's stream.
+// 2-D strided copies between host/device buffers on the handle's stream.  Host <-> device copies
+// of more than a few MB are staged through the handle's pinned buffers; a device -> host copy
+// has completed when copy_2d returns.
 int copy_2d(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
             size_t rows, hipMemcpyKind kind);
 
