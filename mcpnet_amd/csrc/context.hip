@@ -267,6 +267,7 @@ int mcpg_destroy(mcpg_handle h) {
   (void)hipSetDevice(h->device);
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
+  if (h->mm_tail) (void)hipFree(h->mm_tail);
   for (int b = 0; b < 2; ++b) {
     if (h->pin_ev[b]) (void)hipEventDestroy(h->pin_ev[b]);
     if (h->pin[b]) (void)hipHostFree(h->pin[b]);

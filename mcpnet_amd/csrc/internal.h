@@ -42,6 +42,10 @@ struct mcpg_ctx {
   void* pin[2] = {nullptr, nullptr};
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
   bool pin_busy[2] = {false, false};
+  // max-min key kernel: key blocks of the split tail tiles, and resident workgroups per device
+  int* mm_tail = nullptr;
+  size_t mm_tail_bytes = 0;
+  int mm_slots[2] = {0, 0};   // K chunk 16, 32
 };
 
 namespace mcpg {

@@ -311,7 +311,32 @@ struct MKArgs {
   int tri, shared;
   int64_t nbm, nbn, ntiles;
   const uint32_t* order;       // tri mode: tile coordinates in launch order, (bm << 16) | bn
+  // split tail: tiles nfull .. ntiles - 1 run as `splits` workgroups each, over K chunks
+  // [part * kc_per, (part + 1) * kc_per), and fold their keys into tail (one 128 x 128 block of
+  // keys per tile) with atomicMax; mm_finish_tail_kernel then writes their outputs
+  int64_t nfull;
+  int splits;
+  int64_t kc_per;
+  int* tail;
 };
+
+// tile -> (bm, bn): the packed tri order, or the 8-row grouped order of a full grid
+__device__ __forceinline__ void mm_tile_coords(const MKArgs& a, int64_t t, int64_t& bm, int64_t& bn) {
+  if (a.tri) {
+    const uint32_t code = a.order[t];
+    bm = code >> 16;
+    bn = code & 0xffffu;
+  } else {
+    const int64_t group = 8;
+    const int64_t per_group = group * a.nbn;
+    const int64_t g = t / per_group;
+    const int64_t first = g * group;
+    const int64_t gsize = (a.nbm - first) < group ? (a.nbm - first) : group;
+    const int64_t in = t % per_group;
+    bm = first + in % gsize;
+    bn = in / gsize;
+  }
+}
 
 typedef __attribute__((address_space(3))) void* lds_ptr;
 constexpr int kRsrcWord3 = 0x00020000;   // raw (untyped, unswizzled) gfx9-family buffer access
@@ -362,22 +387,13 @@ __global__ __launch_bounds__(kThreads) void maxmin_key_kernel(MKArgs a) {
   using KCh = KeyChunk<KC>;
   extern __shared__ __align__(16) char smem[];
 
-  int64_t bm, bn;
-  if (a.tri) {
-    const uint32_t code = a.order[blockIdx.x];
-    bm = code >> 16;
-    bn = code & 0xffffu;
-  } else {
-    const int64_t L = blockIdx.x;
-    const int64_t group = 8;
-    const int64_t per_group = group * a.nbn;
-    const int64_t g = L / per_group;
-    const int64_t first = g * group;
-    const int64_t gsize = (a.nbm - first) < group ? (a.nbm - first) : group;
-    const int64_t in = L % per_group;
-    bm = first + in % gsize;
-    bn = in / gsize;
+  int64_t tile = blockIdx.x, part = -1;
+  if (tile >= a.nfull) {
+    part = (tile - a.nfull) % a.splits;
+    tile = a.nfull + (tile - a.nfull) / a.splits;
   }
+  int64_t bm, bn;
+  mm_tile_coords(a, tile, bm, bn);
   const int64_t row0 = bm * kBM, col0 = bn * kBN;
   const bool same = a.shared && bm == bn;
 
@@ -416,9 +432,13 @@ __global__ __launch_bounds__(kThreads) void maxmin_key_kernel(MKArgs a) {
 
 
   const int sa = KCh::swz(ty), sb = KCh::swz(tx);
-  const int64_t nk = a.Kp / KC;
-  if (nk > 0) key_dma_issue<KC>(rsrc, lbyte, sbase, rbytes, nload, q0, dst0);
-  for (int64_t kc = 0; kc < nk; ++kc) {
+  const int64_t nk_all = a.Kp / KC;
+  const int64_t kc0 = part < 0 ? 0 : part * a.kc_per;
+  const int64_t nk = part < 0 ? nk_all : (kc0 + a.kc_per < nk_all ? kc0 + a.kc_per : nk_all);
+  if (kc0 < nk)
+    key_dma_issue<KC>(rsrc, lbyte, sbase + (uint32_t)(kc0 * KCh::kRowBytes), rbytes, nload, q0,
+                      dst0 + (kc0 & 1) * KCh::kStageBytes);
+  for (int64_t kc = kc0; kc < nk; ++kc) {
     // this wave's loads of chunk kc have landed; after the barrier, everyone's have, and every
     // wave is done reading the other stage (chunk kc - 1), which the next loads overwrite
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -442,6 +462,15 @@ __global__ __launch_bounds__(kThreads) void maxmin_key_kernel(MKArgs a) {
 #pragma unroll
         for (int n = 0; n < 8; ++n) acc[m][n] = step4(acc[m][n], av[m], bv[n]);
     }
+  }
+
+  if (part >= 0) {   // one K range of a split tail tile: fold into the tile's key block
+    int* T = a.tail + (tile - a.nfull) * (int64_t)(kBM * kBN);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int n = 0; n < 8; ++n) atomicMax(T + (ty + 16 * m) * kBN + tx + 16 * n, acc[m][n]);
+    return;
   }
 
   // epilogue: float and/or key outputs, optional fused ratio, mirror for symmetric off-diagonal tiles
@@ -469,6 +498,39 @@ __global__ __launch_bounds__(kThreads) void maxmin_key_kernel(MKArgs a) {
       }
     }
   }
+}
+
+// The outputs of the split tail tiles from their folded key blocks, as the key kernel's epilogue
+// writes them.
+__global__ void mm_finish_tail_kernel(MKArgs a) {
+  const int64_t ntail = a.ntiles - a.nfull;
+  const int64_t total = ntail * (kBM * kBN);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t tt = t / (kBM * kBN), e = t % (kBM * kBN);
+    int64_t bm, bn;
+    mm_tile_coords(a, a.nfull + tt, bm, bn);
+    const int64_t i = bm * kBM + e / kBN, j = bn * kBN + e % kBN;
+    if (i >= a.M || j >= a.N) continue;
+    const bool mirror = a.tri && bm != bn;
+    const int k = a.tail[t];
+    const float v = key_f32(k);
+    if (a.C) {
+      a.C[i * a.ldc + j] = v;
+      if (mirror) a.C[j * a.ldc + i] = v;
+    }
+    if (a.Ck) {
+      a.Ck[i * a.ldck + j] = k;
+      if (mirror) a.Ck[j * a.ldck + i] = k;
+    }
+    if (a.O) {
+      a.O[i * a.ldo + j] = ratio_of<float>(a.R[i * a.ldr + j], v);
+      if (mirror) a.O[j * a.ldo + i] = ratio_of<float>(a.R[j * a.ldr + i], v);
+    }
+  }
+}
+
+__global__ void fill_i32_kernel(int* p, int64_t n, int v) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) p[t] = v;
 }
 
 template <typename T>
@@ -652,6 +714,9 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
     if (!a.order) return st;
   }
   const int kc = mcpg::env_group("MCPG_MM_KC", 16) == 32 ? 32 : 16;
+  const void* kfn = kc == 16 ? reinterpret_cast<const void*>(&maxmin_key_kernel<16>)
+                             : reinterpret_cast<const void*>(&maxmin_key_kernel<32>);
+  const int smem = 2 * (kc == 16 ? KeyChunk<16>::kStageBytes : KeyChunk<32>::kStageBytes);
   static bool attr_set = false;
   if (!attr_set) {
     MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&maxmin_key_kernel<16>),
@@ -660,17 +725,61 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 2 * KeyChunk<32>::kStageBytes));
     attr_set = true;
   }
+  // Split tail: the tiles beyond the last full round of resident workgroups would leave most
+  // of the GPU idle for a whole tile time; they run instead as several workgroups each over a
+  // range of K, folding exact partial maxima with atomicMax on the keys.
+  int& slots = h->mm_slots[kc == 16 ? 0 : 1];
+  if (slots == 0) {
+    int per_cu = 0, cus = 0;
+    MCPG_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kThreads, smem));
+    MCPG_HIP(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    slots = std::max(1, per_cu * cus);
+  }
+  const int64_t nk = a.Kp / kc;
+  a.nfull = a.ntiles;
+  a.splits = 1;
+  a.kc_per = nk;
+  a.tail = nullptr;
+  {
+    const int64_t full = a.ntiles / slots * slots, ntail = a.ntiles - full;
+    const int64_t splits = ntail > 0 ? std::min<int64_t>(slots / ntail, nk) : 1;
+    if (mcpg::env_group("MCPG_MM_SPLIT", 1) != 0 && full > 0 && splits >= 2) {
+      const size_t need = (size_t)ntail * kBM * kBN * sizeof(int);
+      if (need > h->mm_tail_bytes) {
+        MCPG_HIP(h, hipStreamSynchronize(h->stream));
+        if (h->mm_tail) (void)hipFree(h->mm_tail);
+        h->mm_tail = nullptr;
+        h->mm_tail_bytes = 0;
+        MCPG_HIP(h, hipMalloc(&h->mm_tail, need));
+        h->mm_tail_bytes = need;
+      }
+      a.nfull = full;
+      a.splits = (int)splits;
+      a.kc_per = ceil_div(nk, splits);
+      a.tail = h->mm_tail;
+      mcpg::LaunchScope ls(h, "fill_i32_kernel");
+      hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_of(ntail * kBM * kBN)), dim3(256), 0, h->stream, a.tail,
+                         ntail * kBM * kBN, (int)INT32_MIN);
+    }
+  }
+  const int64_t grid = a.nfull + (a.ntiles - a.nfull) * a.splits;
+  if (grid > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min grid too large");
   {
     mcpg::LaunchScope ls(h, tri ? "maxmin_key_kernel<f32,tri>" : "maxmin_key_kernel<f32>");
     if (kc == 16)
-      hipLaunchKernelGGL(maxmin_key_kernel<16>, dim3((unsigned)a.ntiles), dim3(kThreads), 2 * KeyChunk<16>::kStageBytes,
-                         h->stream, a);
+      hipLaunchKernelGGL(maxmin_key_kernel<16>, dim3((unsigned)grid), dim3(kThreads), smem, h->stream, a);
     else
-      hipLaunchKernelGGL(maxmin_key_kernel<32>, dim3((unsigned)a.ntiles), dim3(kThreads), 2 * KeyChunk<32>::kStageBytes,
-                         h->stream, a);
+      hipLaunchKernelGGL(maxmin_key_kernel<32>, dim3((unsigned)grid), dim3(kThreads), smem, h->stream, a);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return mcpg::hip_error(h, e, "maxmin_key_kernel launch");
+  if (a.nfull < a.ntiles) {
+    mcpg::LaunchScope ls(h, "mm_finish_tail_kernel");
+    hipLaunchKernelGGL(mm_finish_tail_kernel, dim3(grid_of((a.ntiles - a.nfull) * kBM * kBN)), dim3(256), 0, h->stream,
+                       a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return mcpg::hip_error(h, e, "mm_finish_tail_kernel launch");
+  }
   return MCPG_OK;
 }
 

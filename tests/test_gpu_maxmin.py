@@ -140,6 +140,31 @@ def test_mcp3_config1_full(handle, torch_cuda):
     assert bits_equal(out.cpu().numpy(), ref_out)
 
 
+@pytest.mark.parametrize("tri", [True, False])
+def test_split_tail_matches_unsplit(handle, monkeypatch, tri):
+    """Grids larger than one round of resident workgroups run their last tiles split over K with
+    atomicMax on the keys: bit-identical to the unsplit launch and to the oracle."""
+    rng = np.random.default_rng(7 if tri else 8)
+    N, K = (5200, 96) if tri else (3700, 96)
+    A = rng.standard_normal((N, K)).astype(np.float32)
+    A[rng.random((N, K)) < 0.01] = -np.inf
+    B = A if tri else rng.standard_normal((N, K)).astype(np.float32)
+    monkeypatch.setenv("MCPG_MM_SPLIT", "0")
+    ref = handle.maxmin(A, B, symmetric=tri)
+    monkeypatch.setenv("MCPG_MM_SPLIT", "1")
+    handle.set_profiling(True)
+    got = handle.maxmin(A, B, symmetric=tri)
+    names = [n for n, _ in handle.last_timings()]
+    handle.set_profiling(False)
+    assert "mm_finish_tail_kernel" in names          # the split path ran
+    assert bits_equal(got, ref)
+    for rows in (slice(0, 40), slice(N - 40, N)):   # the tail tiles sit at the end of the order
+        assert bits_equal(got[rows], oracle.maxmin(np.ascontiguousarray(A[rows]), B))
+    if tri:
+        M1 = handle.maxmin(A, A, symmetric=True)
+        assert bits_equal(M1, M1.T)
+
+
 def test_maxmin_rejects_bad_arguments(handle):
     import mcpnet_amd
     A = np.ones((4, 5), np.float32)
