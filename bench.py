@@ -34,6 +34,9 @@ PEAK_FP32_TFLOPS = 157.3
 # (i, k, j) triple (v_min_i32 per triple + v_max3_i32 per two triples).
 CUS, LANES, CLOCK_GHZ = 256, 64, 2.4
 PEAK_MAXMIN_GOPS = CUS * LANES * CLOCK_GHZ / 1.5
+# AP-MI point tests: one compare, one ballot and one popcount per 64 points, i.e. 3 lane
+# instructions per test of the reference's quadrant scans
+PEAK_AP_GTESTS = CUS * LANES * CLOCK_GHZ / 3.0
 
 
 def log(msg):
@@ -173,6 +176,18 @@ def bench_ap(args, G, S, seed):
         result["ranks_bit_exact"] = ok
         result["mi_max_abs_diff_vs_oracle"] = float(np.abs(got.astype(np.float64) - ref).max())
         result["speedup_vs_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+        # algorithmic work: the points the reference's quadrant scans compare, counted by the
+        # oracle on the sampled rows and scaled to all G (G + 1) / 2 evaluated pairs
+        tests = oracle.ap_tests_rows(Rh, 0, rows)
+        work = tests / (rows * G) * G * (G + 1) / 2
+        ap_ms = per.get("ap_mi_kernel")
+        if ap_ms:
+            ach = work / (ap_ms / 1e3) / 1e9
+            result["roofline"] = {"bound": "valu", "achieved": ach, "peak": PEAK_AP_GTESTS, "unit": "Gtests/s",
+                                  "frac": ach / PEAK_AP_GTESTS, "traffic": None, "mean_ms": ap_ms,
+                                  "work_per_launch": work, "kernel": "ap_mi_kernel",
+                                  "work_unit": f"point comparisons of the reference's quadrant scans "
+                                               f"({tests / (rows * G):.0f} per pair on rows 0..{rows - 1})"}
     print(json.dumps(result))
     h.close()
 

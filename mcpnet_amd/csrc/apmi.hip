@@ -20,7 +20,7 @@ using mcpg::ceil_div;
 
 constexpr int kMaxWaves = 4;       // waves per workgroup (fewer when S is large)
 constexpr int64_t kLdsBudget = 64 * 1024;
-constexpr int64_t kMaxS = kLdsBudget / 8 - 1;  // one wave: 2 x (S+1) ints of LDS
+constexpr int64_t kMaxS = kLdsBudget / 8 - 1;  // keeps the per-gene histogram and bucket kernels in 32 KB of LDS
 constexpr int kStackSlots = 2;     // stack capacity = 64 * kStackSlots windows
 
 struct Win {
@@ -90,6 +90,23 @@ __global__ __launch_bounds__(256) void cumhist_kernel(const int32_t* __restrict_
   for (int64_t i = threadIdx.x; i <= S; i += blockDim.x) H[g * ldh + i] = hsh[i];
 }
 
+// The y-side of _createSparseMatrix (AdaptivePartitioningMI.hpp:462-493) once per gene instead of
+// once per pair: P[g][cH[r] + t] = s for the t-th sample s of rank r, so that a pair's x ranks
+// grouped by y rank are X[g_x][P[g_y][k]].  Within one rank the order is the atomics' (any order
+// gives the same quadrant counts).
+__global__ __launch_bounds__(256) void ybucket_kernel(const int32_t* __restrict__ R, int64_t ldr, int64_t S,
+                                                      const int32_t* __restrict__ H, int64_t ldh,
+                                                      int32_t* __restrict__ P) {
+  extern __shared__ int cnt[];
+  const int64_t g = blockIdx.x;
+  for (int64_t i = threadIdx.x; i < S; i += blockDim.x) cnt[i] = 0;
+  __syncthreads();
+  for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
+    const int r = R[g * ldr + s];
+    P[g * S + H[g * ldh + r] + atomicAdd(&cnt[r], 1)] = (int32_t)s;
+  }
+}
+
 __device__ __forceinline__ void tri_decode(int64_t Lid, int64_t nb, int64_t* bm, int64_t* bn) {
   double a = 2.0 * (double)nb + 1.0;
   int64_t r = (int64_t)((a - sqrt(a * a - 8.0 * (double)Lid)) * 0.5);
@@ -107,6 +124,7 @@ struct APArgs {
   int64_t ldr;
   const int32_t* H;   // cumulative histograms, G x (S+1)
   int64_t ldh;
+  const int32_t* P;   // samples grouped by rank, G x S (ybucket_kernel)
   int64_t G, S;
   float* MI;
   int64_t ldmi;
@@ -122,43 +140,39 @@ struct APArgs {
 };
 
 // Quadrant counts of window w split at (mx, my) over xg[b0 .. b1), lower/upper y halves split at bm.
-__device__ __forceinline__ void count_quadrants(const int* __restrict__ xg, int b0, int bm, int b1, int x1, int mx,
-                                                int x2, int lane, int& n0, int& n1, int& n2, int& n3) {
-  n0 = n1 = n2 = n3 = 0;
+// The lower half [b0, bm) and the upper half [bm, b1) are scanned separately, 64 points at a
+// time: two ballots (left, right; lanes past the half hold -1 and match neither, since x1 >= 0)
+// and two scalar popcounts per 64 points.
+__device__ __forceinline__ void count_half(const int* __restrict__ xg, int b0, int b1, int x1, int mx, int x2, int lane,
+                                           int& nl, int& nr) {
+  nl = nr = 0;
   for (int base = b0; base < b1; base += 64) {
     const int idx = base + lane;
-    const bool valid = idx < b1;
-    const int x = valid ? xg[idx] : -1;
-    const bool inL = valid && x >= x1 && x < mx;
-    const bool inR = valid && x >= mx && x < x2;
-    const bool low = idx < bm;
-    n0 += __popcll(__ballot(inL && low));
-    n1 += __popcll(__ballot(inR && low));
-    n2 += __popcll(__ballot(inL && !low));
-    n3 += __popcll(__ballot(inR && !low));
+    const int x = idx < b1 ? xg[idx] : -1;
+    nl += __popcll(__builtin_amdgcn_ballot_w64(x >= x1 && x < mx));
+    nr += __popcll(__builtin_amdgcn_ballot_w64(x >= mx && x < x2));
   }
+}
+
+__device__ __forceinline__ void count_quadrants(const int* __restrict__ xg, int b0, int bm, int b1, int x1, int mx,
+                                                int x2, int lane, int& n0, int& n1, int& n2, int& n3) {
+  count_half(xg, b0, bm, x1, mx, x2, lane, n0, n1);
+  count_half(xg, bm, b1, x1, mx, x2, lane, n2, n3);
 }
 
 // Evaluates AdaptivePartitionRankMIKernel2::operator()(x=gx, y=gy) (AdaptivePartitioningMI.hpp:553-645).
 // Returns the FP64 value mi/S + log2(S).  If rec != nullptr, leaf (nxy, nx, ny) triples are
 // recorded in summation order (up to max_leaves) and *nleaves receives the total count.
-__device__ double ap_pair(const APArgs& a, int64_t gx, int64_t gy, int* xg, int* cnt, int lane, int32_t* rec,
-                          int32_t* nleaves) {
+__device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_t gx, int64_t gy, int* xg, int lane,
+                          int32_t* rec, int32_t* nleaves) {
   const int S = (int)a.S;
-  const int32_t* __restrict__ cHX = a.H + gx * a.ldh;
-  const int32_t* __restrict__ cHY = a.H + gy * a.ldh;
+  const int32_t* __restrict__ cHX = H + gx * a.ldh;
+  const int32_t* __restrict__ cHY = H + gy * a.ldh;
   const int32_t* __restrict__ xr = a.R + gx * a.ldr;
-  const int32_t* __restrict__ yr = a.R + gy * a.ldr;
 
-  // _createSparseMatrix (:462-493): bucket x ranks by y rank
-  for (int i = lane; i < S; i += 64) cnt[i] = 0;
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  for (int i = lane; i < S; i += 64) {
-    const int y = yr[i];
-    const int off = cHY[y] + atomicAdd(&cnt[y], 1);
-    xg[off] = xr[i];
-  }
+  // _createSparseMatrix (:462-493): x ranks grouped by y rank, through the y gene's bucket order
+  const int32_t* __restrict__ py = a.P + gy * S;
+  for (int i = lane; i < S; i += 64) xg[i] = xr[py[i]];
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
@@ -246,18 +260,19 @@ __device__ double ap_pair(const APArgs& a, int64_t gx, int64_t gy, int* xg, int*
   return mi / (double)S + log2((double)S);
 }
 
-__global__ __launch_bounds__(64 * kMaxWaves) void ap_mi_kernel(APArgs a) {
+// H is passed on its own, restrict-qualified: the histogram entries a window reads are
+// wave-uniform and, with no store that may alias them, load through the scalar cache.
+__global__ __launch_bounds__(64 * kMaxWaves) void ap_mi_kernel(APArgs a, const int32_t* __restrict__ H) {
   extern __shared__ int lds[];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int waves = blockDim.x >> 6;
-  int* xg = lds + (int64_t)w * 2 * (a.S + 1);
-  int* cnt = xg + (a.S + 1);
+  int* xg = lds + (int64_t)w * (a.S + 1);
   const int64_t total_waves = (int64_t)gridDim.x * waves;
   const int64_t wave_id = (int64_t)blockIdx.x * waves + w;
   if (a.npairs > 0) {
     for (int64_t n = wave_id; n < a.npairs; n += total_waves) {
       int32_t nl = 0;
-      const double v = ap_pair(a, a.px[n], a.py[n], xg, cnt, lane, a.cells + n * 3 * (int64_t)a.max_leaves, &nl);
+      const double v = ap_pair(a, H, a.px[n], a.py[n], xg, lane, a.cells + n * 3 * (int64_t)a.max_leaves, &nl);
       if (lane == 0) {
         a.nleaves[n] = nl;
         a.mi_out[n] = v;
@@ -269,7 +284,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void ap_mi_kernel(APArgs a) {
   for (int64_t p = wave_id; p < P; p += total_waves) {
     int64_t i, j;
     tri_decode(p, a.G, &i, &j);
-    const double v = ap_pair(a, i, j, xg, cnt, lane, nullptr, nullptr);
+    const double v = ap_pair(a, H, i, j, xg, lane, nullptr, nullptr);
     if (lane == 0) {
       const float f = (float)v;
       a.MI[i * a.ldmi + j] = f;
@@ -414,6 +429,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   mcpg::Sizer sz;
   if (ptrs == MCPG_HOST) sz.add<int32_t>(G * S);
   sz.add<int32_t>(G * (S + 1));
+  sz.add<int32_t>(G * S);
   if (!cells_mode && ptrs == MCPG_HOST) sz.add<float>(G * G);
   if (cells_mode) {
     sz.add<int32_t>(npairs); sz.add<int32_t>(npairs);
@@ -430,6 +446,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
     dR = t; lr = S;
   }
   int32_t* dH = cv.take<int32_t>(G * (S + 1));
+  int32_t* dP = cv.take<int32_t>(G * S);
   float* dMI = MI;
   int64_t lm = ldmi;
   if (!cells_mode && ptrs == MCPG_HOST) { dMI = cv.take<float>(G * G); lm = G; }
@@ -439,10 +456,16 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
     hipLaunchKernelGGL(cumhist_kernel, dim3((unsigned)G), dim3(256), (S + 1) * sizeof(int), h->stream, dR, lr, S, dH,
                        S + 1, h->d_flags);
   }
-  // ranks index the histograms: refuse out-of-range ranks before the pair kernel reads them
+  // ranks index the histograms: refuse out-of-range ranks before the bucket and pair kernels read them
   if ((st = check_flags(h, "mcpg_ap_mi_i32"))) return st;
+  {
+    mcpg::LaunchScope ls(h, "ybucket_kernel");
+    hipLaunchKernelGGL(ybucket_kernel, dim3((unsigned)G), dim3(256), S * sizeof(int), h->stream, dR, lr, S, dH, S + 1,
+                       dP);
+  }
   APArgs a{};
-  a.R = dR; a.ldr = lr; a.H = dH; a.ldh = S + 1; a.G = G; a.S = S; a.MI = dMI; a.ldmi = lm; a.flags = h->d_flags;
+  a.R = dR; a.ldr = lr; a.H = dH; a.ldh = S + 1; a.P = dP; a.G = G; a.S = S; a.MI = dMI; a.ldmi = lm;
+  a.flags = h->d_flags;
   int32_t *d_px = nullptr, *d_py = nullptr, *d_cells = nullptr, *d_nl = nullptr;
   double* d_mi = nullptr;
   if (cells_mode) {
@@ -455,13 +478,14 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
     a.mi_out = d_mi;
   }
   const int64_t work = cells_mode ? npairs : G * (G + 1) / 2;
-  const int64_t per_wave = 2 * (S + 1) * (int64_t)sizeof(int);
+  const int64_t per_wave = (S + 1) * (int64_t)sizeof(int);
   const int waves = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxWaves, kLdsBudget / per_wave));
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, waves), 256 * 64 / waves));
   const size_t lds = (size_t)waves * per_wave;
   {
     mcpg::LaunchScope ls(h, "ap_mi_kernel");
-    hipLaunchKernelGGL(ap_mi_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds, h->stream, a);
+    hipLaunchKernelGGL(ap_mi_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds, h->stream, a,
+                       (const int32_t*)dH);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return mcpg::hip_error(h, e, "ap_mi_kernel launch");

@@ -38,6 +38,7 @@ _SIGS = {
     "orc_ap_mi_pair": (c_double, [_P, _P, c_int64, c_int64, c_int64, _P, c_int32, _P]),
     "orc_ap_mi": (None, [_P, c_int64, c_int64, _P]),
     "orc_ap_mi_rows": (None, [_P, c_int64, c_int64, c_int64, c_int64, _P]),
+    "orc_ap_tests_rows": (c_int64, [_P, c_int64, c_int64, c_int64, c_int64]),
     "orc_ap_mi_cells": (c_double, [_P, c_int64, c_int64, c_int64, c_int64, _P, c_int32, _P]),
     "orc_maxmin_f32": (None, [_P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64]),
     "orc_maxmin_f64": (None, [_P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64]),
@@ -233,6 +234,12 @@ def ap_mi(R):
     MI = np.empty((G, G), dtype=np.float32)
     lib().orc_ap_mi(R.ctypes.data, G, S, MI.ctypes.data)
     return MI
+
+
+def ap_tests_rows(R, row0, nrows):
+    """Point comparisons the reference's quadrant scans make for rows [row0, row0 + nrows) x all genes."""
+    R = _c(R, np.int32)
+    return int(lib().orc_ap_tests_rows(R.ctypes.data, R.shape[0], R.shape[1], row0, nrows))
 
 
 def ap_mi_rows(R, row0, nrows):

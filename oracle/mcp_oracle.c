@@ -398,22 +398,24 @@ void orc_cumhist_i32(const int32_t* R, int64_t G, int64_t S, int32_t* H) {
 
 typedef struct { int64_t x1, y1, x2, y2, n; int whole; } owin;
 
-/* _getNumPointsInWindow (AdaptivePartitioningMI.hpp:495-523) */
-static int64_t points_in(const int32_t* cX, const int32_t* cY, const int32_t* xg, const owin* w) {
+/* _getNumPointsInWindow (AdaptivePartitioningMI.hpp:495-523).  tests (nullable) accumulates the
+ * number of points the scan compares, the reference's unit of AP-MI work. */
+static int64_t points_in(const int32_t* cX, const int32_t* cY, const int32_t* xg, const owin* w, int64_t* tests) {
   if (w->x1 >= w->x2 || w->y1 >= w->y2) return 0;
   int64_t nx = cX[w->x2] - cX[w->x1];
   int64_t ny = cY[w->y2] - cY[w->y1];
   if (nx == 0 || ny == 0) return 0;
   int64_t beg = cY[w->y1], end = beg + ny, n = 0;
   for (int64_t i = beg; i < end; ++i) n += (xg[i] >= w->x1) && (xg[i] < w->x2);
+  if (tests) *tests += ny;
   return n;
 }
 
 /* AdaptivePartitionRankMIKernel2::operator() (AdaptivePartitioningMI.hpp:553-645) for the pair
  * (x = row gx, y = row gy) of rank matrix R (G x S) with cumulative histograms Hc (G x (S+1)).
  * Leaves (nxy, nx, ny) are recorded in summation order when cells != NULL. */
-double orc_ap_mi_pair(const int32_t* R, const int32_t* Hc, int64_t S, int64_t gx, int64_t gy, int32_t* cells,
-                      int32_t max_leaves, int32_t* nleaves) {
+static double ap_pair_impl(const int32_t* R, const int32_t* Hc, int64_t S, int64_t gx, int64_t gy, int32_t* cells,
+                           int32_t max_leaves, int32_t* nleaves, int64_t* tests) {
   const int32_t* cX = Hc + gx * (S + 1);
   const int32_t* cY = Hc + gy * (S + 1);
   const int32_t* xr = R + gx * S;
@@ -436,7 +438,7 @@ double orc_ap_mi_pair(const int32_t* R, const int32_t* Hc, int64_t S, int64_t gx
     owin w = st[sp - 1];
     int64_t mx = (w.x1 + w.x2) >> 1, my = (w.y1 + w.y2) >> 1;
     owin q[4] = {{w.x1, w.y1, mx, my, 0, 0}, {mx, w.y1, w.x2, my, 0, 0}, {w.x1, my, mx, w.y2, 0, 0}, {mx, my, w.x2, w.y2, 0, 0}};
-    for (int i = 0; i < 4; ++i) q[i].n = points_in(cX, cY, xg, &q[i]);
+    for (int i = 0; i < 4; ++i) q[i].n = points_in(cX, cY, xg, &q[i], tests);
     double e = (double)w.n / 4.0, chi = 0.0, d;
     for (int i = 0; i < 4; ++i) {
       d = (double)q[i].n - e;
@@ -474,6 +476,31 @@ double orc_ap_mi_pair(const int32_t* R, const int32_t* Hc, int64_t S, int64_t gx
   free(st); free(temp); free(xg);
   if (nleaves) *nleaves = leaf;
   return mi / (double)S + log2((double)S);
+}
+
+double orc_ap_mi_pair(const int32_t* R, const int32_t* Hc, int64_t S, int64_t gx, int64_t gy, int32_t* cells,
+                      int32_t max_leaves, int32_t* nleaves) {
+  return ap_pair_impl(R, Hc, S, gx, gy, cells, max_leaves, nleaves, NULL);
+}
+
+/* Point comparisons of the reference's quadrant scans (every _getNumPointsInWindow call that scans)
+ * over the pairs orc_ap_mi_rows evaluates for rows [row0, row0 + nrows): the algorithmic work
+ * of AP-MI. */
+int64_t orc_ap_tests_rows(const int32_t* R, int64_t G, int64_t S, int64_t row0, int64_t nrows) {
+  int32_t* Hc = (int32_t*)malloc(sizeof(int32_t) * G * (S + 1));
+  orc_cumhist_i32(R, G, S, Hc);
+  int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 8) collapse(2) reduction(+ : total)
+  for (int64_t r = 0; r < nrows; ++r)
+    for (int64_t j = 0; j < G; ++j) {
+      const int64_t i = row0 + r;
+      int64_t t = 0;
+      if (i <= j) ap_pair_impl(R, Hc, S, i, j, NULL, 0, NULL, &t);
+      else ap_pair_impl(R, Hc, S, j, i, NULL, 0, NULL, &t);
+      total += t;
+    }
+  free(Hc);
+  return total;
 }
 
 /* InnerProduct<..., AdaptivePartitionRankMIKernel2, ..., true>: i <= j evaluated as (x=i, y=j),

@@ -129,9 +129,11 @@ def test_ap_identical_ranks_closed_form(S):
     assert cells[:, 0].sum() == S
 
 
-def ap_python(R, x, y):
+def ap_python(R, x, y, work=None):
     """Independent recursive restatement of AdaptivePartitionRankMIKernel2 (depth-first, children
-    in order 0..3 visited last-first), returning leaves in summation order and the MI."""
+    in order 0..3 visited last-first), returning leaves in summation order and the MI.  work
+    (a one-element list) accumulates the points a reference scan compares: ny for every quadrant
+    count of a non-degenerate window with nx, ny > 0 (AdaptivePartitioningMI.hpp:495-523)."""
     S = R.shape[1]
     cnt = np.zeros((R.shape[0], S + 1), np.int64)
     for g in range(R.shape[0]):
@@ -142,6 +144,10 @@ def ap_python(R, x, y):
     leaves = []
 
     def count(x1, y1, x2, y2):
+        if work is not None and x1 < x2 and y1 < y2:
+            nx, ny = int(cH[x, x2] - cH[x, x1]), int(cH[y, y2] - cH[y, y1])
+            if nx and ny:
+                work[0] += ny
         return sum(1 for a, b in pts if x1 <= a < x2 and y1 <= b < y2)
 
     def term(w):
@@ -186,6 +192,20 @@ def test_ap_matches_independent_restatement(seed):
         ref_cells, ref_v = ap_python(R, x, y)
         np.testing.assert_array_equal(cells, ref_cells)
         assert abs(v - ref_v) < 1e-12
+
+
+def test_ap_work_counter_matches_restatement():
+    rng = np.random.default_rng(17)
+    S = 48
+    R = np.stack([rng.permutation(S) for _ in range(5)]).astype(np.int32)
+    R[3] = np.argsort(R[1] + rng.integers(0, 5, S)).astype(np.int32)
+    want = 0
+    for i in (1, 2):
+        for j in range(5):
+            w = [0]
+            ap_python(R, min(i, j), max(i, j), w)
+            want += w[0]
+    assert oracle.ap_tests_rows(R, 1, 2) == want > 0
 
 
 def test_cumhist_identity_for_permutations():
