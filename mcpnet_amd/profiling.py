@@ -7,6 +7,10 @@
                                            [--copy-mib N] [--title T] [--command C]
       condenses rocprofv3 output (kernel_stats.csv, counter_collection.csv) into a short
       markdown table: calls and mean duration per kernel, and HBM KiB per dispatch.
+  python -m mcpnet_amd.profiling traffic --pmc-fetch DIR --pmc-write DIR [--pmc-*-copy DIR]
+                                         [--config N --update FILE]
+      scaled HBM bytes per dispatch (FETCH + WRITE) per kernel, printed as JSON or merged into
+      FILE under the key config<N> (bench.py reads profiles/traffic.json that way).
 """
 from __future__ import annotations
 
@@ -53,7 +57,12 @@ def _short(name: str) -> str:
             if key == "bspline_mi_kernel":
                 return key + {"0": "<fp32>", "1": "<bf16>", "2": "<bf16x3>"}.get(tmpl[1:2], "")
             return key
-    return name[:60]
+    if name.startswith("void at::") or name.startswith("at::"):
+        return name[:60]
+    # other kernels of this library: drop the return type, namespaces and the argument list
+    base = name[5:] if name.startswith("void ") else name
+    base = base.replace("(anonymous namespace)::", "").split("(")[0].split("<")[0]
+    return base.rsplit("::", 1)[-1][:60]
 
 
 def _counter(d: str, counter: str):
@@ -130,6 +139,8 @@ def main(argv=None) -> None:
         s.add_argument("--copy-mib", type=int, default=1024)
         s.add_argument("--title", default="rocprofv3 summary")
         s.add_argument("--command", default="")
+        s.add_argument("--config", type=int, help="traffic: workload number the values are filed under")
+        s.add_argument("--update", help="traffic: JSON file to merge the values into, keyed config<N>")
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_main(a.mib)
@@ -137,7 +148,19 @@ def main(argv=None) -> None:
         sys.stdout.write(summarize(a.trace, a.pmc_fetch, a.pmc_write, a.pmc_fetch_copy, a.pmc_write_copy, a.copy_mib,
                                    a.title, a.command) + "\n")
     else:
-        print(json.dumps(traffic(a.pmc_fetch, a.pmc_write, a.pmc_fetch_copy, a.pmc_write_copy, a.copy_mib), indent=1))
+        t = traffic(a.pmc_fetch, a.pmc_write, a.pmc_fetch_copy, a.pmc_write_copy, a.copy_mib)
+        if a.update and a.config is not None:
+            try:
+                with open(a.update) as f:
+                    doc = json.load(f)
+            except (OSError, ValueError):
+                doc = {}
+            doc[f"config{a.config}"] = t
+            with open(a.update, "w") as f:
+                json.dump(doc, f, indent=1)
+                f.write("\n")
+        else:
+            print(json.dumps(t, indent=1))
 
 
 if __name__ == "__main__":
