@@ -162,6 +162,20 @@ struct Layout {
   __host__ __device__ int64_t row_of(int64_t g, int b) const {
     return (g / gw) * kGroupRows + (g % gw) * bins + b;
   }
+  // For 10 bins (8 genes per 80-row group) the BF16 planes of the MI kernels with the register
+  // epilogue are stored twice, in two row orders inside each group.  Row order R (operand A, the
+  // output rows) puts genes 2q and 2q + 1 at the rows 16i + 4q + r (i < 5, r < 4) that MFMA
+  // output lanes 16q .. 16q + 15 hold; column order C (operand B) puts bins 5h .. 5h + 4 of gene c
+  // at the columns 16j + c + 8h (j < 5) that lane c + 8h holds.  A lane then holds all its
+  // values of two gene pairs, half of each pair's table, the other half in lane ^ 8.
+  __host__ __device__ int64_t row_of_r(int64_t g, int b) const {
+    const int gl = (int)(g % 8), m = 10 * (gl & 1) + b;
+    return (g / 8) * kGroupRows + 16 * (m / 4) + 4 * (gl >> 1) + (m & 3);
+  }
+  __host__ __device__ int64_t row_of_c(int64_t g, int b) const {
+    const int gl = (int)(g % 8);
+    return (g / 8) * kGroupRows + 16 * (b % 5) + gl + 8 * (b / 5);
+  }
   // Element (row, sample) of a K-blocked panel plane: each 80-row group stores its samples in
   // blocks of 32, so one group's K step is one contiguous 80 x 32 slice.
   __host__ __device__ int64_t at(int64_t row, int64_t s) const {
@@ -202,6 +216,8 @@ struct PrepOut {
   float* w32;          // group-tiled FP32 weights (nullable)
   uint16_t* whi;       // group-tiled BF16 hi (nullable)
   uint16_t* wlo;       // group-tiled BF16 lo (nullable)
+  uint16_t* whi_c;     // non-null: whi / wlo in row order R, these two in row order C (10 bins)
+  uint16_t* wlo_c;
   float* wref;         // reference layout G x ldw (nullable)
   int64_t ldw;
   double* H64;         // marginal entropy per gene (nullable)
@@ -215,7 +231,8 @@ struct PrepOut {
 
 // One workgroup per gene: min-max scaling (MinMaxScale, BSplineMI.hpp:81-115), basis weights
 // (BSplineWeightsKernel::operator(), :248-262) and the FP64 marginal entropy (entropy1d_2, :204-224).
-template <int NB, int NO>
+// TWO: BF16 planes in the row orders R and C (o.whi_c, 10 bins) instead of Layout::row_of.
+template <int NB, int NO, bool TWO>
 __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __restrict__ X, int64_t ldx, Layout L,
                                                                 Spline sp, int do_scale, PrepOut o) {
   const int64_t g = blockIdx.x;
@@ -285,20 +302,40 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
 #pragma unroll
   for (int b = 0; b < NB; ++b) acc[b] = 0.0;
   float w[NB];
+  // TWO: element (row, s) of a plane is sbase(s) + 32 * (row within the gene's 80-row group),
+  // and that row depends on b and on the gene's place in its group only (Layout::row_of_r,
+  // row_of_c); 32-bit offsets, as the planes are kept under 4 GiB.
+  const int gl = (int)(g % 8);
+  const uint32_t grp_base = (uint32_t)((g / 8) * (L.Sp / kKStep));
   for (int64_t s = tid; s < S; s += kThreads) {
     const float t = do_scale ? scale * (x[s] - offset) : x[s];
     if (o.xs) o.xs[g * S + s] = t;
     basis_values<NB, NO>(sp, t, w, nullptr);
+    const uint32_t sbase = (grp_base + (uint32_t)(s / kKStep)) * (kGroupRows * kKStep) + (uint32_t)(s % kKStep);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       if (b < bins) {
         acc[b] += (double)w[b];
-        const int64_t e = L.at(L.row_of(g, b), s);
-        if (o.w32) o.w32[e] = w[b];
-        if (o.whi) {
+        if constexpr (TWO) {
           const uint16_t hi = f32_to_bf16_rne(w[b]);
-          o.whi[e] = hi;
-          if (o.wlo) o.wlo[e] = f32_to_bf16_rne(w[b] - bf16_to_f32(hi));
+          const int m = 10 * (gl & 1) + b;
+          const uint32_t er = sbase + (uint32_t)(16 * (m >> 2) + 4 * (gl >> 1) + (m & 3)) * kKStep;
+          const uint32_t ec = sbase + (uint32_t)(16 * (b % 5) + gl + 8 * (b / 5)) * kKStep;
+          o.whi[er] = hi;
+          o.whi_c[ec] = hi;
+          if (o.wlo) {
+            const uint16_t lo = f32_to_bf16_rne(w[b] - bf16_to_f32(hi));
+            o.wlo[er] = lo;
+            o.wlo_c[ec] = lo;
+          }
+        } else {
+          const int64_t e = L.at(L.row_of(g, b), s);
+          if (o.w32) o.w32[e] = w[b];
+          if (o.whi) {
+            const uint16_t hi = f32_to_bf16_rne(w[b]);
+            o.whi[e] = hi;
+            if (o.wlo) o.wlo[e] = f32_to_bf16_rne(w[b] - bf16_to_f32(hi));
+          }
         }
         if (o.wref) o.wref[g * o.ldw + (int64_t)b * S + s] = w[b];
       }
@@ -330,7 +367,7 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
 
 // Reference-layout weight rows -> group-tiled panels (+ FP64 copy of the stored marginal entropy).
 __global__ void weights_import_kernel(const float* __restrict__ W, int64_t ldw, Layout L, float* w32, uint16_t* whi,
-                                      uint16_t* wlo, double* H64, int* flags) {
+                                      uint16_t* wlo, uint16_t* whi_c, uint16_t* wlo_c, double* H64, int* flags) {
   const int64_t g = blockIdx.y;
   const int64_t total = (int64_t)L.bins * L.S;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= total; e += (int64_t)gridDim.x * blockDim.x) {
@@ -346,8 +383,16 @@ __global__ void weights_import_kernel(const float* __restrict__ W, int64_t ldw, 
     if (w32) w32[ei] = v;
     if (whi) {
       const uint16_t hi = f32_to_bf16_rne(v);
-      whi[ei] = hi;
-      if (wlo) wlo[ei] = f32_to_bf16_rne(v - bf16_to_f32(hi));
+      const uint16_t lo = f32_to_bf16_rne(v - bf16_to_f32(hi));
+      if (whi_c) {   // row orders R and C (see Layout::row_of_r)
+        const int64_t er = L.at(L.row_of_r(g, b), s), ec = L.at(L.row_of_c(g, b), s);
+        whi[er] = hi;
+        whi_c[ec] = hi;
+        if (wlo) { wlo[er] = lo; wlo_c[ec] = lo; }
+      } else {
+        whi[ei] = hi;
+        if (wlo) wlo[ei] = lo;
+      }
     }
   }
 }
@@ -359,6 +404,8 @@ struct MIArgs {
   const float* w32;
   const uint16_t* whi;
   const uint16_t* wlo;
+  const uint16_t* whi_c;   // operand-B planes in row order C (register-epilogue kernels)
+  const uint16_t* wlo_c;
   const double* H64;
   float* MI;
   int64_t ldmi;
@@ -481,8 +528,9 @@ __device__ __forceinline__ void dma_issue_part(__amdgpu_buffer_rsrc_t rsrc, uint
 // All-pairs MI: 160 x 160 workgroup tile (16 x 16 genes for 10 bins), four waves of 80 x 80
 // (5 x 5 blocks of 16 x 16 MFMA outputs).  FP32 operands are staged through registers into padded
 // LDS panels (two workgroups per CU); BF16 planes arrive through an NST-stage ring of direct loads
-// (NST - 1 K steps in flight while one is computed).
-template <int P, int NST, bool ER>
+// (NST - 1 K steps in flight while one is computed).  RE (10 bins, BF16 planes in row orders R
+// and C, see Layout::row_of_r): the epilogue sums each gene pair in registers.
+template <int P, int NST, bool ER, bool RE>
 __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
   extern __shared__ __align__(16) char smem[];
 
@@ -550,7 +598,9 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
     // tiles) are split evenly over the four waves; each wave takes a contiguous run of one
     // slice's 16-row chunks.
     const int swave = __builtin_amdgcn_readfirstlane(wave);
-    const int nslices = (diag ? 1 : 2) * np;                   // 1, 2 or 4
+    // a diagonal tile reads one panel for both operands, unless they come in two row orders
+    const bool one_panel = diag && !RE;
+    const int nslices = (one_panel ? 1 : 2) * np;              // 1, 2 or 4
     const int wps = 4 / nslices;                               // waves per slice
     const int slice = swave / wps;
     const int nper = (kDmaPerPlane + wps - 1) / wps;
@@ -559,8 +609,10 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
     const int lrow = lane >> 2;
     const uint32_t lbyte = (lrow * kKStep + ((((lane & 3) ^ ((lrow & 8) ? 3 : 0))) << 3)) * 2;
     const uint32_t plane_bytes = (uint32_t)(a.L.rows * a.L.Sp * 2);
-    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
-        (void*)((P == kBF16X3 && (slice % np) == 1) ? a.wlo : a.whi), (short)0, plane_bytes, kRsrcWord3);
+    const bool lo_plane = P == kBF16X3 && (slice % np) == 1;
+    const uint16_t* plane = (RE && slice / np == 1) ? (lo_plane ? a.wlo_c : a.whi_c) : (lo_plane ? a.wlo : a.whi);
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void*)plane, (short)0, plane_bytes, kRsrcWord3);
     const uint32_t grp_stride = (uint32_t)(nk * kGroupRows * kKStep * 2);      // bytes between groups
     const uint32_t kstep_bytes = (uint32_t)(kGroupRows * kKStep * 2);
     const uint32_t sbase0 = (uint32_t)(((slice / np) == 0 ? rowA : rowB) / kGroupRows) * grp_stride;
@@ -593,7 +645,7 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
       char* const next_dst = smem + (ER ? kk % NST : (kk + NST - 1) % NST) * kStage + dst_off;
       const int next_count = next < nk ? count : 0;
       const char* sA = st;
-      const char* sB = diag ? st : st + np * kPlaneB16;
+      const char* sB = one_panel ? st : st + np * kPlaneB16;
       // B fragments stay in registers for the whole K step; A fragments are read per block row
       // (with ER: all of them up front).
       bf16x8 bh[5], bl[5], ahs[5], als[5];
@@ -633,6 +685,39 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
     __syncthreads();
   }
 
+  if constexpr (RE) {
+    // ---- register epilogue: lane (16q + c + 8h) holds rows of genes 2q, 2q + 1 (row order R)
+    // against bins 5h .. 5h + 4 of gene c (row order C); t = p log2 p summed per gene pair, the
+    // two column halves joined across lanes c and c + 8
+    float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float part = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+          const float p = acc[i][j][r] * a.norm;
+          part += p > 0.0f ? p * log2f(p) : 0.0f;
+        }
+        if (4 * i + r < 10) s0 += part;   // row 4i + r of the lane's 20: gene 2q for the first 10
+        else s1 += part;
+      }
+    s0 += __shfl_xor(s0, 8);
+    s1 += __shfl_xor(s1, 8);
+    if (lr < 8) {
+      const int64_t gj = tj * a.L.gt + wc * 8 + lr;
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int64_t gi = ti * a.L.gt + wr * 8 + 2 * lq + hh;
+        if (gi >= a.L.G || gj >= a.L.G || (diag && gi > gj)) continue;
+        const float mi = (float)((a.H64[gi] + a.H64[gj]) + (double)(hh ? s1 : s0));
+        a.MI[gi * a.ldmi + gj] = mi;
+        if (gi != gj) a.MI[gj * a.ldmi + gi] = mi;
+      }
+    }
+    return;
+  }
   // ---- epilogue: t = p log2 p per joint-table entry, then one gene pair per lane
   float* E = reinterpret_cast<float*>(smem);
   const int bins = a.L.bins, gw = a.L.gw;
@@ -851,21 +936,36 @@ int check_spline_args(mcpg_ctx* h, int bins, int order) {
 int prep_launch(mcpg_ctx* h, const float* dX, int64_t ldx, const Layout& L, const Spline& sp, int do_scale,
                 const PrepOut& o) {
   mcpg::LaunchScope ls(h, "bspline_prep_kernel");
-  if (sp.bins <= 16 && sp.order <= 4)
-    hipLaunchKernelGGL((bspline_prep_kernel<16, 4>), dim3((unsigned)L.G), dim3(kThreads), 0, h->stream, dX, ldx, L, sp,
-                       do_scale, o);
-  else
-    hipLaunchKernelGGL((bspline_prep_kernel<kMaxBins, kMaxOrder>), dim3((unsigned)L.G), dim3(kThreads), 0, h->stream, dX,
-                       ldx, L, sp, do_scale, o);
+  if (o.whi_c && (L.bins != 10 || L.gw != 8 || (uint64_t)L.rows * (uint64_t)L.Sp * 2 >= (1ull << 32)))
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: two-order planes need 10 bins and < 4 GiB");
+#define PREP_GO(NB, NO, TWO)                                                                                 \
+  hipLaunchKernelGGL((bspline_prep_kernel<NB, NO, TWO>), dim3((unsigned)L.G), dim3(kThreads), 0, h->stream, dX, \
+                     ldx, L, sp, do_scale, o)
+  const bool small = sp.bins <= 16 && sp.order <= 4;
+  if (o.whi_c) {
+    if (small) PREP_GO(16, 4, true);
+    else PREP_GO(kMaxBins, kMaxOrder, true);
+  } else {
+    if (small) PREP_GO(16, 4, false);
+    else PREP_GO(kMaxBins, kMaxOrder, false);
+  }
+#undef PREP_GO
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return mcpg::hip_error(h, e, "bspline_prep_kernel launch");
   return MCPG_OK;
 }
 
+// BF16 planes in the two row orders of Layout::row_of_r / row_of_c (whi_c non-null) select the
+// kernels with the register epilogue.
 int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi, const uint16_t* wlo,
-              const double* H64, const Layout& L, float* MI, int64_t ldmi) {
+              const uint16_t* whi_c, const uint16_t* wlo_c, const double* H64, const Layout& L, float* MI,
+              int64_t ldmi) {
   MIArgs a;
-  a.w32 = w32; a.whi = whi; a.wlo = wlo; a.H64 = H64; a.MI = MI; a.ldmi = ldmi; a.L = L;
+  a.w32 = w32; a.whi = whi; a.wlo = wlo; a.whi_c = whi_c; a.wlo_c = wlo_c;
+  a.H64 = H64; a.MI = MI; a.ldmi = ldmi; a.L = L;
+  const bool re = whi_c != nullptr;
+  if (re && (precision == MCPG_FP32 || L.bins != 10))
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: two-order planes need 10 bins and BF16");
   a.ntri = L.ntiles * (L.ntiles + 1) / 2;
   a.norm = 1.0f / (float)L.S;
   if (a.ntri > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "MI grid too large");
@@ -883,14 +983,16 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
   const bool er = mcpg::env_group("MCPG_MI_ER", 1) != 0;
   static bool lds_attr_set = false;
   if (!lds_attr_set) {
-#define MI_ATTR(P, N, E)                                                                                  \
-  MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_kernel<P, N, E>),            \
+#define MI_ATTR(P, N, E, R)                                                                               \
+  MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_kernel<P, N, E, R>),         \
                                   hipFuncAttributeMaxDynamicSharedMemorySize, mi_smem_bytes<P, N>()))
-    MI_ATTR(kFP32, 2, false);
-    MI_ATTR(kBF16, 2, false);
-    MI_ATTR(kBF16, 4, true);
-    MI_ATTR(kBF16X3, 2, false);
-    MI_ATTR(kBF16X3, 2, true);
+    MI_ATTR(kFP32, 2, false, false);
+    MI_ATTR(kBF16, 2, false, false);
+    MI_ATTR(kBF16, 4, true, false);
+    MI_ATTR(kBF16, 4, true, true);
+    MI_ATTR(kBF16X3, 2, false, false);
+    MI_ATTR(kBF16X3, 2, true, false);
+    MI_ATTR(kBF16X3, 2, true, true);
 #undef MI_ATTR
     lds_attr_set = true;
   }
@@ -898,13 +1000,17 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
     const char* name = precision == MCPG_FP32 ? "bspline_mi_kernel<fp32>"
                        : precision == MCPG_BF16 ? "bspline_mi_kernel<bf16>" : "bspline_mi_kernel<bf16x3>";
     mcpg::LaunchScope ls(h, name);
-#define MI_GO(P, N, E) \
-  hipLaunchKernelGGL((bspline_mi_kernel<P, N, E>), dim3(grid), dim3(kThreads), (mi_smem_bytes<P, N>()), h->stream, a)
-    if (precision == MCPG_FP32) MI_GO(kFP32, 2, false);
-    else if (precision == MCPG_BF16 && er) MI_GO(kBF16, 4, true);
-    else if (precision == MCPG_BF16) MI_GO(kBF16, 2, false);
-    else if (er) MI_GO(kBF16X3, 2, true);
-    else MI_GO(kBF16X3, 2, false);
+#define MI_GO(P, N, E, R)                                                                                 \
+  hipLaunchKernelGGL((bspline_mi_kernel<P, N, E, R>), dim3(grid), dim3(kThreads), (mi_smem_bytes<P, N>()), \
+                     h->stream, a)
+    if (re && !er) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: two-order planes need early release");
+    if (precision == MCPG_FP32) MI_GO(kFP32, 2, false, false);
+    else if (precision == MCPG_BF16 && re) MI_GO(kBF16, 4, true, true);
+    else if (precision == MCPG_BF16 && er) MI_GO(kBF16, 4, true, false);
+    else if (precision == MCPG_BF16) MI_GO(kBF16, 2, false, false);
+    else if (re) MI_GO(kBF16X3, 2, true, true);
+    else if (er) MI_GO(kBF16X3, 2, true, false);
+    else MI_GO(kBF16X3, 2, false, false);
 #undef MI_GO
   }
   e = hipGetLastError();
@@ -916,13 +1022,25 @@ struct WeightBufs {
   float* w32 = nullptr;
   uint16_t* whi = nullptr;
   uint16_t* wlo = nullptr;
+  uint16_t* whi_c = nullptr;   // row order C copies (two_orders)
+  uint16_t* wlo_c = nullptr;
   double* H64 = nullptr;
 };
 
+// BF16 MI kernels with early release and 10 bins read their planes in two row orders and sum
+// gene pairs in registers (MCPG_MI_ER=0 or MCPG_MI_RE=0 select the one-order planes).
+bool two_orders(int precision, const Layout& L) {
+  return precision != MCPG_FP32 && L.bins == 10 && L.gw == 8 &&
+         (uint64_t)L.rows * (uint64_t)L.Sp * 2 < (1ull << 32) && mcpg::env_group("MCPG_MI_ER", 1) != 0 &&
+         mcpg::env_group("MCPG_MI_RE", 1) != 0;
+}
+
 void size_weight_bufs(mcpg::Sizer& sz, const Layout& L, int precision) {
   const size_t n = (size_t)L.rows * L.Sp;
+  const int copies = two_orders(precision, L) ? 2 : 1;
   if (precision == MCPG_FP32) sz.add<float>(n);
-  else { sz.add<uint16_t>(n); if (precision == MCPG_BF16X3) sz.add<uint16_t>(n); }
+  else
+    for (int c = 0; c < copies; ++c) { sz.add<uint16_t>(n); if (precision == MCPG_BF16X3) sz.add<uint16_t>(n); }
   sz.add<double>(L.ntiles * L.gt);
 }
 
@@ -930,7 +1048,14 @@ WeightBufs take_weight_bufs(mcpg::Carver& cv, const Layout& L, int precision) {
   WeightBufs w;
   const size_t n = (size_t)L.rows * L.Sp;
   if (precision == MCPG_FP32) w.w32 = cv.take<float>(n);
-  else { w.whi = cv.take<uint16_t>(n); if (precision == MCPG_BF16X3) w.wlo = cv.take<uint16_t>(n); }
+  else {
+    w.whi = cv.take<uint16_t>(n);
+    if (precision == MCPG_BF16X3) w.wlo = cv.take<uint16_t>(n);
+    if (two_orders(precision, L)) {
+      w.whi_c = cv.take<uint16_t>(n);
+      if (precision == MCPG_BF16X3) w.wlo_c = cv.take<uint16_t>(n);
+    }
+  }
   w.H64 = cv.take<double>(L.ntiles * L.gt);
   return w;
 }
@@ -938,8 +1063,8 @@ WeightBufs take_weight_bufs(mcpg::Carver& cv, const Layout& L, int precision) {
 int clear_weight_bufs(mcpg_ctx* h, const WeightBufs& w, const Layout& L) {
   const size_t n = (size_t)L.rows * L.Sp;
   if (w.w32) MCPG_HIP(h, hipMemsetAsync(w.w32, 0, n * sizeof(float), h->stream));
-  if (w.whi) MCPG_HIP(h, hipMemsetAsync(w.whi, 0, n * sizeof(uint16_t), h->stream));
-  if (w.wlo) MCPG_HIP(h, hipMemsetAsync(w.wlo, 0, n * sizeof(uint16_t), h->stream));
+  for (uint16_t* p : {w.whi, w.wlo, w.whi_c, w.wlo_c})
+    if (p) MCPG_HIP(h, hipMemsetAsync(p, 0, n * sizeof(uint16_t), h->stream));
   return MCPG_OK;
 }
 
@@ -1081,9 +1206,10 @@ int mcpg_bspline_mi_weights_f32(mcpg_handle h, int ptrs, const float* W, int64_t
   {
     mcpg::LaunchScope ls(h, "weights_import_kernel");
     hipLaunchKernelGGL(weights_import_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(wrow, 256), 64), (unsigned)G),
-                       dim3(256), 0, h->stream, dW, lw, L, wb.w32, wb.whi, wb.wlo, wb.H64, h->d_flags);
+                       dim3(256), 0, h->stream, dW, lw, L, wb.w32, wb.whi, wb.wlo, wb.whi_c, wb.wlo_c, wb.H64,
+                       h->d_flags);
   }
-  if ((st = mi_launch(h, precision, wb.w32, wb.whi, wb.wlo, wb.H64, L, dMI, lm))) return st;
+  if ((st = mi_launch(h, precision, wb.w32, wb.whi, wb.wlo, wb.whi_c, wb.wlo_c, wb.H64, L, dMI, lm))) return st;
   if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, G * 4, G * 4, G, hipMemcpyDeviceToHost))) return st;
   if ((st = mcpg::finish(h, "mcpg_bspline_mi_weights_f32"))) return st;
   return check_flags_nonfinite(h, "mcpg_bspline_mi_weights_f32");
@@ -1125,9 +1251,10 @@ int mcpg_bspline_mi_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int6
   if ((st = clear_weight_bufs(h, wb, L))) return st;
   if ((st = mcpg::reset_flags(h))) return st;
   PrepOut o{};
-  o.w32 = wb.w32; o.whi = wb.whi; o.wlo = wb.wlo; o.H64 = wb.H64; o.flags = h->d_flags;
+  o.w32 = wb.w32; o.whi = wb.whi; o.wlo = wb.wlo; o.whi_c = wb.whi_c; o.wlo_c = wb.wlo_c;
+  o.H64 = wb.H64; o.flags = h->d_flags;
   if ((st = prep_launch(h, dX, lx, L, sp, 1, o))) return st;
-  if ((st = mi_launch(h, precision, wb.w32, wb.whi, wb.wlo, wb.H64, L, dMI, lm))) return st;
+  if ((st = mi_launch(h, precision, wb.w32, wb.whi, wb.wlo, wb.whi_c, wb.wlo_c, wb.H64, L, dMI, lm))) return st;
   if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, G * 4, G * 4, G, hipMemcpyDeviceToHost))) return st;
   if ((st = mcpg::finish(h, "mcpg_bspline_mi_f32"))) return st;
   return check_flags_nonfinite(h, "mcpg_bspline_mi_f32");
