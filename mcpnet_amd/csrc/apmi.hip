@@ -1,13 +1,14 @@
 // Adaptive-partitioning mutual information on rank-transformed rows, plus the rank, jitter and
 // cumulative-histogram transforms that feed it.
 //
-// One wavefront evaluates one gene pair at a time.  The pair's points are bucketed by y rank
-// into LDS (xGroupedByY), and the window recursion of the reference runs as a LIFO stack held
-// in the wave's registers (entry k lives in lane k % 64, slot k / 64), so every control
-// decision is wave-uniform.  A popped window is split at its mid ranks and its four quadrant
-// counts come from one pass over the window's y range: each lane tests one point and the
-// counts are ballot + popcount over the wave.  Counts are integers and match the reference
-// exactly; leaf terms enter the FP64 sum in the reference's LIFO order.
+// One wavefront evaluates one gene pair at a time, taking pairs in runs of 16 from a global
+// counter.  The pair's points are bucketed by y rank into LDS (xGroupedByY), and the window
+// recursion of the reference runs as a LIFO stack held in the wave's registers (entry k lives
+// in lane k % 64, slot k / 64), so every control decision is wave-uniform.  A popped window is
+// split at its mid ranks and its four quadrant counts come from one pass over the window's y
+// range: each lane tests one point and the counts are ballot + popcount over the wave.  Counts
+// are integers and match the reference exactly; leaf terms are evaluated a lane each and enter
+// the FP64 sum one by one in the reference's LIFO order.
 #include <cmath>
 
 #include "internal.h"
@@ -71,8 +72,11 @@ __global__ __launch_bounds__(256) void rank_kernel(const float* __restrict__ X, 
 }
 
 // IntegralCumulativeHistogramKernel (AdaptivePartitioningMI.hpp:422-439) for one row per block.
+// ident (nullable): ident[g] = 1 if the row's histogram is the identity (the ranks are a
+// permutation of 0 .. S-1), else 0.
 __global__ __launch_bounds__(256) void cumhist_kernel(const int32_t* __restrict__ R, int64_t ldr, int64_t S,
-                                                       int32_t* __restrict__ H, int64_t ldh, int* flags) {
+                                                       int32_t* __restrict__ H, int64_t ldh, int* flags,
+                                                       int32_t* __restrict__ ident) {
   extern __shared__ int hsh[];
   const int64_t g = blockIdx.x;
   for (int64_t i = threadIdx.x; i <= S; i += blockDim.x) hsh[i] = 0;
@@ -87,7 +91,15 @@ __global__ __launch_bounds__(256) void cumhist_kernel(const int32_t* __restrict_
     for (int64_t i = 2; i <= S; ++i) hsh[i] += hsh[i - 1];
   }
   __syncthreads();
-  for (int64_t i = threadIdx.x; i <= S; i += blockDim.x) H[g * ldh + i] = hsh[i];
+  int same = 1;
+  for (int64_t i = threadIdx.x; i <= S; i += blockDim.x) {
+    H[g * ldh + i] = hsh[i];
+    same &= hsh[i] == (int)i;
+  }
+  if (ident) {
+    same = __syncthreads_and(same);
+    if (threadIdx.x == 0) ident[g] = same;
+  }
 }
 
 // The y-side of _createSparseMatrix (AdaptivePartitioningMI.hpp:462-493) once per gene instead of
@@ -137,12 +149,17 @@ struct APArgs {
   int32_t* nleaves;
   double* mi_out;
   int* flags;
+  unsigned long long* queue;   // all-pairs mode: next unclaimed pair index, zero at launch
+  const int32_t* ident;        // per gene: 1 if its histogram is the identity (cumhist_kernel)
 };
+
+constexpr int kChunk = 16;     // consecutive pairs a wave claims from the queue at a time
 
 // Quadrant counts of window w split at (mx, my) over xg[b0 .. b1), lower/upper y halves split at bm.
 // The lower half [b0, bm) and the upper half [bm, b1) are scanned separately, 64 points at a
 // time: two ballots (left, right; lanes past the half hold -1 and match neither, since x1 >= 0)
-// and two scalar popcounts per 64 points.
+// and two scalar popcounts per 64 points.  (One pass with four ballots per 64 points measured
+// slower: 9.5 against 8.3 ms at 2000 x 1000.)
 __device__ __forceinline__ void count_half(const int* __restrict__ xg, int b0, int b1, int x1, int mx, int x2, int lane,
                                            int& nl, int& nr) {
   nl = nr = 0;
@@ -153,26 +170,55 @@ __device__ __forceinline__ void count_half(const int* __restrict__ xg, int b0, i
     nr += __popcll(__builtin_amdgcn_ballot_w64(x >= mx && x < x2));
   }
 }
-
 __device__ __forceinline__ void count_quadrants(const int* __restrict__ xg, int b0, int bm, int b1, int x1, int mx,
                                                 int x2, int lane, int& n0, int& n1, int& n2, int& n3) {
   count_half(xg, b0, bm, x1, mx, x2, lane, n0, n1);
   count_half(xg, bm, b1, x1, mx, x2, lane, n2, n3);
 }
 
+__device__ __forceinline__ double readlane_f64(double v, int k) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, k);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), k);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
 // Evaluates AdaptivePartitionRankMIKernel2::operator()(x=gx, y=gy) (AdaptivePartitioningMI.hpp:553-645).
-// Returns the FP64 value mi/S + log2(S).  If rec != nullptr, leaf (nxy, nx, ny) triples are
-// recorded in summation order (up to max_leaves) and *nleaves receives the total count.
+// Returns the FP64 value mi/S + log2S (log2S = log2(S), computed once by the caller).  If
+// rec != nullptr, leaf (nxy, nx, ny) triples are recorded in summation order (up to max_leaves)
+// and *nleaves receives the total count.
 __device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_t gx, int64_t gy, int* xg, int lane,
-                          int32_t* rec, int32_t* nleaves) {
+                          double log2S, int32_t* rec, int32_t* nleaves) {
   const int S = (int)a.S;
   const int32_t* __restrict__ cHX = H + gx * a.ldh;
   const int32_t* __restrict__ cHY = H + gy * a.ldh;
   const int32_t* __restrict__ xr = a.R + gx * a.ldr;
+  // a gene whose ranks are a permutation has the identity histogram: its entries need no load
+  const bool idX = rfl(a.ident[gx]) != 0, idY = rfl(a.ident[gy]) != 0;
+  auto hx = [&](int i) { return idX ? i : cHX[i]; };
+  auto hy = [&](int i) { return idY ? i : cHY[i]; };
 
   // _createSparseMatrix (:462-493): x ranks grouped by y rank, through the y gene's bucket order
+  // (P holds sample indices in [0, S)); eight loads per lane in flight at a time
   const int32_t* __restrict__ py = a.P + gy * S;
-  for (int i = lane; i < S; i += 64) xg[i] = xr[py[i]];
+  for (int base = 0; base < S; base += 64 * 8) {
+    int v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + 64 * u + lane;
+      v[u] = i < S ? py[i] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + 64 * u + lane;
+      v[u] = i < S ? xr[v[u]] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + 64 * u + lane;
+      if (i < S) xg[i] = v[u];
+    }
+  }
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
@@ -202,29 +248,41 @@ __device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_
       }
     return w;
   };
-  auto term = [&](const Win& w, int32_t& leaf) -> double {
-    const int64_t nx = (int64_t)(cHX[w.x2] - cHX[w.x1]);
-    const int64_t ny = (int64_t)(cHY[w.y2] - cHY[w.y1]);
+  // Leaf terms (compute_for_window, :525-538) are batched: leaf k of a batch is held by lane k,
+  // a full batch (or the last one) evaluates n log2(n / (nx ny)) on all lanes at once, and the
+  // terms then enter the FP64 sum one by one in leaf order, as in the reference's loop.
+  double mi = 0.0;
+  int32_t leaf = 0;
+  int pend = 0, pn = 0, pnx = 0, pny = 0;
+  auto flush = [&]() {
+    double t = 0.0;
+    if (lane < pend) t = (double)pn * log2((double)pn / (double)((int64_t)pnx * (int64_t)pny));
+    for (int k = 0; k < pend; ++k) mi += readlane_f64(t, k);
+    pend = 0;
+  };
+  auto term = [&](const Win& w) {
+    const int nx = hx(w.x2) - hx(w.x1);
+    const int ny = hy(w.y2) - hy(w.y1);
     if (rec && lane == 0 && leaf < a.max_leaves) {
       rec[3 * leaf + 0] = w.n;
-      rec[3 * leaf + 1] = (int32_t)nx;
-      rec[3 * leaf + 2] = (int32_t)ny;
+      rec[3 * leaf + 1] = nx;
+      rec[3 * leaf + 2] = ny;
     }
     ++leaf;
-    return (double)w.n * log2((double)w.n / (double)(nx * ny));
+    if (lane == pend) { pn = w.n; pnx = nx; pny = ny; }
+    ++pend;
   };
 
   push(Win{0, 0, S, S, S});
-  double mi = 0.0;
-  int32_t leaf = 0;
   int overflow = 0;
   while (sp > 0) {
+    if (pend > 60) flush();   // a window adds at most four leaves, so a batch never exceeds 64
     const Win w = top();
     const int mx = (w.x1 + w.x2) >> 1;
     const int my = (w.y1 + w.y2) >> 1;
     Win q[4] = {Win{w.x1, w.y1, mx, my, 0}, Win{mx, w.y1, w.x2, my, 0}, Win{w.x1, my, mx, w.y2, 0},
                 Win{mx, my, w.x2, w.y2, 0}};
-    const int b0 = rfl(cHY[w.y1]), bm = rfl(cHY[my]), b1 = rfl(cHY[w.y2]);
+    const int b0 = rfl(hy(w.y1)), bm = rfl(hy(my)), b1 = rfl(hy(w.y2));
     count_quadrants(xg, b0, bm, b1, w.x1, mx, w.x2, lane, q[0].n, q[1].n, q[2].n, q[3].n);
     const double e = (double)w.n / 4.0;
     double chi = 0.0;
@@ -245,19 +303,20 @@ __device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_
           if (sp >= 64 * kStackSlots) { overflow = 1; continue; }
           push(c);
         } else if (c.n > 0) {
-          mi += term(c, leaf);
+          term(c);
         }
       }
     } else if (w.n > 0) {
-      mi += term(w, leaf);
+      term(w);
       --sp;
     } else {
       --sp;  // unreachable for pushed windows (n > 2); keeps the loop finite
     }
   }
+  flush();
   if (overflow && lane == 0) atomicOr(a.flags, 4);
   if (nleaves) *nleaves = leaf;
-  return mi / (double)S + log2((double)S);
+  return mi / (double)S + log2S;
 }
 
 // H is passed on its own, restrict-qualified: the histogram entries a window reads are
@@ -269,10 +328,12 @@ __global__ __launch_bounds__(64 * kMaxWaves) void ap_mi_kernel(APArgs a, const i
   int* xg = lds + (int64_t)w * (a.S + 1);
   const int64_t total_waves = (int64_t)gridDim.x * waves;
   const int64_t wave_id = (int64_t)blockIdx.x * waves + w;
+  const double log2S = log2((double)a.S);
   if (a.npairs > 0) {
     for (int64_t n = wave_id; n < a.npairs; n += total_waves) {
       int32_t nl = 0;
-      const double v = ap_pair(a, H, a.px[n], a.py[n], xg, lane, a.cells + n * 3 * (int64_t)a.max_leaves, &nl);
+      const double v =
+          ap_pair(a, H, a.px[n], a.py[n], xg, lane, log2S, a.cells + n * 3 * (int64_t)a.max_leaves, &nl);
       if (lane == 0) {
         a.nleaves[n] = nl;
         a.mi_out[n] = v;
@@ -280,15 +341,28 @@ __global__ __launch_bounds__(64 * kMaxWaves) void ap_mi_kernel(APArgs a, const i
     }
     return;
   }
+  // All pairs (i <= j, row-major over the upper triangle), claimed kChunk at a time from a
+  // global counter so that waves whose pairs recurse less take more of them; a wave leaves once
+  // the counter has passed the last pair.
   const int64_t P = a.G * (a.G + 1) / 2;
-  for (int64_t p = wave_id; p < P; p += total_waves) {
+  for (;;) {
+    unsigned long long claim = 0;
+    if (lane == 0) claim = atomicAdd(a.queue, (unsigned long long)kChunk);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)claim);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(claim >> 32));
+    const int64_t base = (int64_t)(((uint64_t)hi << 32) | lo);
+    if (base >= P) break;
+    const int64_t end = base + kChunk < P ? base + kChunk : P;
     int64_t i, j;
-    tri_decode(p, a.G, &i, &j);
-    const double v = ap_pair(a, H, i, j, xg, lane, nullptr, nullptr);
-    if (lane == 0) {
-      const float f = (float)v;
-      a.MI[i * a.ldmi + j] = f;
-      a.MI[j * a.ldmi + i] = f;
+    tri_decode(base, a.G, &i, &j);
+    for (int64_t p = base; p < end; ++p) {
+      const double v = ap_pair(a, H, i, j, xg, lane, log2S, nullptr, nullptr);
+      if (lane == 0) {
+        const float f = (float)v;
+        a.MI[i * a.ldmi + j] = f;
+        a.MI[j * a.ldmi + i] = f;
+      }
+      if (++j == a.G) { ++i; j = i; }
     }
   }
 }
@@ -411,7 +485,7 @@ int mcpg_cumhist_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64
   {
     mcpg::LaunchScope ls(h, "cumhist_kernel");
     hipLaunchKernelGGL(cumhist_kernel, dim3((unsigned)G), dim3(256), (S + 1) * sizeof(int), h->stream, dR, lr, S, dH, lh,
-                       h->d_flags);
+                       h->d_flags, (int32_t*)nullptr);
   }
   if (ptrs == MCPG_HOST &&
       (st = mcpg::copy_2d(h, H, ldh * 4, dH, (S + 1) * 4, (S + 1) * 4, G, hipMemcpyDeviceToHost)))
@@ -430,6 +504,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   if (ptrs == MCPG_HOST) sz.add<int32_t>(G * S);
   sz.add<int32_t>(G * (S + 1));
   sz.add<int32_t>(G * S);
+  sz.add<int32_t>(G);
   if (!cells_mode && ptrs == MCPG_HOST) sz.add<float>(G * G);
   if (cells_mode) {
     sz.add<int32_t>(npairs); sz.add<int32_t>(npairs);
@@ -447,6 +522,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   }
   int32_t* dH = cv.take<int32_t>(G * (S + 1));
   int32_t* dP = cv.take<int32_t>(G * S);
+  int32_t* dId = cv.take<int32_t>(G);
   float* dMI = MI;
   int64_t lm = ldmi;
   if (!cells_mode && ptrs == MCPG_HOST) { dMI = cv.take<float>(G * G); lm = G; }
@@ -454,7 +530,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   {
     mcpg::LaunchScope ls(h, "cumhist_kernel");
     hipLaunchKernelGGL(cumhist_kernel, dim3((unsigned)G), dim3(256), (S + 1) * sizeof(int), h->stream, dR, lr, S, dH,
-                       S + 1, h->d_flags);
+                       S + 1, h->d_flags, dId);
   }
   // ranks index the histograms: refuse out-of-range ranks before the bucket and pair kernels read them
   if ((st = check_flags(h, "mcpg_ap_mi_i32"))) return st;
@@ -466,6 +542,8 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   APArgs a{};
   a.R = dR; a.ldr = lr; a.H = dH; a.ldh = S + 1; a.P = dP; a.G = G; a.S = S; a.MI = dMI; a.ldmi = lm;
   a.flags = h->d_flags;
+  a.queue = reinterpret_cast<unsigned long long*>(h->d_flags + 8);   // zeroed by reset_flags
+  a.ident = dId;
   int32_t *d_px = nullptr, *d_py = nullptr, *d_cells = nullptr, *d_nl = nullptr;
   double* d_mi = nullptr;
   if (cells_mode) {
@@ -480,8 +558,16 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   const int64_t work = cells_mode ? npairs : G * (G + 1) / 2;
   const int64_t per_wave = (S + 1) * (int64_t)sizeof(int);
   const int waves = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxWaves, kLdsBudget / per_wave));
-  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, waves), 256 * 64 / waves));
   const size_t lds = (size_t)waves * per_wave;
+  int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, waves), 256 * 64 / waves));
+  if (!cells_mode) {   // queue mode: only as many workgroups as stay resident together
+    int cus = 0, occ = 0;
+    MCPG_HIP(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    MCPG_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&ap_mi_kernel),
+                                                             64 * waves, lds));
+    blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, (int64_t)waves * kChunk),
+                                                    (int64_t)std::max(cus, 1) * std::max(occ, 1)));
+  }
   {
     mcpg::LaunchScope ls(h, "ap_mi_kernel");
     hipLaunchKernelGGL(ap_mi_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds, h->stream, a,

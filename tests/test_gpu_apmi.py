@@ -74,6 +74,23 @@ def test_cells_with_tied_ranks(handle):
         assert abs(mi[n] - ref_mi) <= 1e-9
 
 
+def test_mixed_permutation_and_tied_rows(handle):
+    """Rows whose ranks are a permutation (identity histogram, read without loads) paired with
+    rows that have ties (histogram read from memory), both ways round, and the whole matrix."""
+    rng = np.random.default_rng(7)
+    R = ranks(6, 300, 21)
+    R[3:] = rng.integers(0, 60, (3, 300)).astype(np.int32)
+    px = np.array([0, 3, 1, 4, 2, 5, 3], np.int32)
+    py = np.array([3, 0, 4, 1, 5, 2, 3], np.int32)
+    cells, mi = handle.ap_mi_cells(R, px, py)
+    for n in range(len(px)):
+        ref_cells, ref_mi = oracle.ap_mi_cells(R, px[n], py[n])
+        assert np.array_equal(cells[n], ref_cells), n
+        assert abs(mi[n] - ref_mi) <= 1e-9
+    got = handle.ap_mi(R)
+    assert np.abs(got.astype(np.float64) - oracle.ap_mi(R)).max() <= 1e-6
+
+
 def test_rejects_out_of_range_ranks(handle):
     import mcpnet_amd
     R = np.zeros((3, 10), np.int32)
