@@ -30,6 +30,17 @@ def test_gradient_vs_fp64_oracle(handle):
     assert rel <= 1e-3, rel
 
 
+def test_valu_kernels_match_mfma_kernels(handle, monkeypatch):
+    """MCPG_ADJ_MFMA=0 selects the VALU pair-table and gather kernels: same loss and gradient
+    as the MFMA kernels up to FP32 summation order."""
+    X, pi, pj, m = setup(70, 260, 0.2, 9)
+    dX1, L1 = handle.bspline_mi_backward(X, pi, pj, m)
+    monkeypatch.setenv("MCPG_ADJ_MFMA", "0")
+    dX0, L0 = handle.bspline_mi_backward(X, pi, pj, m)
+    assert abs(L1 - L0) <= 1e-6 * max(1.0, abs(L0))
+    assert np.linalg.norm(dX1 - dX0) / np.linalg.norm(dX0) <= 1e-5
+
+
 def test_gradient_vs_finite_differences(handle):
     X, pi, pj, m = setup(40, 200, 0.3, 7)
     dX, _ = handle.bspline_mi_backward(X, pi, pj, m)

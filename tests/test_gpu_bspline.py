@@ -51,6 +51,21 @@ def test_mi_vs_fp64_oracle(handle, precision, G, S):
     assert np.array_equal(got, got.T)
 
 
+@pytest.mark.parametrize("env", [{"MCPG_MI_RE": "0"}, {"MCPG_MI_ER": "0"}])
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16"])
+def test_mi_one_row_order_kernels(handle, monkeypatch, env, precision):
+    """The BF16 kernels that read one row order (LDS epilogue; with or without early release),
+    selected per call by environment variable, against the FP64 oracle and the default path."""
+    X = gen(70, 300, seed=41)
+    default = handle.bspline_mi(X, precision=precision).astype(np.float64)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    got = handle.bspline_mi(X, precision=precision).astype(np.float64)
+    assert np.abs(got - oracle.bspline_mi_f64(X)).max() <= TOL[precision]
+    assert np.abs(got - default).max() <= (1e-5 if precision == "bf16x3" else 1e-4)   # FP32 sum order
+    assert np.array_equal(got, got.T)
+
+
 def test_mi_bins_and_order_variants(handle):
     X = gen(40, 300, seed=77)
     for bins, order in [(7, 2), (16, 4), (20, 3), (4, 4)]:
