@@ -66,6 +66,15 @@ def test_mi_one_row_order_kernels(handle, monkeypatch, env, precision):
     assert np.array_equal(got, got.T)
 
 
+def test_mi_tile_order_switches_keep_bits(handle, monkeypatch):
+    """MCPG_MI_GROUP / MCPG_MI_XCD reorder the tiles only: bit-identical MI."""
+    X = gen(300, 200, seed=43)
+    ref = handle.bspline_mi(X, precision="bf16x3")
+    monkeypatch.setenv("MCPG_MI_GROUP", "3")
+    monkeypatch.setenv("MCPG_MI_XCD", "1")
+    assert np.array_equal(handle.bspline_mi(X, precision="bf16x3"), ref)
+
+
 def test_mi_bins_and_order_variants(handle):
     X = gen(40, 300, seed=77)
     for bins, order in [(7, 2), (16, 4), (20, 3), (4, 4)]:

@@ -173,3 +173,21 @@ def test_maxmin_rejects_bad_arguments(handle):
         handle.maxmin(A, B, symmetric=True)   # symmetric needs the same operand
     with pytest.raises(ValueError):
         handle.maxmin(A, np.ones((4, 6), np.float32))
+
+
+@pytest.mark.parametrize("env", [{"MCPG_MM_KC": "32"}, {"MCPG_MM_GROUP": "3", "MCPG_MM_XCD": "1"}])
+def test_tuning_switches_keep_bits(handle, monkeypatch, env):
+    """K chunk, tile-group side and XCD deal change the schedule only: bit-identical results,
+    for a rectangular product and a symmetric (triangle) one."""
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((700, 300)).astype(np.float32)
+    B = rng.standard_normal((450, 300)).astype(np.float32)
+    Y = rng.standard_normal((500, 500)).astype(np.float32)
+    Y = np.ascontiguousarray(np.maximum(Y, Y.T))
+    ref_rect = handle.maxmin(A, B)
+    ref_tri = handle.maxmin(Y, Y, symmetric=True)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    assert bits_equal(handle.maxmin(A, B), ref_rect)
+    assert bits_equal(handle.maxmin(Y, Y, symmetric=True), ref_tri)
+    assert bits_equal(ref_rect[:30], oracle.maxmin(np.ascontiguousarray(A[:30]), B))
