@@ -554,18 +554,35 @@ __global__ void emax_kernel(const T* __restrict__ A, int64_t lda, const T* __res
   }
 }
 
-// flags[0] |= 1 if A is not bitwise symmetric
+// flags[0] |= 1 if A is not bitwise symmetric.  One workgroup per 64 x 64 tile (I, J) with
+// I <= J (2-D grid; the others leave at once): tile (I, J) goes to LDS from coalesced row reads,
+// tile (J, I) is read coalesced and compared with it transposed, so each element is read once.
+constexpr int kSymTile = 64;
+
 template <typename T, typename Bits = typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type>
-__global__ void symmetry_check_kernel(const T* __restrict__ A, int64_t lda, int64_t N, int* flags) {
-  const int64_t total = N * N;
-  int bad = 0;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = t / N, j = t % N;
-    if (j > i) {
-      if (__builtin_bit_cast(Bits, A[i * lda + j]) != __builtin_bit_cast(Bits, A[j * lda + i])) bad = 1;
-    }
+__global__ __launch_bounds__(256) void symmetry_check_kernel(const T* __restrict__ A, int64_t lda, int64_t N,
+                                                              int* flags) {
+  if (blockIdx.y > blockIdx.x) return;
+  __shared__ Bits tile[kSymTile][kSymTile + 1];
+  const int64_t i0 = (int64_t)blockIdx.y * kSymTile, j0 = (int64_t)blockIdx.x * kSymTile;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < kSymTile; r += 4) {
+    const int64_t i = i0 + r, j = j0 + tx;
+    tile[r][tx] = (i < N && j < N) ? __builtin_bit_cast(Bits, A[i * lda + j]) : Bits(0);
   }
-  if (bad) atomicOr(flags, 1);
+  __syncthreads();
+  int bad = 0;
+  for (int r = ty; r < kSymTile; r += 4) {
+    const int64_t j = j0 + r, i = i0 + tx;   // A[j][i] against A[i][j] = tile[tx][r]
+    if (i < N && j < N && __builtin_bit_cast(Bits, A[j * lda + i]) != tile[tx][r]) bad = 1;
+  }
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(flags, 1);
+}
+
+template <typename T>
+void launch_symmetry_check(mcpg_ctx* h, const T* A, int64_t lda, int64_t N) {
+  const unsigned nb = (unsigned)ceil_div(N, kSymTile);
+  hipLaunchKernelGGL(symmetry_check_kernel<T>, dim3(nb, nb), dim3(256), 0, h->stream, A, lda, N, h->d_flags);
 }
 
 // DPI decision and mask application (include/mcp/filter/dpi.hpp:56-66, :86-105;
@@ -852,7 +869,7 @@ int maxmins_f32(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N, 
     if ((st = mcpg::reset_flags(h))) return st;
     {
       mcpg::LaunchScope ls(h, "symmetry_check_kernel");
-      hipLaunchKernelGGL(symmetry_check_kernel<float>, dim3(1024), dim3(256), 0, h->stream, dA, la, N, h->d_flags);
+      launch_symmetry_check<float>(h, dA, la, N);
     }
     int flags[1];
     if ((st = mcpg::read_flags(h, flags, 1))) return st;
@@ -1010,7 +1027,7 @@ int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N,
     if ((st = mcpg::reset_flags(h))) return st;
     {
       mcpg::LaunchScope ls(h, "symmetry_check_kernel");
-      hipLaunchKernelGGL(symmetry_check_kernel<double>, dim3(1024), dim3(256), 0, h->stream, dA, la, N, h->d_flags);
+      launch_symmetry_check<double>(h, dA, la, N);
     }
     int flags[1];
     if ((st = mcpg::read_flags(h, flags, 1))) return st;
@@ -1200,7 +1217,7 @@ int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N,
       if ((st = mcpg::reset_flags(h))) return st;
       {
         mcpg::LaunchScope ls(h, "symmetry_check_kernel");
-        hipLaunchKernelGGL(symmetry_check_kernel<float>, dim3(1024), dim3(256), 0, h->stream, dA, la, N, h->d_flags);
+        launch_symmetry_check<float>(h, dA, la, N);
       }
       int flags[1];
       if ((st = mcpg::read_flags(h, flags, 1))) return st;
@@ -1271,7 +1288,7 @@ int mcp_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, int k, 
       if ((st = mcpg::reset_flags(h))) return st;
       {
         mcpg::LaunchScope ls(h, "symmetry_check_kernel");
-        hipLaunchKernelGGL(symmetry_check_kernel<T>, dim3(1024), dim3(256), 0, h->stream, dA, la, N, h->d_flags);
+        launch_symmetry_check<T>(h, dA, la, N);
       }
       int flags[1];
       if ((st = mcpg::read_flags(h, flags, 1))) return st;
