@@ -157,17 +157,22 @@ constexpr int kChunk = 16;     // consecutive pairs a wave claims from the queue
 
 // Quadrant counts of window w split at (mx, my) over xg[b0 .. b1), lower/upper y halves split at bm.
 // The lower half [b0, bm) and the upper half [bm, b1) are scanned separately, 64 points at a
-// time: two ballots (left, right; lanes past the half hold -1 and match neither, since x1 >= 0)
-// and two scalar popcounts per 64 points.  (One pass with four ballots per 64 points measured
-// slower: 9.5 against 8.3 ms at 2000 x 1000.)
+// time: two ballots and two scalar popcounts per 64 points.  x in [lo, hi) is tested as
+// (unsigned)(x - lo) < hi - lo, one compare per ballot; lanes past the half read the array's
+// padding (kXgPad entries past S) and take x = -1, which no window contains.  (One pass with
+// four ballots per 64 points measured slower: 9.5 against 8.3 ms at 2000 x 1000.)
+constexpr int kXgPad = 64;
+
 __device__ __forceinline__ void count_half(const int* __restrict__ xg, int b0, int b1, int x1, int mx, int x2, int lane,
                                            int& nl, int& nr) {
   nl = nr = 0;
+  const uint32_t wl = (uint32_t)(mx - x1), wr = (uint32_t)(x2 - mx);
   for (int base = b0; base < b1; base += 64) {
     const int idx = base + lane;
-    const int x = idx < b1 ? xg[idx] : -1;
-    nl += __popcll(__builtin_amdgcn_ballot_w64(x >= x1 && x < mx));
-    nr += __popcll(__builtin_amdgcn_ballot_w64(x >= mx && x < x2));
+    const int v = xg[idx];
+    const int x = idx < b1 ? v : -1;
+    nl += __popcll(__builtin_amdgcn_ballot_w64((uint32_t)(x - x1) < wl));
+    nr += __popcll(__builtin_amdgcn_ballot_w64((uint32_t)(x - mx) < wr));
   }
 }
 __device__ __forceinline__ void count_quadrants(const int* __restrict__ xg, int b0, int bm, int b1, int x1, int mx,
@@ -325,7 +330,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void ap_mi_kernel(APArgs a, const i
   extern __shared__ int lds[];
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int waves = blockDim.x >> 6;
-  int* xg = lds + (int64_t)w * (a.S + 1);
+  int* xg = lds + (int64_t)w * (a.S + kXgPad);
   const int64_t total_waves = (int64_t)gridDim.x * waves;
   const int64_t wave_id = (int64_t)blockIdx.x * waves + w;
   const double log2S = log2((double)a.S);
@@ -556,7 +561,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
     a.mi_out = d_mi;
   }
   const int64_t work = cells_mode ? npairs : G * (G + 1) / 2;
-  const int64_t per_wave = (S + 1) * (int64_t)sizeof(int);
+  const int64_t per_wave = (S + kXgPad) * (int64_t)sizeof(int);
   const int waves = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxWaves, kLdsBudget / per_wave));
   const size_t lds = (size_t)waves * per_wave;
   int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, waves), 256 * 64 / waves));
