@@ -22,7 +22,10 @@ using mcpg::ceil_div;
 constexpr int kMaxWaves = 4;       // waves per workgroup (fewer when S is large)
 constexpr int64_t kLdsBudget = 64 * 1024;
 constexpr int64_t kMaxS = kLdsBudget / 8 - 1;  // keeps the per-gene histogram and bucket kernels in 32 KB of LDS
-constexpr int kStackSlots = 2;     // stack capacity = 64 * kStackSlots windows
+// Stack capacity = 64 * kStackSlots windows.  A pop pushes at most four children, so the stack
+// grows by at most 3 per level, and every split halves both rank ranges: at most ceil(log2 S)
+// = 13 levels for S <= 8191, so at most 40 entries (the overflow flag stays as a guard).
+constexpr int kStackSlots = 1;
 
 struct Win {
   int x1, y1, x2, y2, n;
