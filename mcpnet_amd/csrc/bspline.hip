@@ -867,6 +867,7 @@ struct GeneArgs {
   const int* off;     // CSR offsets, G+1
   const int* nbr;     // CSR entries: (j, pair index, orientation) x nnz
   const double* cw;   // per gene: sum of m over its entries, in CSR order
+  float* accg;        // MFMA path: neighbour sums, G x bins x Sp
   const float* Gm;
   const float* m;
   const double* pmarg;
@@ -971,96 +972,103 @@ __global__ __launch_bounds__(256) void bspline_gene_adjoint_kernel(GeneArgs a) {
   }
 }
 
-// The same for bins <= 16, the neighbour sum on MFMA: acc_i[b][s] = sum_e G_e[b][c] W_j[c][s] is
-// a 16 x 16-sample output block of v_mfma_f32_16x16x4_f32 with K over c (four products; zero
-// beyond bins).  Lane (r, k) supplies G_e[r][4t + k] and W_j[4t + k][s0 + r] in product t, and
-// holds entry (4k + q, s0 + r) in output register q.  A pass covers 256 samples: four blocks
-// per wave, through every neighbour; its accumulators then go through LDS to the per-sample
-// derivative contraction of bspline_gene_adjoint_kernel.  (Eight blocks per wave took 200
-// VGPRs and 41 KB of LDS, two waves per SIMD: 4.3 against 2.7 ms at 5000 x 500.)
+// The same for bins <= 16 in two kernels.  bspline_gene_accumulate_kernel forms the neighbour
+// sum acc_i[b][s] = sum_e G_e[b][c] W_j[c][s] on MFMA: a 16 x 16-sample output block of
+// v_mfma_f32_16x16x4_f32 with K over c (four products; zero beyond bins).  Lane (r, k) supplies
+// G_e[r][4t + k] and W_j[4t + k][s0 + r] in product t, and holds entry (4k + q, s0 + r) in output
+// register q.  One workgroup per (256-sample pass, gene), four 16-sample blocks per wave, through
+// every neighbour; the sums go to a.accg (G x bins x Sp).  bspline_gene_epilogue_kernel then runs
+// the per-sample derivative contraction.  Apart, the accumulation needs few registers and keeps
+// more loads of the gathered W_j rows in flight (one kernel with its epilogue: 2.7 ms at 5000 x
+// 500 with four blocks per wave, 4.3 ms with eight).
 constexpr int kAdjPassBlocks = 4;                        // 16-sample blocks per wave per pass
 constexpr int kAdjPass = 4 * kAdjPassBlocks * 16;        // samples per pass (256)
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void bspline_gene_adjoint_mfma_kernel(GeneArgs a) {
-  const int64_t i = blockIdx.x;
+__global__ __launch_bounds__(256) void bspline_gene_accumulate_kernel(GeneArgs a) {
+  const int64_t i = blockIdx.y;
+  const int64_t pass0 = (int64_t)blockIdx.x * kAdjPass;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
+  const int bins = a.L.bins;
+  const int64_t Sp = a.L.Sp;
+  const int64_t step = (int64_t)kGroupRows * kKStep;
+  const int64_t nkb = Sp / kKStep;
+  const int64_t nblk = Sp / 16;
+  const int e0 = a.off[i], e1 = a.off[i + 1];
+  floatx4 acc[kAdjPassBlocks];
+#pragma unroll
+  for (int u = 0; u < kAdjPassBlocks; ++u) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int64_t blk0 = pass0 / 16 + wave * kAdjPassBlocks;
+  // the next entry's (j, n, orientation) loads while this one's operands are used
+  int jn = 0, nn = 0, on = 0;
+  if (e0 < e1) { jn = a.nbr[3 * e0 + 0]; nn = a.nbr[3 * e0 + 1]; on = a.nbr[3 * e0 + 2]; }
+  for (int e = e0; e < e1; ++e) {
+    const int j = jn, n = nn, ori = on;
+    if (e + 1 < e1) { jn = a.nbr[3 * e + 3]; nn = a.nbr[3 * e + 4]; on = a.nbr[3 * e + 5]; }
+    const float* g = a.Gm + (int64_t)n * bins * bins;
+    float ga[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int c = 4 * t + lk;
+      // orientation 0: gene i is the row gene of the pair's table; 1: the column gene
+      ga[t] = (lr < bins && c < bins) ? (ori == 0 ? g[lr * bins + c] : g[c * bins + lr]) : 0.0f;
+    }
+    const int64_t rj = a.L.row_of(j, 0);
+    const float* wj = a.w32 + (rj / kGroupRows) * nkb * step + (rj % kGroupRows) * kKStep;
+#pragma unroll
+    for (int u = 0; u < kAdjPassBlocks; ++u) {
+      const int64_t blk = blk0 + u;
+      if (blk >= nblk) break;
+      const int64_t s = blk * 16 + lr;
+      const float* w = wj + (s / kKStep) * step + (s % kKStep);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int c = 4 * t + lk;
+        const float wb = c < bins ? w[c * kKStep] : 0.0f;
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(ga[t], wb, acc[u], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kAdjPassBlocks; ++u) {
+    const int64_t blk = blk0 + u;
+    if (blk >= nblk) break;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int b = 4 * lk + q;
+      if (b < bins) a.accg[(i * bins + b) * Sp + blk * 16 + lr] = acc[u][q];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void bspline_gene_epilogue_kernel(GeneArgs a) {
+  const int64_t i = blockIdx.x;
+  const int tid = threadIdx.x;
   const int bins = a.L.bins;
   const int64_t S = a.L.S, Sp = a.L.Sp;
   __shared__ double shd[4];
   __shared__ float phi_i[16];
-  __shared__ float accs[16 * kAdjPass];                  // [b][sample in pass]
   __shared__ float gsh[kMaxSPerThread * 256];            // g_s for the whole row (S <= 2048)
-
-  const int e0 = a.off[i], e1 = a.off[i + 1];
   if (tid < bins) {
     const double p = a.pmarg[i * bins + tid];
     phi_i[tid] = p > 0.0 ? (float)(a.cw[i] * (-(log2(p) + 1.4426950408889634))) : 0.0f;
   }
-  const int64_t step = (int64_t)kGroupRows * kKStep;
-  const int64_t nkb = Sp / kKStep;
-  const int64_t nblk = Sp / 16;
+  __syncthreads();
+  // g_s = dL/dxhat_s = sum_b dL/dW[b][s] * N'_b(xhat_s),  dL/dW = (phi_i - acc) / S
   const float invS = 1.0f / (float)S;
   double s_lo = 0.0, s_hi = 0.0;
-  for (int64_t pass0 = 0; pass0 < Sp; pass0 += kAdjPass) {
-    floatx4 acc[kAdjPassBlocks];
+  for (int64_t s = tid; s < S; s += 256) {
+    const float t = a.xs[i * S + s];
+    float w[16], lower[17], d[16];
+    basis_values<16, 4>(a.sp, t, w, lower);
+    basis_derivs<16>(a.sp, lower, d);
+    float gv = 0.0f;
 #pragma unroll
-    for (int u = 0; u < kAdjPassBlocks; ++u) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int64_t blk0 = pass0 / 16 + wave * kAdjPassBlocks;
-    // the next entry's (j, n, orientation) loads while this one's operands are used
-    int jn = 0, nn = 0, on = 0;
-    if (e0 < e1) { jn = a.nbr[3 * e0 + 0]; nn = a.nbr[3 * e0 + 1]; on = a.nbr[3 * e0 + 2]; }
-    for (int e = e0; e < e1; ++e) {
-      const int j = jn, n = nn, ori = on;
-      if (e + 1 < e1) { jn = a.nbr[3 * e + 3]; nn = a.nbr[3 * e + 4]; on = a.nbr[3 * e + 5]; }
-      const float* g = a.Gm + (int64_t)n * bins * bins;
-      float ga[4];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int c = 4 * t + lk;
-        // orientation 0: gene i is the row gene of the pair's table; 1: the column gene
-        ga[t] = (lr < bins && c < bins) ? (ori == 0 ? g[lr * bins + c] : g[c * bins + lr]) : 0.0f;
-      }
-      const int64_t rj = a.L.row_of(j, 0);
-      const float* wj = a.w32 + (rj / kGroupRows) * nkb * step + (rj % kGroupRows) * kKStep;
-#pragma unroll
-      for (int u = 0; u < kAdjPassBlocks; ++u) {
-        const int64_t blk = blk0 + u;
-        if (blk >= nblk) break;
-        const int64_t s = blk * 16 + lr;
-        const float* w = wj + (s / kKStep) * step + (s % kKStep);
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int c = 4 * t + lk;
-          const float wb = c < bins ? w[c * kKStep] : 0.0f;
-          acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(ga[t], wb, acc[u], 0, 0, 0);
-        }
-      }
-    }
-    __syncthreads();   // the previous pass's readers of accs are done
-#pragma unroll
-    for (int u = 0; u < kAdjPassBlocks; ++u) {
-      const int sl = (wave * kAdjPassBlocks + u) * 16 + lr;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) accs[(4 * lk + q) * kAdjPass + sl] = acc[u][q];
-    }
-    __syncthreads();
-    // g_s = dL/dxhat_s = sum_b dL/dW[b][s] * N'_b(xhat_s),  dL/dW = (phi_i - acc) / S
-    for (int sl = tid; sl < kAdjPass; sl += 256) {
-      const int64_t s = pass0 + sl;
-      if (s >= S) break;
-      const float t = a.xs[i * S + s];
-      float w[16], lower[17], d[16];
-      basis_values<16, 4>(a.sp, t, w, lower);
-      basis_derivs<16>(a.sp, lower, d);
-      float gv = 0.0f;
-#pragma unroll
-      for (int b = 0; b < 16; ++b)
-        if (b < bins) gv += (phi_i[b] - accs[b * kAdjPass + sl]) * invS * d[b];
-      gsh[s] = gv;
-      s_lo += (double)gv * ((double)t - 1.0);
-      s_hi += (double)gv * (double)t;
-    }
+    for (int b = 0; b < 16; ++b)
+      if (b < bins) gv += (phi_i[b] - a.accg[(i * bins + b) * Sp + s]) * invS * d[b];
+    gsh[s] = gv;
+    s_lo += (double)gv * ((double)t - 1.0);
+    s_hi += (double)gv * (double)t;
   }
   const double sum_lo = block_sum<double>(s_lo, shd);
   const double sum_hi = block_sum<double>(s_hi, shd);
@@ -1459,6 +1467,8 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
   size_weight_bufs(sz, L, MCPG_FP32);
   sz.add<int>(np1); sz.add<int>(np1); sz.add<float>(np1);
   sz.add<int>(G + 1); sz.add<int>(nbr.size()); sz.add<double>(G);
+  const bool mfma = mcpg::env_group("MCPG_ADJ_MFMA", 1) != 0;
+  if (mfma) sz.add<float>(G * bins * L.Sp);
   sz.add<float>(np1 * bins * bins); sz.add<double>(np1);
   sz.add<double>(G * bins); sz.add<float>(G * S); sz.add<float>(G); sz.add<int>(G); sz.add<int>(G);
   void* base = mcpg::workspace(h, sz.off, &st);
@@ -1481,6 +1491,7 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
   int* d_off = cv.take<int>(G + 1);
   int* d_nbr = cv.take<int>(nbr.size());
   double* d_cw = cv.take<double>(G);
+  float* d_accg = mfma ? cv.take<float>(G * bins * L.Sp) : nullptr;
   float* d_Gm = cv.take<float>(np1 * bins * bins);
   double* d_Lp = cv.take<double>(np1);
   double* d_pm = cv.take<double>(G * bins);
@@ -1507,7 +1518,7 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
     pa.w32 = wb.w32; pa.L = L; pa.pi = d_pi; pa.pj = d_pj; pa.m = d_m; pa.npairs = npairs; pa.H64 = wb.H64;
     pa.norm = 1.0f / (float)S; pa.Gm = d_Gm; pa.Lpart = d_Lp;
     mcpg::LaunchScope ls(h, "bspline_pair_adjoint_kernel");
-    if (mcpg::env_group("MCPG_ADJ_MFMA", 1) != 0)
+    if (mfma)
       hipLaunchKernelGGL(bspline_pair_adjoint_mfma_kernel, dim3((unsigned)ceil_div(npairs, 4)), dim3(256), 0, h->stream,
                          pa);
     else
@@ -1515,13 +1526,15 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
                          pa);
   }
   GeneArgs ga;
-  ga.w32 = wb.w32; ga.L = L; ga.sp = sp; ga.off = d_off; ga.nbr = d_nbr; ga.cw = d_cw; ga.Gm = d_Gm; ga.m = d_m; ga.pmarg = d_pm;
+  ga.w32 = wb.w32; ga.L = L; ga.sp = sp; ga.off = d_off; ga.nbr = d_nbr; ga.cw = d_cw; ga.accg = d_accg; ga.Gm = d_Gm; ga.m = d_m; ga.pmarg = d_pm;
   ga.xs = d_xs; ga.scl = d_scl; ga.amin = d_amin; ga.amax = d_amax; ga.dX = ddX; ga.lddx = ldd;
   {
     mcpg::LaunchScope ls(h, "bspline_gene_adjoint_kernel");
-    if (mcpg::env_group("MCPG_ADJ_MFMA", 1) != 0)
-      hipLaunchKernelGGL(bspline_gene_adjoint_mfma_kernel, dim3((unsigned)G), dim3(256), 0, h->stream, ga);
-    else
+    if (mfma) {
+      hipLaunchKernelGGL(bspline_gene_accumulate_kernel, dim3((unsigned)ceil_div(L.Sp, kAdjPass), (unsigned)G),
+                         dim3(256), 0, h->stream, ga);
+      hipLaunchKernelGGL(bspline_gene_epilogue_kernel, dim3((unsigned)G), dim3(256), 0, h->stream, ga);
+    } else
       hipLaunchKernelGGL((bspline_gene_adjoint_kernel<16, 4>), dim3((unsigned)G), dim3(256), 0, h->stream, ga);
   }
   if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, dX, lddx * 4, ddX, S * 4, S * 4, G, hipMemcpyDeviceToHost))) return st;
