@@ -978,13 +978,13 @@ __global__ __launch_bounds__(256) void bspline_gene_adjoint_kernel(GeneArgs a) {
 // G_e[r][4t + k] and W_j[4t + k][s0 + r] in product t, and holds entry (4k + q, s0 + r) in output
 // register q.  One workgroup per (256-sample pass, gene), four 16-sample blocks per wave, through
 // every neighbour; the sums go to a.accg (G x bins x Sp).  bspline_gene_epilogue_kernel then runs
-// the per-sample derivative contraction.  Apart, the accumulation needs few registers and keeps
-// more loads of the gathered W_j rows in flight (one kernel with its epilogue: 2.7 ms at 5000 x
-// 500 with four blocks per wave, 4.3 ms with eight).
+// the per-sample derivative contraction.  Apart, the accumulation fits 60 VGPRs, eight waves per
+// SIMD, and keeps more loads of the gathered W_j rows in flight (at 5000 x 500: 1.7 ms for both
+// kernels; one kernel with its epilogue took 2.7 ms with four blocks per wave, 4.3 ms with eight).
 constexpr int kAdjPassBlocks = 4;                        // 16-sample blocks per wave per pass
 constexpr int kAdjPass = 4 * kAdjPassBlocks * 16;        // samples per pass (256)
 
-__global__ __launch_bounds__(256) void bspline_gene_accumulate_kernel(GeneArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void bspline_gene_accumulate_kernel(GeneArgs a) {
   const int64_t i = blockIdx.y;
   const int64_t pass0 = (int64_t)blockIdx.x * kAdjPass;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
