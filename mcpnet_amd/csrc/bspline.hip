@@ -976,8 +976,9 @@ __global__ __launch_bounds__(256) void bspline_gene_adjoint_kernel(GeneArgs a) {
 // sum acc_i[b][s] = sum_e G_e[b][c] W_j[c][s] on MFMA: a 16 x 16-sample output block of
 // v_mfma_f32_16x16x4_f32 with K over c (four products; zero beyond bins).  Lane (r, k) supplies
 // G_e[r][4t + k] and W_j[4t + k][s0 + r] in product t, and holds entry (4k + q, s0 + r) in output
-// register q.  One workgroup per (256-sample pass, gene), four 16-sample blocks per wave, through
-// every neighbour; the sums go to a.accg (G x bins x Sp).  bspline_gene_epilogue_kernel then runs
+// register q.  One workgroup per (256-sample pass, gene), 64 samples per wave as four output
+// blocks whose sample r of block v is the wave's sample 4r + v (one float4 per W_j row and lane),
+// through every neighbour; the sums go to a.accg (G x bins x Sp).  bspline_gene_epilogue_kernel then runs
 // the per-sample derivative contraction.  Apart, the accumulation fits 60 VGPRs, eight waves per
 // SIMD, and keeps more loads of the gathered W_j rows in flight (at 5000 x 500: 1.7 ms for both
 // kernels; one kernel with its epilogue took 2.7 ms with four blocks per wave, 4.3 ms with eight).
@@ -986,19 +987,21 @@ constexpr int kAdjPass = 4 * kAdjPassBlocks * 16;        // samples per pass (25
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void bspline_gene_accumulate_kernel(GeneArgs a) {
   const int64_t i = blockIdx.y;
-  const int64_t pass0 = (int64_t)blockIdx.x * kAdjPass;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int bins = a.L.bins;
   const int64_t Sp = a.L.Sp;
   const int64_t step = (int64_t)kGroupRows * kKStep;
   const int64_t nkb = Sp / kKStep;
-  const int64_t nblk = Sp / 16;
   const int e0 = a.off[i], e1 = a.off[i + 1];
+  // this wave's 64 samples: lane r holds samples s4 .. s4 + 3, sample s4 + v in output block v
+  const int64_t s4 = (int64_t)blockIdx.x * kAdjPass + wave * (kAdjPassBlocks * 16) + 4 * lr;
+  const bool sok = s4 < Sp;   // Sp is a multiple of 32, so s4 + 3 < Sp too
+  const int64_t soff = sok ? (s4 / kKStep) * step + (s4 % kKStep) : 0;
   floatx4 acc[kAdjPassBlocks];
 #pragma unroll
-  for (int u = 0; u < kAdjPassBlocks; ++u) acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const int64_t blk0 = pass0 / 16 + wave * kAdjPassBlocks;
+  for (int v = 0; v < kAdjPassBlocks; ++v) acc[v] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
   // the next entry's (j, n, orientation) loads while this one's operands are used
   int jn = 0, nn = 0, on = 0;
   if (e0 < e1) { jn = a.nbr[3 * e0 + 0]; nn = a.nbr[3 * e0 + 1]; on = a.nbr[3 * e0 + 2]; }
@@ -1006,38 +1009,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     const int j = jn, n = nn, ori = on;
     if (e + 1 < e1) { jn = a.nbr[3 * e + 3]; nn = a.nbr[3 * e + 4]; on = a.nbr[3 * e + 5]; }
     const float* g = a.Gm + (int64_t)n * bins * bins;
+    const int64_t rj = a.L.row_of(j, 0);
+    const float* wj = a.w32 + (rj / kGroupRows) * nkb * step + (rj % kGroupRows) * kKStep + soff;
     float ga[4];
+    floatx4 wv[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
       const int c = 4 * t + lk;
       // orientation 0: gene i is the row gene of the pair's table; 1: the column gene
       ga[t] = (lr < bins && c < bins) ? (ori == 0 ? g[lr * bins + c] : g[c * bins + lr]) : 0.0f;
+      wv[t] = (sok && c < bins) ? *reinterpret_cast<const floatx4*>(wj + c * kKStep) : zero;
     }
-    const int64_t rj = a.L.row_of(j, 0);
-    const float* wj = a.w32 + (rj / kGroupRows) * nkb * step + (rj % kGroupRows) * kKStep;
 #pragma unroll
-    for (int u = 0; u < kAdjPassBlocks; ++u) {
-      const int64_t blk = blk0 + u;
-      if (blk >= nblk) break;
-      const int64_t s = blk * 16 + lr;
-      const float* w = wj + (s / kKStep) * step + (s % kKStep);
+    for (int v = 0; v < kAdjPassBlocks; ++v)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int c = 4 * t + lk;
-        const float wb = c < bins ? w[c * kKStep] : 0.0f;
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(ga[t], wb, acc[u], 0, 0, 0);
-      }
-    }
+      for (int t = 0; t < 4; ++t) acc[v] = __builtin_amdgcn_mfma_f32_16x16x4f32(ga[t], wv[t][v], acc[v], 0, 0, 0);
   }
+  if (!sok) return;
 #pragma unroll
-  for (int u = 0; u < kAdjPassBlocks; ++u) {
-    const int64_t blk = blk0 + u;
-    if (blk >= nblk) break;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int b = 4 * lk + q;
-      if (b < bins) a.accg[(i * bins + b) * Sp + blk * 16 + lr] = acc[u][q];
-    }
+  for (int q = 0; q < 4; ++q) {
+    const int b = 4 * lk + q;
+    if (b < bins)
+      *reinterpret_cast<floatx4*>(a.accg + (i * bins + b) * Sp + s4) = floatx4{acc[0][q], acc[1][q], acc[2][q], acc[3][q]};
   }
 }
 
