@@ -326,8 +326,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(see DESIGN.md for how the work would shard); rerun with --gpus 1", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path (the products would "
+              "shard by row panels with an all-gather of M1's keys); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -336,8 +336,8 @@ def main():
     from mcpnet_amd import grn
 
     cid = args.config
-    G = args.genes or grn.CONFIGS[cid]["G"]
-    S = args.samples or grn.CONFIGS[cid]["S"]
+    G = args.genes or grn.WORKLOADS[cid]["G"]
+    S = args.samples or grn.WORKLOADS[cid]["S"]
     bins = 10
     if args.precision is None:
         args.precision = "fp32" if cid == 1 else "bf16x3"

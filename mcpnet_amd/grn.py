@@ -22,7 +22,7 @@ _M2 = np.uint64(0x94D049BB133111EB)
 CONFIG_SEED_BASE = 0x4D43504E00
 
 # Workloads (genes, samples) of the five benchmark configurations.
-CONFIGS = {
+WORKLOADS = {
     1: dict(G=1000, S=1000, what="FP32 B-spline MI + MCP3"),
     2: dict(G=2000, S=1024, what="BF16 / BF16x3 B-spline MI + MCP2/3/4"),
     3: dict(G=20000, S=2000, what="BF16x3 B-spline MI + MCP3 + ratio"),
@@ -92,7 +92,7 @@ def grn_expression(G: int, S: int, seed: int, parents: int = 3, noise_sd: float 
 
 def expression(cid: int) -> np.ndarray:
     """Expression matrix of benchmark configuration `cid` (1..5)."""
-    c = CONFIGS[cid]
+    c = WORKLOADS[cid]
     return grn_expression(c["G"], c["S"], config_seed(cid))[0]
 
 
