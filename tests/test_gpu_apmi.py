@@ -101,7 +101,7 @@ def test_rejects_out_of_range_ranks(handle):
 
 
 @pytest.mark.slow
-def test_config4_full(handle, torch_cuda):
+def test_ap_mi_2000_genes_1000_samples_full(handle, torch_cuda):
     """Config 4 (2000 x 1000): GPU jitter + rank + AP-MI vs the CPU oracle on the same ranks."""
     X = grn.expression(4)
     seed = grn.config_seed(4)

@@ -128,7 +128,7 @@ def test_mi_rejects_bad_input(handle):
 
 @pytest.mark.slow
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
-def test_config2_error_report(handle, torch_cuda, precision):
+def test_mi_2000_genes_1024_samples_error_report(handle, torch_cuda, precision):
     """Config 2 (2000 x 1024): MI error against the FP64 oracle on all pairs."""
     X = grn.expression(2)
     got = handle.bspline_mi(torch_cuda.from_numpy(X).cuda(), precision=precision).cpu().numpy().astype(np.float64)
@@ -140,7 +140,7 @@ def test_config2_error_report(handle, torch_cuda, precision):
 
 
 @pytest.mark.slow
-def test_config3_sampled_pairs(handle, torch_cuda):
+def test_mi_20000_genes_2000_samples_sampled_pairs(handle, torch_cuda):
     """Config 3 (20000 x 2000) BF16x3: exact symmetry everywhere; all pairs of 300 sampled genes vs FP64."""
     import torch
     X = grn.expression(3)

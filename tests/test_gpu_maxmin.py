@@ -128,7 +128,7 @@ def test_mcp_device_pointers(handle, torch_cuda, dt):
 
 
 @pytest.mark.slow
-def test_mcp3_config1_full(handle, torch_cuda):
+def test_mcp3_1000_genes_full(handle, torch_cuda):
     """MCP3 on a 1000-gene FP32 MI matrix (1000 samples), bit-exact against the CPU oracle."""
     X = grn.expression(1)
     with_gpu = handle.bspline_mi(torch_cuda.from_numpy(X).cuda(), precision="fp32")
