@@ -102,7 +102,7 @@ def test_rejects_out_of_range_ranks(handle):
 
 @pytest.mark.slow
 def test_ap_mi_2000_genes_1000_samples_full(handle, torch_cuda):
-    """Config 4 (2000 x 1000): GPU jitter + rank + AP-MI vs the CPU oracle on the same ranks."""
+    """2000 genes x 1000 samples: GPU jitter + rank + AP-MI vs the CPU oracle on the same ranks."""
     X = grn.expression(4)
     seed = grn.config_seed(4)
     Xd = torch_cuda.from_numpy(X).cuda()
