@@ -112,5 +112,5 @@ def test_ap_mi_2000_genes_1000_samples_full(handle, torch_cuda):
     got = handle.ap_mi(R).cpu().numpy().astype(np.float64)
     ref = oracle.ap_mi(Rh).astype(np.float64)
     err = np.abs(got - ref)
-    print(f"config4 AP-MI: max |dMI| vs oracle {err.max():.3e}, identical {np.mean(err == 0):.6f}")
+    print(f"2000x1000 AP-MI: max |dMI| vs oracle {err.max():.3e}, identical {np.mean(err == 0):.6f}")
     assert err.max() <= 1e-6

@@ -129,19 +129,19 @@ def test_mi_rejects_bad_input(handle):
 @pytest.mark.slow
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
 def test_mi_2000_genes_1024_samples_error_report(handle, torch_cuda, precision):
-    """Config 2 (2000 x 1024): MI error against the FP64 oracle on all pairs."""
+    """2000 genes x 1024 samples: MI error against the FP64 oracle on all pairs."""
     X = grn.expression(2)
     got = handle.bspline_mi(torch_cuda.from_numpy(X).cuda(), precision=precision).cpu().numpy().astype(np.float64)
     ref = oracle.bspline_mi_f64(X)
     err = np.abs(got - ref)
-    print(f"config2 {precision}: max |dMI| {err.max():.3e} mean {err.mean():.3e}")
+    print(f"2000x1024 {precision}: max |dMI| {err.max():.3e} mean {err.mean():.3e}")
     assert err.max() <= TOL[precision]
     assert np.array_equal(got, got.T)
 
 
 @pytest.mark.slow
 def test_mi_20000_genes_2000_samples_sampled_pairs(handle, torch_cuda):
-    """Config 3 (20000 x 2000) BF16x3: exact symmetry everywhere; all pairs of 300 sampled genes vs FP64."""
+    """20000 genes x 2000 samples, BF16x3: exact symmetry everywhere; all pairs of 300 sampled genes vs FP64."""
     import torch
     X = grn.expression(3)
     mi = handle.bspline_mi(torch_cuda.from_numpy(X).cuda(), precision="bf16x3")
@@ -154,5 +154,5 @@ def test_mi_20000_genes_2000_samples_sampled_pairs(handle, torch_cuda):
     gj = torch.from_numpy(genes[lj]).long().cuda()
     got = mi[gi, gj].cpu().numpy().astype(np.float64)
     err = np.abs(got - ref)
-    print(f"config3 bf16x3 sampled: max |dMI| {err.max():.3e} mean {err.mean():.3e}")
+    print(f"20000x2000 bf16x3 sampled: max |dMI| {err.max():.3e} mean {err.mean():.3e}")
     assert err.max() <= 1e-4
