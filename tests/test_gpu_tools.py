@@ -205,6 +205,39 @@ def test_mcpnet_ensemble(tmp_path):
     assert not Path(f"{prefix}.mcp_1.000_0.000_0.000_0.000.exp").exists()   # (1,0,0,0) is skipped
 
 
+def test_mcpnet_tf_mode_on_reference_gene_and_tf_lists(tmp_path):
+    """mcpnet --tf-input with the reference's own lists: the yeast TF names of
+    data/infergs_yeast_tfs.txt against the first 600 gene names of data/gnw2000_names.txt (both
+    copied to tests/golden/; most TF names are absent there), on a seeded symmetric MI-like
+    matrix: TF rows and MCP2/3/4 bit-exact against the oracle's masked products."""
+    gold = Path(__file__).with_name("golden")
+    names = (gold / "gnw2000_names.txt").read_text().split()[:600]
+    tf_list = (gold / "infergs_yeast_tfs.txt").read_text().split()
+    tf_set = set(tf_list)
+    tf_idx = [i for i, g in enumerate(names) if g in tf_set]
+    rng = np.random.default_rng(5)
+    A = rng.random((600, 600)) ** 3
+    A = np.maximum(A, A.T)
+    src = tmp_path / "mi.exp"
+    write_exp(src, A, names)
+    prefix = tmp_path / "gnw"
+    out = run("mcpnet", "-i", src, "-o", prefix, "--mi-input", "--tf-input", gold / "infergs_yeast_tfs.txt",
+              "-m", "1", "2", "3", "--format", "exp")
+    assert f"TFs specified {len(tf_list)}, found {len(tf_idx)}" in out
+    Ain = read_exp(src)[1]
+    np.fill_diagonal(Ain, 0.0)
+    mask = np.full(600, -np.finfo(np.float64).max)
+    mask[tf_idx] = np.finfo(np.float64).max
+    T = np.ascontiguousarray(Ain[tf_idx])
+    mm = oracle.maxmin
+    G1 = mm(Ain, Ain, mask)
+    want = {2: mm(T, Ain, mask), 3: mm(T, G1, mask), 4: mm(mm(T, Ain, mask), G1, mask)}
+    for k in (2, 3, 4):
+        got_names, got = read_exp(f"{prefix}_tf_inmi.mcp{k}.exp")
+        assert got_names == [names[t] for t in tf_idx]
+        assert np.array_equal(got, oracle.ratio(T, want[k])), k
+
+
 def test_mcpnet_tf_mode(tmp_path):
     """mcpnet --tf-input: MCP2/3/4 from the TF rows (compute_mcp{2,3,4}_tfs with transitions 1, 2, 3)
     and the TF ensemble (compute_maxmins_tfs, combo on the TF rows, compute_stouffer_tf), in
