@@ -17,6 +17,7 @@ BIN = ROOT / "mcpnet_amd" / "bin"
 R10 = Path(__file__).with_name("golden") / "r10c10.exp"
 R10_H5 = Path(__file__).with_name("golden") / "r10c10.h5"
 R128_H5 = Path(__file__).with_name("golden") / "r128c1000.h5"
+R127_H5 = Path(__file__).with_name("golden") / "r127c999.h5"
 
 
 def read_exp(path):
@@ -71,6 +72,24 @@ def test_mcpnet_on_reference_h5(tmp_path):
     A = mi.astype(np.float32)
     got = h5_to_exp(str(prefix) + "_bs.mcp3.h5")[1].astype(np.float32)
     assert np.array_equal(got, oracle.mcp(A, 3)[0])
+
+
+def test_mcpnet_on_odd_sized_reference_h5(tmp_path):
+    """mcpnet -i data/r127c999.h5 (127 genes x 999 samples: neither a tile nor a K-step multiple)
+    --mi-method 0 -m 1 2 3: MI against the FP64 oracle, MCP2/3/4 bit-exact on the written MI."""
+    prefix = tmp_path / "r127"
+    run("mcpnet", "-i", R127_H5, "-o", prefix, "--mi-method", "0", "-m", "1", "2", "3", "--single")
+    X = h5_to_exp(R127_H5)[1].astype(np.float32)
+    assert X.shape == (127, 999)
+    names, mi = h5_to_exp(str(prefix) + "_bs.mi.h5")
+    assert len(names) == 127
+    ref = oracle.bspline_mi_f64(X)
+    np.fill_diagonal(ref, 0.0)
+    assert np.abs(mi - ref).max() <= 1e-5
+    A = mi.astype(np.float32)
+    for k in (2, 3, 4):
+        got = h5_to_exp(str(prefix) + f"_bs.mcp{k}.h5")[1].astype(np.float32)
+        assert np.array_equal(got, oracle.mcp(A, k)[0]), k
 
 
 def test_mi_adaptive_partitioning_double(tmp_path):
