@@ -230,31 +230,24 @@ __device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 
-  // register-resident LIFO stack
-  int sx1[kStackSlots], sy1[kStackSlots], sx2[kStackSlots], sy2[kStackSlots], sn[kStackSlots];
+  // register-resident LIFO stack: entry k in lane k, with x1, x2 and y1, y2 packed as 16-bit
+  // halves (ranks are below 2^13): a push writes three registers in one lane, a read is three
+  // readlanes
+  int sxs = 0, sys = 0, sn = 0;
   int sp = 0;  // number of entries
   bool whole = true;
   auto push = [&](const Win& w) {
-    const int slot = sp >> 6, ln = sp & 63;
-#pragma unroll
-    for (int k = 0; k < kStackSlots; ++k)
-      if (k == slot && lane == ln) { sx1[k] = w.x1; sy1[k] = w.y1; sx2[k] = w.x2; sy2[k] = w.y2; sn[k] = w.n; }
+    if (lane == sp) {
+      sxs = (w.x1 << 16) | w.x2;
+      sys = (w.y1 << 16) | w.y2;
+      sn = w.n;
+    }
     ++sp;
   };
   auto top = [&]() {
-    const int idx = sp - 1;
-    const int slot = idx >> 6, ln = idx & 63;
-    Win w{0, 0, 0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < kStackSlots; ++k)
-      if (k == slot) {
-        w.x1 = __builtin_amdgcn_readlane(sx1[k], ln);
-        w.y1 = __builtin_amdgcn_readlane(sy1[k], ln);
-        w.x2 = __builtin_amdgcn_readlane(sx2[k], ln);
-        w.y2 = __builtin_amdgcn_readlane(sy2[k], ln);
-        w.n = __builtin_amdgcn_readlane(sn[k], ln);
-      }
-    return w;
+    const int ln = sp - 1;
+    const int xs = __builtin_amdgcn_readlane(sxs, ln), ys = __builtin_amdgcn_readlane(sys, ln);
+    return Win{xs >> 16, ys >> 16, xs & 0xffff, ys & 0xffff, __builtin_amdgcn_readlane(sn, ln)};
   };
   // Leaf terms (compute_for_window, :525-538) are batched: leaf k of a batch is held by lane k,
   // a full batch (or the last one) evaluates n log2(n / (nx ny)) on all lanes at once, and the
