@@ -742,9 +742,10 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 2 * KeyChunk<32>::kStageBytes));
     attr_set = true;
   }
-  // Split tail: the tiles beyond the last full round of resident workgroups would leave most
-  // of the GPU idle for a whole tile time; they run instead as several workgroups each over a
-  // range of K, folding exact partial maxima with atomicMax on the keys.
+  // Split tail: the tiles beyond the last full round of resident workgroups (all tiles, for a
+  // grid smaller than one round) would leave most of the GPU idle for a whole tile time; they
+  // run instead as several workgroups each over a range of K, folding exact partial maxima with
+  // atomicMax on the keys.
   int& slots = h->mm_slots[kc == 16 ? 0 : 1];
   if (slots == 0) {
     int per_cu = 0, cus = 0;
@@ -760,7 +761,7 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
   {
     const int64_t full = a.ntiles / slots * slots, ntail = a.ntiles - full;
     const int64_t splits = ntail > 0 ? std::min<int64_t>(slots / ntail, nk) : 1;
-    if (mcpg::env_group("MCPG_MM_SPLIT", 1) != 0 && full > 0 && splits >= 2) {
+    if (mcpg::env_group("MCPG_MM_SPLIT", 1) != 0 && splits >= 2) {
       const size_t need = (size_t)ntail * kBM * kBN * sizeof(int);
       if (need > h->mm_tail_bytes) {
         MCPG_HIP(h, hipStreamSynchronize(h->stream));
