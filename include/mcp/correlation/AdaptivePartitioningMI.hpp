@@ -1,37 +1,55 @@
 // Adaptive-partitioning MI: the reference's kernel functor names
 // (include/mcp/correlation/AdaptivePartitioningMI.hpp:40-647 of MCPNet) with the patterns that
 // apply them specialised to the GPU:
-//   add_noise<T>::operator()(in, out)                                   -> mcpg_add_noise_f32
+//   add_noise<T, Distribution, Generator>::operator()(in, out)          -> mcpg_add_noise_f32
 //   Transform<M<IT>, IntegralCumulativeHistogramKernel<IT,0>, M<IT>>    -> mcpg_cumhist_i32
 //   InnerProduct<M<IT>, AdaptivePartitionRankMIKernel2<IT,OT,0>, M<OT>> -> mcpg_ap_mi_i32
 // The reference's add_noise draws from per-thread std::default_random_engine streams of splash
 // (absent); here the jitter is a seeded counter-based stream, identical for any thread count.
+// Like the reference header, it brings in splash/kernel/random.hpp for the
+// random_number_generator its callers construct.
 #pragma once
 
 #include <cstdint>
+#include <random>
+#include <type_traits>
 #include <vector>
 
 #include "mcpgpu/convert.hpp"
 #include "splash/ds/aligned_matrix.hpp"
-#include "splash/kernel/rank.hpp"
-#include "splash/pattern/patterns.hpp"
+#include "splash/kernel/random.hpp"
+#include "splash/transform/rank.hpp"
+#include "splash/patterns/pattern.hpp"
 
 namespace mcp {
 namespace kernel {
 
-// add_noise (AdaptivePartitioningMI.hpp:40-114): out = in + U[mn, mx); only [0, 1e-5) is
-// supported on the GPU, the range the reference's callers use (mi.cpp:130, mcpnet_helper.hpp:113).
-template <typename IT>
+// add_noise (AdaptivePartitioningMI.hpp:37-114): out = in + U[mn, mx), with the reference's
+// template parameters and constructor (mcpnet_helper.hpp:111-114, mi.cpp:128-131).  The jitter
+// is the GPU's counter-based stream under gen.seed(); only the range the reference's callers
+// use, [0, 1e-5), is supported, and any other range is an MCPG_ERR_UNSUPPORTED error.
+template <typename IT, typename Distribution = std::uniform_real_distribution<IT>,
+          typename Generator = std::default_random_engine>
 class add_noise {
-  uint64_t seed_;
+  static_assert(std::is_same<Distribution, std::uniform_real_distribution<IT>>::value,
+                "the GPU jitter is uniform");
+  uint64_t seed_ = splash::kernel::random_number_generator<Generator>::default_seed;
+  IT mn_ = 0.0, mx_ = 0.00001;
 
  public:
-  explicit add_noise(uint64_t seed = 0x4D43504E04ull, IT const& mn = 0.0, IT const& mx = 0.00001) : seed_(seed) {
-    if (!(mn == static_cast<IT>(0.0) && mx == static_cast<IT>(0.00001)))
-      throw mcpgpu::error(MCPG_ERR_UNSUPPORTED, "add_noise: the GPU jitter is U[0, 1e-5)");
+  add_noise() {}
+  add_noise(splash::kernel::random_number_generator<Generator>& gen, IT const& mn = 0.0, IT const& mx = 1.0)
+      : seed_(gen.seed()), mn_(mn), mx_(mx) {}
+  virtual ~add_noise() {}
+  void copy_parameters(const add_noise& o) {
+    seed_ = o.seed_;
+    mn_ = o.mn_;
+    mx_ = o.mx_;
   }
-  void copy_parameters(const add_noise& o) { seed_ = o.seed_; }
+  uint64_t seed() const { return seed_; }
   void operator()(const splash::ds::aligned_matrix<IT>& in, splash::ds::aligned_matrix<IT>& out) const {
+    if (!(mn_ == static_cast<IT>(0.0) && mx_ == static_cast<IT>(0.00001)))
+      throw mcpgpu::error(MCPG_ERR_UNSUPPORTED, "add_noise: the GPU jitter is U[0, 1e-5)");
     const size_t G = in.rows(), S = in.columns();
     std::vector<float> x = mcpgpu::packed<float>(in), y(G * S);
     mcpgpu::check(mcpg_add_noise_f32(mcpgpu::thread_handle(), MCPG_HOST, x.data(), G, S, S, seed_, y.data(), S),

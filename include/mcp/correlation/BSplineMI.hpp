@@ -14,7 +14,7 @@
 
 #include "mcpgpu/convert.hpp"
 #include "splash/ds/aligned_matrix.hpp"
-#include "splash/pattern/patterns.hpp"
+#include "splash/patterns/pattern.hpp"
 
 namespace mcp {
 namespace kernel {

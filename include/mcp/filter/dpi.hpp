@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "splash/ds/aligned_matrix.hpp"
-#include "splash/pattern/patterns.hpp"
+#include "splash/patterns/pattern.hpp"
 
 namespace mcp {
 namespace kernel {

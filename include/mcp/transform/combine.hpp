@@ -7,7 +7,7 @@
 #include <type_traits>
 
 #include "splash/ds/aligned_matrix.hpp"
-#include "splash/pattern/patterns.hpp"
+#include "splash/patterns/pattern.hpp"
 
 namespace mcp {
 namespace kernel {

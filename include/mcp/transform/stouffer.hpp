@@ -10,7 +10,7 @@
 
 #include "splash/ds/aligned_matrix.hpp"
 #include "splash/kernel/gaussian.hpp"
-#include "splash/pattern/patterns.hpp"
+#include "splash/patterns/pattern.hpp"
 
 namespace mcp {
 namespace kernel {

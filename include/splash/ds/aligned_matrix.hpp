@@ -60,8 +60,9 @@ class aligned_matrix {
     if (ld_ == 0) ld_ = per;
     const size_t bytes = rows_ * ld_ * sizeof(T);
     if (bytes) {
-      data_ = static_cast<T*>(std::aligned_alloc(kAlign, (bytes + kAlign - 1) / kAlign * kAlign));
-      if (!data_) throw std::bad_alloc();
+      void* p = nullptr;
+      if (posix_memalign(&p, kAlign, (bytes + kAlign - 1) / kAlign * kAlign) != 0 || !p) throw std::bad_alloc();
+      data_ = static_cast<T*>(p);
       std::memset(data_, 0, bytes);
     }
   }

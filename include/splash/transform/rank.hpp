@@ -8,7 +8,7 @@
 #include <vector>
 
 #include "mcpgpu/convert.hpp"
-#include "splash/pattern/patterns.hpp"
+#include "splash/patterns/pattern.hpp"
 
 namespace splash {
 namespace kernel {

@@ -163,6 +163,20 @@ def _empty_like_kind(kind, shape, dtype, ref):
     return np.empty(shape, dtype=dtype)
 
 
+def _out_mat(out, kind, shape, dtype, ref, name):
+    """An output operand: allocated like the inputs when not given, else checked to be of the
+    inputs' kind (host or device), the expected shape and dtype, with contiguous rows."""
+    if out is None:
+        out = _empty_like_kind(kind, shape, dtype, ref)
+    m = _Mat(out, dtype, name)
+    if m.kind != kind:
+        raise TypeError(f"{name}: the inputs are {'device' if kind == MCPG_DEVICE else 'host'} arrays, so the output "
+                        f"must be one too")
+    if m.shape != tuple(shape):
+        raise ValueError(f"{name}: expected shape {tuple(shape)}, got {m.shape}")
+    return out, m
+
+
 def _same_kind(*mats):
     kinds = {m.kind for m in mats if m is not None}
     if len(kinds) > 1:
@@ -241,8 +255,7 @@ class Handle:
     def minmax_scale(self, X, out=None):
         x = _Mat(X, np.float32, "X")
         G, S = x.shape
-        out = _empty_like_kind(x.kind, (G, S), np.float32, X) if out is None else out
-        y = _Mat(out, np.float32, "out")
+        out, y = _out_mat(out, x.kind, (G, S), np.float32, X, "out")
         self._pre(x.kind)
         self._check(self.lib.mcpg_minmax_scale_f32(self.h, x.kind, x.ptr, G, S, x.ld, y.ptr, y.ld), "mcpg_minmax_scale_f32")
         return out
@@ -250,8 +263,7 @@ class Handle:
     def bspline_weights(self, Xs, bins=10, order=3, out=None):
         x = _Mat(Xs, np.float32, "Xs")
         G, S = x.shape
-        out = _empty_like_kind(x.kind, (G, bins * S + 1), np.float32, Xs) if out is None else out
-        w = _Mat(out, np.float32, "out")
+        out, w = _out_mat(out, x.kind, (G, bins * S + 1), np.float32, Xs, "out")
         self._pre(x.kind)
         self._check(self.lib.mcpg_bspline_weights_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order, w.ptr, w.ld),
                     "mcpg_bspline_weights_f32")
@@ -260,8 +272,7 @@ class Handle:
     def bspline_mi_weights(self, W, S, bins=10, precision="fp32", out=None):
         w = _Mat(W, np.float32, "W")
         G = w.shape[0]
-        out = _empty_like_kind(w.kind, (G, G), np.float32, W) if out is None else out
-        m = _Mat(out, np.float32, "out")
+        out, m = _out_mat(out, w.kind, (G, G), np.float32, W, "out")
         self._pre(w.kind)
         self._check(self.lib.mcpg_bspline_mi_weights_f32(self.h, w.kind, w.ptr, G, S, w.ld, bins,
                                                          PRECISIONS[precision], m.ptr, m.ld),
@@ -271,8 +282,7 @@ class Handle:
     def bspline_mi(self, X, bins=10, order=3, precision="fp32", out=None):
         x = _Mat(X, np.float32, "X")
         G, S = x.shape
-        out = _empty_like_kind(x.kind, (G, G), np.float32, X) if out is None else out
-        m = _Mat(out, np.float32, "out")
+        out, m = _out_mat(out, x.kind, (G, G), np.float32, X, "out")
         _same_kind(x, m)
         self._pre(x.kind)
         self._check(self.lib.mcpg_bspline_mi_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order,
@@ -288,8 +298,7 @@ class Handle:
         mm = np.ascontiguousarray(m, dtype=np.float32)
         if not (pi.shape == pj.shape == mm.shape):
             raise ValueError("pair arrays must have the same length")
-        out = _empty_like_kind(x.kind, (G, S), np.float32, X) if out is None else out
-        d = _Mat(out, np.float32, "out")
+        out, d = _out_mat(out, x.kind, (G, S), np.float32, X, "out")
         L = c_double()
         self._pre(x.kind)
         self._check(self.lib.mcpg_bspline_mi_backward_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order,
@@ -302,8 +311,7 @@ class Handle:
     def add_noise(self, X, seed, out=None):
         x = _Mat(X, np.float32, "X")
         G, S = x.shape
-        out = _empty_like_kind(x.kind, (G, S), np.float32, X) if out is None else out
-        y = _Mat(out, np.float32, "out")
+        out, y = _out_mat(out, x.kind, (G, S), np.float32, X, "out")
         self._pre(x.kind)
         self._check(self.lib.mcpg_add_noise_f32(self.h, x.kind, x.ptr, G, S, x.ld, c_uint64(seed), y.ptr, y.ld),
                     "mcpg_add_noise_f32")
@@ -312,8 +320,7 @@ class Handle:
     def rank(self, X, out=None):
         x = _Mat(X, np.float32, "X")
         G, S = x.shape
-        out = _empty_like_kind(x.kind, (G, S), np.int32, X) if out is None else out
-        r = _Mat(out, np.int32, "out")
+        out, r = _out_mat(out, x.kind, (G, S), np.int32, X, "out")
         self._pre(x.kind)
         self._check(self.lib.mcpg_rank_f32(self.h, x.kind, x.ptr, G, S, x.ld, r.ptr, r.ld), "mcpg_rank_f32")
         return out
@@ -321,8 +328,7 @@ class Handle:
     def cumhist(self, R, out=None):
         r = _Mat(R, np.int32, "R")
         G, S = r.shape
-        out = _empty_like_kind(r.kind, (G, S + 1), np.int32, R) if out is None else out
-        hm = _Mat(out, np.int32, "out")
+        out, hm = _out_mat(out, r.kind, (G, S + 1), np.int32, R, "out")
         self._pre(r.kind)
         self._check(self.lib.mcpg_cumhist_i32(self.h, r.kind, r.ptr, G, S, r.ld, hm.ptr, hm.ld), "mcpg_cumhist_i32")
         return out
@@ -330,8 +336,7 @@ class Handle:
     def ap_mi(self, R, out=None):
         r = _Mat(R, np.int32, "R")
         G, S = r.shape
-        out = _empty_like_kind(r.kind, (G, G), np.float32, R) if out is None else out
-        m = _Mat(out, np.float32, "out")
+        out, m = _out_mat(out, r.kind, (G, G), np.float32, R, "out")
         self._pre(r.kind)
         self._check(self.lib.mcpg_ap_mi_i32(self.h, r.kind, r.ptr, G, S, r.ld, m.ptr, m.ld), "mcpg_ap_mi_i32")
         return out
@@ -368,8 +373,7 @@ class Handle:
         if km is not None and km.shape[0] != K:
             raise ValueError("kmask length must equal K")
         kind = _same_kind(a, b, km)
-        out = _empty_like_kind(kind, (M, N), dt, A) if out is None else out
-        c = _Mat(out, dt, "out")
+        out, c = _out_mat(out, kind, (M, N), dt, A, "out")
         fn = self.lib.mcpg_maxmin_f64 if dt == np.float64 else self.lib.mcpg_maxmin_f32
         self._pre(kind)
         self._check(fn(self.h, kind, a.ptr, a.ld, b.ptr, b.ld, km.ptr if km else None, M, N, K, int(symmetric),
@@ -383,8 +387,7 @@ class Handle:
         if a.shape != m.shape:
             raise ValueError("ratio operands must have the same shape")
         kind = _same_kind(a, m)
-        out = _empty_like_kind(kind, a.shape, dt, A) if out is None else out
-        o = _Mat(out, dt, "out")
+        out, o = _out_mat(out, kind, a.shape, dt, A, "out")
         fn = self.lib.mcpg_ratio_f64 if dt == np.float64 else self.lib.mcpg_ratio_f32
         self._pre(kind)
         self._check(fn(self.h, kind, a.ptr, a.ld, m.ptr, m.ld, a.shape[0], a.shape[1], o.ptr, o.ld), "mcpg_ratio")
@@ -397,8 +400,7 @@ class Handle:
         if a.shape != b.shape:
             raise ValueError("operands must have the same shape")
         kind = _same_kind(a, b)
-        out = _empty_like_kind(kind, a.shape, dt, A) if out is None else out
-        o = _Mat(out, dt, "out")
+        out, o = _out_mat(out, kind, a.shape, dt, A, "out")
         fn = self.lib.mcpg_elementwise_max_f64 if dt == np.float64 else self.lib.mcpg_elementwise_max_f32
         self._pre(kind)
         self._check(fn(self.h, kind, a.ptr, a.ld, b.ptr, b.ld, a.shape[0], a.shape[1], o.ptr, o.ld),
@@ -412,12 +414,10 @@ class Handle:
         N = a.shape[0]
         if a.shape[1] != N:
             raise ValueError("MCP needs a square matrix")
-        out = _empty_like_kind(a.kind, (N, N), dt, A) if out is None else out
-        o = _Mat(out, dt, "out")
+        out, o = _out_mat(out, a.kind, (N, N), dt, A, "out")
         mm = None
         if want_maxmin or maxmin_out is not None:
-            maxmin_out = _empty_like_kind(a.kind, (N, N), dt, A) if maxmin_out is None else maxmin_out
-            mm = _Mat(maxmin_out, dt, "maxmin_out")
+            maxmin_out, mm = _out_mat(maxmin_out, a.kind, (N, N), dt, A, "maxmin_out")
         fn = self.lib.mcpg_mcp_f64 if dt == np.float64 else self.lib.mcpg_mcp_f32
         self._pre(a.kind)
         self._check(fn(self.h, a.kind, a.ptr, a.ld, N, int(k), mm.ptr if mm else None, mm.ld if mm else 0, o.ptr, o.ld),
@@ -438,8 +438,9 @@ class Handle:
         R = G
         if tfs is not None:
             tf = _Mat(tfs, dt, "tfs")
-            if tf.shape[0] != G:
+            if tf.shape != (G,):
                 raise ValueError("tfs needs one value per gene")
+            _same_kind(x, tf)
             vals = tfs.detach().cpu().numpy() if hasattr(tfs, "detach") else np.asarray(tfs)
             R = int(np.count_nonzero(vals > 0))
         res = {}
@@ -490,8 +491,7 @@ class Handle:
             if m is not None and m.shape != (M, N):
                 raise ValueError("operands must have the same shape")
         kind = _same_kind(mi, *mats)
-        out = _empty_like_kind(kind, (M, N), dt, MI) if out is None else out
-        o = _Mat(out, dt, "out")
+        out, o = _out_mat(out, kind, (M, N), dt, MI, "out")
         args = []
         for m in mats:
             args += [m.ptr if m else None, m.ld if m else 0]
@@ -507,8 +507,7 @@ class Handle:
         N = x.shape[0]
         if x.shape[1] != N:
             raise ValueError("Stouffer needs a square matrix")
-        out = _empty_like_kind(x.kind, (N, N), dt, X) if out is None else out
-        o = _Mat(out, dt, "out")
+        out, o = _out_mat(out, x.kind, (N, N), dt, X, "out")
         fn = self.lib.mcpg_stouffer_f64 if dt == np.float64 else self.lib.mcpg_stouffer_f32
         self._pre(x.kind)
         self._check(fn(self.h, x.kind, x.ptr, x.ld, N, int(exclude_ones), o.ptr, o.ld), "mcpg_stouffer")
@@ -519,8 +518,7 @@ class Handle:
         dt = np.float64 if str(getattr(X, "dtype", "")).endswith("float64") else np.float32
         x = _Mat(X, dt, "X")
         M, N = x.shape
-        out = _empty_like_kind(x.kind, (M, N), dt, X) if out is None else out
-        o = _Mat(out, dt, "out")
+        out, o = _out_mat(out, x.kind, (M, N), dt, X, "out")
         fn = self.lib.mcpg_stouffer_tf_f64 if dt == np.float64 else self.lib.mcpg_stouffer_tf_f32
         self._pre(x.kind)
         self._check(fn(self.h, x.kind, x.ptr, x.ld, M, N, o.ptr, o.ld), "mcpg_stouffer_tf")
@@ -538,8 +536,7 @@ class Handle:
         if tf is not None and tf.shape != (N,):
             raise ValueError("tfs must have one entry per gene")
         kind = _same_kind(a, tf)
-        out = _empty_like_kind(kind, (N, N), dt, A) if out is None else out
-        o = _Mat(out, dt, "out")
+        out, o = _out_mat(out, kind, (N, N), dt, A, "out")
         mask = None
         mk = None
         if want_mask:

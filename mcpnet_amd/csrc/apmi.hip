@@ -26,6 +26,7 @@ constexpr int64_t kMaxS = kLdsBudget / 8 - 1;  // keeps the per-gene histogram a
 // grows by at most 3 per level, and every split halves both rank ranges: at most ceil(log2 S)
 // = 13 levels for S <= 8191, so at most 40 entries (the overflow flag stays as a guard).
 constexpr int kStackSlots = 1;
+static_assert(kStackSlots == 1, "the stack's push and top address lane sp directly: one 64-lane slot");
 
 struct Win {
   int x1, y1, x2, y2, n;
@@ -301,7 +302,7 @@ __device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_
         const Win& c = q[i];
         const bool splittable = ((c.x1 + 1 < c.x2) || (c.y1 + 1 < c.y2)) && (c.n > 2);
         if (splittable) {
-          if (sp >= 64 * kStackSlots) { overflow = 1; continue; }
+          if (sp >= 64) { overflow = 1; continue; }
           push(c);
         } else if (c.n > 0) {
           term(c);

@@ -8,7 +8,7 @@
 #include "common.hpp"
 #include "mcp/correlation/AdaptivePartitioningMI.hpp"
 #include "mcp/correlation/BSplineMI.hpp"
-#include "splash/kernel/rank.hpp"
+#include "splash/transform/rank.hpp"
 
 namespace {
 
@@ -35,7 +35,10 @@ int run(const cli::Parser& p) {
   if (method == 1) {
     t = tool::Timer();
     Mat noisy(input.rows(), input.columns());
-    mcp::kernel::add_noise<DataType> noise_adder(p.integer("--noise-seed"));
+    using NoiseKernelType = mcp::kernel::add_noise<DataType, std::uniform_real_distribution<DataType>,
+                                                   std::default_random_engine>;
+    splash::kernel::random_number_generator<> gen(p.integer("--noise-seed"));
+    NoiseKernelType noise_adder(gen, 0.0, 0.00001);
     noise_adder(input, noisy);
     printf("Noise added in %f sec\n", t.seconds());
     t = tool::Timer();
