@@ -10,7 +10,8 @@ durations from the library's event bracketing of each launch.
 
 Prints ONE JSON line.  value = MI gene-pairs/s of the MI stage; mcp3_maxmin_gops = 1.6e13
 algorithmic max-min ops / MCP3 stage time (BASELINE's unit: one (min, max) per (i, k, j) triple,
-M*N*K per product).
+M*N*K per product); mcp3_maxmin_gops_executed counts only the triples the triangle launches
+compute (about half).
 """
 from __future__ import annotations
 
@@ -60,6 +61,26 @@ def cpu_model():
     return "unknown"
 
 
+SIMD_FLAGS = ("sse4_2", "avx", "avx2", "fma", "avx512f", "avx512bw", "avx512vl", "avx512dq", "avx512_bf16",
+              "avx512_vnni", "amx_bf16")
+
+
+def cpu_simd_flags():
+    """The SIMD extensions of the host CPU the baselines may use (from /proc/cpuinfo)."""
+    try:
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("flags"):
+                have = set(ln.split(":", 1)[1].split())
+                return [f for f in SIMD_FLAGS if f in have]
+    except OSError:
+        pass
+    return []
+
+
+def cpu_host():
+    return {"model": cpu_model(), "simd": cpu_simd_flags(), "threads": omp_threads()}
+
+
 def cpu_baselines(X, S, bins, mi_host, budget_s):
     """Bounded CPU samples: the FP32 B-spline MI port (reference inner product) and the
     reference's own max_of_pairmin (oracle/_ref) on the same data."""
@@ -79,7 +100,7 @@ def cpu_baselines(X, S, bins, mi_host, budget_s):
             break
         rows = int(min(C, rows * max(2.0, 0.9 * budget_s / max(t, 1e-3))))
     out["mi"] = {
-        "value": rows * C / t, "unit": "gene-pairs/s", "cores": omp_threads(), "kind": "port",
+        "value": rows * C / t, "unit": "gene-pairs/s", "cores": omp_threads(), "kind": "port", "host": cpu_host(),
         "sample": f"FP32 B-spline MI inner product (100 dots of S={S} + log2 per pair, OpenMP) on "
                   f"{rows} x {C} gene pairs of the same data, {t:.1f} s; host {cpu_model()}",
     }
@@ -97,6 +118,7 @@ def cpu_baselines(X, S, bins, mi_host, budget_s):
             r = int(min(N, r * max(2.0, 0.9 * budget_s / max(t, 1e-3))))
         out["maxmin"] = {
             "value": r * N * N / t / 1e9, "unit": "Gop/s", "cores": omp_threads(), "kind": "reference",
+            "host": cpu_host(),
             "sample": f"reference max_of_pairmin (simd_maxmin.hpp, float = scalar path) over rows 0..{r - 1} "
                       f"of MI (x) MI, N = K = {N}, {t:.1f} s",
         }
@@ -171,6 +193,7 @@ def bench_ap(args, G, S, seed):
         t = time.perf_counter() - t0
         got = MI[:rows].cpu().numpy()
         result["cpu_baseline"] = {"value": rows * G / t, "unit": "gene-pairs/s", "cores": omp_threads(), "kind": "port",
+                                  "host": cpu_host(),
                                   "sample": f"AdaptivePartitionRankMIKernel2 restatement (OpenMP) on rows 0..{rows - 1} "
                                             f"x {G} genes, {t:.1f} s; host {cpu_model()}"}
         result["ranks_bit_exact"] = ok
@@ -432,6 +455,8 @@ def main():
     # dominant kernel = largest time per step
     per_step = {n: float(np.sum(v)) / args.steps for n, v in kern.items()}
     dominant = max(per_step, key=per_step.get) if per_step else None
+    # HBM bytes per launch: FETCH_SIZE and WRITE_SIZE of the same kernel on the same workload
+    # (rocprofv3 --pmc runs, copy-calibrated), kept in profiles/traffic.json
     traffic_file = ROOT / "profiles" / "traffic.json"
     if traffic_file.exists():
         try:
@@ -440,8 +465,17 @@ def main():
                 t = tr.get(n, tr.get(n.replace(",tri", "")))
                 if t is not None:
                     roof[n]["traffic"] = t
+                    roof[n]["traffic_source"] = "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
         except (OSError, ValueError):
             pass
+    for n, r in roof.items():
+        r["hbm_gbs"] = r["traffic"] / (r["mean_ms"] / 1e3) / 1e9 if r.get("traffic") else None
+        r["hbm_frac"] = r["hbm_gbs"] / PEAK_HBM_GBS if r["hbm_gbs"] else None
+    # max-min work the launches executed: a triangle launch produces G (G + 1) / 2 outputs and
+    # mirrors them, a full launch G^2, each over K = G
+    mm_launches = {n: len(v) / args.steps for n, v in kern.items() if n.startswith("maxmin_key_kernel")}
+    mcp_ops_executed = sum(c * (G * (G + 1) / 2 if n.endswith(",tri>") else G * G) * G
+                           for n, c in mm_launches.items())
 
     # ---- FP32 MI pass for the BF16-split / FP32 speed ratio and accuracy on sampled pairs
     extra = {}
@@ -517,6 +551,12 @@ def main():
                                + ", ".join(f"MCP{k}" for k in mcp_orders) + " (max-min products + ratio)",
                    "genes": G, "samples": S, "bins": bins, "spline_order": 3, "parallelism": "single GPU"},
         "mcp3_maxmin_gops": mcp3_gops,
+        "mcp3_maxmin_gops_executed": mcp_ops_executed / t_mcp / 1e9,
+        "skipped_work": (f"mcp3_maxmin_gops credits G^3 = {G ** 3:.3g} (min, max) triples per product (BASELINE's unit). "
+                         f"A, A(x)A and A(x)(A(x)A) are symmetric, so a triangle launch (',tri') computes only the upper "
+                         f"triangle of its output and mirrors it: {mcp_ops_executed:.3g} executed triples per step "
+                         f"({', '.join(f'{c:g} x {n}' for n, c in mm_launches.items())}); "
+                         f"mcp3_maxmin_gops_executed is that count / MCP stage time"),
         "stage_ms": {"mi": t_mi * 1e3, "mcp3": t_mcp * 1e3},
         "wall_s_timed_region": t_wall,
         "roofline": dict(roof.get(dominant, {}), kernel=dominant) if dominant in roof else None,

@@ -121,6 +121,33 @@ def test_mcp_matches_oracle_bit_exact(tmp_path, method, k):
     assert np.array_equal(read_exp(mm)[1], ref_mm)
 
 
+def test_mcp_from_partials(tmp_path):
+    """-m 5 (mcp.cpp:618-655): max-min of the --first and --second partials, row . row, then the
+    ratio against the MI input.  The second partial is read from --second (the reference re-reads
+    --first, mcp.cpp:192); the partials differ here, so reading --first twice would fail."""
+    X = grn.grn_expression(40, 200, 12)[0]
+    A = oracle.bspline_mi_f64(X)
+    np.fill_diagonal(A, 0.0)
+    src = tmp_path / "mi.exp"
+    write_exp(src, A)
+    m1 = tmp_path / "m1.exp"
+    run("mcp", "-i", src, "-o", tmp_path / "mcp2.exp", "-m", 1, "-x", m1)     # first partial: A (x) A
+    out = tmp_path / "mcp_p.exp"
+    mm = tmp_path / "mm_p.exp"
+    run("mcp", "-i", src, "-o", out, "-m", 5, "--first", m1, "--second", src, "-x", mm)
+    Ain = read_exp(src)[1]
+    M1 = read_exp(m1)[1]
+    ref_mm = oracle.maxmin(M1, Ain)
+    assert not np.array_equal(ref_mm, oracle.maxmin(M1, M1))
+    assert np.array_equal(read_exp(mm)[1], ref_mm)
+    assert np.array_equal(read_exp(out)[1], oracle.ratio(Ain, ref_mm))
+    tf_file = tmp_path / "tfs.txt"
+    tf_file.write_text("g0\ng3\n")
+    res = subprocess.run([str(BIN / "mcp"), "-i", str(src), "-o", str(out), "-m", "5", "--first", str(m1),
+                          "--second", str(src), "-f", str(tf_file)], capture_output=True, text=True, timeout=120)
+    assert res.returncode != 0 and "partials" in res.stdout + res.stderr
+
+
 def test_mcp_tf_masked(tmp_path):
     X = grn.grn_expression(30, 200, 6)[0]
     A = oracle.bspline_mi_f64(X)
