@@ -183,6 +183,20 @@ int mcpg_mcp_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, int64_t N
 int mcpg_mcp_f64(mcpg_handle h, int ptrs, const double* A, int64_t lda, int64_t N, int k,
                  double* maxmin_out, int64_t ldm, double* mcp_out, int64_t ldo);
 
+/* Allocation-free MCP2/3/4.  No counterpart in the reference: this entry point is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It computes what mcpg_mcp_f32 computes, with the same
+ * kernels and the same bits, on DEVICE pointers only (there is no ptrs argument), and allocates
+ * nothing: the order keys of A and of M1, the triangle tile order and the split-tail key blocks
+ * live in a static 64 MiB device array of the library, which each GPU the library runs on holds
+ * until the process ends.  Limit: the keys take 4 * ceil(N/128)*128 * ceil(N/32)*32 bytes per
+ * matrix (one matrix for k = 2, two for k = 3, 4), so N <= 4064 for k = 2 and N <= 2848 for
+ * k = 3, 4; a larger N returns MCPG_ERR_UNSUPPORTED.  Every handle on one GPU shares that GPU's
+ * array and each call overwrites it: calls that overlap on one GPU (two host threads, two
+ * handles) race and must be serialised by the caller. */
+int mcpg_mcp_f32_noalloc(mcpg_handle h, const float* A, int64_t lda, int64_t N, int k,
+                         float* maxmin_out, int64_t ldm, float* mcp_out, int64_t ldo);
+
 /* The mcpnet stage chain (apps/pipeline/mcpnet.cpp:286-515 with compute_mi / compute_ap_mi, the
  * diagonal, filter_local_mat_rows_by_tfs and compute_mcp{2,3,4}[_tfs], mcpnet_helper.hpp:56-435)
  * in one call, every intermediate in device memory: with MCPG_HOST pointers the only transfers

@@ -63,6 +63,7 @@ _SIGS = {
     "mcpg_elementwise_max_f64": [_P, c_int, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_mcp_f32": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64, _P, c_int64],
     "mcpg_mcp_f64": [_P, c_int, _P, c_int64, c_int64, c_int, _P, c_int64, _P, c_int64],
+    "mcpg_mcp_f32_noalloc": [_P, _P, c_int64, c_int64, c_int, _P, c_int64, _P, c_int64],
     "mcpg_maxmins_f32": [_P, c_int, _P, c_int64, c_int64, _P, c_int64, _P, c_int64, _P, c_int64],
     "mcpg_maxmins_f64": [_P, c_int, _P, c_int64, c_int64, _P, c_int64, _P, c_int64, _P, c_int64],
     "mcpg_combo_f32": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, _P, _P, c_int64],

@@ -258,6 +258,7 @@ int mcpg_create(mcpg_handle* out, int device) {
     delete h;
     return MCPG_ERR_OUT_OF_MEMORY;
   }
+  mcpg::init_scratch(h);
   *out = h;
   return MCPG_OK;
 }

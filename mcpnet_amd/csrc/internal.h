@@ -46,6 +46,10 @@ struct mcpg_ctx {
   int* mm_tail = nullptr;
   size_t mm_tail_bytes = 0;
   int mm_slots[2] = {0, 0};   // K chunk 16, 32
+  // device address of the static scratch of the allocation-free entry point on this handle's
+  // device (set by mcpg_create), and the error if it could not be taken
+  void* scratch = nullptr;
+  int scratch_err = 0;
 };
 
 namespace mcpg {
@@ -56,6 +60,10 @@ __host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + 
 
 int set_error(mcpg_ctx* h, int status, const std::string& msg);
 int hip_error(mcpg_ctx* h, hipError_t e, const char* where);
+
+// Takes the device address of the allocation-free entry point's static scratch for the handle's
+// device (current when called); on failure h->scratch stays null and h->scratch_err says why.
+void init_scratch(mcpg_ctx* h);
 
 // Ensures the workspace holds at least `bytes`; returns its base pointer (or nullptr).
 void* workspace(mcpg_ctx* h, size_t bytes, int* status);

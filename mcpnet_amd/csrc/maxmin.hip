@@ -18,7 +18,10 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <cstdio>
 #include <limits>
+#include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -710,9 +713,33 @@ int pad_keys(mcpg_ctx* h, const KeyMat& km, int64_t cols) {
   return MCPG_OK;
 }
 
-// C / Ck / O over [0, M) x [0, N) of A (x) B, with K = A.ldk (== B.ldk)
+// Buffers a caller provides in place of the handle's cached tile orders and tail block (the
+// allocation-free entry point): room for one triangle tile order and for split-tail key blocks.
+// With too little tail room the product runs unsplit, which gives the same bits.
+struct MMScratch {
+  uint32_t* order = nullptr;
+  int64_t order_cap = 0;   // entries
+  int* tail = nullptr;
+  size_t tail_bytes = 0;
+};
+
+// The order mcpg::tri_order builds without the XCD deal (blocks of group x group tiles in
+// row-major order, tiles row-major inside a block, upper triangle only), written on the device
+// by one thread: the allocation-free path has no cached table.
+__global__ void tri_order_kernel(uint32_t* __restrict__ out, int64_t nb, int group) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t n = 0;
+  for (int64_t bi = 0; bi < nb; bi += group)
+    for (int64_t bj = 0; bj < nb; bj += group)
+      for (int64_t ti = bi; ti < bi + group && ti < nb; ++ti)
+        for (int64_t tj = bj > ti ? bj : ti; tj < bj + group && tj < nb; ++tj) out[n++] = (uint32_t)((ti << 16) | tj);
+}
+
+// C / Ck / O over [0, M) x [0, N) of A (x) B, with K = A.ldk (== B.ldk).  With `sc`, the tile
+// order and the tail blocks live in the caller's buffers and nothing is allocated.
 int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M, int64_t N, int tri, float* C,
-                       int64_t ldc, int* Ck, int64_t ldck, const float* R, int64_t ldr, float* O, int64_t ldo) {
+                       int64_t ldc, int* Ck, int64_t ldck, const float* R, int64_t ldr, float* O, int64_t ldo,
+                       const MMScratch* sc = nullptr) {
   if (M <= 0 || N <= 0) return MCPG_OK;
   MKArgs a;
   a.A = A.d; a.lda = A.ldk; a.B = B.d; a.ldb = B.ldk; a.a_bytes = A.bytes(); a.b_bytes = B.bytes();
@@ -725,7 +752,14 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
   a.ntiles = tri ? a.nbm * (a.nbm + 1) / 2 : a.nbm * a.nbn;
   if (a.ntiles > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min grid too large");
   a.order = nullptr;
-  if (tri) {
+  if (tri && sc) {
+    if (a.ntiles > sc->order_cap || a.nbm > 65535)
+      return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min tile order exceeds the scratch buffer");
+    const int group = std::max(mcpg::env_group("MCPG_MM_GROUP", 8), 1);
+    mcpg::LaunchScope ls(h, "tri_order_kernel");
+    hipLaunchKernelGGL(tri_order_kernel, dim3(1), dim3(64), 0, h->stream, sc->order, a.nbm, group);
+    a.order = sc->order;
+  } else if (tri) {
     int st;
     a.order = mcpg::tri_order(h, a.nbm, mcpg::env_group("MCPG_MM_GROUP", 8), mcpg::env_group("MCPG_MM_XCD", 0), &st);
     if (!a.order) return st;
@@ -761,9 +795,11 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
   {
     const int64_t full = a.ntiles / slots * slots, ntail = a.ntiles - full;
     const int64_t splits = ntail > 0 ? std::min<int64_t>(slots / ntail, nk) : 1;
-    if (mcpg::env_group("MCPG_MM_SPLIT", 1) != 0 && splits >= 2) {
-      const size_t need = (size_t)ntail * kBM * kBN * sizeof(int);
-      if (need > h->mm_tail_bytes) {
+    const size_t need = (size_t)ntail * kBM * kBN * sizeof(int);
+    if (mcpg::env_group("MCPG_MM_SPLIT", 1) != 0 && splits >= 2 && (!sc || need <= sc->tail_bytes)) {
+      if (sc) {
+        a.tail = sc->tail;
+      } else if (need > h->mm_tail_bytes) {
         MCPG_HIP(h, hipStreamSynchronize(h->stream));
         if (h->mm_tail) (void)hipFree(h->mm_tail);
         h->mm_tail = nullptr;
@@ -774,7 +810,7 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
       a.nfull = full;
       a.splits = (int)splits;
       a.kc_per = ceil_div(nk, splits);
-      a.tail = h->mm_tail;
+      if (!sc) a.tail = h->mm_tail;
       mcpg::LaunchScope ls(h, "fill_i32_kernel");
       hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_of(ntail * kBM * kBN)), dim3(256), 0, h->stream, a.tail,
                          ntail * kBM * kBN, (int)INT32_MIN);
@@ -1176,10 +1212,30 @@ int ratio_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* Mm, int
   return mcpg::finish(h, "mcpg_ratio");
 }
 
+// Static scratch of the allocation-free MCP entry point (mcpg_mcp_f32_noalloc).  Each device the
+// library runs on holds its own copy until the process ends; host code reaches it only through
+// the per-device address taken with hipGetSymbolAddress at handle creation (mcpg_ctx::scratch).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+// Scratch plan of mcp_f32_keys with device operands: keys of A, keys of M1 (k >= 3), one
+// triangle tile order; the rest of `cap` bytes (if any) holds split-tail key blocks.
+size_t mcp_scratch_bytes(int64_t N, int k) {
+  const KeyMat ka = key_geometry(N, N);
+  mcpg::Sizer sz;
+  sz.add<int>(ka.rows_pad * ka.ldk);
+  if (k >= 3) sz.add<int>(ka.rows_pad * ka.ldk);
+  const int64_t nb = ceil_div(N, kBM);
+  sz.add<uint32_t>(nb * (nb + 1) / 2);
+  return sz.off;
+}
+
 // MCP2/3/4 on FP32 order keys: keys of A once; M1 = A (x) A stays in key form on the device; the
-// last product writes the max-min matrix only when asked for and fuses the ratio.
+// last product writes the max-min matrix only when asked for and fuses the ratio.  With
+// `scratch_base` (device operands only) every buffer is carved from those `scratch_cap` bytes and
+// nothing is allocated.
 int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N, int k, float* mm_out, int64_t ldm,
-                 float* out, int64_t ldo) {
+                 float* out, int64_t ldo, void* scratch_base = nullptr, size_t scratch_cap = 0) {
   const size_t NN = (size_t)N * N;
   KeyMat ka = key_geometry(N, N), k1 = key_geometry(N, N);
   int st;
@@ -1188,8 +1244,25 @@ int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N,
   if (host) { sz.add<float>(NN); sz.add<float>(NN); if (mm_out) sz.add<float>(NN); }
   sz.add<int>(ka.rows_pad * ka.ldk);
   if (k >= 3) sz.add<int>(k1.rows_pad * k1.ldk);
-  void* base = mcpg::workspace(h, sz.off, &st);
-  if (!base) return st;
+  MMScratch msc;
+  const MMScratch* sc = nullptr;
+  void* base = nullptr;
+  if (scratch_base) {
+    if (host) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "the scratch path takes device operands only");
+    const int64_t nb = ceil_div(N, kBM);
+    const size_t keys = sz.off;
+    sz.add<uint32_t>(nb * (nb + 1) / 2);
+    if (sz.off > scratch_cap) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "MCP operands exceed the scratch buffer");
+    base = scratch_base;
+    msc.order = reinterpret_cast<uint32_t*>(static_cast<char*>(base) + keys);
+    msc.order_cap = nb * (nb + 1) / 2;
+    msc.tail = reinterpret_cast<int*>(static_cast<char*>(base) + sz.off);
+    msc.tail_bytes = scratch_cap - sz.off;
+    sc = &msc;
+  } else {
+    base = mcpg::workspace(h, sz.off, &st);
+    if (!base) return st;
+  }
   mcpg::Carver cv(base);
   const float* dA = A;
   int64_t la = lda;
@@ -1207,15 +1280,21 @@ int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N,
   ka.d = cv.take<int>(ka.rows_pad * ka.ldk);
   if ((st = prep_keys(h, dA, la, N, nullptr, ka))) return st;
   if (k == 2) {
-    if ((st = launch_maxmin_keys(h, ka, ka, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo))) return st;
+    if ((st = launch_maxmin_keys(h, ka, ka, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
   } else {
     k1.d = cv.take<int>(k1.rows_pad * k1.ldk);
     if ((st = pad_keys(h, k1, N))) return st;
     // the first product A (x) A is symmetric by construction
-    if ((st = launch_maxmin_keys(h, ka, ka, N, N, 1, nullptr, 0, k1.d, k1.ldk, nullptr, 0, nullptr, 0))) return st;
+    if ((st = launch_maxmin_keys(h, ka, ka, N, N, 1, nullptr, 0, k1.d, k1.ldk, nullptr, 0, nullptr, 0, sc))) return st;
     if (k == 3) {
-      // A (x) M1 is exactly symmetric when A is (max/min never round); check A bitwise.
-      if ((st = mcpg::reset_flags(h))) return st;
+      // A (x) M1 is exactly symmetric when A is (max/min never round); check A bitwise.  The
+      // scratch path clears the flag with a kernel (no memset node for callers that capture).
+      if (sc) {
+        mcpg::LaunchScope ls(h, "fill_i32_kernel");
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, h->stream, h->d_flags, 16, 0);
+      } else if ((st = mcpg::reset_flags(h))) {
+        return st;
+      }
       {
         mcpg::LaunchScope ls(h, "symmetry_check_kernel");
         launch_symmetry_check<float>(h, dA, la, N);
@@ -1223,9 +1302,9 @@ int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N,
       int flags[1];
       if ((st = mcpg::read_flags(h, flags, 1))) return st;
       const int sym = flags[0] == 0;
-      if ((st = launch_maxmin_keys(h, ka, k1, N, N, sym, dM, lm, nullptr, 0, dA, la, dO, lo))) return st;
+      if ((st = launch_maxmin_keys(h, ka, k1, N, N, sym, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
     } else {
-      if ((st = launch_maxmin_keys(h, k1, k1, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo))) return st;
+      if ((st = launch_maxmin_keys(h, k1, k1, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
     }
   }
   if (host) {
@@ -1307,9 +1386,73 @@ int mcp_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, int k, 
   return mcpg::finish(h, "mcpg_mcp");
 }
 
+// MCP2/3/4 of mcp_f32_keys on device operands with every buffer in `scratch`.
+int mcp_noalloc_entry(mcpg_ctx* h, const float* A, int64_t lda, int64_t N, int k, float* mm_out, int64_t ldm, float* out,
+                      int64_t ldo) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (k < 2 || k > 4) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "MCP order must be 2, 3 or 4");
+  if (N < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (N == 0) return MCPG_OK;
+  if (!A || !out) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (lda < N || ldo < N || (mm_out && ldm < N)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (!h->scratch)
+    return mcpg::set_error(h, MCPG_ERR_HIP, std::string("static scratch unavailable: hipGetSymbolAddress: ") +
+                                                hipGetErrorString(static_cast<hipError_t>(h->scratch_err)));
+  if (mcp_scratch_bytes(N, k) > kScratchBytes) {
+    char buf[160];
+    snprintf(buf, sizeof buf, "MCP%d of %lld genes needs %zu bytes of scratch; the static scratch holds %zu", k,
+             (long long)N, mcp_scratch_bytes(N, k), kScratchBytes);
+    return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, buf);
+  }
+  MCPG_HIP(h, hipSetDevice(h->device));
+  // launch timing would create events; this entry point records none
+  const bool prof = h->profiling;
+  h->profiling = false;
+  const int st = mcp_f32_keys(h, false, A, lda, N, k, mm_out, ldm, out, ldo, h->scratch, kScratchBytes);
+  h->profiling = prof;
+  return st;
+}
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
 }  // namespace
 
+namespace mcpg {
+
+// Called by mcpg_create with the handle's device current: the device address of `scratch` on
+// that device, taken once per device and kept for the process.
+void init_scratch(mcpg_ctx* h) {
+  h->scratch = nullptr;
+  h->scratch_err = hipSuccess;
+  if (h->device < 0 || h->device >= kMaxDevices) {
+    h->scratch_err = hipErrorInvalidDevice;
+    return;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[h->device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      h->scratch_err = e != hipSuccess ? e : hipErrorInvalidSymbol;
+      return;
+    }
+    g_scratch_ptr[h->device] = p;
+  }
+  h->scratch = g_scratch_ptr[h->device];
+}
+
+}  // namespace mcpg
+
 extern "C" {
+
+int mcpg_mcp_f32_noalloc(mcpg_handle h, const float* A, int64_t lda, int64_t N, int k, float* maxmin_out, int64_t ldm,
+                         float* mcp_out, int64_t ldo) {
+  return mcp_noalloc_entry(h, A, lda, N, k, maxmin_out, ldm, mcp_out, ldo);
+}
 
 int mcpg_maxmin_f32(mcpg_handle h, int ptrs, const float* A, int64_t lda, const float* B, int64_t ldb,
                     const float* kmask, int64_t M, int64_t N, int64_t K, int symmetric, float* C, int64_t ldc) {
