@@ -46,6 +46,7 @@ struct mcpg_ctx {
   int* mm_tail = nullptr;
   size_t mm_tail_bytes = 0;
   int mm_slots[2] = {0, 0};   // K chunk 16, 32
+  int mm64_slots[2] = {0, 0}; // FP64 products, K chunk 8, 16 doubles
   // device address of the static scratch of the allocation-free entry point on this handle's
   // device (set by mcpg_create), and the error if it could not be taken
   void* scratch = nullptr;

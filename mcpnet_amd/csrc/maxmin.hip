@@ -13,8 +13,10 @@
 //   mask of the masked kernel is folded in there).  K advances in chunks of 16 keys per row that
 //   arrive in LDS by direct buffer loads, two chunks in flight, into XOR-swizzled rows whose
 //   operand reads are bank-conflict free.  An MCP pipeline keeps M1 in key form on the device.
-// * FP64: K advances in chunks of 128 bytes per row, staged through registers into LDS rows
-//   padded to 144 bytes; the next chunk is prefetched while the current one is consumed.
+// * FP64: the operands are copied once to padded value matrices (val_prep_kernel, -inf padding,
+//   K mask folded in) and K chunks of 16 doubles per row arrive in LDS by the same direct loads
+//   (maxmin_f64_kernel).  The earlier register-staged kernel (maxmin_tile_kernel: chunks of 128
+//   bytes staged through registers into LDS rows padded to 144 bytes) stays as a cross-check.
 #include <cfloat>
 #include <climits>
 #include <cmath>
@@ -536,6 +538,227 @@ __global__ void fill_i32_kernel(int* p, int64_t n, int v) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) p[t] = v;
 }
 
+// ---------------------------------------------------------------- FP64 padded-value path
+// FP64 operands are copied once into padded value matrices (rows padded to the tile, row length
+// a multiple of the largest K chunk, padding -inf, the K mask folded in), so that K chunks arrive
+// in LDS by direct buffer loads exactly as the FP32 keys do.  v_min_f64 / v_max_f64 are written
+// as single instructions: as library calls the compiler quiets every LDS operand first
+// (v_max_f64 x, x, x), an eighth more VALU work.  Split-tail partial maxima meet in a block of
+// 64-bit order keys through atomicMax.
+constexpr int kValAlign = 16;   // row length of value matrices (doubles): a multiple of the largest K chunk
+
+__device__ __forceinline__ double vmin_f64(double a, double b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double vmax_f64(double a, double b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ long long f64_key(double x) {
+  const long long b = __double_as_longlong(x);
+  return b ^ ((b >> 63) & 0x7fffffffffffffffll);
+}
+__device__ __forceinline__ double key_f64(long long k) {
+  return __longlong_as_double(k ^ ((k >> 63) & 0x7fffffffffffffffll));
+}
+
+__global__ void val_prep_kernel(const double* __restrict__ X, int64_t ldx, int64_t rows, int64_t K,
+                                const double* __restrict__ kmask, double* __restrict__ out, int64_t ld, int64_t rows_pad) {
+  const int64_t total = rows_pad * ld;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / ld, k = t % ld;
+    double v = -INFINITY;
+    if (r < rows && k < K) {
+      v = X[r * ldx + k];
+      if (kmask) v = vmin_f64(v, kmask[k]);
+    }
+    out[t] = v;
+  }
+}
+
+// -inf into the padding of a value matrix whose [0, rows) x [0, cols) part a product writes
+__global__ void val_pad_kernel(double* __restrict__ out, int64_t ld, int64_t rows, int64_t cols, int64_t rows_pad) {
+  const int64_t pad_cols = ld - cols;
+  const int64_t total = rows * pad_cols + (rows_pad - rows) * ld;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    if (t < rows * pad_cols) out[(t / pad_cols) * ld + cols + t % pad_cols] = -INFINITY;
+    else out[rows * ld + (t - rows * pad_cols)] = -INFINITY;
+  }
+}
+
+__global__ void fill_i64_kernel(long long* p, int64_t n, long long v) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) p[t] = v;
+}
+
+struct MVArgs {
+  const double* A; int64_t lda;   // padded values, rows padded to kBM, lda a multiple of the K chunk
+  const double* B; int64_t ldb;
+  uint32_t a_bytes, b_bytes;      // buffer ranges
+  int64_t M, N, Kp;               // Kp: padded K (doubles)
+  double* C; int64_t ldc;         // output (nullable)
+  double* Cp; int64_t ldcp;       // second output, e.g. a padded operand of the next product (nullable)
+  const double* R; int64_t ldr;   // ratio numerator (nullable: no fused ratio)
+  double* O; int64_t ldo;         // ratio output
+  int tri, shared;
+  int64_t nbm, nbn, ntiles;
+  const uint32_t* order;          // tri mode: tile coordinates in launch order
+  int64_t nfull;                  // split tail, as for the FP32 keys
+  int splits;
+  int64_t kc_per;
+  long long* tail;
+};
+
+__device__ __forceinline__ void mv_tile_coords(const MVArgs& a, int64_t t, int64_t& bm, int64_t& bn) {
+  if (a.tri) {
+    const uint32_t code = a.order[t];
+    bm = code >> 16;
+    bn = code & 0xffffu;
+  } else {
+    const int64_t group = 8;
+    const int64_t per_group = group * a.nbn;
+    const int64_t g = t / per_group;
+    const int64_t first = g * group;
+    const int64_t gsize = (a.nbm - first) < group ? (a.nbm - first) : group;
+    const int64_t in = t % per_group;
+    bm = first + in % gsize;
+    bn = in / gsize;
+  }
+}
+
+__device__ __forceinline__ void mv_store(const MVArgs& a, int64_t i, int64_t j, bool mirror, double v) {
+  if (a.C) {
+    a.C[i * a.ldc + j] = v;
+    if (mirror) a.C[j * a.ldc + i] = v;
+  }
+  if (a.Cp) {
+    a.Cp[i * a.ldcp + j] = v;
+    if (mirror) a.Cp[j * a.ldcp + i] = v;
+  }
+  if (a.O) {
+    a.O[i * a.ldo + j] = ratio_of<double>(a.R[i * a.ldr + j], v);
+    if (mirror) a.O[j * a.ldo + i] = ratio_of<double>(a.R[j * a.ldr + i], v);
+  }
+}
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+// KD doubles of K per stage: LDS panel rows of KD * 8 bytes, with the swizzle of the FP32 key
+// rows of the same length (KeyChunk<2 KD>).
+template <int KD>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void maxmin_f64_kernel(MVArgs a) {
+  using KCh = KeyChunk<2 * KD>;
+  extern __shared__ __align__(16) char smem[];
+
+  int64_t tile = blockIdx.x, part = -1;
+  if (tile >= a.nfull) {
+    part = (tile - a.nfull) % a.splits;
+    tile = a.nfull + (tile - a.nfull) / a.splits;
+  }
+  int64_t bm, bn;
+  mv_tile_coords(a, tile, bm, bn);
+  const int64_t row0 = bm * kBM, col0 = bn * kBN;
+  const bool same = a.shared && bm == bn;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tx = tid & 15, ty = tid >> 4;
+
+  double acc[8][8];
+#pragma unroll
+  for (int m = 0; m < 8; ++m)
+#pragma unroll
+    for (int n = 0; n < 8; ++n) acc[m][n] = -DBL_MAX;   // numeric_limits<double>::lowest()
+
+  const int swave = __builtin_amdgcn_readfirstlane(wave);
+  const int panel = same ? 0 : (swave >> 1);
+  const int nload = same ? KCh::kLoadsPerPanel / 4 : KCh::kLoadsPerPanel / 2;
+  const int q0 = same ? swave * nload : (swave & 1) * nload;
+  const int64_t prow0 = panel == 0 ? row0 : col0;
+  const int64_t ld = panel == 0 ? a.lda : a.ldb;
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)(panel == 0 ? a.A : a.B), (short)0, panel == 0 ? a.a_bytes : a.b_bytes, kRsrcWord3);
+  const int lrow = lane / KCh::kChunks, pc = lane % KCh::kChunks;
+  uint32_t lbyte[2];
+#pragma unroll
+  for (int par = 0; par < 2; ++par) {
+    const int row = par * KCh::kRowsPerLoad + lrow;
+    lbyte[par] = (uint32_t)(lrow * ld * 8 + ((pc ^ KCh::swz(row)) << 4));
+  }
+  const uint32_t rbytes = (uint32_t)(KCh::kRowsPerLoad * ld * 8);
+  const uint32_t sbase = (uint32_t)(prow0 * ld * 8);
+  char* const dst0 = smem + panel * KCh::kPanelBytes;
+
+  const int sa = KCh::swz(ty), sb = KCh::swz(tx);
+  const int64_t nk_all = a.Kp / KD;
+  const int64_t kc0 = part < 0 ? 0 : part * a.kc_per;
+  const int64_t nk = part < 0 ? nk_all : (kc0 + a.kc_per < nk_all ? kc0 + a.kc_per : nk_all);
+  if (kc0 < nk)
+    key_dma_issue<2 * KD>(rsrc, lbyte, sbase + (uint32_t)(kc0 * KCh::kRowBytes), rbytes, nload, q0,
+                          dst0 + (kc0 & 1) * KCh::kStageBytes);
+  for (int64_t kc = kc0; kc < nk; ++kc) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (kc + 1 < nk)
+      key_dma_issue<2 * KD>(rsrc, lbyte, sbase + (uint32_t)((kc + 1) * KCh::kRowBytes), rbytes, nload, q0,
+                            dst0 + ((kc + 1) & 1) * KCh::kStageBytes);
+    const char* As = smem + (kc & 1) * KCh::kStageBytes;
+    const char* Bs = same ? As : As + KCh::kPanelBytes;
+#pragma unroll
+    for (int c = 0; c < KCh::kChunks; ++c) {
+      // the A fragments of the chunk stay in registers; each B fragment is read as it is used
+      v2d av[8];
+      const int oa = ty * KCh::kRowBytes + ((c ^ sa) << 4);
+      const int ob = tx * KCh::kRowBytes + ((c ^ sb) << 4);
+#pragma unroll
+      for (int m = 0; m < 8; ++m) av[m] = *reinterpret_cast<const v2d*>(As + oa + 16 * m * KCh::kRowBytes);
+#pragma unroll
+      for (int n = 0; n < 8; ++n) {
+        const v2d bv = *reinterpret_cast<const v2d*>(Bs + ob + 16 * n * KCh::kRowBytes);
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+          acc[m][n] = vmax_f64(acc[m][n], vmax_f64(vmin_f64(av[m].x, bv.x), vmin_f64(av[m].y, bv.y)));
+      }
+    }
+  }
+
+  if (part >= 0) {   // one K range of a split tail tile: fold into the tile's key block
+    long long* T = a.tail + (tile - a.nfull) * (int64_t)(kBM * kBN);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+      for (int n = 0; n < 8; ++n) atomicMax(T + (ty + 16 * m) * kBN + tx + 16 * n, f64_key(acc[m][n]));
+    return;
+  }
+
+  const bool mirror = a.tri && bm != bn;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) {
+    const int64_t i = row0 + ty + 16 * m;
+    if (i >= a.M) continue;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+      const int64_t j = col0 + tx + 16 * n;
+      if (j >= a.N) continue;
+      mv_store(a, i, j, mirror, acc[m][n]);
+    }
+  }
+}
+
+__global__ void mv_finish_tail_kernel(MVArgs a) {
+  const int64_t ntail = a.ntiles - a.nfull;
+  const int64_t total = ntail * (kBM * kBN);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t tt = t / (kBM * kBN), e = t % (kBM * kBN);
+    int64_t bm, bn;
+    mv_tile_coords(a, a.nfull + tt, bm, bn);
+    const int64_t i = bm * kBM + e / kBN, j = bn * kBN + e % kBN;
+    if (i >= a.M || j >= a.N) continue;
+    mv_store(a, i, j, a.tri && bm != bn, key_f64(a.tail[t]));
+  }
+}
+
 template <typename T>
 __global__ void ratio_kernel(const T* __restrict__ A, int64_t lda, const T* __restrict__ Mm, int64_t ldm,
                              int64_t M, int64_t N, T* __restrict__ O, int64_t ldo) {
@@ -871,6 +1094,167 @@ int run_keys(mcpg_ctx* h, const KeyPlan& p, const float* A, int64_t lda, const f
   return launch_maxmin_keys(h, p.ka, p.kb, p.ka.rows, p.kb.rows, symmetric, C, ldc, nullptr, 0, nullptr, 0, nullptr, 0);
 }
 
+// ---------------------------------------------------------------- FP64 padded-value path, host side
+struct ValMat {
+  double* d = nullptr;
+  int64_t rows = 0, rows_pad = 0, ld = 0;
+  uint32_t bytes() const { return (uint32_t)(rows_pad * ld * 8); }
+};
+
+ValMat val_geometry(int64_t rows, int64_t K) {
+  ValMat vm;
+  vm.rows = rows;
+  vm.rows_pad = ceil_div(rows, kBM) * kBM;
+  vm.ld = std::max<int64_t>(ceil_div(K, kValAlign) * kValAlign, kValAlign);
+  return vm;
+}
+
+int val_check(mcpg_ctx* h, const ValMat& vm) {
+  if ((uint64_t)vm.rows_pad * (uint64_t)vm.ld * 8 >= (1ull << 32))
+    return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "FP64 max-min operand exceeds 4 GiB");
+  return MCPG_OK;
+}
+
+int prep_vals(mcpg_ctx* h, const double* X, int64_t ldx, int64_t K, const double* kmask, const ValMat& vm) {
+  mcpg::LaunchScope ls(h, "val_prep_kernel");
+  hipLaunchKernelGGL(val_prep_kernel, dim3(grid_of(vm.rows_pad * vm.ld)), dim3(256), 0, h->stream, X, ldx, vm.rows, K,
+                     kmask, vm.d, vm.ld, vm.rows_pad);
+  return MCPG_OK;
+}
+
+int pad_vals(mcpg_ctx* h, const ValMat& vm, int64_t cols) {
+  const int64_t total = vm.rows * (vm.ld - cols) + (vm.rows_pad - vm.rows) * vm.ld;
+  if (total == 0) return MCPG_OK;
+  mcpg::LaunchScope ls(h, "val_pad_kernel");
+  hipLaunchKernelGGL(val_pad_kernel, dim3(grid_of(total)), dim3(256), 0, h->stream, vm.d, vm.ld, vm.rows, cols,
+                     vm.rows_pad);
+  return MCPG_OK;
+}
+
+// C / Cp / O over [0, M) x [0, N) of A (x) B on padded FP64 values, K = A.ld (== B.ld)
+int launch_maxmin_vals(mcpg_ctx* h, const ValMat& A, const ValMat& B, int64_t M, int64_t N, int tri, double* C,
+                       int64_t ldc, double* Cp, int64_t ldcp, const double* R, int64_t ldr, double* O, int64_t ldo) {
+  if (M <= 0 || N <= 0) return MCPG_OK;
+  MVArgs a;
+  a.A = A.d; a.lda = A.ld; a.B = B.d; a.ldb = B.ld; a.a_bytes = A.bytes(); a.b_bytes = B.bytes();
+  a.M = M; a.N = N; a.Kp = A.ld;
+  a.C = C; a.ldc = ldc; a.Cp = Cp; a.ldcp = ldcp; a.R = R; a.ldr = ldr; a.O = O; a.ldo = ldo;
+  a.tri = tri;
+  a.shared = A.d == B.d ? 1 : 0;
+  a.nbm = ceil_div(M, kBM);
+  a.nbn = ceil_div(N, kBN);
+  a.ntiles = tri ? a.nbm * (a.nbm + 1) / 2 : a.nbm * a.nbn;
+  if (a.ntiles > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min grid too large");
+  a.order = nullptr;
+  if (tri) {
+    int st;
+    a.order = mcpg::tri_order(h, a.nbm, mcpg::env_group("MCPG_MM_GROUP", 8), mcpg::env_group("MCPG_MM_XCD", 0), &st);
+    if (!a.order) return st;
+  }
+  const int kd = mcpg::env_group("MCPG_MM64_KD", 16) == 8 ? 8 : 16;
+  const void* kfn = kd == 8 ? reinterpret_cast<const void*>(&maxmin_f64_kernel<8>)
+                            : reinterpret_cast<const void*>(&maxmin_f64_kernel<16>);
+  const int smem = 2 * (kd == 8 ? KeyChunk<16>::kStageBytes : KeyChunk<32>::kStageBytes);
+  static bool attr_set = false;
+  if (!attr_set) {
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&maxmin_f64_kernel<8>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 2 * KeyChunk<16>::kStageBytes));
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&maxmin_f64_kernel<16>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 2 * KeyChunk<32>::kStageBytes));
+    attr_set = true;
+  }
+  int& slots = h->mm64_slots[kd == 8 ? 0 : 1];
+  if (slots == 0) {
+    int per_cu = 0, cus = 0;
+    MCPG_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kThreads, smem));
+    MCPG_HIP(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    slots = std::max(1, per_cu * cus);
+  }
+  const int64_t nk = a.Kp / kd;
+  a.nfull = a.ntiles;
+  a.splits = 1;
+  a.kc_per = nk;
+  a.tail = nullptr;
+  {
+    const int64_t full = a.ntiles / slots * slots, ntail = a.ntiles - full;
+    const int64_t splits = ntail > 0 ? std::min<int64_t>(slots / ntail, nk) : 1;
+    if (mcpg::env_group("MCPG_MM_SPLIT", 1) != 0 && splits >= 2) {
+      const size_t need = (size_t)ntail * kBM * kBN * sizeof(long long);
+      if (need > h->mm_tail_bytes) {
+        MCPG_HIP(h, hipStreamSynchronize(h->stream));
+        if (h->mm_tail) (void)hipFree(h->mm_tail);
+        h->mm_tail = nullptr;
+        h->mm_tail_bytes = 0;
+        MCPG_HIP(h, hipMalloc(&h->mm_tail, need));
+        h->mm_tail_bytes = need;
+      }
+      a.nfull = full;
+      a.splits = (int)splits;
+      a.kc_per = ceil_div(nk, splits);
+      a.tail = reinterpret_cast<long long*>(h->mm_tail);
+      mcpg::LaunchScope ls(h, "fill_i64_kernel");
+      hipLaunchKernelGGL(fill_i64_kernel, dim3(grid_of(ntail * kBM * kBN)), dim3(256), 0, h->stream, a.tail,
+                         ntail * kBM * kBN, (long long)LLONG_MIN);
+    }
+  }
+  const int64_t grid = a.nfull + (a.ntiles - a.nfull) * a.splits;
+  if (grid > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min grid too large");
+  {
+    mcpg::LaunchScope ls(h, tri ? "maxmin_f64_kernel<tri>" : "maxmin_f64_kernel");
+    if (kd == 8)
+      hipLaunchKernelGGL(maxmin_f64_kernel<8>, dim3((unsigned)grid), dim3(kThreads), smem, h->stream, a);
+    else
+      hipLaunchKernelGGL(maxmin_f64_kernel<16>, dim3((unsigned)grid), dim3(kThreads), smem, h->stream, a);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mcpg::hip_error(h, e, "maxmin_f64_kernel launch");
+  if (a.nfull < a.ntiles) {
+    mcpg::LaunchScope ls(h, "mv_finish_tail_kernel");
+    hipLaunchKernelGGL(mv_finish_tail_kernel, dim3(grid_of((a.ntiles - a.nfull) * kBM * kBN)), dim3(256), 0, h->stream,
+                       a);
+    e = hipGetLastError();
+    if (e != hipSuccess) return mcpg::hip_error(h, e, "mv_finish_tail_kernel launch");
+  }
+  return MCPG_OK;
+}
+
+// The FP64 products run on padded values unless MCPG_MM64_TILE=1 selects the register-staged tile
+// kernel (kept as a cross-check; the same bits).
+bool f64_tile_kernel() { return mcpg::env_group("MCPG_MM64_TILE", 0) == 1; }
+
+// C (and/or O) = A (x) B for raw FP64 operands through padded copies carved from `cv`
+struct ValPlan {
+  ValMat va, vb;
+  bool shareB = false;
+};
+
+ValPlan plan_vals(bool same_operand, bool masked, int64_t M, int64_t N, int64_t K) {
+  ValPlan p;
+  p.va = val_geometry(M, K);
+  p.shareB = same_operand && !masked;
+  p.vb = p.shareB ? p.va : val_geometry(N, K);
+  return p;
+}
+
+void size_vals(mcpg::Sizer& sz, const ValPlan& p) {
+  sz.add<double>(p.va.rows_pad * p.va.ld);
+  if (!p.shareB) sz.add<double>(p.vb.rows_pad * p.vb.ld);
+}
+
+void take_vals(mcpg::Carver& cv, ValPlan& p) {
+  p.va.d = cv.take<double>(p.va.rows_pad * p.va.ld);
+  p.vb.d = p.shareB ? p.va.d : cv.take<double>(p.vb.rows_pad * p.vb.ld);
+}
+
+int run_vals(mcpg_ctx* h, const ValPlan& p, const double* A, int64_t lda, const double* B, int64_t ldb,
+             const double* kmask, int64_t K, int tri, double* C, int64_t ldc, const double* R, int64_t ldr, double* O,
+             int64_t ldo) {
+  int st;
+  if ((st = prep_vals(h, A, lda, K, kmask, p.va))) return st;
+  if (!p.shareB && (st = prep_vals(h, B, ldb, K, nullptr, p.vb))) return st;
+  return launch_maxmin_vals(h, p.va, p.vb, p.va.rows, p.vb.rows, tri, C, ldc, nullptr, 0, R, ldr, O, ldo);
+}
+
 // compute_maxmins (apps/pipeline/mcpnet_helper.hpp:436-460) on FP32 keys: M1 = A (x) A,
 // M2 = A (x) M1, M3 = M1 (x) M1, each written only when its output pointer is set.
 int maxmins_f32(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N, float* M1, int64_t ld1, float* M2,
@@ -951,8 +1335,12 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
   if (tfs) sz.add<T>(N);
   sz.add<T>(NN);                                  // W
   if (tfs) sz.add<T>(NN);                         // WT
+  const bool vals = !f32 && !f64_tile_kernel();
+  ValMat vx = val_geometry(N, N), vt = val_geometry(N, N);
+  if (vals && (st = val_check(h, vx))) return st;
   if (f32) { sz.add<int>(kx.rows_pad * kx.ldk); if (tfs) sz.add<int>(kt.rows_pad * kt.ldk); }
   else { sz.add<T>(NN); if (tfs) sz.add<T>(N); }  // diagonal-excluded copy, TF column mask
+  if (vals) { sz.add<double>(vx.rows_pad * vx.ld); if (tfs) sz.add<double>(vt.rows_pad * vt.ld); }
   sz.add<unsigned long long>(1);
   void* base = mcpg::workspace(h, sz.off, &st);
   if (!base) return st;
@@ -1001,9 +1389,14 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
       mcpg::LaunchScope ls(h, "diag_copy_kernel");
       hipLaunchKernelGGL(diag_copy_kernel<T>, dim3(grid_of(NN)), dim3(256), 0, h->stream, dA, la, N, (T)-INFINITY, Ad);
     }
-    if ((st = launch_maxmin<TrF64>(h, (const double*)Ad, N, (const double*)Ad, N, nullptr, N, N, N, 1, (double*)W, N,
-                                   nullptr, 0, nullptr, 0)))
+    if (vals) {
+      vx.d = cv.take<double>(vx.rows_pad * vx.ld);
+      if ((st = prep_vals(h, (const double*)Ad, N, N, nullptr, vx))) return st;
+      if ((st = launch_maxmin_vals(h, vx, vx, N, N, 1, (double*)W, N, nullptr, 0, nullptr, 0, nullptr, 0))) return st;
+    } else if ((st = launch_maxmin<TrF64>(h, (const double*)Ad, N, (const double*)Ad, N, nullptr, N, N, N, 1,
+                                          (double*)W, N, nullptr, 0, nullptr, 0))) {
       return st;
+    }
     if (tfs) {
       std::vector<T> mask(N), htf(N);
       if (host) std::copy(tfs, tfs + N, htf.begin());
@@ -1011,9 +1404,15 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
       for (int64_t k = 0; k < N; ++k) mask[k] = htf[k] > (T)0 ? std::numeric_limits<T>::max() : (T)-INFINITY;
       T* dmask = cv.take<T>(N);
       MCPG_HIP(h, hipMemcpy(dmask, mask.data(), N * sizeof(T), hipMemcpyHostToDevice));
-      if ((st = launch_maxmin<TrF64>(h, (const double*)Ad, N, (const double*)Ad, N, (const double*)dmask, N, N, N, 1,
-                                     (double*)WT, N, nullptr, 0, nullptr, 0)))
+      // max over TF y of min(A[x][y], A[z][y]): the mask on one operand suffices (min is associative)
+      if (vals) {
+        vt.d = cv.take<double>(vt.rows_pad * vt.ld);
+        if ((st = prep_vals(h, (const double*)Ad, N, N, (const double*)dmask, vt))) return st;
+        if ((st = launch_maxmin_vals(h, vt, vx, N, N, 1, (double*)WT, N, nullptr, 0, nullptr, 0, nullptr, 0))) return st;
+      } else if ((st = launch_maxmin<TrF64>(h, (const double*)Ad, N, (const double*)Ad, N, (const double*)dmask, N, N,
+                                            N, 1, (double*)WT, N, nullptr, 0, nullptr, 0))) {
         return st;
+      }
     }
   }
   unsigned long long* dcount = cv.take<unsigned long long>(1);
@@ -1036,9 +1435,63 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
   return MCPG_OK;
 }
 
-// compute_maxmins on FP64 values
+// compute_maxmins on padded FP64 values: A padded once, M1 written both to the caller and into a
+// padded operand of the other two products
+int maxmins_f64_vals(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N, double* M1, int64_t ld1,
+                     double* M2, int64_t ld2, double* M3, int64_t ld3) {
+  const size_t NN = (size_t)N * N;
+  ValMat va = val_geometry(N, N), v1 = val_geometry(N, N);
+  int st;
+  if ((st = val_check(h, va))) return st;
+  mcpg::Sizer sz;
+  if (host) { sz.add<double>(NN); for (double* p : {M1, M2, M3}) if (p) sz.add<double>(NN); }
+  sz.add<double>(va.rows_pad * va.ld);
+  sz.add<double>(v1.rows_pad * v1.ld);
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const double* dA = A;
+  int64_t la = lda;
+  double* dM[3] = {M1, M2, M3};
+  int64_t ldM[3] = {ld1, ld2, ld3};
+  if (host) {
+    double* a = cv.take<double>(NN);
+    if ((st = mcpg::copy_2d(h, a, N * 8, A, lda * 8, N * 8, N, hipMemcpyHostToDevice))) return st;
+    dA = a; la = N;
+    for (int q = 0; q < 3; ++q)
+      if (dM[q]) { dM[q] = cv.take<double>(NN); ldM[q] = N; }
+  }
+  va.d = cv.take<double>(va.rows_pad * va.ld);
+  v1.d = cv.take<double>(v1.rows_pad * v1.ld);
+  if ((st = prep_vals(h, dA, la, N, nullptr, va))) return st;
+  if ((st = pad_vals(h, v1, N))) return st;
+  if ((st = launch_maxmin_vals(h, va, va, N, N, 1, dM[0], ldM[0], v1.d, v1.ld, nullptr, 0, nullptr, 0))) return st;
+  if (dM[1]) {
+    if ((st = mcpg::reset_flags(h))) return st;
+    {
+      mcpg::LaunchScope ls(h, "symmetry_check_kernel");
+      launch_symmetry_check<double>(h, dA, la, N);
+    }
+    int flags[1];
+    if ((st = mcpg::read_flags(h, flags, 1))) return st;
+    if ((st = launch_maxmin_vals(h, va, v1, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0, nullptr, 0, nullptr, 0)))
+      return st;
+  }
+  if (dM[2] && (st = launch_maxmin_vals(h, v1, v1, N, N, 1, dM[2], ldM[2], nullptr, 0, nullptr, 0, nullptr, 0)))
+    return st;
+  if (host) {
+    double* out[3] = {M1, M2, M3};
+    const int64_t ldo[3] = {ld1, ld2, ld3};
+    for (int q = 0; q < 3; ++q)
+      if (out[q] && (st = mcpg::copy_2d(h, out[q], ldo[q] * 8, dM[q], N * 8, N * 8, N, hipMemcpyDeviceToHost))) return st;
+  }
+  return mcpg::finish(h, "mcpg_maxmins");
+}
+
+// compute_maxmins on FP64 values with the register-staged tile kernel
 int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N, double* M1, int64_t ld1, double* M2,
                 int64_t ld2, double* M3, int64_t ld3) {
+  if (!f64_tile_kernel()) return maxmins_f64_vals(h, host, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
   const size_t NN = (size_t)N * N;
   mcpg::Sizer sz;
   if (host) { sz.add<double>(NN); for (double* p : {M2, M3}) if (p) sz.add<double>(NN); }
@@ -1124,12 +1577,17 @@ int maxmin_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* B, int
   constexpr bool f32 = std::is_same<T, float>::value;
   const bool host = ptrs == MCPG_HOST;
   const bool shareB = (A == B && M == N && lda == ldb);
-  // FP32 runs on order keys (sized here with the host staging); FP64 reads the values directly
+  // FP32 runs on order keys, FP64 on padded values (both sized here with the host staging)
   KeyPlan kp;
+  ValPlan vp;
+  const bool vals = !f32 && !f64_tile_kernel();
+  int st;
   if constexpr (f32) {
     kp = plan_keys(shareB, kmask != nullptr, M, N, K);
-    int st;
     if ((st = key_check(h, kp.ka)) || (st = key_check(h, kp.kb))) return st;
+  } else if (vals) {
+    vp = plan_vals(shareB, kmask != nullptr, M, N, K);
+    if ((st = val_check(h, vp.va)) || (st = val_check(h, vp.vb))) return st;
   }
   mcpg::Sizer sz;
   if (host) {
@@ -1139,18 +1597,20 @@ int maxmin_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* B, int
     sz.add<T>(M * N);
   }
   if constexpr (f32) size_keys(sz, kp);
-  int st;
+  if (vals) size_vals(sz, vp);
   void* base = sz.off ? mcpg::workspace(h, sz.off, &st) : h->ws;
   if (sz.off && !base) return st;
   mcpg::Carver cv(base);
+  auto run = [&](const T* a, int64_t la, const T* b, int64_t lb, const T* m, T* c, int64_t lc) -> int {
+    if constexpr (f32) return run_keys(h, kp, a, la, b, lb, m, K, symmetric, c, lc);
+    else if (vals)
+      return run_vals(h, vp, a, la, b, lb, m, K, symmetric, c, lc, nullptr, 0, nullptr, 0);
+    else return launch_maxmin<Tr>(h, a, la, b, lb, m, M, N, K, symmetric, c, lc, nullptr, 0, nullptr, 0);
+  };
   if (!host) {
-    if constexpr (f32) {
-      take_keys(cv, kp);
-      st = run_keys(h, kp, A, lda, B, ldb, kmask, K, symmetric, C, ldc);
-    } else {
-      st = launch_maxmin<Tr>(h, A, lda, B, ldb, kmask, M, N, K, symmetric, C, ldc, nullptr, 0, nullptr, 0);
-    }
-    if (st) return st;
+    if constexpr (f32) take_keys(cv, kp);
+    if (vals) take_vals(cv, vp);
+    if ((st = run(A, lda, B, ldb, kmask, C, ldc))) return st;
     return mcpg::finish(h, "mcpg_maxmin");
   }
   T* dA = cv.take<T>(M * K);
@@ -1158,13 +1618,13 @@ int maxmin_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* B, int
   T* dm = kmask ? cv.take<T>(K) : nullptr;
   T* dC = cv.take<T>(M * N);
   if constexpr (f32) take_keys(cv, kp);
+  if (vals) take_vals(cv, vp);
   if (K > 0) {
     if ((st = mcpg::copy_2d(h, dA, K * sizeof(T), A, lda * sizeof(T), K * sizeof(T), M, hipMemcpyHostToDevice))) return st;
     if (!shareB && (st = mcpg::copy_2d(h, dB, K * sizeof(T), B, ldb * sizeof(T), K * sizeof(T), N, hipMemcpyHostToDevice))) return st;
     if (kmask) MCPG_HIP(h, hipMemcpyAsync(dm, kmask, K * sizeof(T), hipMemcpyHostToDevice, h->stream));
   }
-  if constexpr (f32) st = run_keys(h, kp, dA, K, dB, K, dm, K, symmetric, dC, N);
-  else st = launch_maxmin<Tr>(h, dA, K, dB, K, dm, M, N, K, symmetric, dC, N, nullptr, 0, nullptr, 0);
+  st = run(dA, K, dB, K, dm, dC, N);
   if (st) return st;
   if ((st = mcpg::copy_2d(h, C, ldc * sizeof(T), dC, N * sizeof(T), N * sizeof(T), M, hipMemcpyDeviceToHost))) return st;
   return mcpg::finish(h, "mcpg_maxmin");
@@ -1314,6 +1774,65 @@ int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N,
   return mcpg::finish(h, "mcpg_mcp");
 }
 
+// MCP2/3/4 on padded FP64 values: A padded once; M1 = A (x) A is written straight into a padded
+// operand for the second product; the last product writes the max-min matrix only when asked for
+// and fuses the ratio.
+int mcp_f64_vals(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N, int k, double* mm_out, int64_t ldm,
+                 double* out, int64_t ldo) {
+  const size_t NN = (size_t)N * N;
+  ValMat va = val_geometry(N, N), v1 = val_geometry(N, N);
+  int st;
+  if ((st = val_check(h, va))) return st;
+  mcpg::Sizer sz;
+  if (host) { sz.add<double>(NN); sz.add<double>(NN); if (mm_out) sz.add<double>(NN); }
+  sz.add<double>(va.rows_pad * va.ld);
+  if (k >= 3) sz.add<double>(v1.rows_pad * v1.ld);
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const double* dA = A;
+  int64_t la = lda;
+  double* dO = out;
+  int64_t lo = ldo;
+  double* dM = mm_out;
+  int64_t lm = ldm;
+  if (host) {
+    double* a = cv.take<double>(NN);
+    if ((st = mcpg::copy_2d(h, a, N * 8, A, lda * 8, N * 8, N, hipMemcpyHostToDevice))) return st;
+    dA = a; la = N;
+    dO = cv.take<double>(NN); lo = N;
+    if (mm_out) { dM = cv.take<double>(NN); lm = N; }
+  }
+  va.d = cv.take<double>(va.rows_pad * va.ld);
+  if ((st = prep_vals(h, dA, la, N, nullptr, va))) return st;
+  if (k == 2) {
+    if ((st = launch_maxmin_vals(h, va, va, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo))) return st;
+  } else {
+    v1.d = cv.take<double>(v1.rows_pad * v1.ld);
+    if ((st = pad_vals(h, v1, N))) return st;
+    // the first product A (x) A is symmetric by construction
+    if ((st = launch_maxmin_vals(h, va, va, N, N, 1, nullptr, 0, v1.d, v1.ld, nullptr, 0, nullptr, 0))) return st;
+    if (k == 3) {
+      // A (x) M1 is exactly symmetric when A is (max/min never round); check A bitwise.
+      if ((st = mcpg::reset_flags(h))) return st;
+      {
+        mcpg::LaunchScope ls(h, "symmetry_check_kernel");
+        launch_symmetry_check<double>(h, dA, la, N);
+      }
+      int flags[1];
+      if ((st = mcpg::read_flags(h, flags, 1))) return st;
+      if ((st = launch_maxmin_vals(h, va, v1, N, N, flags[0] == 0, dM, lm, nullptr, 0, dA, la, dO, lo))) return st;
+    } else {
+      if ((st = launch_maxmin_vals(h, v1, v1, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo))) return st;
+    }
+  }
+  if (host) {
+    if ((st = mcpg::copy_2d(h, out, ldo * 8, dO, N * 8, N * 8, N, hipMemcpyDeviceToHost))) return st;
+    if (mm_out && (st = mcpg::copy_2d(h, mm_out, ldm * 8, dM, N * 8, N * 8, N, hipMemcpyDeviceToHost))) return st;
+  }
+  return mcpg::finish(h, "mcpg_mcp");
+}
+
 template <typename T>
 int mcp_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, int k, T* mm_out, int64_t ldm, T* out,
               int64_t ldo) {
@@ -1328,6 +1847,7 @@ int mcp_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, int k, 
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
   if constexpr (std::is_same<T, float>::value) return mcp_f32_keys(h, ptrs == MCPG_HOST, A, lda, N, k, mm_out, ldm, out, ldo);
+  else if (!f64_tile_kernel()) return mcp_f64_vals(h, ptrs == MCPG_HOST, A, lda, N, k, mm_out, ldm, out, ldo);
   const size_t NN = (size_t)N * N;
   const bool host = ptrs == MCPG_HOST;
   mcpg::Sizer sz;

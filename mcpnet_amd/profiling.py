@@ -11,6 +11,10 @@
                                          [--config N --update FILE]
       scaled HBM bytes per dispatch (FETCH + WRITE) per kernel, printed as JSON or merged into
       FILE under the key config<N> (bench.py reads profiles/traffic.json that way).
+  python -m mcpnet_amd.profiling maxmin --genes N [--dtype f32|f64] [--reps R] [--mcp K]
+      max-min products of a seeded symmetric N x N matrix with diagonal 0 on the GPU (the
+      triangle launch, R times, or MCP_K): the workload for kernel traces of the product alone;
+      prints each launch's duration from the library's own event timing.
 """
 from __future__ import annotations
 
@@ -32,6 +36,31 @@ def copy_main(mib: int) -> None:
         y.copy_(x)
     torch.cuda.synchronize()
     print(f"copied {mib} MiB x 3", flush=True)
+
+
+def maxmin_main(genes: int, dtype: str, reps: int, mcp: int) -> None:
+    import numpy as np
+    import torch
+
+    import mcpnet_amd
+
+    dt = np.float64 if dtype == "f64" else np.float32
+    rng = np.random.default_rng(0x4D4D)
+    A = rng.random((genes, genes), dtype=np.float64).astype(dt)
+    A = np.maximum(A, A.T)
+    np.fill_diagonal(A, 0.0)
+    h = mcpnet_amd.Handle(0)
+    h.set_profiling(True)
+    Ad = torch.from_numpy(A).cuda()
+    C = torch.empty_like(Ad)
+    for r in range(reps):
+        if mcp:
+            h.mcp(Ad, mcp, out=C)
+        else:
+            h.maxmin(Ad, Ad, symmetric=True, out=C)
+        for name, ms in h.last_timings():
+            print(f"{genes} genes {dtype} rep {r}: {name} {ms:.3f} ms", flush=True)
+    h.close()
 
 
 def _db(d: str):
@@ -129,6 +158,11 @@ def main(argv=None) -> None:
     sub = ap.add_subparsers(dest="cmd", required=True)
     c = sub.add_parser("copy")
     c.add_argument("--mib", type=int, default=1024)
+    m = sub.add_parser("maxmin")
+    m.add_argument("--genes", type=int, default=2000)
+    m.add_argument("--dtype", choices=["f32", "f64"], default="f64")
+    m.add_argument("--reps", type=int, default=3)
+    m.add_argument("--mcp", type=int, default=0, choices=[0, 2, 3, 4])
     for name in ("summarize", "traffic"):
         s = sub.add_parser(name)
         s.add_argument("--trace")
@@ -144,6 +178,8 @@ def main(argv=None) -> None:
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy_main(a.mib)
+    elif a.cmd == "maxmin":
+        maxmin_main(a.genes, a.dtype, a.reps, a.mcp)
     elif a.cmd == "summarize":
         sys.stdout.write(summarize(a.trace, a.pmc_fetch, a.pmc_write, a.pmc_fetch_copy, a.pmc_write_copy, a.copy_mib,
                                    a.title, a.command) + "\n")

@@ -191,3 +191,54 @@ def test_tuning_switches_keep_bits(handle, monkeypatch, env):
     assert bits_equal(handle.maxmin(A, B), ref_rect)
     assert bits_equal(handle.maxmin(Y, Y, symmetric=True), ref_tri)
     assert bits_equal(ref_rect[:30], oracle.maxmin(np.ascontiguousarray(A[:30]), B))
+
+
+def _names(handle, fn):
+    handle.set_profiling(True)
+    try:
+        out = fn()
+        names = [n for n, _ in handle.last_timings()]
+    finally:
+        handle.set_profiling(False)
+    return out, names
+
+
+@pytest.mark.parametrize("shape,tri", [((1500, 1500, 1500), True), ((700, 900, 400), False),
+                                       ((4224, 4224, 96), True), ((2900, 2900, 64), False)])
+def test_f64_split_tail_matches_unsplit(handle, monkeypatch, shape, tri):
+    """FP64 tiles beyond the last full round of resident workgroups (every tile of a grid smaller
+    than one round) run split over K, their partial maxima folded with a 64-bit atomicMax on
+    order keys: bit-identical to the unsplit launch and to the oracle."""
+    M, N, K = shape
+    rng = np.random.default_rng(M + K)
+    A = rng.standard_normal((M, K))
+    A[rng.random((M, K)) < 0.01] = -np.inf
+    B = A if tri else rng.standard_normal((N, K))
+    monkeypatch.setenv("MCPG_MM_SPLIT", "0")
+    ref = handle.maxmin(A, B, symmetric=tri)
+    monkeypatch.setenv("MCPG_MM_SPLIT", "1")
+    got, names = _names(handle, lambda: handle.maxmin(A, B, symmetric=tri))
+    if M <= 1500:                        # far below one round of resident workgroups: always split
+        assert "mv_finish_tail_kernel" in names
+    assert bits_equal(got, ref)
+    for rows in (slice(0, 24), slice(M - 24, M)):
+        assert bits_equal(got[rows], oracle.maxmin(np.ascontiguousarray(A[rows]), B))
+
+
+@pytest.mark.parametrize("env", [{"MCPG_MM64_KD": "8"}, {"MCPG_MM64_TILE": "1"}])
+def test_f64_kernels_agree(handle, monkeypatch, env):
+    """The direct-load FP64 kernel at both K chunks and the register-staged tile kernel give the
+    same bits, for a masked rectangular product, a triangle product and MCP3."""
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal((500, 333))
+    B = rng.standard_normal((260, 333))
+    km = np.where(rng.random(333) < 0.5, np.finfo(np.float64).max, np.finfo(np.float64).min)
+    Y = mi_like(300, 8).astype(np.float64)
+    ref = (handle.maxmin(A, B, kmask=km), handle.maxmin(Y, Y, symmetric=True), handle.mcp(Y, 3, want_maxmin=True))
+    assert bits_equal(ref[0], oracle.maxmin(A, B, km))
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    got = (handle.maxmin(A, B, kmask=km), handle.maxmin(Y, Y, symmetric=True), handle.mcp(Y, 3, want_maxmin=True))
+    assert bits_equal(got[0], ref[0])
+    assert bits_equal(got[1], ref[1])
+    assert bits_equal(got[2][0], ref[2][0]) and bits_equal(got[2][1], ref[2][1])
