@@ -30,7 +30,22 @@ static_assert(kStackSlots == 1, "the stack's push and top address lane sp direct
 
 struct Win {
   int x1, y1, x2, y2, n;
+  int node = 0, depth = 0;   // the window's cell in the quadrant pyramid (node of level `depth`)
 };
+
+// Quadrant pyramid.  Every window the recursion can pop is a cell of a fixed dyadic grid: the
+// root is [0, S) x [0, S) and each split halves both rank ranges at (lo + hi) >> 1, so a window
+// at depth d is one of the 4^d cells whose x and y ranges are depth-d intervals of that bisection
+// of [0, S).  A pair's point counts on the depth-kPyrDepth grid, summed up level by level, give
+// the four quadrant counts of every window above that depth with one LDS read, exactly as the
+// scans would count them.  Cells are numbered in Morton order (x bit below y bit at each level),
+// so the four children of node k of level L are nodes 4k .. 4k + 3 of level L + 1, in the order
+// of the reference's quadrants (x low / y low, x high / y low, x low / y high, x high / y high).
+constexpr int kPyrDepth = 5;
+__host__ __device__ constexpr int pyr_offset(int level) { return ((1 << (2 * level)) - 4) / 3; }   // level >= 1
+constexpr int kPyrCells = pyr_offset(kPyrDepth + 1);   // levels 1 .. kPyrDepth: 4 + 16 + ... + 4^5 = 1364
+// Counts are 16-bit (a window holds at most S <= 8191 points); level offsets are multiples of 4
+// cells, so the four children of a node are one aligned 8-byte LDS read.
 
 __device__ __forceinline__ int rfl(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -123,6 +138,65 @@ __global__ __launch_bounds__(256) void ybucket_kernel(const int32_t* __restrict_
   }
 }
 
+// Z[g][s]: the bisection path of rank R[g][s] down kPyrDepth levels (bit kPyrDepth - 1 - l set
+// when the rank falls in the upper half at level l + 1), its bits spread to the even positions
+// of a Morton code.  A pair's cell at the deepest level is Z[gx][s] | Z[gy][s] << 1.
+__global__ void apcode_kernel(const int32_t* __restrict__ R, int64_t ldr, int64_t G, int64_t S,
+                              uint16_t* __restrict__ Z) {
+  const int64_t total = G * S;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t g = t / S, s = t % S;
+    const int r = R[g * ldr + s];
+    int lo = 0, hi = (int)S, code = 0;
+    for (int l = 0; l < kPyrDepth; ++l) {
+      const int m = (lo + hi) >> 1;
+      const int up = r >= m;
+      code |= up << (2 * (kPyrDepth - 1 - l));
+      if (up) lo = m;
+      else hi = m;
+    }
+    Z[t] = (uint16_t)code;
+  }
+}
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// The pair's point counts on the deepest grid (LDS atomics on the 32-bit words that hold two
+// 16-bit cells), then each coarser level as sums of four children.
+__device__ __forceinline__ uint32_t sum4_u16(uint2 q) {
+  return (q.x & 0xffffu) + (q.x >> 16) + (q.y & 0xffffu) + (q.y >> 16);
+}
+__device__ __forceinline__ void build_pyramid(uint16_t* __restrict__ pyr, const uint16_t* __restrict__ zx,
+                                              const uint16_t* __restrict__ zy, int S, int lane) {
+  static_assert(pyr_offset(kPyrDepth) % 2 == 0, "the deepest level starts on a 32-bit word");
+  uint32_t* leaf = reinterpret_cast<uint32_t*>(pyr + pyr_offset(kPyrDepth));
+#pragma unroll
+  for (int i = lane; i < (1 << (2 * kPyrDepth)) / 2; i += 64) leaf[i] = 0u;
+  wave_sync();
+  for (int base = 0; base < S; base += 64 * 4) {
+    int c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int i = base + 64 * u + lane;
+      c[u] = i < S ? ((int)zx[i] | ((int)zy[i] << 1)) : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (c[u] >= 0) atomicAdd(&leaf[c[u] >> 1], 1u << ((c[u] & 1) << 4));
+  }
+  wave_sync();
+#pragma unroll
+  for (int L = kPyrDepth - 1; L >= 1; --L) {
+    const uint2* dn = reinterpret_cast<const uint2*>(pyr + pyr_offset(L + 1));
+    uint16_t* up = pyr + pyr_offset(L);
+    for (int k = lane; k < (1 << (2 * L)); k += 64) up[k] = (uint16_t)sum4_u16(dn[k]);
+    wave_sync();
+  }
+}
+
 __device__ __forceinline__ void tri_decode(int64_t Lid, int64_t nb, int64_t* bm, int64_t* bn) {
   double a = 2.0 * (double)nb + 1.0;
   int64_t r = (int64_t)((a - sqrt(a * a - 8.0 * (double)Lid)) * 0.5);
@@ -155,6 +229,7 @@ struct APArgs {
   int* flags;
   unsigned long long* queue;   // all-pairs mode: next unclaimed pair index, zero at launch
   const int32_t* ident;        // per gene: 1 if its histogram is the identity (cumhist_kernel)
+  const uint16_t* Z;           // per gene and sample: the Morton x code of its rank's pyramid cell (apcode_kernel)
 };
 
 constexpr int kChunk = 16;     // consecutive pairs a wave claims from the queue at a time
@@ -166,23 +241,50 @@ constexpr int kChunk = 16;     // consecutive pairs a wave claims from the queue
 // padding (kXgPad entries past S) and take x = -1, which no window contains.  (One pass with
 // four ballots per 64 points measured slower: 9.5 against 8.3 ms at 2000 x 1000.)
 constexpr int kXgPad = 64;
+// LDS bytes per wave: the quadrant pyramid, then the grouped points (16-bit ranks), each rounded
+// to 16 bytes
+__host__ __device__ constexpr int node_offset(int level) { return ((1 << (2 * level)) - 1) / 3; }
+constexpr int kTreeNodes = node_offset(kPyrDepth);   // levels 0 .. kPyrDepth - 1: 341 nodes
+constexpr int kStageItems = 64;
+__host__ __device__ inline int64_t ap_pyr_bytes() { return (2 * kPyrCells + 15) & ~15; }
+// the tree pass's staged leaves and node words share the space of the grouped points, which
+// only the sequential walk uses
+__host__ __device__ inline int64_t ap_tree_bytes() { return (8 * kStageItems + 4 * kTreeNodes + 15) & ~15; }
+__host__ __device__ inline int64_t ap_wave_bytes(int64_t S) {
+  const int64_t xg = (2 * (S + kXgPad) + 15) & ~(int64_t)15;
+  return ap_pyr_bytes() + (xg > ap_tree_bytes() ? xg : ap_tree_bytes());
+}
 
-__device__ __forceinline__ void count_half(const int* __restrict__ xg, int b0, int b1, int x1, int mx, int x2, int lane,
+__device__ __forceinline__ void count_half(const uint16_t* __restrict__ xg, int b0, int b1, int x1, int mx, int x2, int lane,
                                            int& nl, int& nr) {
   nl = nr = 0;
   const uint32_t wl = (uint32_t)(mx - x1), wr = (uint32_t)(x2 - mx);
   for (int base = b0; base < b1; base += 64) {
     const int idx = base + lane;
-    const int v = xg[idx];
+    const int v = (int)xg[idx];
     const int x = idx < b1 ? v : -1;
     nl += __popcll(__builtin_amdgcn_ballot_w64((uint32_t)(x - x1) < wl));
     nr += __popcll(__builtin_amdgcn_ballot_w64((uint32_t)(x - mx) < wr));
   }
 }
-__device__ __forceinline__ void count_quadrants(const int* __restrict__ xg, int b0, int bm, int b1, int x1, int mx,
+__device__ __forceinline__ void count_quadrants(const uint16_t* __restrict__ xg, int b0, int bm, int b1, int x1, int mx,
                                                 int x2, int lane, int& n0, int& n1, int& n2, int& n3) {
   count_half(xg, b0, bm, x1, mx, x2, lane, n0, n1);
   count_half(xg, bm, b1, x1, mx, x2, lane, n2, n3);
+}
+
+// The reference's test (AdaptivePartitioningMI.hpp:608-619) computes, in double, e = n / 4,
+// chi = sum (n_i - e)^2 / e and splits when chi > 7.815.  Every step before the division is exact,
+// so chi is D / (4 n) rounded once, with D = sum (4 n_i - n)^2 an integer (D <= 12 n^2 < 2^31).
+// A correctly rounded quotient exceeds the double c = 7.815 = M 2^-50 exactly when the exact one
+// exceeds the midpoint (2M + 1) 2^-51 between c and the next double (it never equals it: that
+// would need 2^49 | n), i.e. when D > floor(n (2M + 1) / 2^49).
+constexpr double kChi2 = 7.815;
+constexpr uint64_t kChi2Mid = 2 * (uint64_t)(kChi2 * 0x1p50) + 1;   // 2M + 1: 54 bits
+static_assert((double)(kChi2Mid >> 1) * 0x1p-50 == kChi2, "7.815 is M 2^-50 for an integer M");
+__host__ __device__ __forceinline__ int chi2_threshold(int n) {
+  const uint64_t hi = (uint64_t)n * (kChi2Mid >> 32), lo = (uint64_t)n * (kChi2Mid & 0xffffffffull);
+  return (int)((hi + (lo >> 32)) >> 17);
 }
 
 __device__ __forceinline__ double readlane_f64(double v, int k) {
@@ -192,12 +294,191 @@ __device__ __forceinline__ double readlane_f64(double v, int k) {
   return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
-// Evaluates AdaptivePartitionRankMIKernel2::operator()(x=gx, y=gy) (AdaptivePartitioningMI.hpp:553-645).
-// Returns the FP64 value mi/S + log2S (log2S = log2(S), computed once by the caller).  If
-// rec != nullptr, leaf (nxy, nx, ny) triples are recorded in summation order (up to max_leaves)
-// and *nleaves receives the total count.
-__device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_t gx, int64_t gy, int* xg, int lane,
-                          double log2S, int32_t* rec, int32_t* nleaves) {
+// Rank intervals of node k of pyramid level L: the bisection of [0, S) along the node's Morton
+// path (x bit below y bit at each level).
+__device__ __forceinline__ void node_bounds(int S, int L, int k, int& x1, int& x2, int& y1, int& y2) {
+  int xl = 0, xh = S, yl = 0, yh = S;
+  for (int l = 0; l < L; ++l) {
+    const int sh = 2 * (L - 1 - l);
+    const int mx = (xl + xh) >> 1, my = (yl + yh) >> 1;
+    if ((k >> sh) & 1) xl = mx; else xh = mx;
+    if ((k >> (sh + 1)) & 1) yl = my; else yh = my;
+  }
+  x1 = xl; x2 = xh; y1 = yl; y2 = yh;
+}
+
+// The LIFO walk of the reference over the windows above the pyramid's deepest level, evaluated
+// level by level with one lane per node instead of one window at a time.  A node's word in LDS:
+// bit 0 visited (the walk pops it), bit 1 split, bits 2-5 the children it pushes, bits 6-9 the
+// children it adds as leaves at once, bits 16-23 the number of leaves its subtree adds, bits
+// 24-31 the position of its first leaf in the walk's order.  Popping a split window adds its
+// non-splittable children with points (children 0 .. 3 in order), then walks its pushed
+// children last-pushed first, so a split node's leaves are [its leaf children] followed by the
+// subtrees of its pushed children 3, 2, 1, 0.  The leaves (n, nx, ny) are staged in that order.
+// Returns their number, or -1 when the walk would pop a window below the pyramid or add more than
+// kStageItems leaves (the caller then walks the pair window by window).  Needs S >= 2^(kPyrDepth+1),
+// so that every interval down to the pyramid's deepest level holds two ranks or more and a window
+// is splittable exactly when it holds more than two points.
+template <typename HX, typename HY>
+__device__ int tree_pass(const uint16_t* __restrict__ pyr, uint32_t* __restrict__ nodes, uint2* __restrict__ stage,
+                         int S, int lane, HX&& hx, HY&& hy) {
+  auto child_counts = [&](int L, int k, int (&cc)[4]) {
+    const uint2 c = *reinterpret_cast<const uint2*>(pyr + pyr_offset(L + 1) + 4 * k);
+    cc[0] = (int)(c.x & 0xffffu); cc[1] = (int)(c.x >> 16); cc[2] = (int)(c.y & 0xffffu); cc[3] = (int)(c.y >> 16);
+  };
+  // 1. top-down: visited, split, pushed and leaf children
+  int deepest = 0;
+  for (int L = 0; L < kPyrDepth; ++L) {
+    const int level_nodes = 1 << (2 * L);
+    bool pushes = false;
+    for (int k0 = 0; k0 < level_nodes; k0 += 64) {
+      const int k = k0 + lane;
+      if (k < level_nodes) {
+        uint32_t word = 0;
+        const bool visited = L == 0 || ((nodes[node_offset(L - 1) + (k >> 2)] >> (2 + (k & 3))) & 1u);
+        if (visited) {
+          const int n = L == 0 ? S : (int)pyr[pyr_offset(L) + k];
+          int cc[4];
+          child_counts(L, k, cc);
+          int dev = 0;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int t = 4 * cc[i] - n;
+            dev += t * t;
+          }
+          const bool split = L == 0 || dev > chi2_threshold(n);
+          uint32_t pushed = 0, leafm = 0;
+          if (split) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              if (cc[i] > 2) pushed |= 1u << i;
+              else if (cc[i] > 0) leafm |= 1u << i;
+            }
+          }
+          word = 1u | ((uint32_t)split << 1) | (pushed << 2) | (leafm << 6);
+          pushes |= pushed != 0;
+        }
+        nodes[node_offset(L) + k] = word;
+      }
+    }
+    wave_sync();
+    deepest = L;
+    if (__builtin_amdgcn_ballot_w64(pushes) == 0) break;
+    if (L == kPyrDepth - 1) return -1;   // a window below the pyramid would be popped
+  }
+  // 2. bottom-up: leaves per visited subtree (saturating at 255)
+  for (int L = deepest; L >= 0; --L) {
+    const int level_nodes = 1 << (2 * L);
+    for (int k0 = 0; k0 < level_nodes; k0 += 64) {
+      const int k = k0 + lane;
+      if (k < level_nodes) {
+        const uint32_t word = nodes[node_offset(L) + k];
+        if (word & 1u) {
+          int cnt = 1;
+          if (word & 2u) {
+            cnt = __builtin_popcount((word >> 6) & 0xfu);
+            const uint32_t pushed = (word >> 2) & 0xfu;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (pushed & (1u << i)) cnt += (int)((nodes[node_offset(L + 1) + 4 * k + i] >> 16) & 0xffu);
+          }
+          nodes[node_offset(L) + k] = word | ((uint32_t)(cnt < 255 ? cnt : 255) << 16);
+        }
+      }
+    }
+    wave_sync();
+  }
+  const int total = (int)((nodes[0] >> 16) & 0xffu);
+  if (total > kStageItems) return -1;
+  // 3. top-down: leaf positions; split nodes stage their leaf children and place their pushed ones
+  for (int L = 0; L <= deepest; ++L) {
+    const int level_nodes = 1 << (2 * L);
+    for (int k0 = 0; k0 < level_nodes; k0 += 64) {
+      const int k = k0 + lane;
+      if (k < level_nodes) {
+        const uint32_t word = nodes[node_offset(L) + k];
+        if (word & 1u) {
+          const int pos = (int)(word >> 24);
+          int x1, x2, y1, y2;
+          if (word & 2u) {
+            int cc[4];
+            child_counts(L, k, cc);
+            const uint32_t leafm = (word >> 6) & 0xfu, pushed = (word >> 2) & 0xfu;
+            int q = pos;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              if (leafm & (1u << i)) {
+                node_bounds(S, L + 1, 4 * k + i, x1, x2, y1, y2);
+                stage[q++] = make_uint2((uint32_t)cc[i] | ((uint32_t)(hx(x2) - hx(x1)) << 16),
+                                        (uint32_t)(hy(y2) - hy(y1)));
+              }
+            }
+#pragma unroll
+            for (int i = 3; i >= 0; --i) {
+              if (pushed & (1u << i)) {
+                uint32_t* cw = nodes + node_offset(L + 1) + 4 * k + i;
+                const uint32_t c = *cw;
+                *cw = (c & 0x00ffffffu) | ((uint32_t)q << 24);
+                q += (int)((c >> 16) & 0xffu);
+              }
+            }
+          } else {
+            node_bounds(S, L, k, x1, x2, y1, y2);
+            const int n = L == 0 ? S : (int)pyr[pyr_offset(L) + k];
+            stage[pos] = make_uint2((uint32_t)n | ((uint32_t)(hx(x2) - hx(x1)) << 16), (uint32_t)(hy(y2) - hy(y1)));
+          }
+        }
+      }
+    }
+    wave_sync();
+  }
+  return total;
+}
+
+// Leaf terms (compute_for_window, :525-538) of consecutive pairs are batched per wave: leaf k of
+// the batch is held by lane k, and a batch (flushed before it can pass 64 leaves, when 64 pairs
+// have completed, and at the end) evaluates n log2(n / (nx ny)) on all lanes at once.  Each
+// pair's terms then enter its own FP64 sum one by one in leaf order, as in the reference's loop:
+// completed pair k of the batch (lane k) keeps its first leaf, its sum over leaves of earlier
+// batches and where its result goes.  One evaluation then serves several pairs.
+struct LeafBatch {
+  int pend = 0, pn = 0, pnx = 0, pny = 0;   // leaves; lane k: leaf k
+  int done = 0;                              // completed pairs waiting for the evaluation
+  int p_first = 0, p_x = 0, p_y = 0;         // lane k: pair k's first leaf and its genes (or output slot)
+  double p_mi = 0.0;                         // lane k: pair k's sum over leaves of earlier batches
+  int cur_first = 0;                         // first leaf of the pair in progress
+  double cur_mi = 0.0;                       // its sum over leaves of earlier batches
+};
+
+// Evaluates the batch: the completed pairs' sums are finished and handed to store(x, y, value)
+// (value = mi/S + log2S), the pair in progress keeps its partial sum.
+template <typename Store>
+__device__ __forceinline__ void flush_batch(LeafBatch& b, int lane, int S, double log2S, Store&& store) {
+  double t = 0.0;
+  if (lane < b.pend) t = (double)b.pn * log2((double)b.pn / (double)((int64_t)b.pnx * (int64_t)b.pny));
+  int l = 0;
+  for (int k = 0; k < b.done; ++k) {
+    const int end = k + 1 < b.done ? __builtin_amdgcn_readlane(b.p_first, k + 1) : b.cur_first;
+    double m = readlane_f64(b.p_mi, k);
+    for (; l < end; ++l) m += readlane_f64(t, l);
+    store(__builtin_amdgcn_readlane(b.p_x, k), __builtin_amdgcn_readlane(b.p_y, k), m / (double)S + log2S);
+  }
+  for (; l < b.pend; ++l) b.cur_mi += readlane_f64(t, l);
+  b.pend = 0;
+  b.done = 0;
+  b.cur_first = 0;
+}
+
+// Evaluates AdaptivePartitionRankMIKernel2::operator()(x=gx, y=gy) (AdaptivePartitioningMI.hpp:553-645)
+// into the wave's leaf batch; the pair is recorded there as completed under (ox, oy), and its value
+// mi/S + log2S reaches `store` when the batch is evaluated.  If rec != nullptr, leaf (nxy, nx, ny)
+// triples are recorded in summation order (up to max_leaves) and *nleaves receives the count.
+template <typename Store>
+__device__ void ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_t gx, int64_t gy, uint16_t* pyr,
+                        uint16_t* xg, int lane, double log2S, LeafBatch& b, int ox, int oy, Store&& store, int32_t* rec,
+                        int32_t* nleaves) {
+  uint2* stage = reinterpret_cast<uint2*>(xg);
+  uint32_t* nodes = reinterpret_cast<uint32_t*>(stage + kStageItems);
   const int S = (int)a.S;
   const int32_t* __restrict__ cHX = H + gx * a.ldh;
   const int32_t* __restrict__ cHY = H + gy * a.ldh;
@@ -207,33 +488,78 @@ __device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_
   auto hx = [&](int i) { return idX ? i : cHX[i]; };
   auto hy = [&](int i) { return idY ? i : cHY[i]; };
 
-  // _createSparseMatrix (:462-493): x ranks grouped by y rank, through the y gene's bucket order
-  // (P holds sample indices in [0, S)); eight loads per lane in flight at a time
-  const int32_t* __restrict__ py = a.P + gy * S;
-  for (int base = 0; base < S; base += 64 * 8) {
-    int v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = base + 64 * u + lane;
-      v[u] = i < S ? py[i] : 0;
+  build_pyramid(pyr, a.Z + gx * S, a.Z + gy * S, S, lane);
+
+  auto complete = [&]() {   // the pair is complete: record it in the batch
+    if (lane == b.done) {
+      b.p_first = b.cur_first;
+      b.p_mi = b.cur_mi;
+      b.p_x = ox;
+      b.p_y = oy;
     }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = base + 64 * u + lane;
-      v[u] = i < S ? xr[v[u]] : 0;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int i = base + 64 * u + lane;
-      if (i < S) xg[i] = v[u];
+    ++b.done;
+    b.cur_first = b.pend;
+    b.cur_mi = 0.0;
+    if (b.done == 64) flush_batch(b, lane, S, log2S, store);
+  };
+  if (S >= (2 << kPyrDepth)) {
+    const int nl = tree_pass(pyr, nodes, stage, S, lane, hx, hy);
+    if (nl >= 0) {
+      if (b.pend + nl > 64) flush_batch(b, lane, S, log2S, store);
+      if (lane < nl) {
+        const uint2 it = stage[lane];
+        if (rec && lane < a.max_leaves) {
+          rec[3 * lane + 0] = (int32_t)(it.x & 0xffffu);
+          rec[3 * lane + 1] = (int32_t)(it.x >> 16);
+          rec[3 * lane + 2] = (int32_t)it.y;
+        }
+      }
+      const int src = lane - b.pend;
+      if (src >= 0 && src < nl) {
+        const uint2 it = stage[src];
+        b.pn = (int)(it.x & 0xffffu);
+        b.pnx = (int)(it.x >> 16);
+        b.pny = (int)it.y;
+      }
+      b.pend += nl;
+      if (nleaves) *nleaves = nl;
+      wave_sync();   // the staged leaves are read before the next pair reuses the space
+      complete();
+      return;
     }
   }
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+
+  // _createSparseMatrix (:462-493): x ranks grouped by y rank, through the y gene's bucket order
+  // (P holds sample indices in [0, S)); eight loads per lane in flight at a time.  Only windows
+  // below the pyramid scan points, so a pair builds this on its first such window.
+  bool grouped = false;
+  auto group_points = [&]() {
+    const int32_t* __restrict__ py = a.P + gy * S;
+    for (int base = 0; base < S; base += 64 * 8) {
+      int v[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = base + 64 * u + lane;
+        v[u] = i < S ? py[i] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = base + 64 * u + lane;
+        v[u] = i < S ? xr[v[u]] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = base + 64 * u + lane;
+        if (i < S) xg[i] = (uint16_t)v[u];
+      }
+    }
+    wave_sync();
+    grouped = true;
+  };
 
   // register-resident LIFO stack: entry k in lane k, with x1, x2 and y1, y2 packed as 16-bit
-  // halves (ranks are below 2^13): a push writes three registers in one lane, a read is three
-  // readlanes
+  // halves (ranks are below 2^13) and n, the pyramid node and the depth in the third register
+  // (13, 10 and 5 bits): a push writes three registers in one lane, a read is three readlanes
   int sxs = 0, sys = 0, sn = 0;
   int sp = 0;  // number of entries
   bool whole = true;
@@ -241,27 +567,20 @@ __device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_
     if (lane == sp) {
       sxs = (w.x1 << 16) | w.x2;
       sys = (w.y1 << 16) | w.y2;
-      sn = w.n;
+      sn = w.n | (w.node << 13) | (w.depth << 23);
     }
     ++sp;
   };
   auto top = [&]() {
     const int ln = sp - 1;
     const int xs = __builtin_amdgcn_readlane(sxs, ln), ys = __builtin_amdgcn_readlane(sys, ln);
-    return Win{xs >> 16, ys >> 16, xs & 0xffff, ys & 0xffff, __builtin_amdgcn_readlane(sn, ln)};
+    const int nn = __builtin_amdgcn_readlane(sn, ln);
+    Win w{xs >> 16, ys >> 16, xs & 0xffff, ys & 0xffff, nn & 0x1fff};
+    w.node = (nn >> 13) & 0x3ff;
+    w.depth = nn >> 23;
+    return w;
   };
-  // Leaf terms (compute_for_window, :525-538) are batched: leaf k of a batch is held by lane k,
-  // a full batch (or the last one) evaluates n log2(n / (nx ny)) on all lanes at once, and the
-  // terms then enter the FP64 sum one by one in leaf order, as in the reference's loop.
-  double mi = 0.0;
   int32_t leaf = 0;
-  int pend = 0, pn = 0, pnx = 0, pny = 0;
-  auto flush = [&]() {
-    double t = 0.0;
-    if (lane < pend) t = (double)pn * log2((double)pn / (double)((int64_t)pnx * (int64_t)pny));
-    for (int k = 0; k < pend; ++k) mi += readlane_f64(t, k);
-    pend = 0;
-  };
   auto term = [&](const Win& w) {
     const int nx = hx(w.x2) - hx(w.x1);
     const int ny = hy(w.y2) - hy(w.y1);
@@ -271,30 +590,42 @@ __device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_
       rec[3 * leaf + 2] = ny;
     }
     ++leaf;
-    if (lane == pend) { pn = w.n; pnx = nx; pny = ny; }
-    ++pend;
+    if (lane == b.pend) { b.pn = w.n; b.pnx = nx; b.pny = ny; }
+    ++b.pend;
   };
 
   push(Win{0, 0, S, S, S});
   int overflow = 0;
   while (sp > 0) {
-    if (pend > 60) flush();   // a window adds at most four leaves, so a batch never exceeds 64
+    if (b.pend > 60) flush_batch(b, lane, S, log2S, store);   // a window adds at most four leaves
     const Win w = top();
     const int mx = (w.x1 + w.x2) >> 1;
     const int my = (w.y1 + w.y2) >> 1;
     Win q[4] = {Win{w.x1, w.y1, mx, my, 0}, Win{mx, w.y1, w.x2, my, 0}, Win{w.x1, my, mx, w.y2, 0},
                 Win{mx, my, w.x2, w.y2, 0}};
-    const int b0 = rfl(hy(w.y1)), bm = rfl(hy(my)), b1 = rfl(hy(w.y2));
-    count_quadrants(xg, b0, bm, b1, w.x1, mx, w.x2, lane, q[0].n, q[1].n, q[2].n, q[3].n);
-    const double e = (double)w.n / 4.0;
-    double chi = 0.0;
+    if (w.depth < kPyrDepth) {
+      // the window's four quadrants are the children of its pyramid node
+      const uint2 c = *reinterpret_cast<const uint2*>(pyr + pyr_offset(w.depth + 1) + 4 * w.node);
+      const uint32_t c0 = (uint32_t)rfl((int)c.x), c1 = (uint32_t)rfl((int)c.y);
+      q[0].n = (int)(c0 & 0xffffu); q[1].n = (int)(c0 >> 16); q[2].n = (int)(c1 & 0xffffu); q[3].n = (int)(c1 >> 16);
+    } else {
+      if (!grouped) group_points();
+      const int b0 = rfl(hy(w.y1)), bm = rfl(hy(my)), b1 = rfl(hy(w.y2));
+      count_quadrants(xg, b0, bm, b1, w.x1, mx, w.x2, lane, q[0].n, q[1].n, q[2].n, q[3].n);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const double d = (double)q[i].n - e;
-      chi += d * d;
+      q[i].depth = w.depth + 1;
+      q[i].node = w.depth + 1 <= kPyrDepth ? 4 * w.node + i : 0;
     }
-    chi /= e;
-    if (chi > 7.815 || whole) {
+    // chi-square > 7.815 (:608-619), decided exactly in integer arithmetic
+    int dev = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int t = 4 * q[i].n - w.n;
+      dev += t * t;
+    }
+    if (dev > chi2_threshold(w.n) || whole) {
       --sp;
       whole = false;  // only the first window is `whole`; pushed children are copies with whole = false
 #pragma unroll
@@ -315,57 +646,67 @@ __device__ double ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_
       --sp;  // unreachable for pushed windows (n > 2); keeps the loop finite
     }
   }
-  flush();
   if (overflow && lane == 0) atomicOr(a.flags, 4);
   if (nleaves) *nleaves = leaf;
-  return mi / (double)S + log2S;
+  complete();
 }
 
 // H is passed on its own, restrict-qualified: the histogram entries a window reads are
-// wave-uniform and, with no store that may alias them, load through the scalar cache.
-__global__ __launch_bounds__(64 * kMaxWaves) void ap_mi_kernel(APArgs a, const int32_t* __restrict__ H) {
+// wave-uniform and, with no store that may alias them, load through the scalar cache.  CELLS:
+// the listed pairs with their leaves recorded; otherwise every pair of the upper triangle.
+template <bool CELLS>
+__global__ __launch_bounds__(64 * kMaxWaves) __attribute__((amdgpu_waves_per_eu(8)))
+void ap_mi_kernel(APArgs a, const int32_t* __restrict__ H) {
   extern __shared__ int lds[];
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int waves = blockDim.x >> 6;
-  int* xg = lds + (int64_t)w * (a.S + kXgPad);
+  uint16_t* pyr = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(lds) + (int64_t)w * ap_wave_bytes(a.S));
+  uint16_t* xg = reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(pyr) + ap_pyr_bytes());
   const int64_t total_waves = (int64_t)gridDim.x * waves;
   const int64_t wave_id = (int64_t)blockIdx.x * waves + w;
   const double log2S = log2((double)a.S);
-  if (a.npairs > 0) {
+  const int S = (int)a.S;
+  LeafBatch b;
+  if constexpr (CELLS) {
+    auto store = [&](int n, int, double v) {
+      if (lane == 0) a.mi_out[n] = v;
+    };
     for (int64_t n = wave_id; n < a.npairs; n += total_waves) {
       int32_t nl = 0;
-      const double v =
-          ap_pair(a, H, a.px[n], a.py[n], xg, lane, log2S, a.cells + n * 3 * (int64_t)a.max_leaves, &nl);
-      if (lane == 0) {
-        a.nleaves[n] = nl;
-        a.mi_out[n] = v;
-      }
+      ap_pair(a, H, a.px[n], a.py[n], pyr, xg, lane, log2S, b, (int)n, 0, store,
+              a.cells + n * 3 * (int64_t)a.max_leaves, &nl);
+      if (lane == 0) a.nleaves[n] = nl;
     }
+    flush_batch(b, lane, S, log2S, store);
     return;
-  }
-  // All pairs (i <= j, row-major over the upper triangle), claimed kChunk at a time from a
-  // global counter so that waves whose pairs recurse less take more of them; a wave leaves once
-  // the counter has passed the last pair.
-  const int64_t P = a.G * (a.G + 1) / 2;
-  for (;;) {
-    unsigned long long claim = 0;
-    if (lane == 0) claim = atomicAdd(a.queue, (unsigned long long)kChunk);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)claim);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(claim >> 32));
-    const int64_t base = (int64_t)(((uint64_t)hi << 32) | lo);
-    if (base >= P) break;
-    const int64_t end = base + kChunk < P ? base + kChunk : P;
-    int64_t i, j;
-    tri_decode(base, a.G, &i, &j);
-    for (int64_t p = base; p < end; ++p) {
-      const double v = ap_pair(a, H, i, j, xg, lane, log2S, nullptr, nullptr);
+  } else {
+    auto store = [&](int i, int j, double v) {
       if (lane == 0) {
         const float f = (float)v;
-        a.MI[i * a.ldmi + j] = f;
-        a.MI[j * a.ldmi + i] = f;
+        a.MI[(int64_t)i * a.ldmi + j] = f;
+        a.MI[(int64_t)j * a.ldmi + i] = f;
       }
-      if (++j == a.G) { ++i; j = i; }
+    };
+    // All pairs (i <= j, row-major over the upper triangle), claimed kChunk at a time from a
+    // global counter so that waves whose pairs recurse less take more of them; a wave leaves
+    // once the counter has passed the last pair.
+    const int64_t P = a.G * (a.G + 1) / 2;
+    for (;;) {
+      unsigned long long claim = 0;
+      if (lane == 0) claim = atomicAdd(a.queue, (unsigned long long)kChunk);
+      const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)claim);
+      const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(claim >> 32));
+      const int64_t base = (int64_t)(((uint64_t)hi << 32) | lo);
+      if (base >= P) break;
+      const int64_t end = base + kChunk < P ? base + kChunk : P;
+      int64_t i, j;
+      tri_decode(base, a.G, &i, &j);
+      for (int64_t p = base; p < end; ++p) {
+        ap_pair(a, H, i, j, pyr, xg, lane, log2S, b, (int)i, (int)j, store, nullptr, nullptr);
+        if (++j == a.G) { ++i; j = i; }
+      }
     }
+    flush_batch(b, lane, S, log2S, store);
   }
 }
 
@@ -507,6 +848,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   sz.add<int32_t>(G * (S + 1));
   sz.add<int32_t>(G * S);
   sz.add<int32_t>(G);
+  sz.add<uint16_t>(G * S);
   if (!cells_mode && ptrs == MCPG_HOST) sz.add<float>(G * G);
   if (cells_mode) {
     sz.add<int32_t>(npairs); sz.add<int32_t>(npairs);
@@ -525,6 +867,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   int32_t* dH = cv.take<int32_t>(G * (S + 1));
   int32_t* dP = cv.take<int32_t>(G * S);
   int32_t* dId = cv.take<int32_t>(G);
+  uint16_t* dZ = cv.take<uint16_t>(G * S);
   float* dMI = MI;
   int64_t lm = ldmi;
   if (!cells_mode && ptrs == MCPG_HOST) { dMI = cv.take<float>(G * G); lm = G; }
@@ -541,11 +884,17 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
     hipLaunchKernelGGL(ybucket_kernel, dim3((unsigned)G), dim3(256), S * sizeof(int), h->stream, dR, lr, S, dH, S + 1,
                        dP);
   }
+  {
+    mcpg::LaunchScope ls(h, "apcode_kernel");
+    hipLaunchKernelGGL(apcode_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(G * S, 256), 65536)), dim3(256), 0,
+                       h->stream, dR, lr, G, S, dZ);
+  }
   APArgs a{};
   a.R = dR; a.ldr = lr; a.H = dH; a.ldh = S + 1; a.P = dP; a.G = G; a.S = S; a.MI = dMI; a.ldmi = lm;
   a.flags = h->d_flags;
   a.queue = reinterpret_cast<unsigned long long*>(h->d_flags + 8);   // zeroed by reset_flags
   a.ident = dId;
+  a.Z = dZ;
   int32_t *d_px = nullptr, *d_py = nullptr, *d_cells = nullptr, *d_nl = nullptr;
   double* d_mi = nullptr;
   if (cells_mode) {
@@ -558,22 +907,26 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
     a.mi_out = d_mi;
   }
   const int64_t work = cells_mode ? npairs : G * (G + 1) / 2;
-  const int64_t per_wave = (S + kXgPad) * (int64_t)sizeof(int);
+  const int64_t per_wave = ap_wave_bytes(S);
   const int waves = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxWaves, kLdsBudget / per_wave));
   const size_t lds = (size_t)waves * per_wave;
   int64_t blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, waves), 256 * 64 / waves));
   if (!cells_mode) {   // queue mode: only as many workgroups as stay resident together
     int cus = 0, occ = 0;
     MCPG_HIP(h, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
-    MCPG_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&ap_mi_kernel),
+    MCPG_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void*>(&ap_mi_kernel<false>),
                                                              64 * waves, lds));
     blocks = std::max<int64_t>(1, std::min<int64_t>(ceil_div(work, (int64_t)waves * kChunk),
                                                     (int64_t)std::max(cus, 1) * std::max(occ, 1)));
   }
   {
     mcpg::LaunchScope ls(h, "ap_mi_kernel");
-    hipLaunchKernelGGL(ap_mi_kernel, dim3((unsigned)blocks), dim3(64 * waves), lds, h->stream, a,
-                       (const int32_t*)dH);
+    if (cells_mode)
+      hipLaunchKernelGGL(ap_mi_kernel<true>, dim3((unsigned)blocks), dim3(64 * waves), lds, h->stream, a,
+                         (const int32_t*)dH);
+    else
+      hipLaunchKernelGGL(ap_mi_kernel<false>, dim3((unsigned)blocks), dim3(64 * waves), lds, h->stream, a,
+                         (const int32_t*)dH);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return mcpg::hip_error(h, e, "ap_mi_kernel launch");

@@ -130,7 +130,7 @@ int copy_2d(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitc
 }
 
 int reset_flags(mcpg_ctx* h) {
-  MCPG_HIP(h, hipMemsetAsync(h->d_flags, 0, 16 * sizeof(int), h->stream));
+  MCPG_HIP(h, hipMemsetAsync(h->d_flags, 0, kFlagWords * sizeof(int), h->stream));
   return MCPG_OK;
 }
 
@@ -253,7 +253,7 @@ int mcpg_create(mcpg_handle* out, int device) {
   h->device = device;
   if (hipSetDevice(device) != hipSuccess) { delete h; return MCPG_ERR_NO_DEVICE; }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { delete h; return MCPG_ERR_HIP; }
-  if (hipMalloc(&h->d_flags, 16 * sizeof(int)) != hipSuccess) {
+  if (hipMalloc(&h->d_flags, mcpg::kFlagWords * sizeof(int)) != hipSuccess) {
     (void)hipStreamDestroy(h->stream);
     delete h;
     return MCPG_ERR_OUT_OF_MEMORY;

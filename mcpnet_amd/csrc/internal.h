@@ -16,7 +16,7 @@ struct mcpg_ctx {
   std::string last_error;
   void* ws = nullptr;        // grow-only device workspace
   size_t ws_bytes = 0;
-  int* d_flags = nullptr;    // small device array of error flags
+  int* d_flags = nullptr;    // small device array: error flags (words 0-7), pair-queue counters (words 8-31)
   int last_launches = 0;
   const char* last_kernel = "";
   // per-launch timing of the most recent call (only when profiling is on)
@@ -93,6 +93,8 @@ struct Sizer {
 // has completed when copy_2d returns.
 int copy_2d(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
             size_t rows, hipMemcpyKind kind);
+
+constexpr int kFlagWords = 32;   // ints in mcpg_ctx::d_flags
 
 // Clears the device error flags and reads them back (synchronising the stream).
 int reset_flags(mcpg_ctx* h);

@@ -1751,7 +1751,7 @@ int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N,
       // scratch path clears the flag with a kernel (no memset node for callers that capture).
       if (sc) {
         mcpg::LaunchScope ls(h, "fill_i32_kernel");
-        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, h->stream, h->d_flags, 16, 0);
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, h->stream, h->d_flags, mcpg::kFlagWords, 0);
       } else if ((st = mcpg::reset_flags(h))) {
         return st;
       }
