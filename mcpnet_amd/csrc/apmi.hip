@@ -141,11 +141,18 @@ __global__ __launch_bounds__(256) void ybucket_kernel(const int32_t* __restrict_
 // Z[g][s]: the bisection path of rank R[g][s] down kPyrDepth levels (bit kPyrDepth - 1 - l set
 // when the rank falls in the upper half at level l + 1), its bits spread to the even positions
 // of a Morton code.  A pair's cell at the deepest level is Z[gx][s] | Z[gy][s] << 1.
+// Rows are padded to ldz (a multiple of the pyramid builder's 256-sample step) with code 0.
+constexpr int kZStep = 256;
+__host__ __device__ inline int64_t ap_ldz(int64_t S) { return (S + kZStep - 1) / kZStep * kZStep; }
 __global__ void apcode_kernel(const int32_t* __restrict__ R, int64_t ldr, int64_t G, int64_t S,
-                              uint16_t* __restrict__ Z) {
-  const int64_t total = G * S;
+                              uint16_t* __restrict__ Z, int64_t ldz) {
+  const int64_t total = G * ldz;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t g = t / S, s = t % S;
+    const int64_t g = t / ldz, s = t % ldz;
+    if (s >= S) {
+      Z[t] = 0;
+      continue;
+    }
     const int r = R[g * ldr + s];
     int lo = 0, hi = (int)S, code = 0;
     for (int l = 0; l < kPyrDepth; ++l) {
@@ -176,17 +183,21 @@ __device__ __forceinline__ void build_pyramid(uint16_t* __restrict__ pyr, const 
 #pragma unroll
   for (int i = lane; i < (1 << (2 * kPyrDepth)) / 2; i += 64) leaf[i] = 0u;
   wave_sync();
-  for (int base = 0; base < S; base += 64 * 4) {
-    int c[4];
+  // the rows of Z are padded to a multiple of kZStep samples with code 0: every lane adds its
+  // four points without a bounds test, and cell 0 gives the padding back afterwards
+  const int Sp = (int)ap_ldz(S);
+  for (int base = 0; base < Sp; base += kZStep) {
+    uint32_t zxv[4], zyv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int i = base + 64 * u + lane;
-      c[u] = i < S ? ((int)zx[i] | ((int)zy[i] << 1)) : -1;
+      zxv[u] = zx[base + 64 * u + lane];
+      zyv[u] = zy[base + 64 * u + lane];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (c[u] >= 0) atomicAdd(&leaf[c[u] >> 1], 1u << ((c[u] & 1) << 4));
+    for (int u = 0; u < 4; ++u) atomicAdd(&leaf[(zxv[u] >> 1) | zyv[u]], 1u << ((zxv[u] & 1u) << 4));
   }
+  wave_sync();
+  if (lane == 0 && Sp > S) leaf[0] -= (uint32_t)(Sp - S);
   wave_sync();
 #pragma unroll
   for (int L = kPyrDepth - 1; L >= 1; --L) {
@@ -230,6 +241,7 @@ struct APArgs {
   unsigned long long* queue;   // all-pairs mode: next unclaimed pair index, zero at launch
   const int32_t* ident;        // per gene: 1 if its histogram is the identity (cumhist_kernel)
   const uint16_t* Z;           // per gene and sample: the Morton x code of its rank's pyramid cell (apcode_kernel)
+  int64_t ldz;
 };
 
 constexpr int kChunk = 16;     // consecutive pairs a wave claims from the queue at a time
@@ -488,7 +500,7 @@ __device__ void ap_pair(const APArgs& a, const int32_t* __restrict__ H, int64_t 
   auto hx = [&](int i) { return idX ? i : cHX[i]; };
   auto hy = [&](int i) { return idY ? i : cHY[i]; };
 
-  build_pyramid(pyr, a.Z + gx * S, a.Z + gy * S, S, lane);
+  build_pyramid(pyr, a.Z + gx * a.ldz, a.Z + gy * a.ldz, S, lane);
 
   auto complete = [&]() {   // the pair is complete: record it in the batch
     if (lane == b.done) {
@@ -848,7 +860,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   sz.add<int32_t>(G * (S + 1));
   sz.add<int32_t>(G * S);
   sz.add<int32_t>(G);
-  sz.add<uint16_t>(G * S);
+  sz.add<uint16_t>(G * ap_ldz(S));
   if (!cells_mode && ptrs == MCPG_HOST) sz.add<float>(G * G);
   if (cells_mode) {
     sz.add<int32_t>(npairs); sz.add<int32_t>(npairs);
@@ -867,7 +879,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   int32_t* dH = cv.take<int32_t>(G * (S + 1));
   int32_t* dP = cv.take<int32_t>(G * S);
   int32_t* dId = cv.take<int32_t>(G);
-  uint16_t* dZ = cv.take<uint16_t>(G * S);
+  uint16_t* dZ = cv.take<uint16_t>(G * ap_ldz(S));
   float* dMI = MI;
   int64_t lm = ldmi;
   if (!cells_mode && ptrs == MCPG_HOST) { dMI = cv.take<float>(G * G); lm = G; }
@@ -886,8 +898,8 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   }
   {
     mcpg::LaunchScope ls(h, "apcode_kernel");
-    hipLaunchKernelGGL(apcode_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(G * S, 256), 65536)), dim3(256), 0,
-                       h->stream, dR, lr, G, S, dZ);
+    hipLaunchKernelGGL(apcode_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(G * ap_ldz(S), 256), 65536)), dim3(256),
+                       0, h->stream, dR, lr, G, S, dZ, ap_ldz(S));
   }
   APArgs a{};
   a.R = dR; a.ldr = lr; a.H = dH; a.ldh = S + 1; a.P = dP; a.G = G; a.S = S; a.MI = dMI; a.ldmi = lm;
@@ -895,6 +907,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   a.queue = reinterpret_cast<unsigned long long*>(h->d_flags + 8);   // zeroed by reset_flags
   a.ident = dId;
   a.Z = dZ;
+  a.ldz = ap_ldz(S);
   int32_t *d_px = nullptr, *d_py = nullptr, *d_cells = nullptr, *d_nl = nullptr;
   double* d_mi = nullptr;
   if (cells_mode) {
