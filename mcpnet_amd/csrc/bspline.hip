@@ -439,6 +439,8 @@ constexpr int kDmaRows = 16;                          // rows covered by one 1 K
 constexpr int kDmaPerPlane = kTileRows / kDmaRows;    // 10
 // buffer resource word 3 for raw (untyped, unswizzled) gfx9-family buffer accesses
 constexpr int kRsrcWord3 = 0x00020000;
+// s_waitcnt immediate (gfx9 encoding) for lgkmcnt(0) with the vector-memory and export counts free
+constexpr int kWaitLgkm0 = 0xc07f;
 
 template <int P>
 constexpr int mi_planes() { return P == kBF16X3 ? 2 : 1; }
@@ -636,7 +638,7 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
       // a bare barrier: __syncthreads()'s release fence would wait for every load in flight
       asm volatile("s_barrier" ::: "memory");
       const char* st = smem + (kk % NST) * kStage;
-      // The loads of a later step are issued two chunks at a time between the MFMA block rows
+      // The loads of a later step are issued a few chunks at a time between the MFMA block rows
       // below, so that a wave never waits long on a full load queue with MFMA work at hand.
       // Without ER they fill the buffer of step kk - 1, which every wave finished before the
       // barrier; with ER the buffer of step kk, once the second barrier has freed it.
@@ -646,32 +648,10 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
       const int next_count = next < nk ? count : 0;
       const char* sA = st;
       const char* sB = one_panel ? st : st + np * kPlaneB16;
-      // B fragments stay in registers for the whole K step; A fragments are read per block row
-      // (with ER: all of them up front).
       bf16x8 bh[5], bl[5], ahs[5], als[5];
-#pragma unroll
-      for (int j = 0; j < 5; ++j) {
-        const int o = (wc * 80 + j * 16) * kRowB16 + frag;
-        bh[j] = *reinterpret_cast<const bf16x8*>(sB + o);
-        if (P == kBF16X3) bl[j] = *reinterpret_cast<const bf16x8*>(sB + kPlaneB16 + o);
-      }
-      if (ER) {
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
-          const int o = (wr * 80 + i * 16) * kRowB16 + frag;
-          ahs[i] = *reinterpret_cast<const bf16x8*>(sA + o);
-          if (P == kBF16X3) als[i] = *reinterpret_cast<const bf16x8*>(sA + kPlaneB16 + o);
-        }
-        // every wave's reads of this buffer are complete after the barrier
-        asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-      }
-#pragma unroll
-      for (int i = 0; i < 5; ++i) {
-        dma_issue_part(rsrc, lbyte, next_src, grp_stride, next_count, c0, 2 * i, next_dst);
-        const int o = (wr * 80 + i * 16) * kRowB16 + frag;
-        const bf16x8 ah = ER ? ahs[i] : *reinterpret_cast<const bf16x8*>(sA + o);
+      // the three products (two for the low halves, then hi x hi) of block row i
+      auto mfma_row = [&](int i, const bf16x8& ah, const bf16x8& al) {
         if (P == kBF16X3) {
-          const bf16x8 al = ER ? als[i] : *reinterpret_cast<const bf16x8*>(sA + kPlaneB16 + o);
 #pragma unroll
           for (int j = 0; j < 5; ++j) {
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
@@ -680,6 +660,80 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
         }
 #pragma unroll
         for (int j = 0; j < 5; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+      };
+      if (ER) {
+        // Every fragment of the step is copied into registers, those of block row 0 first (B hi,
+        // A lo 0, B lo, A hi 0, then rows 1-4), each read held in place by a scheduling barrier,
+        // so that the MFMAs of rows 0 and 1 start as their operands arrive.  The barrier that
+        // frees the buffer follows them, and the next step's loads go out over rows 2-4.
+        // Nothing is outstanding in the LDS/scalar counter here; the explicit wait says so to the
+        // wait-count pass, which otherwise (a scalar load of the epilogue's arguments still in
+        // flight from the prologue) waits for all twenty reads before the first MFMA.
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        const int oa = (wr * 80) * kRowB16 + frag, ob = (wc * 80) * kRowB16 + frag;
+        bh[0] = *reinterpret_cast<const bf16x8*>(sB + ob);
+        if (P == kBF16X3) als[0] = *reinterpret_cast<const bf16x8*>(sA + kPlaneB16 + oa);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 1; j < 5; ++j) {
+          bh[j] = *reinterpret_cast<const bf16x8*>(sB + ob + j * 16 * kRowB16);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        if (P == kBF16X3) {
+#pragma unroll
+          for (int j = 0; j < 5; ++j) {
+            bl[j] = *reinterpret_cast<const bf16x8*>(sB + kPlaneB16 + ob + j * 16 * kRowB16);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
+        ahs[0] = *reinterpret_cast<const bf16x8*>(sA + oa);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 1; i < 5; ++i) {
+          const int o = oa + i * 16 * kRowB16;
+          if (P == kBF16X3) als[i] = *reinterpret_cast<const bf16x8*>(sA + kPlaneB16 + o);
+          ahs[i] = *reinterpret_cast<const bf16x8*>(sA + o);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        constexpr int kRelease = 2;                                            // block row
+        constexpr int kPer = (kDmaPerPlane + 4 - kRelease) / (5 - kRelease);   // chunks per later row
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+          if (i == kRelease) {
+            // every wave's reads of this buffer are complete after the barrier
+            __builtin_amdgcn_sched_barrier(0);
+            asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+          }
+          if (i >= kRelease) {
+#pragma unroll
+            for (int jj = 0; jj < kPer; ++jj) {
+              const int j = (i - kRelease) * kPer + jj;
+              if (j < next_count) {
+                const int c = c0 + j;
+                const uint32_t soff = next_src + (uint32_t)(c / 5) * grp_stride + (uint32_t)(c % 5) * (kDmaRows * kKStep * 2);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr)(next_dst + j * 1024), 16, lbyte, soff, 0, 0);
+              }
+            }
+          }
+          mfma_row(i, ahs[i], als[i]);
+        }
+      } else {
+        // B fragments stay in registers for the whole K step; A fragments are read per block row
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+          const int o = (wc * 80 + j * 16) * kRowB16 + frag;
+          bh[j] = *reinterpret_cast<const bf16x8*>(sB + o);
+          if (P == kBF16X3) bl[j] = *reinterpret_cast<const bf16x8*>(sB + kPlaneB16 + o);
+        }
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+          dma_issue_part(rsrc, lbyte, next_src, grp_stride, next_count, c0, 2 * i, next_dst);
+          const int o = (wr * 80 + i * 16) * kRowB16 + frag;
+          const bf16x8 ah = *reinterpret_cast<const bf16x8*>(sA + o);
+          const bf16x8 al = P == kBF16X3 ? *reinterpret_cast<const bf16x8*>(sA + kPlaneB16 + o) : ah;
+          mfma_row(i, ah, al);
+        }
       }
     }
     __syncthreads();
@@ -1128,7 +1182,7 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
   int st;
   hipError_t e;
   if (L.tr != kTileRows) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: weight layout does not match the MI kernel");
-  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 32), mcpg::env_group("MCPG_MI_XCD", 0), &st);
+  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 64), mcpg::env_group("MCPG_MI_XCD", 0), &st);
   if (!a.order) return st;
   const unsigned grid = (unsigned)a.ntri;
   // BF16 planes: a ring of 2 (BF16X3, 80 KB of LDS) or 4 (BF16, 80 KB) stage buffers with early

@@ -15,6 +15,9 @@
       max-min products of a seeded symmetric N x N matrix with diagonal 0 on the GPU (the
       triangle launch, R times, or MCP_K): the workload for kernel traces of the product alone;
       prints each launch's duration from the library's own event timing.
+  python -m mcpnet_amd.profiling mi --genes N --samples S [--precision P] [--reps R]
+      B-spline MI (10 bins, order 3) of seeded synthetic expression on the GPU, R times: the
+      workload for kernel traces of the MI kernel alone; prints each launch's duration.
 """
 from __future__ import annotations
 
@@ -60,6 +63,24 @@ def maxmin_main(genes: int, dtype: str, reps: int, mcp: int) -> None:
             h.maxmin(Ad, Ad, symmetric=True, out=C)
         for name, ms in h.last_timings():
             print(f"{genes} genes {dtype} rep {r}: {name} {ms:.3f} ms", flush=True)
+    h.close()
+
+
+def mi_main(genes: int, samples: int, precision: str, reps: int) -> None:
+    import torch
+
+    import mcpnet_amd
+    from mcpnet_amd import grn
+
+    X = grn.grn_expression(genes, samples, 0x4D49)[0]
+    h = mcpnet_amd.Handle(0)
+    h.set_profiling(True)
+    Xd = torch.from_numpy(X).cuda()
+    MI = torch.empty((genes, genes), dtype=torch.float32, device="cuda")
+    for r in range(reps):
+        h.bspline_mi(Xd, precision=precision, out=MI)
+        for name, ms in h.last_timings():
+            print(f"{genes} x {samples} {precision} rep {r}: {name} {ms:.3f} ms", flush=True)
     h.close()
 
 
@@ -163,6 +184,11 @@ def main(argv=None) -> None:
     m.add_argument("--dtype", choices=["f32", "f64"], default="f64")
     m.add_argument("--reps", type=int, default=3)
     m.add_argument("--mcp", type=int, default=0, choices=[0, 2, 3, 4])
+    b = sub.add_parser("mi")
+    b.add_argument("--genes", type=int, default=20000)
+    b.add_argument("--samples", type=int, default=2000)
+    b.add_argument("--precision", choices=["fp32", "bf16", "bf16x3"], default="bf16x3")
+    b.add_argument("--reps", type=int, default=3)
     for name in ("summarize", "traffic"):
         s = sub.add_parser(name)
         s.add_argument("--trace")
@@ -180,6 +206,8 @@ def main(argv=None) -> None:
         copy_main(a.mib)
     elif a.cmd == "maxmin":
         maxmin_main(a.genes, a.dtype, a.reps, a.mcp)
+    elif a.cmd == "mi":
+        mi_main(a.genes, a.samples, a.precision, a.reps)
     elif a.cmd == "summarize":
         sys.stdout.write(summarize(a.trace, a.pmc_fetch, a.pmc_write, a.pmc_fetch_copy, a.pmc_write_copy, a.copy_mib,
                                    a.title, a.command) + "\n")
