@@ -99,6 +99,16 @@ int mcpg_bspline_mi_weights_f32(mcpg_handle h, int ptrs, const float* W, int64_t
 int mcpg_bspline_mi_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx,
                         int bins, int order, int precision, float* MI, int64_t ldmi);
 
+/* The same for one block of the G x G MI matrix: MI (nrows x ncols, ldmi >= ncols) receives rows
+ * row0 .. row0+nrows-1 and columns col0 .. col0+ncols-1, bit-identical to those entries of
+ * mcpg_bspline_mi_f32.  This is the unit a rank of the reference's distributed run fills: the
+ * InnerProduct of mcpnet_helper.hpp:90-98 writes the rank's row partition of `dmi` (all G
+ * columns) from the allgathered weights (:84).  Weights and marginals are computed for all G
+ * genes; only the tiles that cover the block are launched. */
+int mcpg_bspline_mi_block_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx,
+                              int bins, int order, int precision, int64_t row0, int64_t nrows, int64_t col0,
+                              int64_t ncols, float* MI, int64_t ldmi);
+
 /* Adjoint of the B-spline MI (no counterpart in the reference, which has no gradients):
  * for L = sum_n m[n] * MI(pi[n], pj[n]) (pairs with pi != pj; the MI is the MCPG_FP32 forward),
  * writes dX[g][s] = dL/dX[g][s] through the min-max scaling (sub-gradient at the first arg-min /
@@ -136,6 +146,13 @@ int mcpg_cumhist_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64
  * the reference's LIFO window order. */
 int mcpg_ap_mi_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,
                    float* MI, int64_t ldmi);
+
+/* One block of the same matrix (rows row0 .. row0+nrows-1, columns col0 .. col0+ncols-1 into MI,
+ * ldmi >= ncols), bit-identical to those entries of mcpg_ap_mi_i32: each entry (i, j) is
+ * kernel(x=min(i,j), y=max(i,j)).  The row-partition unit of mi.cpp:170-177 and
+ * mcpnet_helper.hpp:156-162, where each rank's InnerProduct fills its rows of the distributed MI. */
+int mcpg_ap_mi_block_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,
+                         int64_t row0, int64_t nrows, int64_t col0, int64_t ncols, float* MI, int64_t ldmi);
 
 /* Debug/parity view of the same kernel: for each requested pair (px[n], py[n]) the leaf windows
  * in the order their terms enter the FP64 sum, as (nxy, nx, ny) int32 triples, at most

@@ -48,12 +48,15 @@ _SIGS = {
     "mcpg_bspline_weights_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
     "mcpg_bspline_mi_weights_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
     "mcpg_bspline_mi_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, c_int, _P, c_int64],
+    "mcpg_bspline_mi_block_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, c_int, c_int64, c_int64, c_int64,
+                                  c_int64, _P, c_int64],
     "mcpg_bspline_mi_backward_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, _P, _P, c_int64,
                                      _P, c_int64, POINTER(c_double)],
     "mcpg_add_noise_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_uint64, _P, c_int64],
     "mcpg_rank_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_cumhist_i32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_ap_mi_i32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_ap_mi_block_i32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_ap_mi_cells_i32": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, _P, c_int64, c_int32, _P, _P, _P],
     "mcpg_maxmin_f32": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, c_int, _P, c_int64],
     "mcpg_maxmin_f64": [_P, c_int, _P, c_int64, _P, c_int64, _P, c_int64, c_int64, c_int64, c_int, _P, c_int64],
@@ -178,6 +181,14 @@ def _out_mat(out, kind, shape, dtype, ref, name):
     return out, m
 
 
+def _check_block(G, row0, nrows, col0, ncols):
+    for name, v in (("row0", row0), ("nrows", nrows), ("col0", col0), ("ncols", ncols)):
+        if not isinstance(v, (int, np.integer)) or v < 0:
+            raise ValueError(f"{name} must be a non-negative integer, got {v!r}")
+    if row0 + nrows > G or col0 + ncols > G:
+        raise ValueError(f"block rows {row0}+{nrows}, columns {col0}+{ncols} exceed the {G} x {G} matrix")
+
+
 def _same_kind(*mats):
     kinds = {m.kind for m in mats if m is not None}
     if len(kinds) > 1:
@@ -290,6 +301,19 @@ class Handle:
                                                  PRECISIONS[precision], m.ptr, m.ld), "mcpg_bspline_mi_f32")
         return out
 
+    def bspline_mi_block(self, X, row0, nrows, col0, ncols, bins=10, order=3, precision="fp32", out=None):
+        """Rows row0 .. row0+nrows-1, columns col0 .. col0+ncols-1 of bspline_mi(X)."""
+        x = _Mat(X, np.float32, "X")
+        G, S = x.shape
+        _check_block(G, row0, nrows, col0, ncols)
+        out, m = _out_mat(out, x.kind, (nrows, ncols), np.float32, X, "out")
+        _same_kind(x, m)
+        self._pre(x.kind)
+        self._check(self.lib.mcpg_bspline_mi_block_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order,
+                                                       PRECISIONS[precision], row0, nrows, col0, ncols, m.ptr,
+                                                       max(m.ld, 1)), "mcpg_bspline_mi_block_f32")
+        return out
+
     def bspline_mi_backward(self, X, pi, pj, m, bins=10, order=3, out=None):
         """dL/dX for L = sum_n m[n] * MI(pi[n], pj[n]); returns (dX, L)."""
         x = _Mat(X, np.float32, "X")
@@ -340,6 +364,17 @@ class Handle:
         out, m = _out_mat(out, r.kind, (G, G), np.float32, R, "out")
         self._pre(r.kind)
         self._check(self.lib.mcpg_ap_mi_i32(self.h, r.kind, r.ptr, G, S, r.ld, m.ptr, m.ld), "mcpg_ap_mi_i32")
+        return out
+
+    def ap_mi_block(self, R, row0, nrows, col0, ncols, out=None):
+        """Rows row0 .. row0+nrows-1, columns col0 .. col0+ncols-1 of ap_mi(R)."""
+        r = _Mat(R, np.int32, "R")
+        G, S = r.shape
+        _check_block(G, row0, nrows, col0, ncols)
+        out, m = _out_mat(out, r.kind, (nrows, ncols), np.float32, R, "out")
+        self._pre(r.kind)
+        self._check(self.lib.mcpg_ap_mi_block_i32(self.h, r.kind, r.ptr, G, S, r.ld, row0, nrows, col0, ncols, m.ptr,
+                                                  max(m.ld, 1)), "mcpg_ap_mi_block_i32")
         return out
 
     def ap_mi_cells(self, R, px, py, max_leaves=4096):

@@ -229,6 +229,10 @@ struct APArgs {
   int64_t G, S;
   float* MI;
   int64_t ldmi;
+  // output block: MI holds rows r0 .. r0 + nr - 1 and columns c0 .. c0 + nc - 1 of the G x G
+  // matrix; block = false: the whole matrix, its pairs enumerated over the upper triangle
+  int64_t r0, nr, c0, nc;
+  bool block;
   // debug / cells mode
   const int32_t* px;
   const int32_t* py;
@@ -692,17 +696,20 @@ void ap_mi_kernel(APArgs a, const int32_t* __restrict__ H) {
     flush_batch(b, lane, S, log2S, store);
     return;
   } else {
+    // pair (i, j), i <= j, goes to (i, j) and (j, i) where those fall inside the output block
     auto store = [&](int i, int j, double v) {
       if (lane == 0) {
         const float f = (float)v;
-        a.MI[(int64_t)i * a.ldmi + j] = f;
-        a.MI[(int64_t)j * a.ldmi + i] = f;
+        const uint64_t ri = (uint64_t)(i - a.r0), cj = (uint64_t)(j - a.c0);
+        if (ri < (uint64_t)a.nr && cj < (uint64_t)a.nc) a.MI[(int64_t)ri * a.ldmi + (int64_t)cj] = f;
+        const uint64_t rj = (uint64_t)(j - a.r0), ci = (uint64_t)(i - a.c0);
+        if (i != j && rj < (uint64_t)a.nr && ci < (uint64_t)a.nc) a.MI[(int64_t)rj * a.ldmi + (int64_t)ci] = f;
       }
     };
-    // All pairs (i <= j, row-major over the upper triangle), claimed kChunk at a time from a
-    // global counter so that waves whose pairs recurse less take more of them; a wave leaves
-    // once the counter has passed the last pair.
-    const int64_t P = a.G * (a.G + 1) / 2;
+    // All pairs (i <= j, row-major over the upper triangle; for a block, its positions row-major),
+    // claimed kChunk at a time from a global counter so that waves whose pairs recurse less take
+    // more of them; a wave leaves once the counter has passed the last pair.
+    const int64_t P = a.block ? a.nr * a.nc : a.G * (a.G + 1) / 2;
     for (;;) {
       unsigned long long claim = 0;
       if (lane == 0) claim = atomicAdd(a.queue, (unsigned long long)kChunk);
@@ -711,11 +718,22 @@ void ap_mi_kernel(APArgs a, const int32_t* __restrict__ H) {
       const int64_t base = (int64_t)(((uint64_t)hi << 32) | lo);
       if (base >= P) break;
       const int64_t end = base + kChunk < P ? base + kChunk : P;
-      int64_t i, j;
-      tri_decode(base, a.G, &i, &j);
-      for (int64_t p = base; p < end; ++p) {
-        ap_pair(a, H, i, j, pyr, xg, lane, log2S, b, (int)i, (int)j, store, nullptr, nullptr);
-        if (++j == a.G) { ++i; j = i; }
+      if (a.block) {
+        // position (i, j) computes pair (min, max), the orientation of the whole-matrix launch;
+        // a position below the diagonal whose mirror is also in the block is left to the mirror
+        for (int64_t p = base; p < end; ++p) {
+          const int64_t i = a.r0 + p / a.nc, j = a.c0 + p % a.nc;
+          if (i > j && (uint64_t)(j - a.r0) < (uint64_t)a.nr && (uint64_t)(i - a.c0) < (uint64_t)a.nc) continue;
+          const int64_t x = i < j ? i : j, y = i < j ? j : i;
+          ap_pair(a, H, x, y, pyr, xg, lane, log2S, b, (int)x, (int)y, store, nullptr, nullptr);
+        }
+      } else {
+        int64_t i, j;
+        tri_decode(base, a.G, &i, &j);
+        for (int64_t p = base; p < end; ++p) {
+          ap_pair(a, H, i, j, pyr, xg, lane, log2S, b, (int)i, (int)j, store, nullptr, nullptr);
+          if (++j == a.G) { ++i; j = i; }
+        }
       }
     }
     flush_batch(b, lane, S, log2S, store);
@@ -849,9 +867,11 @@ int mcpg_cumhist_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64
   return check_flags(h, "mcpg_cumhist_i32");
 }
 
+// The ranks' histograms, buckets and cell codes, then the pair kernel over the whole matrix
+// (blk == nullptr), a block of it (blk = {row0, nrows, col0, ncols}) or the listed pairs (cells).
 static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, float* MI, int64_t ldmi,
                   const int32_t* px, const int32_t* py, int64_t npairs, int32_t max_leaves, int32_t* cells,
-                  int32_t* nleaves, double* miv) {
+                  int32_t* nleaves, double* miv, const int64_t* blk = nullptr) {
   MCPG_HIP(h, hipSetDevice(h->device));
   int st;
   const bool cells_mode = npairs > 0;
@@ -861,7 +881,8 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   sz.add<int32_t>(G * S);
   sz.add<int32_t>(G);
   sz.add<uint16_t>(G * ap_ldz(S));
-  if (!cells_mode && ptrs == MCPG_HOST) sz.add<float>(G * G);
+  const int64_t r0 = blk ? blk[0] : 0, nr = blk ? blk[1] : G, c0 = blk ? blk[2] : 0, nc = blk ? blk[3] : G;
+  if (!cells_mode && ptrs == MCPG_HOST) sz.add<float>(nr * nc);
   if (cells_mode) {
     sz.add<int32_t>(npairs); sz.add<int32_t>(npairs);
     sz.add<int32_t>(npairs * 3 * (int64_t)max_leaves); sz.add<int32_t>(npairs); sz.add<double>(npairs);
@@ -882,7 +903,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   uint16_t* dZ = cv.take<uint16_t>(G * ap_ldz(S));
   float* dMI = MI;
   int64_t lm = ldmi;
-  if (!cells_mode && ptrs == MCPG_HOST) { dMI = cv.take<float>(G * G); lm = G; }
+  if (!cells_mode && ptrs == MCPG_HOST) { dMI = cv.take<float>(nr * nc); lm = nc; }
   if ((st = mcpg::reset_flags(h))) return st;
   {
     mcpg::LaunchScope ls(h, "cumhist_kernel");
@@ -903,6 +924,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   }
   APArgs a{};
   a.R = dR; a.ldr = lr; a.H = dH; a.ldh = S + 1; a.P = dP; a.G = G; a.S = S; a.MI = dMI; a.ldmi = lm;
+  a.r0 = r0; a.nr = nr; a.c0 = c0; a.nc = nc; a.block = blk != nullptr;
   a.flags = h->d_flags;
   a.queue = reinterpret_cast<unsigned long long*>(h->d_flags + 8);   // zeroed by reset_flags
   a.ident = dId;
@@ -919,7 +941,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
     a.px = d_px; a.py = d_py; a.npairs = npairs; a.max_leaves = max_leaves; a.cells = d_cells; a.nleaves = d_nl;
     a.mi_out = d_mi;
   }
-  const int64_t work = cells_mode ? npairs : G * (G + 1) / 2;
+  const int64_t work = cells_mode ? npairs : blk ? nr * nc : G * (G + 1) / 2;
   const int64_t per_wave = ap_wave_bytes(S);
   const int waves = (int)std::max<int64_t>(1, std::min<int64_t>(kMaxWaves, kLdsBudget / per_wave));
   const size_t lds = (size_t)waves * per_wave;
@@ -948,7 +970,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
     MCPG_HIP(h, hipMemcpyAsync(nleaves, d_nl, npairs * 4, hipMemcpyDeviceToHost, h->stream));
     MCPG_HIP(h, hipMemcpyAsync(miv, d_mi, npairs * 8, hipMemcpyDeviceToHost, h->stream));
   } else if (ptrs == MCPG_HOST) {
-    if ((st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, G * 4, G * 4, G, hipMemcpyDeviceToHost))) return st;
+    if ((st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, nc * 4, nc * 4, nr, hipMemcpyDeviceToHost))) return st;
   }
   if ((st = mcpg::finish(h, "mcpg_ap_mi_i32"))) return st;
   return check_flags(h, "mcpg_ap_mi_i32");
@@ -966,6 +988,23 @@ int mcpg_ap_mi_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t
   if (ldr < S || ldmi < G) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   return ap_run(h, ptrs, R, G, S, ldr, MI, ldmi, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr);
+}
+
+int mcpg_ap_mi_block_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, int64_t row0,
+                         int64_t nrows, int64_t col0, int64_t ncols, float* MI, int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (row0 < 0 || nrows < 0 || col0 < 0 || ncols < 0 || row0 > G - nrows || col0 > G - ncols)
+    return mcpg::set_error(h, MCPG_ERR_SHAPE, "block outside the G x G matrix");
+  if (G == 0 || nrows == 0 || ncols == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (S > kMaxS) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "adaptive-partitioning MI supports S <= 8191");
+  if (!R || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldr < S || ldmi < ncols) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  const int64_t blk[4] = {row0, nrows, col0, ncols};
+  return ap_run(h, ptrs, R, G, S, ldr, MI, ldmi, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, blk);
 }
 
 int mcpg_ap_mi_cells_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,

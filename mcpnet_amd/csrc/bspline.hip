@@ -413,7 +413,20 @@ struct MIArgs {
   int64_t ntri;      // number of (ti <= tj) tile pairs
   const uint32_t* order;  // tile coordinates in launch order, (ti << 16) | tj
   float norm;        // (float)1 / (float)S, the reference's FT norm_factor (BSplineMI.hpp:319)
+  // output block: MI holds rows r0 .. r0 + nr - 1 and columns c0 .. c0 + nc - 1 of the G x G
+  // matrix (0, G, 0, G for the whole matrix)
+  int64_t r0, nr, c0, nc;
 };
+
+// Stores the MI of gene pair (gi, gj), gi <= gj, at (gi, gj) and (gj, gi) where those fall
+// inside the output block.
+__device__ __forceinline__ void store_mi(const MIArgs& a, int64_t gi, int64_t gj, float mi) {
+  const uint64_t ri = (uint64_t)(gi - a.r0), cj = (uint64_t)(gj - a.c0);
+  if (ri < (uint64_t)a.nr && cj < (uint64_t)a.nc) a.MI[(int64_t)ri * a.ldmi + (int64_t)cj] = mi;
+  if (gi == gj) return;
+  const uint64_t rj = (uint64_t)(gj - a.r0), ci = (uint64_t)(gi - a.c0);
+  if (rj < (uint64_t)a.nr && ci < (uint64_t)a.nc) a.MI[(int64_t)rj * a.ldmi + (int64_t)ci] = mi;
+}
 
 // LDS layouts.
 // * FP32 panels: rows of 32 floats + 4 pad (144 B), staged through registers.
@@ -766,8 +779,7 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
         const int64_t gi = ti * a.L.gt + wr * 8 + 2 * lq + hh;
         if (gi >= a.L.G || gj >= a.L.G || (diag && gi > gj)) continue;
         const float mi = (float)((a.H64[gi] + a.H64[gj]) + (double)(hh ? s1 : s0));
-        a.MI[gi * a.ldmi + gj] = mi;
-        if (gi != gj) a.MI[gj * a.ldmi + gi] = mi;
+        store_mi(a, gi, gj, mi);
       }
     }
     return;
@@ -804,8 +816,7 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
           for (int c = 0; c < bins; ++c) sum += row[c];
         }
         const float mi = (float)((a.H64[gi] + a.H64[gj]) + (double)sum);
-        a.MI[gi * a.ldmi + gj] = mi;
-        if (gi != gj) a.MI[gj * a.ldmi + gi] = mi;
+        store_mi(a, gi, gj, mi);
       }
     }
     __syncthreads();
@@ -1164,12 +1175,40 @@ int prep_launch(mcpg_ctx* h, const float* dX, int64_t ldx, const Layout& L, cons
 
 // BF16 planes in the two row orders of Layout::row_of_r / row_of_c (whi_c non-null) select the
 // kernels with the register epilogue.
+// A block of the G x G MI matrix (rows r0 .. r0 + nr - 1, columns c0 .. c0 + nc - 1) and the device
+// list of the tiles that cover it; nr < 0 stands for the whole matrix.
+struct MIBlock {
+  int64_t r0 = 0, nr = -1, c0 = 0, nc = -1;
+  const uint32_t* order = nullptr;
+  int64_t ntiles = 0;
+};
+
+// The tiles (ti <= tj, packed (ti << 16) | tj) whose gene pairs land in the block at (i, j) or at
+// (j, i), each once: every pair is computed in the orientation the whole-matrix launch uses, so a
+// block holds exactly the whole matrix's values.  A tile position (ti, tj) with ti > tj is dropped
+// when its mirror (tj, ti) is itself a position of the block.
+void block_tiles(const Layout& L, int64_t r0, int64_t nr, int64_t c0, int64_t nc, std::vector<uint32_t>& out) {
+  out.clear();
+  if (nr <= 0 || nc <= 0) return;
+  const int64_t gt = L.gt;
+  const int64_t ra = r0 / gt, rb = (r0 + nr - 1) / gt, ca = c0 / gt, cb = (c0 + nc - 1) / gt;
+  for (int64_t ti = ra; ti <= rb; ++ti)
+    for (int64_t tj = ca; tj <= cb; ++tj) {
+      if (ti > tj && tj >= ra && tj <= rb && ti >= ca && ti <= cb) continue;
+      const int64_t lo = ti < tj ? ti : tj, hi = ti < tj ? tj : ti;
+      out.push_back((uint32_t)((lo << 16) | hi));
+    }
+}
+
 int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi, const uint16_t* wlo,
               const uint16_t* whi_c, const uint16_t* wlo_c, const double* H64, const Layout& L, float* MI,
-              int64_t ldmi) {
+              int64_t ldmi, const MIBlock& blk = MIBlock()) {
   MIArgs a;
   a.w32 = w32; a.whi = whi; a.wlo = wlo; a.whi_c = whi_c; a.wlo_c = wlo_c;
   a.H64 = H64; a.MI = MI; a.ldmi = ldmi; a.L = L;
+  const bool whole = blk.nr < 0;
+  a.r0 = whole ? 0 : blk.r0; a.nr = whole ? L.G : blk.nr;
+  a.c0 = whole ? 0 : blk.c0; a.nc = whole ? L.G : blk.nc;
   const bool re = whi_c != nullptr;
   if (re && (precision == MCPG_FP32 || L.bins != 10))
     return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: two-order planes need 10 bins and BF16");
@@ -1182,8 +1221,14 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
   int st;
   hipError_t e;
   if (L.tr != kTileRows) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: weight layout does not match the MI kernel");
-  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 64), mcpg::env_group("MCPG_MI_XCD", 0), &st);
-  if (!a.order) return st;
+  if (whole) {
+    a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 64), mcpg::env_group("MCPG_MI_XCD", 0), &st);
+    if (!a.order) return st;
+  } else {
+    a.order = blk.order;
+    a.ntri = blk.ntiles;
+  }
+  if (a.ntri == 0) return MCPG_OK;
   const unsigned grid = (unsigned)a.ntri;
   // BF16 planes: a ring of 2 (BF16X3, 80 KB of LDS) or 4 (BF16, 80 KB) stage buffers with early
   // release, two workgroups per CU; MCPG_MI_ER=0 selects the 2-stage ring without it
@@ -1287,6 +1332,65 @@ bool valid_precision(int p) { return p == MCPG_FP32 || p == MCPG_BF16 || p == MC
 
 // gene groups per workgroup tile side of the MI kernel that runs at this precision
 int mi_groups_per_tile(int) { return kTileRows / kGroupRows; }
+
+// compute_mi for one block of the output (the whole matrix: 0, G, 0, G): scaling, weights and
+// marginals of every gene, then the MI launch over the tiles that cover the block.
+int bspline_mi_run(mcpg_ctx* h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int bins, int order,
+                   int precision, int64_t r0, int64_t nr, int64_t c0, int64_t nc, float* MI, int64_t ldmi,
+                   const char* where) {
+  int st;
+  if ((st = check_spline_args(h, bins, order))) return st;
+  if (!valid_precision(precision)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad precision");
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0 || nr == 0 || nc == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (!X || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldx < S || ldmi < nc) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const Layout L = make_layout(G, S, bins, mi_groups_per_tile(precision));
+  const Spline sp = make_spline(bins, order);
+  const bool whole = r0 == 0 && c0 == 0 && nr == G && nc == G;
+  if (!whole) block_tiles(L, r0, nr, c0, nc, h->block_order);
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) { sz.add<float>(G * S); sz.add<float>(nr * nc); }
+  if (!whole) sz.add<uint32_t>(h->block_order.size());
+  size_weight_bufs(sz, L, precision);
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const float* dX = X;
+  int64_t lx = ldx;
+  float* dMI = MI;
+  int64_t lm = ldmi;
+  if (ptrs == MCPG_HOST) {
+    float* t = cv.take<float>(G * S);
+    if ((st = mcpg::copy_2d(h, t, S * 4, X, ldx * 4, S * 4, G, hipMemcpyHostToDevice))) return st;
+    dX = t; lx = S;
+    dMI = cv.take<float>(nr * nc); lm = nc;
+  }
+  MIBlock blk;
+  if (!whole) {
+    uint32_t* d_order = cv.take<uint32_t>(h->block_order.size());
+    MCPG_HIP(h, hipMemcpyAsync(d_order, h->block_order.data(), h->block_order.size() * 4, hipMemcpyHostToDevice,
+                               h->stream));
+    blk.r0 = r0; blk.nr = nr; blk.c0 = c0; blk.nc = nc;
+    blk.order = d_order;
+    blk.ntiles = (int64_t)h->block_order.size();
+  }
+  WeightBufs wb = take_weight_bufs(cv, L, precision);
+  if ((st = clear_weight_bufs(h, wb, L))) return st;
+  if ((st = mcpg::reset_flags(h))) return st;
+  PrepOut o{};
+  o.w32 = wb.w32; o.whi = wb.whi; o.wlo = wb.wlo; o.whi_c = wb.whi_c; o.wlo_c = wb.wlo_c;
+  o.H64 = wb.H64; o.flags = h->d_flags;
+  if ((st = prep_launch(h, dX, lx, L, sp, 1, o))) return st;
+  if ((st = mi_launch(h, precision, wb.w32, wb.whi, wb.wlo, wb.whi_c, wb.wlo_c, wb.H64, L, dMI, lm, blk))) return st;
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, nc * 4, nc * 4, nr, hipMemcpyDeviceToHost)))
+    return st;
+  if ((st = mcpg::finish(h, where))) return st;
+  return check_flags_nonfinite(h, where);
+}
 
 }  // namespace
 
@@ -1426,45 +1530,18 @@ int mcpg_bspline_mi_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int6
                         int precision, float* MI, int64_t ldmi) {
   MCPG_CHECK_HANDLE(h);
   mcpg::begin_call(h);
-  int st;
-  if ((st = check_spline_args(h, bins, order))) return st;
-  if (!valid_precision(precision)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad precision");
-  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
-  if (G == 0) return MCPG_OK;
-  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
-  if (!X || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
-  if (ldx < S || ldmi < G) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
-  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
-  MCPG_HIP(h, hipSetDevice(h->device));
-  const Layout L = make_layout(G, S, bins, mi_groups_per_tile(precision));
-  const Spline sp = make_spline(bins, order);
-  mcpg::Sizer sz;
-  if (ptrs == MCPG_HOST) { sz.add<float>(G * S); sz.add<float>(G * G); }
-  size_weight_bufs(sz, L, precision);
-  void* base = mcpg::workspace(h, sz.off, &st);
-  if (!base) return st;
-  mcpg::Carver cv(base);
-  const float* dX = X;
-  int64_t lx = ldx;
-  float* dMI = MI;
-  int64_t lm = ldmi;
-  if (ptrs == MCPG_HOST) {
-    float* t = cv.take<float>(G * S);
-    if ((st = mcpg::copy_2d(h, t, S * 4, X, ldx * 4, S * 4, G, hipMemcpyHostToDevice))) return st;
-    dX = t; lx = S;
-    dMI = cv.take<float>(G * G); lm = G;
-  }
-  WeightBufs wb = take_weight_bufs(cv, L, precision);
-  if ((st = clear_weight_bufs(h, wb, L))) return st;
-  if ((st = mcpg::reset_flags(h))) return st;
-  PrepOut o{};
-  o.w32 = wb.w32; o.whi = wb.whi; o.wlo = wb.wlo; o.whi_c = wb.whi_c; o.wlo_c = wb.wlo_c;
-  o.H64 = wb.H64; o.flags = h->d_flags;
-  if ((st = prep_launch(h, dX, lx, L, sp, 1, o))) return st;
-  if ((st = mi_launch(h, precision, wb.w32, wb.whi, wb.wlo, wb.whi_c, wb.wlo_c, wb.H64, L, dMI, lm))) return st;
-  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, G * 4, G * 4, G, hipMemcpyDeviceToHost))) return st;
-  if ((st = mcpg::finish(h, "mcpg_bspline_mi_f32"))) return st;
-  return check_flags_nonfinite(h, "mcpg_bspline_mi_f32");
+  return bspline_mi_run(h, ptrs, X, G, S, ldx, bins, order, precision, 0, G, 0, G, MI, ldmi, "mcpg_bspline_mi_f32");
+}
+
+int mcpg_bspline_mi_block_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int bins,
+                              int order, int precision, int64_t row0, int64_t nrows, int64_t col0, int64_t ncols,
+                              float* MI, int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || row0 < 0 || nrows < 0 || col0 < 0 || ncols < 0 || row0 > G - nrows || col0 > G - ncols)
+    return mcpg::set_error(h, MCPG_ERR_SHAPE, "block outside the G x G matrix");
+  return bspline_mi_run(h, ptrs, X, G, S, ldx, bins, order, precision, row0, nrows, col0, ncols, MI, ldmi,
+                        "mcpg_bspline_mi_block_f32");
 }
 
 int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int bins,

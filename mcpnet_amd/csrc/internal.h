@@ -35,6 +35,8 @@ struct mcpg_ctx {
     int ratio = 0, gi = 0, gj = 0, xcd = 0;
   };
   std::vector<TileOrder> orders;
+  // host copy of the tile list of the last block MI launch (kept alive for its upload)
+  std::vector<uint32_t> block_order;
   // > 0 while a composite entry point (mcpg_mcpnet_*) runs its stages through other entry
   // points: the stages then keep the composite call's launch log and timings
   int nest = 0;

@@ -101,3 +101,19 @@ def test_matching_out_reaches_the_library(h):
     out = np.empty((4, 5), np.float32)
     assert h.ratio(A, A, out=out) is out
     assert h.lib.calls == ["mcpg_ratio_f32"]
+
+
+def test_block_outside_matrix_is_rejected(h):
+    X = np.ones((10, 20), np.float32)
+    R = np.zeros((10, 20), np.int32)
+    for args in ((0, 11, 0, 10), (5, 6, 0, 1), (0, 1, 9, 2), (-1, 2, 0, 2), (0, 2, 0, -1)):
+        with pytest.raises(ValueError):
+            h.bspline_mi_block(X, *args)
+        with pytest.raises(ValueError):
+            h.ap_mi_block(R, *args)
+    with pytest.raises(ValueError, match="shape"):
+        h.bspline_mi_block(X, 2, 3, 4, 5, out=np.empty((3, 4), np.float32))
+    assert h.lib.calls == []
+    h.bspline_mi_block(X, 2, 3, 4, 5)
+    h.ap_mi_block(R, 0, 10, 0, 10)
+    assert h.lib.calls == ["mcpg_bspline_mi_block_f32", "mcpg_ap_mi_block_i32"]
