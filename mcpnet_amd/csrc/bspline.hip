@@ -1222,7 +1222,7 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
   hipError_t e;
   if (L.tr != kTileRows) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: weight layout does not match the MI kernel");
   if (whole) {
-    a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 64), mcpg::env_group("MCPG_MI_XCD", 0), &st);
+    a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 32), mcpg::env_group("MCPG_MI_XCD", 0), &st);
     if (!a.order) return st;
   } else {
     a.order = blk.order;
