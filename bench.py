@@ -11,7 +11,9 @@ durations from the library's event bracketing of each launch.
 Prints ONE JSON line.  value = MI gene-pairs/s of the MI stage; mcp3_maxmin_gops = 1.6e13
 algorithmic max-min ops / MCP3 stage time (BASELINE's unit: one (min, max) per (i, k, j) triple,
 M*N*K per product); mcp3_maxmin_gops_executed counts only the triples the triangle launches
-compute (about half).
+compute (about half).  The default run then times the other workloads briefly (1000 x 1000 FP32
+MI + MCP3, 2000 x 1024 BF16X3 MI + MCP2/3/4 + ensemble + DPI, 2000 x 1000 AP-MI, 5000 x 500 MI
+adjoint; three steps each, no CPU legs) and reports them under "secondary_configs".
 """
 from __future__ import annotations
 
@@ -211,8 +213,8 @@ def bench_ap(args, G, S, seed):
                                   "work_per_launch": work, "kernel": "ap_mi_kernel",
                                   "work_unit": f"point comparisons of the reference's quadrant scans "
                                                f"({tests / (rows * G):.0f} per pair on rows 0..{rows - 1})"}
-    print(json.dumps(result))
     h.close()
+    return result
 
 
 def bench_adjoint(args, G, S, seed):
@@ -272,8 +274,8 @@ def bench_adjoint(args, G, S, seed):
         errs = np.array(errs)
         result["fd_check"] = {"entries": int(len(errs)), "median_rel_err": float(np.median(errs)),
                               "max_rel_err": float(errs.max()), "h": 1e-3}
-    print(json.dumps(result))
     h.close()
+    return result
 
 
 def bench_ensemble_dpi(args, h, MI, stream, torch):
@@ -346,6 +348,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 MI comparison pass")
     ap.add_argument("--extras", action="store_true", help="also time the ensemble MCP and the DPI filter "
                                                           "(always on for config 2)")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="config 3 only: skip the short timed runs of configs 1, 2, 4 and 5 that are reported under "
+                         "'secondary_configs'")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -353,12 +358,31 @@ def main():
               "shard by row panels with an all-gather of M1's keys); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    result = run_config(args, args.config)
+    if args.config == 3 and not args.no_secondary and args.genes is None and args.samples is None:
+        # the other workloads, timed briefly (no CPU legs or oracle checks: their parity is in the
+        # GPU tests), so that one default run records every config's throughput
+        sec = {}
+        for cid in (1, 2, 4, 5):
+            sub = argparse.Namespace(**vars(args))
+            sub.steps, sub.warmup, sub.precision = 3, 1, None
+            sub.no_cpu, sub.no_fp32, sub.extras = True, True, False
+            log(f"bench: secondary config {cid}")
+            r = run_config(sub, cid)
+            sec[f"config{cid}"] = {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "dtype", "config",
+                                                     "stage_ms", "kernel_mean_ms", "roofline_kernels",
+                                                     "mcp3_maxmin_gops", "ensemble", "dpi") if k in r}
+        result["secondary_configs"] = sec
+    print(json.dumps(result))
+
+
+def run_config(args, cid):
+    """Runs one workload and returns its JSON record."""
     import torch
 
     import mcpnet_amd
     from mcpnet_amd import grn
 
-    cid = args.config
     G = args.genes or grn.WORKLOADS[cid]["G"]
     S = args.samples or grn.WORKLOADS[cid]["S"]
     bins = 10
@@ -570,8 +594,8 @@ def main():
         result["speedup_vs_cpu_mi"] = mi_rate / cpu["mi"]["value"]
     if cpu.get("maxmin"):
         result["speedup_vs_cpu_maxmin"] = mcp3_gops / cpu["maxmin"]["value"]
-    print(json.dumps(result))
     h.close()
+    return result
 
 
 if __name__ == "__main__":

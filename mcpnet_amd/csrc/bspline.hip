@@ -1641,7 +1641,7 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
     PairArgs pa;
     pa.w32 = wb.w32; pa.L = L; pa.pi = d_pi; pa.pj = d_pj; pa.m = d_m; pa.npairs = npairs; pa.H64 = wb.H64;
     pa.norm = 1.0f / (float)S; pa.Gm = d_Gm; pa.Lpart = d_Lp;
-    mcpg::LaunchScope ls(h, "bspline_pair_adjoint_kernel");
+    mcpg::LaunchScope ls(h, mfma ? "bspline_pair_adjoint_mfma_kernel" : "bspline_pair_adjoint_kernel");
     if (mfma)
       hipLaunchKernelGGL(bspline_pair_adjoint_mfma_kernel, dim3((unsigned)ceil_div(npairs, 4)), dim3(256), 0, h->stream,
                          pa);
@@ -1652,14 +1652,17 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
   GeneArgs ga;
   ga.w32 = wb.w32; ga.L = L; ga.sp = sp; ga.off = d_off; ga.nbr = d_nbr; ga.cw = d_cw; ga.accg = d_accg; ga.Gm = d_Gm; ga.m = d_m; ga.pmarg = d_pm;
   ga.xs = d_xs; ga.scl = d_scl; ga.amin = d_amin; ga.amax = d_amax; ga.dX = ddX; ga.lddx = ldd;
-  {
-    mcpg::LaunchScope ls(h, "bspline_gene_adjoint_kernel");
-    if (mfma) {
+  if (mfma) {
+    {
+      mcpg::LaunchScope ls(h, "bspline_gene_accumulate_kernel");
       hipLaunchKernelGGL(bspline_gene_accumulate_kernel, dim3((unsigned)ceil_div(L.Sp, kAdjPass), (unsigned)G),
                          dim3(256), 0, h->stream, ga);
-      hipLaunchKernelGGL(bspline_gene_epilogue_kernel, dim3((unsigned)G), dim3(256), 0, h->stream, ga);
-    } else
-      hipLaunchKernelGGL((bspline_gene_adjoint_kernel<16, 4>), dim3((unsigned)G), dim3(256), 0, h->stream, ga);
+    }
+    mcpg::LaunchScope ls(h, "bspline_gene_epilogue_kernel");
+    hipLaunchKernelGGL(bspline_gene_epilogue_kernel, dim3((unsigned)G), dim3(256), 0, h->stream, ga);
+  } else {
+    mcpg::LaunchScope ls(h, "bspline_gene_adjoint_kernel");
+    hipLaunchKernelGGL((bspline_gene_adjoint_kernel<16, 4>), dim3((unsigned)G), dim3(256), 0, h->stream, ga);
   }
   if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, dX, lddx * 4, ddX, S * 4, S * 4, G, hipMemcpyDeviceToHost))) return st;
   if (L_out) {
