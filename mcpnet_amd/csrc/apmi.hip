@@ -2,13 +2,16 @@
 // cumulative-histogram transforms that feed it.
 //
 // One wavefront evaluates one gene pair at a time, taking pairs in runs of 16 from a global
-// counter.  The pair's points are bucketed by y rank into LDS (xGroupedByY), and the window
-// recursion of the reference runs as a LIFO stack held in the wave's registers (entry k lives
-// in lane k % 64, slot k / 64), so every control decision is wave-uniform.  A popped window is
-// split at its mid ranks and its four quadrant counts come from one pass over the window's y
-// range: each lane tests one point and the counts are ballot + popcount over the wave.  Counts
-// are integers and match the reference exactly; leaf terms are evaluated a lane each and enter
-// the FP64 sum one by one in the reference's LIFO order.
+// counter.  Every window the reference's recursion can pop is a cell of one dyadic grid over the
+// two rank ranges, so the wave first counts the pair's points on that grid to depth 5 and sums
+// them into a quadrant-count pyramid in LDS.  The reference's chi-square split test is decided
+// exactly in integers, and the LIFO walk over the windows above depth 5 is evaluated level by
+// level with one lane per window.  Pairs whose walk goes deeper run it window by window: a LIFO
+// stack in the wave's registers (entry k in lane k), the points bucketed by y rank in LDS
+// (xGroupedByY), and each popped window's four quadrant counts from ballots and popcounts over
+// its y range.  Counts are integers and match the reference exactly; leaf terms are evaluated a
+// lane each, batched across pairs, and enter each pair's FP64 sum one by one in the reference's
+// LIFO order.
 #include <cmath>
 
 #include "internal.h"

@@ -183,7 +183,7 @@ struct Layout {
   }
 };
 
-// groups_per_tile: 80-row gene groups per workgroup tile side (2 for the FP32 kernel, 4 for BF16)
+// groups_per_tile: 80-row gene groups per workgroup tile side (2 for every MI kernel: 160 rows)
 Layout make_layout(int64_t G, int64_t S, int bins, int groups_per_tile = 2) {
   Layout L;
   L.bins = bins;
