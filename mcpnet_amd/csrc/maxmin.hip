@@ -645,6 +645,55 @@ __device__ __forceinline__ void mv_store(const MVArgs& a, int64_t i, int64_t j, 
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
+// acc[m][n] = max(acc[m][n], max(min(a[m].x, b.x), min(a[m].y, b.y))) for m < 8, as one asm
+// block in an order where no instruction reads the result of the one before.  As separate
+// statements the compiler, which cannot see their latency, puts dependent ones next to each
+// other and a hazard wait state between them (a third as many s_nop as min/max instructions).
+template <typename Acc>
+__device__ __forceinline__ void mm_f64_column(Acc& acc, int n, const v2d (&a)[8], const v2d& b) {
+  double t0, t1, t2, t3;
+  asm("v_min_f64 %[t0], %[a0x], %[bx]\n\t"
+      "v_min_f64 %[t1], %[a0y], %[by]\n\t"
+      "v_min_f64 %[t2], %[a1x], %[bx]\n\t"
+      "v_min_f64 %[t3], %[a1y], %[by]\n\t"
+      "v_max_f64 %[t0], %[t0], %[t1]\n\t"
+      "v_max_f64 %[t2], %[t2], %[t3]\n\t"
+      "v_max_f64 %[c0], %[c0], %[t0]\n\t"
+      "v_max_f64 %[c1], %[c1], %[t2]\n\t"
+      "v_min_f64 %[t0], %[a2x], %[bx]\n\t"
+      "v_min_f64 %[t1], %[a2y], %[by]\n\t"
+      "v_min_f64 %[t2], %[a3x], %[bx]\n\t"
+      "v_min_f64 %[t3], %[a3y], %[by]\n\t"
+      "v_max_f64 %[t0], %[t0], %[t1]\n\t"
+      "v_max_f64 %[t2], %[t2], %[t3]\n\t"
+      "v_max_f64 %[c2], %[c2], %[t0]\n\t"
+      "v_max_f64 %[c3], %[c3], %[t2]\n\t"
+      "v_min_f64 %[t0], %[a4x], %[bx]\n\t"
+      "v_min_f64 %[t1], %[a4y], %[by]\n\t"
+      "v_min_f64 %[t2], %[a5x], %[bx]\n\t"
+      "v_min_f64 %[t3], %[a5y], %[by]\n\t"
+      "v_max_f64 %[t0], %[t0], %[t1]\n\t"
+      "v_max_f64 %[t2], %[t2], %[t3]\n\t"
+      "v_max_f64 %[c4], %[c4], %[t0]\n\t"
+      "v_max_f64 %[c5], %[c5], %[t2]\n\t"
+      "v_min_f64 %[t0], %[a6x], %[bx]\n\t"
+      "v_min_f64 %[t1], %[a6y], %[by]\n\t"
+      "v_min_f64 %[t2], %[a7x], %[bx]\n\t"
+      "v_min_f64 %[t3], %[a7y], %[by]\n\t"
+      "v_max_f64 %[t0], %[t0], %[t1]\n\t"
+      "v_max_f64 %[t2], %[t2], %[t3]\n\t"
+      "v_max_f64 %[c6], %[c6], %[t0]\n\t"
+      "v_max_f64 %[c7], %[c7], %[t2]"
+      : [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3),
+        [c0] "+v"(acc[0][n]), [c1] "+v"(acc[1][n]), [c2] "+v"(acc[2][n]), [c3] "+v"(acc[3][n]),
+        [c4] "+v"(acc[4][n]), [c5] "+v"(acc[5][n]), [c6] "+v"(acc[6][n]), [c7] "+v"(acc[7][n])
+      : [a0x] "v"(a[0].x), [a0y] "v"(a[0].y), [a1x] "v"(a[1].x), [a1y] "v"(a[1].y),
+        [a2x] "v"(a[2].x), [a2y] "v"(a[2].y), [a3x] "v"(a[3].x), [a3y] "v"(a[3].y),
+        [a4x] "v"(a[4].x), [a4y] "v"(a[4].y), [a5x] "v"(a[5].x), [a5y] "v"(a[5].y),
+        [a6x] "v"(a[6].x), [a6y] "v"(a[6].y), [a7x] "v"(a[7].x), [a7y] "v"(a[7].y),
+        [bx] "v"(b.x), [by] "v"(b.y));
+}
+
 // KD doubles of K per stage: LDS panel rows of KD * 8 bytes, with the swizzle of the FP32 key
 // rows of the same length (KeyChunk<2 KD>).
 template <int KD>
@@ -707,18 +756,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     const char* Bs = same ? As : As + KCh::kPanelBytes;
 #pragma unroll
     for (int c = 0; c < KCh::kChunks; ++c) {
-      // the A fragments of the chunk stay in registers; each B fragment is read as it is used
+      // the A fragments of the chunk stay in registers; each B fragment is read one use ahead
       v2d av[8];
       const int oa = ty * KCh::kRowBytes + ((c ^ sa) << 4);
       const int ob = tx * KCh::kRowBytes + ((c ^ sb) << 4);
 #pragma unroll
       for (int m = 0; m < 8; ++m) av[m] = *reinterpret_cast<const v2d*>(As + oa + 16 * m * KCh::kRowBytes);
+      v2d bnext = *reinterpret_cast<const v2d*>(Bs + ob);
 #pragma unroll
       for (int n = 0; n < 8; ++n) {
-        const v2d bv = *reinterpret_cast<const v2d*>(Bs + ob + 16 * n * KCh::kRowBytes);
-#pragma unroll
-        for (int m = 0; m < 8; ++m)
-          acc[m][n] = vmax_f64(acc[m][n], vmax_f64(vmin_f64(av[m].x, bv.x), vmin_f64(av[m].y, bv.y)));
+        const v2d bv = bnext;
+        if (n + 1 < 8) bnext = *reinterpret_cast<const v2d*>(Bs + ob + 16 * (n + 1) * KCh::kRowBytes);
+        mm_f64_column(acc, n, av, bv);
       }
     }
   }
