@@ -64,3 +64,23 @@ def test_ap_mi_blocks_equal_whole_matrix(handle):
         gd = handle.ap_mi_block(Rd, r0, nr, c0, nc)
         torch.cuda.synchronize()
         assert same_bits(gd.cpu().numpy(), whole[r0:r0 + nr, c0:c0 + nc]), ("device", r0, nr, c0, nc)
+
+
+@pytest.mark.parametrize("bins,precision", [(7, "fp32"), (7, "bf16x3"), (16, "bf16")])
+def test_bspline_mi_blocks_one_row_order(handle, bins, precision):
+    # bins other than 10 take the one-row-order kernels and their LDS epilogue
+    X = grn.grn_expression(70, 150, 31)[0]
+    whole = handle.bspline_mi(X, bins=bins, precision=precision)
+    for r0, nr, c0, nc in [(0, 70, 0, 70), (3, 20, 40, 30), (50, 20, 0, 25), (10, 40, 10, 40), (69, 1, 69, 1)]:
+        got = handle.bspline_mi_block(X, r0, nr, c0, nc, bins=bins, precision=precision)
+        assert same_bits(got, whole[r0:r0 + nr, c0:c0 + nc]), (bins, precision, r0, nr, c0, nc)
+
+
+def test_ap_mi_blocks_with_tied_ranks(handle):
+    # ranks with ties (not a permutation): the histograms are not the identity
+    rng = np.random.default_rng(12)
+    R = rng.integers(0, 90, size=(40, 90)).astype(np.int32)
+    whole = handle.ap_mi(R)
+    for r0, nr, c0, nc in [(0, 40, 0, 40), (5, 10, 20, 20), (30, 10, 0, 35)]:
+        got = handle.ap_mi_block(R, r0, nr, c0, nc)
+        assert same_bits(got, whole[r0:r0 + nr, c0:c0 + nc]), (r0, nr, c0, nc)
