@@ -16,7 +16,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libmcpgpu.so"
 
 MCPG_HOST, MCPG_DEVICE = 0, 1
-PRECISIONS = {"fp32": 0, "bf16": 1, "bf16x3": 2}
+PRECISIONS = {"fp32": 0, "bf16": 1, "bf16x3": 2, "i8x3": 3}
 STATUS = {
     0: "ok", -1: "invalid argument", -2: "shape mismatch", -3: "non-finite input", -4: "out of device memory",
     -5: "HIP runtime error", -6: "no usable HIP device", -7: "unsupported size",

