@@ -33,6 +33,7 @@ sys.path.insert(0, str(ROOT))
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
+PEAK_INT8_TOPS = 5000.0
 # VALU issue: 256 CUs x 64 lanes x 2.4 GHz; the max-min inner loop spends 1.5 instructions per
 # (i, k, j) triple (v_min_i32 per triple + v_max3_i32 per two triples).
 CUS, LANES, CLOCK_GHZ = 256, 64, 2.4
@@ -342,7 +343,7 @@ def main():
                          "5 = 5000 x 500 B-spline MI adjoint")
     ap.add_argument("--genes", type=int, default=None)
     ap.add_argument("--samples", type=int, default=None)
-    ap.add_argument("--precision", default=None, choices=["fp32", "bf16", "bf16x3"])
+    ap.add_argument("--precision", default=None, choices=["fp32", "bf16", "bf16x3", "i8x3"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of each bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 MI comparison pass")
@@ -374,6 +375,21 @@ def main():
                                                      "mcp3_maxmin_gops", "ensemble", "dpi") if k in r}
         result["secondary_configs"] = sec
     print(json.dumps(result))
+
+
+def mi_kernel_name(precision):
+    return "bspline_mi_i8_kernel" if precision == "i8x3" else f"bspline_mi_kernel<{precision}>"
+
+
+def mi_roofline(precision, flops, ms):
+    """Algorithmic work (2*B^2*S per unordered pair; the split products are not credited) per
+    launch over the launch's mean duration, against the dense MFMA peak of the arithmetic used."""
+    peak, unit = {"fp32": (PEAK_FP32_TFLOPS, "TFLOP/s"), "i8x3": (PEAK_INT8_TOPS, "TOP/s")}.get(
+        precision, (PEAK_BF16_TFLOPS, "TFLOP/s"))
+    ach = flops / (ms / 1e3) / 1e12
+    return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak, "traffic": None,
+            "mean_ms": ms, "work_per_launch": flops,
+            "work_unit": "algorithmic multiply-adds x 2 (2*B^2*S per unordered gene pair; split products not credited)"}
 
 
 def run_config(args, cid):
@@ -456,16 +472,12 @@ def run_config(args, cid):
         v = kern.get(name)
         return float(np.mean(v)) if v else None
 
-    mi_name = f"bspline_mi_kernel<{args.precision}>"
+    mi_name = mi_kernel_name(args.precision)
     mi_ms = mean_ms(mi_name)
     mi_flops = 2.0 * bins * bins * S * P          # 2*B^2*S per unordered pair
-    peak_mi = PEAK_FP32_TFLOPS if args.precision == "fp32" else PEAK_BF16_TFLOPS
     roof = {}
     if mi_ms:
-        ach = mi_flops / (mi_ms / 1e3) / 1e12
-        roof[mi_name] = {"bound": "mfma", "achieved": ach, "peak": peak_mi, "unit": "TFLOP/s", "frac": ach / peak_mi,
-                         "traffic": None, "mean_ms": mi_ms, "work_per_launch": mi_flops,
-                         "work_unit": "algorithmic flops 2*B^2*S per unordered gene pair (split products not credited)"}
+        roof[mi_name] = mi_roofline(args.precision, mi_flops, mi_ms)
     for name, tri in (("maxmin_key_kernel<f32,tri>", True), ("maxmin_key_kernel<f32>", False)):
         ms = mean_ms(name)
         if not ms:
@@ -515,6 +527,27 @@ def run_config(args, cid):
         extra["mi_fp32_pairs_per_s"] = P / t32
         extra["mi_speedup_vs_fp32"] = t32 / t_mi
         del MI32
+    i8 = None
+    if args.precision == "bf16x3" and bins == 10:
+        # the int8 two-digit split on the same matrix: kernel time, pairs/s and (CPU leg) its error
+        MI8 = torch.empty((G, G), dtype=torch.float32, device=dev)
+        h.bspline_mi(Xd, bins=bins, precision="i8x3", out=MI8)
+        ms8, t8 = [], []
+        for _ in range(max(2, min(args.steps, 5))):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            h.bspline_mi(Xd, bins=bins, precision="i8x3", out=MI8)
+            e1.record(stream)
+            ms8 += [ms for n, ms in h.last_timings() if n == mi_kernel_name("i8x3")]
+            torch.cuda.synchronize()
+            t8.append(e0.elapsed_time(e1) / 1e3)
+        t8 = float(np.mean(t8))
+        i8 = {"pairs_per_s": P / t8, "stage_ms": t8 * 1e3, "speedup_vs_bf16x3": t_mi / t8,
+              "dtype": "two int8 digits per weight, w ~= (h + l/254)/127; h*h and h*l + l*h on the int8 MFMA, "
+                       "exact int32 sums",
+              "roofline": mi_roofline("i8x3", mi_flops, float(np.mean(ms8))) if ms8 else None}
+        extra["mi_i8x3"] = i8
+        del MI8
 
     cpu = {}
     if not args.no_cpu:
@@ -532,10 +565,17 @@ def run_config(args, cid):
         err = np.abs(got.astype(np.float64) - ref)
         extra["mi_err_vs_fp64"] = {"max": float(err.max()), "mean": float(err.mean()),
                                    "pairs": int(len(li)), "sample": "all pairs of 200 seeded random genes"}
+        if i8 is not None:
+            h.bspline_mi(Xd, bins=bins, precision="i8x3", out=MI)
+            got8 = MI[torch.from_numpy(genes[li]).long().to(dev),
+                      torch.from_numpy(genes[lj]).long().to(dev)].cpu().numpy()
+            e8 = np.abs(got8.astype(np.float64) - ref)
+            i8["err_vs_fp64"] = {"max": float(e8.max()), "mean": float(e8.mean()), "pairs": int(len(li)),
+                                 "sample": "the same pairs"}
         if G <= 2000:   # small configs: every pair, every precision
             full = oracle.bspline_mi_f64(X)
             errs = {}
-            for prec in ("fp32", "bf16", "bf16x3"):
+            for prec in ("fp32", "bf16", "bf16x3") + (("i8x3",) if bins == 10 else ()):
                 h.bspline_mi(Xd, bins=bins, precision=prec, out=MI)
                 e = np.abs(MI.cpu().numpy().astype(np.float64) - full)
                 errs[prec] = {"max": float(e.max()), "mean": float(e.mean())}
@@ -568,7 +608,8 @@ def run_config(args, cid):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.precision} MFMA (fp32 accumulate) MI; f32 max-min",
+        "dtype": (f"{args.precision} MFMA (fp32 accumulate) MI; f32 max-min" if args.precision != "i8x3" else
+                  "i8x3 int8 MFMA (exact int32 sums) MI; f32 max-min"),
         "data": "synthetic: seeded linear-Gaussian GRN (3 parents/gene), exp, per-gene min-max",
         "config": {"workload": f"config {cid}: {G} genes x {S} samples: B-spline MI (10 bins, order 3, "
                                f"{args.precision}) -> diagonal 0 -> "

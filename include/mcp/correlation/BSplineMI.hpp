@@ -73,7 +73,7 @@ class BSplineEntropyKernel {
 template <typename IT, typename OT = IT>
 class BSplineMIKernel : public BSplineEntropyKernel<IT, OT> {
  public:
-  // contraction arithmetic of the GPU path (MCPG_FP32, MCPG_BF16 or MCPG_BF16X3)
+  // contraction arithmetic of the GPU path (MCPG_FP32, MCPG_BF16, MCPG_BF16X3 or MCPG_I8X3)
   int precision = MCPG_FP32;
   BSplineMIKernel(int const& bins, int const& samples, int prec = MCPG_FP32)
       : BSplineEntropyKernel<IT, OT>(bins, samples), precision(prec) {}

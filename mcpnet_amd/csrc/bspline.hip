@@ -883,8 +883,9 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
 // register epilogue as the BF16 kernels with two row orders; the two accumulator sets (200 of
 // the 256 registers a wave has at two waves per SIMD) leave no room for a step's A fragments, so
 // a K step's buffer stays in LDS until its last block row and NST - 1 steps are in flight.
-template <int NST>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))) void bspline_mi_i8_kernel(MIArgs a) {
+// EARLY: the next step's ten loads are issued together after the barrier (else two per block row).
+template <int NST, bool EARLY>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NST == 2 ? 2 : 1, 2))) void bspline_mi_i8_kernel(MIArgs a) {
   extern __shared__ __align__(16) char smem[];
 
   int64_t ti, tj;
@@ -939,6 +940,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
     const uint32_t next_src = sbase0 + (uint32_t)next * kstep_bytes;
     char* const next_dst = smem + (next % NST) * kStage + dst_off;
     const int next_count = next < nk ? count : 0;
+    if (EARLY) dma_issue(rsrc, lbyte, next_src, grp_stride, next_count, 0, next_dst);
     v4i bh[5], bl[5];
 #pragma unroll
     for (int j = 0; j < 5; ++j) {
@@ -947,7 +949,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-      dma_issue_part(rsrc, lbyte, next_src, grp_stride, next_count, 0, 2 * i, next_dst);
+      if (!EARLY) dma_issue_part(rsrc, lbyte, next_src, grp_stride, next_count, 0, 2 * i, next_dst);
       const v4i ah = *reinterpret_cast<const v4i*>(sA + oa + i * 16 * kRowB16);
       const v4i al = *reinterpret_cast<const v4i*>(sA + kPlaneB16 + oa + i * 16 * kRowB16);
 #pragma unroll
@@ -1410,13 +1412,28 @@ int mi_launch(mcpg_ctx* h, int precision, const WeightBufs& wb, const Layout& L,
     MI_ATTR(kBF16X3, 2, true, false);
     MI_ATTR(kBF16X3, 2, true, true);
 #undef MI_ATTR
-    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_i8_kernel<2>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4 * kPlaneB16));
+#define MI8_ATTR(N, E)                                                                                        \
+  MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_i8_kernel<N, E>),                \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, N * 4 * kPlaneB16))
+    MI8_ATTR(2, false);
+    MI8_ATTR(2, true);
+    MI8_ATTR(3, false);
+    MI8_ATTR(3, true);
+#undef MI8_ATTR
     lds_attr_set = true;
   }
   if (i8) {
+    // MCPG_MI8: 0 = two stages, loads spread over the block rows (default); 1 = two stages, loads
+    // issued together; 2, 3 = the same with three stages (120 KB, one workgroup per CU)
+    const int v = mcpg::env_group("MCPG_MI8", 0);
     mcpg::LaunchScope ls(h, "bspline_mi_i8_kernel");
-    hipLaunchKernelGGL((bspline_mi_i8_kernel<2>), dim3(grid), dim3(kThreads), 2 * 4 * kPlaneB16, h->stream, a);
+#define MI8_GO(N, E)                                                                                           \
+  hipLaunchKernelGGL((bspline_mi_i8_kernel<N, E>), dim3(grid), dim3(kThreads), N * 4 * kPlaneB16, h->stream, a)
+    if (v == 1) MI8_GO(2, true);
+    else if (v == 2) MI8_GO(3, false);
+    else if (v == 3) MI8_GO(3, true);
+    else MI8_GO(2, false);
+#undef MI8_GO
   } else {
     const char* name = precision == MCPG_FP32 ? "bspline_mi_kernel<fp32>"
                        : precision == MCPG_BF16 ? "bspline_mi_kernel<bf16>" : "bspline_mi_kernel<bf16x3>";

@@ -16,7 +16,8 @@ int precision_of(const std::string& s) {
   if (s == "fp32") return MCPG_FP32;
   if (s == "bf16") return MCPG_BF16;
   if (s == "bf16x3") return MCPG_BF16X3;
-  throw std::runtime_error("--precision must be fp32, bf16 or bf16x3");
+  if (s == "i8x3") return MCPG_I8X3;
+  throw std::runtime_error("--precision must be fp32, bf16, bf16x3 or i8x3");
 }
 
 template <typename DataType>
@@ -89,7 +90,7 @@ int main(int argc, char** argv) {
   p.option("-b,--bins", "No. of bins", true, "10");
   p.option("-l,--spline_order", "Spline Order", true, "3");
   p.option("-m,--method", "Algorithm: B-spline MI=0, Adaptive partitioning=1", true, "1");
-  p.option("--precision", "B-spline contraction arithmetic: fp32, bf16 or bf16x3", true, "fp32");
+  p.option("--precision", "B-spline contraction arithmetic: fp32, bf16, bf16x3 or i8x3 (10 bins)", true, "fp32");
   p.option("--noise-seed", "seed of the U[0,1e-5) rank jitter", true, "1296388676612");
   try {
     p.parse(argc, argv);

@@ -59,6 +59,17 @@ def test_mi_bspline_on_reference_data(tmp_path, src):
     assert np.abs(mi - ref).max() <= 1e-5
 
 
+@pytest.mark.parametrize("precision", ["bf16x3", "i8x3"])
+def test_mi_tool_split_precisions(tmp_path, precision):
+    """mi --precision bf16x3 / i8x3 on 60 genes x 500 samples of GRN data, against the FP64 oracle."""
+    X = grn.grn_expression(60, 500, seed=99)[0]
+    src, out = tmp_path / "x.exp", tmp_path / "x.mi.exp"
+    write_exp(src, X)
+    run("mi", "-i", src, "-o", out, "-m", "0", "--single", "--precision", precision)
+    _, mi = read_exp(out)
+    assert np.abs(mi - oracle.bspline_mi_f64(X)).max() <= 1e-4
+
+
 def test_mcpnet_on_reference_h5(tmp_path):
     """mcpnet -i data/r128c1000.h5 --mi-method 0 -m 2: the .h5 MI and MCP3 outputs against the oracle."""
     prefix = tmp_path / "r128"
