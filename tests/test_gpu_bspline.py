@@ -216,3 +216,20 @@ def test_i8x3_limits(handle):
     with pytest.raises(mcpnet_amd.MCPGError) as e:
         handle.bspline_mi_weights(W, S=100, precision="i8x3")
     assert e.value.status == -1
+
+
+@pytest.mark.parametrize("env", [{"MCPG_MI8": "1"}, {"MCPG_MI8": "2"}, {"MCPG_MI8": "3"}])
+@pytest.mark.parametrize("G,S", [(1, 10), (17, 100), (70, 300), (400, 130), (600, 64)])
+def test_i8x3_kernel_schedules_bit_identical(handle, monkeypatch, env, G, S):
+    """The int8 kernel's schedule switches (loads issued together; three stages at one
+    workgroup per CU) give the default kernel's bits, with one (600 x 64) and three (400 x 130)
+    K steps per tile."""
+    X = gen(G, S, seed=G + 7 * S)
+    ref = handle.bspline_mi(X, precision="i8x3")
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    got = handle.bspline_mi(X, precision="i8x3")
+    assert np.array_equal(got, ref)
+    if G >= 70:
+        blk = handle.bspline_mi_block(X, 5, G // 2, G // 3, G - G // 3, precision="i8x3")
+        assert np.array_equal(blk, ref[5:5 + G // 2, G // 3:])
