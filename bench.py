@@ -494,19 +494,23 @@ def run_config(args, cid):
     # HBM bytes per launch: FETCH_SIZE and WRITE_SIZE of the same kernel on the same workload
     # (rocprofv3 --pmc runs, copy-calibrated), kept in profiles/traffic.json
     traffic_file = ROOT / "profiles" / "traffic.json"
+    tr = {}
     if traffic_file.exists():
         try:
             tr = json.loads(traffic_file.read_text()).get(f"config{cid}", {})
-            for n in roof:   # the rocprof names carry no ",tri" launch mode
-                t = tr.get(n, tr.get(n.replace(",tri", "")))
-                if t is not None:
-                    roof[n]["traffic"] = t
-                    roof[n]["traffic_source"] = "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
         except (OSError, ValueError):
-            pass
-    for n, r in roof.items():
+            tr = {}
+
+    def attach_traffic(n, r):
+        t = tr.get(n, tr.get(n.replace(",tri", "")))   # the rocprof names carry no ",tri" launch mode
+        if t is not None:
+            r["traffic"] = t
+            r["traffic_source"] = "profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
         r["hbm_gbs"] = r["traffic"] / (r["mean_ms"] / 1e3) / 1e9 if r.get("traffic") else None
         r["hbm_frac"] = r["hbm_gbs"] / PEAK_HBM_GBS if r["hbm_gbs"] else None
+
+    for n, r in roof.items():
+        attach_traffic(n, r)
     # max-min work the launches executed: a triangle launch produces G (G + 1) / 2 outputs and
     # mirrors them, a full launch G^2, each over K = G
     mm_launches = {n: len(v) / args.steps for n, v in kern.items() if n.startswith("maxmin_key_kernel")}
@@ -546,6 +550,8 @@ def run_config(args, cid):
               "dtype": "two int8 digits per weight, w ~= (h + l/254)/127; h*h and h*l + l*h on the int8 MFMA, "
                        "exact int32 sums",
               "roofline": mi_roofline("i8x3", mi_flops, float(np.mean(ms8))) if ms8 else None}
+        if i8["roofline"]:
+            attach_traffic(mi_kernel_name("i8x3"), i8["roofline"])
         extra["mi_i8x3"] = i8
         del MI8
 

@@ -66,21 +66,22 @@ def maxmin_main(genes: int, dtype: str, reps: int, mcp: int) -> None:
     h.close()
 
 
-def mi_main(genes: int, samples: int, precision: str, reps: int) -> None:
+def mi_main(genes: int, samples: int, precision: str, reps: int, seed: int = 0x4D49) -> None:
     import torch
 
     import mcpnet_amd
     from mcpnet_amd import grn
 
-    X = grn.grn_expression(genes, samples, 0x4D49)[0]
+    X = grn.grn_expression(genes, samples, seed)[0]
     h = mcpnet_amd.Handle(0)
     h.set_profiling(True)
     Xd = torch.from_numpy(X).cuda()
     MI = torch.empty((genes, genes), dtype=torch.float32, device="cuda")
-    for r in range(reps):
-        h.bspline_mi(Xd, precision=precision, out=MI)
-        for name, ms in h.last_timings():
-            print(f"{genes} x {samples} {precision} rep {r}: {name} {ms:.3f} ms", flush=True)
+    for prec in precision.split(","):
+        for r in range(reps):
+            h.bspline_mi(Xd, precision=prec, out=MI)
+            for name, ms in h.last_timings():
+                print(f"{genes} x {samples} {prec} rep {r}: {name} {ms:.3f} ms", flush=True)
     h.close()
 
 
@@ -126,7 +127,31 @@ def _counter(d: str, counter: str):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
-def summarize(trace, fetch, write, fetch_copy, write_copy, copy_mib, title, command) -> str:
+def _busy(d: str):
+    """Per kernel: mean duration (ms), clock (GHz) and MFMA-busy fraction from a rocprofv3 --pmc
+    GRBM_GUI_ACTIVE SQ_VALU_MFMA_BUSY_CYCLES run.  On MI355X GRBM_GUI_ACTIVE is summed over the 8
+    XCDs and SQ_VALU_MFMA_BUSY_CYCLES over the 1024 SIMDs (256 CUs x 4)."""
+    db = _db(d)
+    if db is None:
+        return {}
+    per = defaultdict(dict)
+    names = {}
+    for disp, kn, cn, v, dur in db.execute(
+            "select dispatch_id, kernel_name, counter_name, value, duration from counters_collection"):
+        per[disp][cn] = per[disp].get(cn, 0.0) + float(v)
+        per[disp]["dur"] = float(dur)
+        names[disp] = _short(kn)
+    acc = defaultdict(list)
+    for disp, r in per.items():
+        if "GRBM_GUI_ACTIVE" not in r or r["dur"] <= 0:
+            continue
+        cyc = r["GRBM_GUI_ACTIVE"] / 8.0
+        busy = r.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0) / 1024.0 / cyc if cyc else 0.0
+        acc[names[disp]].append((r["dur"] / 1e6, cyc / (r["dur"] / 1e9) / 1e9, busy))
+    return {k: tuple(sum(x[i] for x in v) / len(v) for i in range(3)) for k, v in acc.items()}
+
+
+def summarize(trace, fetch, write, fetch_copy, write_copy, copy_mib, title, command, busy=None) -> str:
     """Markdown summary of rocprofv3 databases (rocpd SQLite output)."""
     lines = [f"# {title}", "", "Command:", "", "```", command, "```", ""]
     db = _db(trace)
@@ -154,6 +179,16 @@ def summarize(trace, fetch, write, fetch_copy, write_copy, copy_mib, title, comm
                   "| kernel | mean KiB | scaled GB per dispatch |", "|---|---|---|"]
         for k, v in sorted(vals.items()):
             lines.append(f"| {k} | {v:.0f} | {v * 1024 * s / 1e9:.3f} |")
+        lines.append("")
+    if busy:
+        vals = _busy(busy)
+        lines += ["## Clock and MFMA pipe (rocprofv3 --pmc GRBM_GUI_ACTIVE SQ_VALU_MFMA_BUSY_CYCLES)", "",
+                  "Clock = GRBM_GUI_ACTIVE / 8 XCDs / duration; MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / 1024 SIMDs / "
+                  "(GRBM_GUI_ACTIVE / 8).", "",
+                  "| kernel | mean ms (counter run) | clock GHz | MFMA busy |", "|---|---|---|---|"]
+        for k, (ms, ghz, b) in sorted(vals.items()):
+            if b > 0:
+                lines.append(f"| {k} | {ms:.3f} | {ghz:.2f} | {b:.3f} |")
         lines.append("")
     return "\n".join(lines)
 
@@ -187,7 +222,8 @@ def main(argv=None) -> None:
     b = sub.add_parser("mi")
     b.add_argument("--genes", type=int, default=20000)
     b.add_argument("--samples", type=int, default=2000)
-    b.add_argument("--precision", choices=["fp32", "bf16", "bf16x3"], default="bf16x3")
+    b.add_argument("--precision", default="bf16x3", help="fp32, bf16, bf16x3 or i8x3, or a comma-separated list")
+    b.add_argument("--seed", type=lambda v: int(v, 0), default=0x4D49, help="GRN generator seed")
     b.add_argument("--reps", type=int, default=3)
     for name in ("summarize", "traffic"):
         s = sub.add_parser(name)
@@ -197,6 +233,7 @@ def main(argv=None) -> None:
         s.add_argument("--pmc-fetch-copy")
         s.add_argument("--pmc-write-copy")
         s.add_argument("--copy-mib", type=int, default=1024)
+        s.add_argument("--pmc-busy", help="summarize: run with GRBM_GUI_ACTIVE and SQ_VALU_MFMA_BUSY_CYCLES")
         s.add_argument("--title", default="rocprofv3 summary")
         s.add_argument("--command", default="")
         s.add_argument("--config", type=int, help="traffic: workload number the values are filed under")
@@ -207,10 +244,10 @@ def main(argv=None) -> None:
     elif a.cmd == "maxmin":
         maxmin_main(a.genes, a.dtype, a.reps, a.mcp)
     elif a.cmd == "mi":
-        mi_main(a.genes, a.samples, a.precision, a.reps)
+        mi_main(a.genes, a.samples, a.precision, a.reps, a.seed)
     elif a.cmd == "summarize":
         sys.stdout.write(summarize(a.trace, a.pmc_fetch, a.pmc_write, a.pmc_fetch_copy, a.pmc_write_copy, a.copy_mib,
-                                   a.title, a.command) + "\n")
+                                   a.title, a.command, a.pmc_busy) + "\n")
     else:
         t = traffic(a.pmc_fetch, a.pmc_write, a.pmc_fetch_copy, a.pmc_write_copy, a.copy_mib)
         if a.update and a.config is not None:
@@ -219,7 +256,7 @@ def main(argv=None) -> None:
                     doc = json.load(f)
             except (OSError, ValueError):
                 doc = {}
-            doc[f"config{a.config}"] = t
+            doc[f"config{a.config}"] = {**doc.get(f"config{a.config}", {}), **t}
             with open(a.update, "w") as f:
                 json.dump(doc, f, indent=1)
                 f.write("\n")
