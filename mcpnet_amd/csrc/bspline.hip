@@ -241,13 +241,36 @@ struct PrepOut {
 // (BSplineWeightsKernel::operator(), :248-262) and the FP64 marginal entropy (entropy1d_2, :204-224).
 // TWO: BF16 planes in the row orders R and C (o.whi_c, 10 bins) instead of Layout::row_of.
 // I8: the int8 digit planes of the I8X3 kernel (o.q8h ..., row orders R and C, 10 bins) instead.
+// TWO and I8 launches cover every gene slot of the planes (L.ntiles * L.gt) and every sample up to
+// Sp: padding slots and samples are written as zeros, so those planes need no clearing.
 template <int NB, int NO, bool TWO, bool I8>
 __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __restrict__ X, int64_t ldx, Layout L,
                                                                 Spline sp, int do_scale, PrepOut o) {
   const int64_t g = blockIdx.x;
   const int tid = threadIdx.x;
-  const float* x = X + g * ldx;
   const int64_t S = L.S;
+  if constexpr (TWO || I8) {
+    if (g >= L.G) {   // a padding slot of the last gene tile
+      const int gl = (int)(g % 8);
+      const uint32_t grp_base = (uint32_t)((g / 8) * (L.Sp / L.ks));
+      for (int64_t s = tid; s < L.Sp; s += kThreads) {
+        const uint32_t sbase = (grp_base + (uint32_t)(s / L.ks)) * (uint32_t)(kGroupRows * L.ks) + (uint32_t)(s % L.ks);
+        for (int b = 0; b < 10; ++b) {
+          const int m = 10 * (gl & 1) + b;
+          const uint32_t er = sbase + (uint32_t)(16 * (m >> 2) + 4 * (gl >> 1) + (m & 3)) * L.ks;
+          const uint32_t ec = sbase + (uint32_t)(16 * (b % 5) + gl + 8 * (b / 5)) * L.ks;
+          if constexpr (I8) {
+            o.q8h[er] = 0; o.q8l[er] = 0; o.q8h_c[ec] = 0; o.q8l_c[ec] = 0;
+          } else {
+            o.whi[er] = 0; o.whi_c[ec] = 0;
+            if (o.wlo) { o.wlo[er] = 0; o.wlo_c[ec] = 0; }
+          }
+        }
+      }
+      return;
+    }
+  }
+  const float* x = X + g * ldx;
   __shared__ float shf[4];
   __shared__ int shi[4];
   __shared__ double shd[4];
@@ -316,10 +339,17 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
   // row_of_c); 32-bit offsets, as the planes are kept under 4 GiB.
   const int gl = (int)(g % 8);
   const uint32_t grp_base = (uint32_t)((g / 8) * (L.Sp / L.ks));
-  for (int64_t s = tid; s < S; s += kThreads) {
-    const float t = do_scale ? scale * (x[s] - offset) : x[s];
-    if (o.xs) o.xs[g * S + s] = t;
-    basis_values<NB, NO>(sp, t, w, nullptr);
+  // TWO / I8: samples S .. Sp - 1 get zero weights
+  const int64_t s_end = (TWO || I8) ? L.Sp : S;
+  for (int64_t s = tid; s < s_end; s += kThreads) {
+    if ((TWO || I8) && s >= S) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) w[b] = 0.0f;
+    } else {
+      const float t = do_scale ? scale * (x[s] - offset) : x[s];
+      if (o.xs) o.xs[g * S + s] = t;
+      basis_values<NB, NO>(sp, t, w, nullptr);
+    }
     const uint32_t sbase = (grp_base + (uint32_t)(s / L.ks)) * (uint32_t)(kGroupRows * L.ks) + (uint32_t)(s % L.ks);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -358,7 +388,7 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
             if (o.wlo) o.wlo[e] = f32_to_bf16_rne(w[b] - bf16_to_f32(hi));
           }
         }
-        if (o.wref) o.wref[g * o.ldw + (int64_t)b * S + s] = w[b];
+        if (o.wref && s < S) o.wref[g * o.ldw + (int64_t)b * S + s] = w[b];
       }
     }
   }
@@ -993,7 +1023,6 @@ __global__ __launch_bounds__(256) void bspline_pair_adjoint_kernel(PairArgs a) {
   if (n >= a.npairs) return;
   const int64_t i = a.pi[n], j = a.pj[n];
   const int bins = a.L.bins;
-  const int64_t Sp = a.L.Sp;
   // each lane accumulates the full table over its samples, then a wave reduction per entry
   float part[NB];  // reused per row b
   double Lsum = 0.0;
@@ -1098,7 +1127,7 @@ __global__ __launch_bounds__(256) void bspline_gene_adjoint_kernel(GeneArgs a) {
   const int64_t i = blockIdx.x;
   const int tid = threadIdx.x;
   const int bins = a.L.bins;
-  const int64_t S = a.L.S, Sp = a.L.Sp;
+  const int64_t S = a.L.S;
   __shared__ double shd[4];
   __shared__ float gsh[kMaxBins * kMaxBins];
   __shared__ float phi_i[kMaxBins];
@@ -1305,8 +1334,10 @@ int prep_launch(mcpg_ctx* h, const float* dX, int64_t ldx, const Layout& L, cons
     return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: two-order planes need 10 bins and < 4 GiB");
   if (o.q8h && (L.bins != 10 || L.gw != 8 || L.ks != kKStep8))
     return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: int8 digit planes need 10 bins and 64-sample blocks");
+  // two-order and int8 planes: every gene slot of the planes, padding slots written as zeros
+  const unsigned grid = (unsigned)((o.whi_c || o.q8h) ? L.ntiles * L.gt : L.G);
 #define PREP_GO(NB, NO, TWO, I8)                                                                                 \
-  hipLaunchKernelGGL((bspline_prep_kernel<NB, NO, TWO, I8>), dim3((unsigned)L.G), dim3(kThreads), 0, h->stream, dX, \
+  hipLaunchKernelGGL((bspline_prep_kernel<NB, NO, TWO, I8>), dim3(grid), dim3(kThreads), 0, h->stream, dX, \
                      ldx, L, sp, do_scale, o)
   const bool small = sp.bins <= 16 && sp.order <= 4;
   if (o.q8h) {
@@ -1577,7 +1608,8 @@ int bspline_mi_run(mcpg_ctx* h, int ptrs, const float* X, int64_t G, int64_t S, 
     blk.ntiles = (int64_t)h->block_order.size();
   }
   WeightBufs wb = take_weight_bufs(cv, L, precision);
-  if ((st = clear_weight_bufs(h, wb, L))) return st;
+  // the prep kernel writes every element of the two-order and int8 planes, padding included
+  if (!wb.whi_c && !wb.q8[0] && (st = clear_weight_bufs(h, wb, L))) return st;
   if ((st = mcpg::reset_flags(h))) return st;
   PrepOut o{};
   o.w32 = wb.w32; o.whi = wb.whi; o.wlo = wb.wlo; o.whi_c = wb.whi_c; o.wlo_c = wb.wlo_c;
