@@ -905,7 +905,7 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
 // h = rint(127 w) in [0, 127] and l = rint(254 (127 w - h)) in [-127, 127] (absolute error of the
 // split <= 1 / 64516 + 1 ulp).  Per 64-sample K step, v_mfma_i32_16x16x64_i8 accumulates h_i h_j
 // into one int32 set and h_i l_j + l_i h_j into a second; both sums are exact (|HH| <= 127^2 Sp,
-// |X| <= 2 * 127^2 Sp, below 2^31 for Sp < 66000).  The epilogue forms p = (HH + X / 254) / (127^2 S);
+// |X| <= 2 * 127^2 Sp, below 2^31 for Sp <= 66560).  The epilogue forms p = (HH + X / 254) / (127^2 S);
 // the dropped l_i l_j term is below 1 / 254^2 of a digit product.  The int8 MFMA runs at twice the
 // BF16 rate and each digit plane takes one byte per sample, so a K step of 64 samples moves the
 // bytes and costs the MFMA cycles of a 32-sample BF16 step with half of BF16X3's products.
@@ -1553,11 +1553,11 @@ Layout mi_layout(int64_t G, int64_t S, int bins, int precision) {
 }
 
 // I8X3 runs on the two-row-order planes of 10 bins only, and its int32 sums stay exact for
-// padded sample counts below 2^31 / (3 * 127^2) (the largest 64-multiple is 44352).
+// padded sample counts with 2 * 127^2 * Sp < 2^31 (the largest 64-multiple is 66560).
 int check_i8x3(mcpg_ctx* h, int precision, const Layout& L) {
   if (precision != MCPG_I8X3) return MCPG_OK;
   if (L.bins != 10 || L.gw != 8) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "i8x3 precision needs 10 bins");
-  if (L.Sp > 44352) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "i8x3 precision supports at most 44352 samples");
+  if (L.Sp > 66560) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "i8x3 precision supports at most 66560 samples");
   return MCPG_OK;
 }
 

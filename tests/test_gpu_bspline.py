@@ -218,6 +218,26 @@ def test_i8x3_limits(handle):
     assert e.value.status == -1
 
 
+def test_i8x3_int32_sums_at_the_sample_limit(handle):
+    """At the largest supported sample count (66560) with every weight 126.5/127, whose digits
+    are h = 126 and l = 127, the cross sums reach 2 * 126 * 127 * 66560 = 2.13e9, just below
+    2^31: the MI must be the FP64 value 100 p log2 p of the split's joint probability
+    p = (126^2 + 2 * 126 * 127 / 254) / 127^2 (h*h + h*l + l*h; marginal entropies imported as 0).
+    One sample more is refused."""
+    import mcpnet_amd
+    S, bins = 66560, 10
+    W = np.full((2, bins * S + 1), np.float32(126.5 / 127.0), dtype=np.float32)
+    W[:, bins * S] = 0.0
+    got = handle.bspline_mi_weights(W, S=S, precision="i8x3").astype(np.float64)
+    p = (126.0 * 126.0 + 2.0 * 126.0 * 127.0 / 254.0) / 16129.0
+    ref = bins * bins * p * np.log2(p)
+    # FP32 p log2 p near p = 1 is good to about 1e-7 per entry; an overflowed sum would be off by O(1)
+    assert np.abs(got - ref).max() <= 2e-5, (got, ref)
+    with pytest.raises(mcpnet_amd.MCPGError) as e:
+        handle.bspline_mi(gen(2, S + 1), precision="i8x3")
+    assert e.value.status == -7
+
+
 @pytest.mark.parametrize("env", [{"MCPG_MI8": "1"}, {"MCPG_MI8": "2"}, {"MCPG_MI8": "3"}])
 @pytest.mark.parametrize("G,S", [(1, 10), (17, 100), (70, 300), (400, 130), (600, 64)])
 def test_i8x3_kernel_schedules_bit_identical(handle, monkeypatch, env, G, S):
