@@ -1797,8 +1797,9 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
   if (ldx < S || lddx < S) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   if (npairs > (int64_t)INT32_MAX / 2) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "too many pairs");
-  // CSR of the symmetric pair list: gene -> (other gene, pair index, orientation)
-  std::vector<int> off(G + 1, 0), nbr;
+  // CSR of the symmetric pair list: gene -> (other gene, pair index, orientation); the counts
+  // first, the entries below, written straight into the block that goes to the device
+  std::vector<int> off(G + 1, 0);
   for (int64_t n = 0; n < npairs; ++n) {
     if (pi[n] < 0 || pi[n] >= G || pj[n] < 0 || pj[n] >= G)
       return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "pair index out of range");
@@ -1806,19 +1807,7 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
     off[pj[n] + 1]++;
   }
   for (int64_t g = 0; g < G; ++g) off[g + 1] += off[g];
-  nbr.resize(3 * (size_t)off[G] + 3);
-  {
-    std::vector<int> fill(off.begin(), off.end() - 1);
-    for (int64_t n = 0; n < npairs; ++n) {
-      int e = fill[pi[n]]++;
-      nbr[3 * e] = pj[n]; nbr[3 * e + 1] = (int)n; nbr[3 * e + 2] = 0;
-      e = fill[pj[n]]++;
-      nbr[3 * e] = pi[n]; nbr[3 * e + 1] = (int)n; nbr[3 * e + 2] = 1;
-    }
-  }
-  std::vector<double> cw(G, 0.0);   // per gene: sum of m over its entries, in CSR order
-  for (int64_t g = 0; g < G; ++g)
-    for (int e = off[g]; e < off[g + 1]; ++e) cw[g] += (double)m[nbr[3 * e + 1]];
+  const size_t nnbr = 3 * (size_t)off[G] + 3;
   MCPG_HIP(h, hipSetDevice(h->device));
   const Layout L = make_layout(G, S, bins);
   const Spline sp = make_spline(bins, order);
@@ -1827,7 +1816,7 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
   if (ptrs == MCPG_HOST) { sz.add<float>(G * S); sz.add<float>(G * S); }
   size_weight_bufs(sz, L, MCPG_FP32);
   sz.add<int>(np1); sz.add<int>(np1); sz.add<float>(np1);
-  sz.add<int>(G + 1); sz.add<int>(nbr.size()); sz.add<double>(G);
+  sz.add<int>(G + 1); sz.add<int>(nnbr); sz.add<double>(G);
   const bool mfma = mcpg::env_group("MCPG_ADJ_MFMA", 1) != 0;
   if (mfma) sz.add<float>(G * bins * L.Sp);
   sz.add<float>(np1 * bins * bins); sz.add<double>(np1);
@@ -1846,12 +1835,15 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
     ddX = cv.take<float>(G * S); ldd = S;
   }
   WeightBufs wb = take_weight_bufs(cv, L, MCPG_FP32);
+  // pair list, CSR and per-gene weight sums: one block, built on the host and uploaded at once
   int* d_pi = cv.take<int>(np1);
   int* d_pj = cv.take<int>(np1);
   float* d_m = cv.take<float>(np1);
   int* d_off = cv.take<int>(G + 1);
-  int* d_nbr = cv.take<int>(nbr.size());
+  int* d_nbr = cv.take<int>(nnbr);
   double* d_cw = cv.take<double>(G);
+  char* const dblk = reinterpret_cast<char*>(d_pi);
+  const size_t blk_bytes = (size_t)(reinterpret_cast<char*>(d_cw + G) - dblk);
   float* d_accg = mfma ? cv.take<float>(G * bins * L.Sp) : nullptr;
   float* d_Gm = cv.take<float>(np1 * bins * bins);
   double* d_Lp = cv.take<double>(np1);
@@ -1860,14 +1852,43 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
   float* d_scl = cv.take<float>(G);
   int* d_amin = cv.take<int>(G);
   int* d_amax = cv.take<int>(G);
-  if (npairs > 0) {
-    MCPG_HIP(h, hipMemcpyAsync(d_pi, pi, npairs * 4, hipMemcpyHostToDevice, h->stream));
-    MCPG_HIP(h, hipMemcpyAsync(d_pj, pj, npairs * 4, hipMemcpyHostToDevice, h->stream));
-    MCPG_HIP(h, hipMemcpyAsync(d_m, m, npairs * 4, hipMemcpyHostToDevice, h->stream));
+  {
+    // the block in the handle's pinned staging buffer when it fits (else in pageable memory)
+    std::vector<char> pageable;
+    char* hblk = static_cast<char*>(mcpg::pinned_staging(h, 0, blk_bytes, &st));
+    if (st) return st;
+    if (!hblk) {
+      pageable.resize(blk_bytes);
+      hblk = pageable.data();
+    }
+    auto host_of = [&](const void* d) { return hblk + (reinterpret_cast<const char*>(d) - dblk); };
+    int* hnbr = reinterpret_cast<int*>(host_of(d_nbr));
+    double* hcw = reinterpret_cast<double*>(host_of(d_cw));
+    if (npairs > 0) {
+      std::memcpy(host_of(d_pi), pi, npairs * 4);
+      std::memcpy(host_of(d_pj), pj, npairs * 4);
+      std::memcpy(host_of(d_m), m, npairs * 4);
+    }
+    std::memcpy(host_of(d_off), off.data(), (G + 1) * 4);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int64_t n = 0; n < npairs; ++n) {
+      int e = fill[pi[n]]++;
+      hnbr[3 * e] = pj[n]; hnbr[3 * e + 1] = (int)n; hnbr[3 * e + 2] = 0;
+      e = fill[pj[n]]++;
+      hnbr[3 * e] = pi[n]; hnbr[3 * e + 1] = (int)n; hnbr[3 * e + 2] = 1;
+    }
+    hnbr[nnbr - 3] = hnbr[nnbr - 2] = hnbr[nnbr - 1] = 0;
+    for (int64_t g = 0; g < G; ++g) {   // per gene: sum of m over its entries, in CSR order
+      double c = 0.0;
+      for (int e = off[g]; e < off[g + 1]; ++e) c += (double)m[hnbr[3 * e + 1]];
+      hcw[g] = c;
+    }
+    if (pageable.empty()) {
+      if ((st = mcpg::pinned_upload(h, 0, dblk, blk_bytes))) return st;
+    } else {
+      MCPG_HIP(h, hipMemcpy(dblk, hblk, blk_bytes, hipMemcpyHostToDevice));
+    }
   }
-  MCPG_HIP(h, hipMemcpyAsync(d_off, off.data(), (G + 1) * 4, hipMemcpyHostToDevice, h->stream));
-  MCPG_HIP(h, hipMemcpyAsync(d_nbr, nbr.data(), nbr.size() * 4, hipMemcpyHostToDevice, h->stream));
-  MCPG_HIP(h, hipMemcpyAsync(d_cw, cw.data(), G * 8, hipMemcpyHostToDevice, h->stream));
   if ((st = clear_weight_bufs(h, wb, L))) return st;
   if ((st = mcpg::reset_flags(h))) return st;
   PrepOut o{};
@@ -1903,8 +1924,15 @@ int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_
   }
   if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, dX, lddx * 4, ddX, S * 4, S * 4, G, hipMemcpyDeviceToHost))) return st;
   if (L_out) {
-    std::vector<double> lp(np1, 0.0);
-    if (npairs > 0) MCPG_HIP(h, hipMemcpyAsync(lp.data(), d_Lp, npairs * 8, hipMemcpyDeviceToHost, h->stream));
+    // per-pair loss terms through the second pinned buffer when they fit, summed in pair order
+    std::vector<double> pageable;
+    double* lp = static_cast<double*>(mcpg::pinned_staging(h, 1, (size_t)np1 * 8, &st));
+    if (st) return st;
+    if (!lp) {
+      pageable.assign(np1, 0.0);
+      lp = pageable.data();
+    }
+    if (npairs > 0) MCPG_HIP(h, hipMemcpyAsync(lp, d_Lp, npairs * 8, hipMemcpyDeviceToHost, h->stream));
     if ((st = mcpg::finish(h, "mcpg_bspline_mi_backward_f32"))) return st;
     double Ls = 0.0;
     for (int64_t n = 0; n < npairs; ++n) Ls += lp[n];

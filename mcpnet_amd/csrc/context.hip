@@ -80,6 +80,28 @@ static int ensure_pinned(mcpg_ctx* h) {
   return MCPG_OK;
 }
 
+void* pinned_staging(mcpg_ctx* h, int b, size_t bytes, int* st) {
+  *st = MCPG_OK;
+  if (bytes > kPinBytes) return nullptr;
+  if ((*st = ensure_pinned(h))) return nullptr;
+  if (h->pin_busy[b]) {
+    const hipError_t e = hipEventSynchronize(h->pin_ev[b]);
+    if (e != hipSuccess) {
+      *st = hip_error(h, e, "hipEventSynchronize");
+      return nullptr;
+    }
+    h->pin_busy[b] = false;
+  }
+  return h->pin[b];
+}
+
+int pinned_upload(mcpg_ctx* h, int b, void* dst, size_t bytes) {
+  MCPG_HIP(h, hipMemcpyAsync(dst, h->pin[b], bytes, hipMemcpyHostToDevice, h->stream));
+  MCPG_HIP(h, hipEventRecord(h->pin_ev[b], h->stream));
+  h->pin_busy[b] = true;
+  return MCPG_OK;
+}
+
 int copy_2d(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
             size_t rows, hipMemcpyKind kind) {
   if (rows == 0 || width_bytes == 0) return MCPG_OK;

@@ -93,6 +93,11 @@ struct Sizer {
 // 2-D strided copies between host/device buffers on the handle's stream.  Host <-> device copies
 // of more than a few MB are staged through the handle's pinned buffers; a device -> host copy
 // has completed when copy_2d returns.
+// Pinned staging buffer b (0 or 1) of the handle for a host block of at most 32 MB (nullptr with
+// *st == MCPG_OK when the block is larger), once the copy that last used it has finished;
+// pinned_upload sends its first `bytes` to dst on the handle's stream.
+void* pinned_staging(mcpg_ctx* h, int b, size_t bytes, int* st);
+int pinned_upload(mcpg_ctx* h, int b, void* dst, size_t bytes);
 int copy_2d(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
             size_t rows, hipMemcpyKind kind);
 
