@@ -60,37 +60,55 @@ __host__ __device__ __forceinline__ uint64_t splitmix64(uint64_t seed, uint64_t 
   return z ^ (z >> 31);
 }
 
-__global__ void noise_kernel(const float* __restrict__ X, int64_t ldx, int64_t G, int64_t S, uint64_t seed,
+// Sample s of gene g gets draw g * S + s of the stream; rows by blockIdx.y (gridDim.y <= 65535
+// per launch, offset by g0), samples by blockIdx.x, so no 64-bit division per element.
+__global__ void noise_kernel(const float* __restrict__ X, int64_t ldx, int64_t g0, int64_t S, uint64_t seed,
                              float* __restrict__ Y, int64_t ldy) {
-  const int64_t total = G * S;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t g = t / S, s = t % S;
-    const double u = (double)(splitmix64(seed, (uint64_t)t) >> 11) * 0x1.0p-53;
+  const int64_t g = g0 + blockIdx.y;
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < S; s += (int64_t)gridDim.x * blockDim.x) {
+    const double u = (double)(splitmix64(seed, (uint64_t)(g * S + s)) >> 11) * 0x1.0p-53;
     const float j = (float)(u * 1e-5);
     Y[g * ldy + s] = X[g * ldx + s] + j;
   }
 }
 
-// Descending rank, ties by ascending index: r_s = #{t : x_t > x_s} + #{t < s : x_t == x_s}.
-__global__ __launch_bounds__(256) void rank_kernel(const float* __restrict__ X, int64_t ldx, int64_t S,
+// Descending rank, ties by ascending index: r_s = #{t : x_t > x_s} + #{t < s : x_t == x_s}, i.e.
+// the position of s when the samples are sorted by (value descending, index ascending).  One
+// block per gene sorts the 64-bit keys (~u(x_s) << 32 | s) ascending with a bitonic network in
+// LDS, where u is the unsigned order key of the float (-0 taken as +0, so they tie as under ==);
+// n = S rounded up to a power of two, padding keys all ones.
+__global__ __launch_bounds__(256) void rank_kernel(const float* __restrict__ X, int64_t ldx, int64_t S, int n,
                                                     int32_t* __restrict__ R, int64_t ldr, int* flags) {
-  extern __shared__ float xsh[];
+  extern __shared__ uint64_t key[];
   const int64_t g = blockIdx.x;
-  for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
-    const float v = X[g * ldx + s];
-    if (!isfinite(v)) atomicOr(flags, 1);
-    xsh[s] = v;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    uint64_t k = ~0ull;
+    if (i < S) {
+      const float v = X[g * ldx + i];
+      if (!isfinite(v)) atomicOr(flags, 1);
+      const uint32_t b = __float_as_uint(v + 0.0f);   // -0 + 0 = +0
+      const uint32_t u = b ^ ((b >> 31) ? 0xffffffffu : 0x80000000u);
+      k = ((uint64_t)~u << 32) | (uint32_t)i;
+    }
+    key[i] = k;
   }
   __syncthreads();
-  for (int64_t s = threadIdx.x; s < S; s += blockDim.x) {
-    const float v = xsh[s];
-    int r = 0;
-    for (int64_t t = 0; t < S; ++t) {
-      const float u = xsh[t];
-      r += (u > v) || (u == v && t < s);
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const int l = i ^ j;
+        if (l > i) {
+          const uint64_t a = key[i], b = key[l];
+          if ((a > b) == ((i & k) == 0)) {
+            key[i] = b;
+            key[l] = a;
+          }
+        }
+      }
+      __syncthreads();
     }
-    R[g * ldr + s] = r;
   }
+  for (int i = threadIdx.x; i < S; i += blockDim.x) R[g * ldr + (int64_t)(key[i] & 0xffffffffu)] = i;
 }
 
 // IntegralCumulativeHistogramKernel (AdaptivePartitioningMI.hpp:422-439) for one row per block.
@@ -785,8 +803,10 @@ int mcpg_add_noise_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64
   }
   {
     mcpg::LaunchScope ls(h, "noise_kernel");
-    hipLaunchKernelGGL(noise_kernel, dim3((unsigned)std::min<int64_t>(ceil_div(G * S, 256), 65536)), dim3(256), 0,
-                       h->stream, dX, lx, G, S, seed, dY, ly);
+    const unsigned gx = (unsigned)std::min<int64_t>(ceil_div(S, 256), 1024);
+    for (int64_t g0 = 0; g0 < G; g0 += 65535)
+      hipLaunchKernelGGL(noise_kernel, dim3(gx, (unsigned)std::min<int64_t>(G - g0, 65535)), dim3(256), 0, h->stream,
+                         dX, lx, g0, S, seed, dY, ly);
   }
   if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, Y, ldy * 4, dY, S * 4, S * 4, G, hipMemcpyDeviceToHost))) return st;
   return mcpg::finish(h, "mcpg_add_noise_f32");
@@ -820,9 +840,17 @@ int mcpg_rank_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S,
   }
   if ((st = mcpg::reset_flags(h))) return st;
   {
+    int n = 2;
+    while (n < S) n <<= 1;
+    static bool attr = false;   // up to 16384 keys of 8 bytes: 128 KB of LDS
+    if (!attr) {
+      MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&rank_kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 16384 * 8));
+      attr = true;
+    }
     mcpg::LaunchScope ls(h, "rank_kernel");
-    hipLaunchKernelGGL(rank_kernel, dim3((unsigned)G), dim3(256), S * sizeof(float), h->stream, dX, lx, S, dR, lr,
-                       h->d_flags);
+    hipLaunchKernelGGL(rank_kernel, dim3((unsigned)G), dim3(256), (size_t)n * sizeof(uint64_t), h->stream, dX, lx, S,
+                       n, dR, lr, h->d_flags);
   }
   if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, R, ldr * 4, dR, S * 4, S * 4, G, hipMemcpyDeviceToHost))) return st;
   if ((st = mcpg::finish(h, "mcpg_rank_f32"))) return st;

@@ -25,6 +25,23 @@ def test_jitter_and_rank_bit_exact(handle):
     assert np.array_equal(handle.rank(T), grn.rank_desc(T))
 
 
+@pytest.mark.parametrize("S", [1, 2, 3, 257, 1000, 5000, 16384])
+def test_rank_with_ties_and_signed_zeros(handle, S):
+    """Descending ranks with ties by ascending index on rows with few distinct values, -0 and +0
+    (equal under ==), negatives and a row of one value, at sizes that pad the sort to the next
+    power of two and at the largest supported S."""
+    rng = np.random.default_rng(S)
+    X = np.round(rng.normal(size=(5, S)) * 2.0).astype(np.float32)   # about a dozen distinct values
+    X[1, ::3] = -0.0
+    X[1, 1::3] = 0.0
+    X[2] = 7.0
+    X[3] = rng.normal(size=S).astype(np.float32)
+    X[4] = -np.abs(X[4])
+    assert np.array_equal(handle.rank(X), grn.rank_desc(X))
+    J = handle.add_noise(X, seed=99)
+    assert np.array_equal(J, grn.jitter(X, 99))
+
+
 def test_cumhist(handle):
     R = ranks(20, 300, 5)
     assert np.array_equal(handle.cumhist(R), oracle.cumhist(R))
