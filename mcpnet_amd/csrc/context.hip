@@ -102,6 +102,48 @@ int pinned_upload(mcpg_ctx* h, int b, void* dst, size_t bytes) {
   return MCPG_OK;
 }
 
+int Download::start(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
+                    size_t rows) {
+  if (rows == 0 || width_bytes == 0) return MCPG_OK;
+  if (width_bytes * rows < kPinMin || width_bytes > kPinBytes)
+    return copy_2d(h, dst, dpitch, src, spitch, width_bytes, rows, hipMemcpyDeviceToHost);
+  if (!h->side) MCPG_HIP(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+  for (int b = 0; b < 2; ++b)
+    if (!h->side_pin[b]) MCPG_HIP(h, hipHostMalloc(&h->side_pin[b], kPinBytes, hipHostMallocDefault));
+  for (int b = 0; b < 3; ++b)
+    if (!h->side_ev[b]) MCPG_HIP(h, hipEventCreateWithFlags(&h->side_ev[b], hipEventDisableTiming));
+  MCPG_HIP(h, hipEventRecord(h->side_ev[2], h->stream));
+  MCPG_HIP(h, hipStreamWaitEvent(h->side, h->side_ev[2], 0));
+  const size_t per = kPinBytes / width_bytes, nchunks = (rows + per - 1) / per;
+  char* d = static_cast<char*>(dst);
+  const char* s = static_cast<const char*>(src);
+  th = std::thread([=]() {
+    auto enqueue = [&](size_t c) -> hipError_t {
+      const int b = (int)(c & 1);
+      const size_t r0 = c * per, n = std::min(per, rows - r0);
+      hipError_t e = hipMemcpy2DAsync(h->side_pin[b], width_bytes, s + r0 * spitch, spitch, width_bytes, n,
+                                      hipMemcpyDeviceToHost, h->side);
+      return e != hipSuccess ? e : hipEventRecord(h->side_ev[b], h->side);
+    };
+    hipError_t e = enqueue(0);
+    for (size_t c = 0; e == hipSuccess && c < nchunks; ++c) {
+      if (c + 1 < nchunks && (e = enqueue(c + 1)) != hipSuccess) break;
+      const int b = (int)(c & 1);
+      const size_t r0 = c * per, n = std::min(per, rows - r0);
+      if ((e = hipEventSynchronize(h->side_ev[b])) != hipSuccess) break;
+      copy_rows_parallel(d + r0 * dpitch, dpitch, static_cast<const char*>(h->side_pin[b]), width_bytes, width_bytes, n);
+    }
+    if (e != hipSuccess) status = MCPG_ERR_HIP;
+  });
+  return MCPG_OK;
+}
+
+int Download::wait(mcpg_ctx* h) {
+  if (th.joinable()) th.join();
+  if (status) return set_error(h, status, "side-stream download failed");
+  return MCPG_OK;
+}
+
 int copy_2d(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
             size_t rows, hipMemcpyKind kind) {
   if (rows == 0 || width_bytes == 0) return MCPG_OK;
@@ -294,7 +336,11 @@ int mcpg_destroy(mcpg_handle h) {
   for (int b = 0; b < 2; ++b) {
     if (h->pin_ev[b]) (void)hipEventDestroy(h->pin_ev[b]);
     if (h->pin[b]) (void)hipHostFree(h->pin[b]);
+    if (h->side_pin[b]) (void)hipHostFree(h->side_pin[b]);
   }
+  for (hipEvent_t e : h->side_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->side) (void)hipStreamDestroy(h->side);
   if (h->ws) (void)hipFree(h->ws);
   for (auto& o : h->orders) (void)hipFree(o.d);
   if (h->d_flags) (void)hipFree(h->d_flags);

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "mcpgpu/mcpgpu.h"
@@ -44,6 +45,11 @@ struct mcpg_ctx {
   void* pin[2] = {nullptr, nullptr};
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
   bool pin_busy[2] = {false, false};
+  // a second stream (default priority) with its own two pinned buffers, for a device -> host
+  // download that overlaps the kernels queued after it on `stream` (see mcpg::Download)
+  hipStream_t side = nullptr;
+  void* side_pin[2] = {nullptr, nullptr};
+  hipEvent_t side_ev[3] = {nullptr, nullptr, nullptr};   // the two buffers, and the start
   // max-min key kernel: key blocks of the split tail tiles, and resident workgroups per device
   int* mm_tail = nullptr;
   size_t mm_tail_bytes = 0;
@@ -98,6 +104,19 @@ struct Sizer {
 // pinned_upload sends its first `bytes` to dst on the handle's stream.
 void* pinned_staging(mcpg_ctx* h, int b, size_t bytes, int* st);
 int pinned_upload(mcpg_ctx* h, int b, void* dst, size_t bytes);
+// A device -> host copy of `rows` rows that runs on the handle's side stream, after the work
+// queued on the handle's stream so far, through the side pinned buffers, driven by a host thread:
+// start() returns at once, so later kernels on the handle's stream overlap it; wait() joins it
+// and returns its status.  Copies of at most 8 MB are done synchronously by start().
+struct Download {
+  std::thread th;
+  int status = MCPG_OK;
+  int start(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes, size_t rows);
+  int wait(mcpg_ctx* h);
+  ~Download() {
+    if (th.joinable()) th.join();
+  }
+};
 int copy_2d(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size_t spitch, size_t width_bytes,
             size_t rows, hipMemcpyKind kind);
 

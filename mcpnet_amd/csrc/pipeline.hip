@@ -214,7 +214,11 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
     hipLaunchKernelGGL(gather_rows_kernel<T>, dim3(grid_of(R * G)), dim3(256), 0, h->stream, dmi, G, drows, R, G, dmit,
                        G);
   }
-  if (mi_out && (st = mcpg::copy_2d(h, mi_out, ldmi * sizeof(T), dmi, G * sizeof(T), G * sizeof(T), G, out_kind)))
+  // the MI download overlaps the max-min products (host outputs: on the side stream)
+  mcpg::Download mi_dl;
+  if (mi_out && host && (st = mi_dl.start(h, mi_out, ldmi * sizeof(T), dmi, G * sizeof(T), G * sizeof(T), G)))
+    return st;
+  if (mi_out && !host && (st = mcpg::copy_2d(h, mi_out, ldmi * sizeof(T), dmi, G * sizeof(T), G * sizeof(T), G, out_kind)))
     return st;
 
   // ---- MCP2/3/4 (compute_mcp{2,3,4}, or compute_mcp{2,3,4}_tfs with tf_gene_transition k - 1)
@@ -256,6 +260,7 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
       return st;
     if ((st = mcpg::finish(h, "mcpg_mcpnet"))) return st;   // the output copy completes before bout is freed
   }
+  if ((st = mi_dl.wait(h))) return st;
   return mcpg::finish(h, "mcpg_mcpnet");
 }
 

@@ -27,6 +27,21 @@ def test_bspline_chain_matches_stages(handle, precision):
         assert bits_equal(res[k], handle.mcp(MI, k)[0]), k
 
 
+@pytest.mark.slow
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_host_chain_with_overlapped_mi_download(handle, dt):
+    """3000 genes with host buffers: the MI (36 or 72 MB) comes back on the side stream in two or
+    three 32 MB chunks while the max-min products run; every output equals the stage entry
+    points' bits."""
+    X = grn.grn_expression(3000, 96, 34)[0].astype(dt)
+    res = handle.mcpnet(X, mi_method=0, orders=(2, 3), precision="bf16x3")
+    MI = handle.bspline_mi(X.astype(np.float32), precision="bf16x3").astype(dt)
+    np.fill_diagonal(MI, 0.0)
+    assert bits_equal(res["mi"], MI)
+    for k in (2, 3):
+        assert bits_equal(res[k], handle.mcp(MI, k)[0]), k
+
+
 def test_ap_chain_matches_stages(handle):
     X = grn.grn_expression(70, 256, 32)[0]
     res = handle.mcpnet(X, mi_method=1, noise_seed=99, orders=(3,), diagonal=-1.0)
