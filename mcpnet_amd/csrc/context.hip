@@ -118,6 +118,12 @@ int Download::start(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size
   char* d = static_cast<char*>(dst);
   const char* s = static_cast<const char*>(src);
   th = std::thread([=]() {
+    // a new host thread starts on device 0: make the handle's device current
+    if (hipSetDevice(h->device) != hipSuccess) {
+      status = MCPG_ERR_HIP;
+      (void)hipStreamSynchronize(h->side);
+      return;
+    }
     auto enqueue = [&](size_t c) -> hipError_t {
       const int b = (int)(c & 1);
       const size_t r0 = c * per, n = std::min(per, rows - r0);
@@ -133,7 +139,9 @@ int Download::start(mcpg_ctx* h, void* dst, size_t dpitch, const void* src, size
       if ((e = hipEventSynchronize(h->side_ev[b])) != hipSuccess) break;
       copy_rows_parallel(d + r0 * dpitch, dpitch, static_cast<const char*>(h->side_pin[b]), width_bytes, width_bytes, n);
     }
-    if (e != hipSuccess) status = MCPG_ERR_HIP;
+    // nothing queued on the side stream outlives the thread, also after an error
+    const hipError_t es = hipStreamSynchronize(h->side);
+    if (e != hipSuccess || es != hipSuccess) status = MCPG_ERR_HIP;
   });
   return MCPG_OK;
 }
