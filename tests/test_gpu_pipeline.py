@@ -16,7 +16,7 @@ def bits_equal(a, b):
     return a.shape == b.shape and np.array_equal(a.view(np.uint8), b.view(np.uint8))
 
 
-@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "i8x3"])
 def test_bspline_chain_matches_stages(handle, precision):
     X = grn.grn_expression(150, 320, 31)[0]
     res = handle.mcpnet(X, mi_method=0, precision=precision)
