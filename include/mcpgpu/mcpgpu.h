@@ -49,7 +49,7 @@ enum mcpg_precision {
   MCPG_BF16 = 1,   /* BF16 weights, BF16 MFMA, FP32 accumulation */
   MCPG_BF16X3 = 2, /* hi+lo BF16 split, 3 MFMA products (hi*hi + hi*lo + lo*hi), FP32 accumulation */
   MCPG_I8X3 = 3    /* two int8 digits per weight, w ~= (h + l/254)/127; products h*h and h*l + l*h
-                      on the int8 MFMA (v_mfma_i32_16x16x64_i8) with exact int32 sums; 10 bins,
+                      on the int8 MFMA (v_mfma_i32_16x16x64_i8) with exact int32 sums;
                       S <= 66560.  Not in the reference (its MI is FP32/FP64 on the CPU). */
 };
 

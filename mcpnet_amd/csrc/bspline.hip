@@ -360,13 +360,19 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
           const float w127 = w[b] * 127.0f;
           const float hq = rintf(w127);
           const float lq = fminf(fmaxf(rintf((w127 - hq) * 254.0f), -127.0f), 127.0f);
-          const int m = 10 * (gl & 1) + b;
-          const uint32_t er = sbase + (uint32_t)(16 * (m >> 2) + 4 * (gl >> 1) + (m & 3)) * kKStep8;
-          const uint32_t ec = sbase + (uint32_t)(16 * (b % 5) + gl + 8 * (b / 5)) * kKStep8;
-          o.q8h[er] = (int8_t)hq;
-          o.q8l[er] = (int8_t)lq;
-          o.q8h_c[ec] = (int8_t)hq;
-          o.q8l_c[ec] = (int8_t)lq;
+          if (o.q8h_c) {   // two row orders (10 bins)
+            const int m = 10 * (gl & 1) + b;
+            const uint32_t er = sbase + (uint32_t)(16 * (m >> 2) + 4 * (gl >> 1) + (m & 3)) * kKStep8;
+            const uint32_t ec = sbase + (uint32_t)(16 * (b % 5) + gl + 8 * (b / 5)) * kKStep8;
+            o.q8h[er] = (int8_t)hq;
+            o.q8l[er] = (int8_t)lq;
+            o.q8h_c[ec] = (int8_t)hq;
+            o.q8l_c[ec] = (int8_t)lq;
+          } else {         // one row order (Layout::row_of)
+            const int64_t e = L.at(L.row_of(g, b), s);
+            o.q8h[e] = (int8_t)hq;
+            o.q8l[e] = (int8_t)lq;
+          }
         } else if constexpr (TWO) {
           const uint16_t hi = f32_to_bf16_rne(w[b]);
           const int m = 10 * (gl & 1) + b;
@@ -441,11 +447,17 @@ __global__ void weights_import_kernel(const float* __restrict__ W, int64_t ldw, 
       const float w127 = v * 127.0f;
       const float hq = rintf(w127);
       const float lq = fminf(fmaxf(rintf((w127 - hq) * 254.0f), -127.0f), 127.0f);
-      const int64_t er = L.at(L.row_of_r(g, b), s), ec = L.at(L.row_of_c(g, b), s);
-      q8h[er] = (int8_t)hq;
-      q8l[er] = (int8_t)lq;
-      q8h_c[ec] = (int8_t)hq;
-      q8l_c[ec] = (int8_t)lq;
+      if (q8h_c) {   // two row orders (10 bins)
+        const int64_t er = L.at(L.row_of_r(g, b), s), ec = L.at(L.row_of_c(g, b), s);
+        q8h[er] = (int8_t)hq;
+        q8l[er] = (int8_t)lq;
+        q8h_c[ec] = (int8_t)hq;
+        q8l_c[ec] = (int8_t)lq;
+      } else {
+        const int64_t e = L.at(L.row_of(g, b), s);
+        q8h[e] = (int8_t)hq;
+        q8l[e] = (int8_t)lq;
+      }
       continue;
     }
     const int64_t ei = L.at(L.row_of(g, b), s);
@@ -527,6 +539,8 @@ constexpr int kWaitLgkm0 = 0xc07f;
 
 template <int P>
 constexpr int mi_planes() { return P == kBF16X3 ? 2 : 1; }
+// LDS of the int8 kernel: NST stages of four 10 KB plane slices, reused by the LDS epilogue
+constexpr int mi8_smem_bytes(int nst) { return nst * 4 * kPlaneB16 > kEpiBytes ? nst * 4 * kPlaneB16 : kEpiBytes; }
 template <int P>
 constexpr int mi_stage_bytes() { return 2 * mi_planes<P>() * kPlaneB16; }   // panels A and B
 // NST: K steps held in LDS (one being computed, NST - 1 in flight)
@@ -643,6 +657,52 @@ __device__ __forceinline__ void re_epilogue(const MIArgs& a, const floatx4 (&acc
       const float mi = (float)((a.H64[gi] + a.H64[gj]) + (double)(hh ? s1 : s0));
       store_mi(a, gi, gj, mi);
     }
+  }
+}
+
+// LDS epilogue of the kernels with one row order (any bin count): each wave's t = p log2 p tile
+// goes to LDS (two waves at a time, over the stage buffers), then each lane sums one gene pair's
+// bins x bins block, and MI = (H_i + H_j) + sum t in FP64.  Every thread of the block calls it.
+__device__ __forceinline__ void lds_epilogue(const MIArgs& a, const floatx4 (&acc)[5][5], int64_t ti, int64_t tj,
+                                             char* smem) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1, lr = lane & 15, lq = lane >> 4;
+  const bool diag = ti == tj;
+  float* E = reinterpret_cast<float*>(smem);
+  const int bins = a.L.bins, gw = a.L.gw;
+  for (int phase = 0; phase < 2; ++phase) {
+    if (wr == phase) {
+      float* Ew = E + wc * kEpiWave;
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float p = acc[i][j][r] * a.norm;
+            const float t = p > 0.0f ? p * log2f(p) : 0.0f;
+            Ew[(i * 16 + lq * 4 + r) * kEpiStride + j * 16 + lr] = t;
+          }
+    }
+    __syncthreads();
+    if (wr == phase) {
+      const float* Ew = E + wc * kEpiWave;
+      for (int pr = lane; pr < gw * gw; pr += 64) {
+        const int pi = pr / gw, pj = pr % gw;
+        const int64_t gi = ti * a.L.gt + wr * gw + pi;
+        const int64_t gj = tj * a.L.gt + wc * gw + pj;
+        if (gi >= a.L.G || gj >= a.L.G) continue;
+        if (diag && gi > gj) continue;
+        float sum = 0.0f;
+        for (int r = 0; r < bins; ++r) {
+          const float* row = Ew + (pi * bins + r) * kEpiStride + pj * bins;
+          for (int c = 0; c < bins; ++c) sum += row[c];
+        }
+        const float mi = (float)((a.H64[gi] + a.H64[gj]) + (double)sum);
+        store_mi(a, gi, gj, mi);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -862,43 +922,7 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
     re_epilogue(a, acc, ti, tj);
     return;
   }
-  // ---- epilogue: t = p log2 p per joint-table entry, then one gene pair per lane
-  float* E = reinterpret_cast<float*>(smem);
-  const int bins = a.L.bins, gw = a.L.gw;
-  for (int phase = 0; phase < 2; ++phase) {
-    if (wr == phase) {
-      float* Ew = E + wc * kEpiWave;
-#pragma unroll
-      for (int i = 0; i < 5; ++i)
-#pragma unroll
-        for (int j = 0; j < 5; ++j)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const float p = acc[i][j][r] * a.norm;
-            const float t = p > 0.0f ? p * log2f(p) : 0.0f;
-            Ew[(i * 16 + lq * 4 + r) * kEpiStride + j * 16 + lr] = t;
-          }
-    }
-    __syncthreads();
-    if (wr == phase) {
-      const float* Ew = E + wc * kEpiWave;
-      for (int pr = lane; pr < gw * gw; pr += 64) {
-        const int pi = pr / gw, pj = pr % gw;
-        const int64_t gi = ti * a.L.gt + wr * gw + pi;
-        const int64_t gj = tj * a.L.gt + wc * gw + pj;
-        if (gi >= a.L.G || gj >= a.L.G) continue;
-        if (diag && gi > gj) continue;
-        float sum = 0.0f;
-        for (int r = 0; r < bins; ++r) {
-          const float* row = Ew + (pi * bins + r) * kEpiStride + pj * bins;
-          for (int c = 0; c < bins; ++c) sum += row[c];
-        }
-        const float mi = (float)((a.H64[gi] + a.H64[gj]) + (double)sum);
-        store_mi(a, gi, gj, mi);
-      }
-    }
-    __syncthreads();
-  }
+  lds_epilogue(a, acc, ti, tj, smem);
 }
 
 // I8X3: each weight is split into two int8 digits, w ~= (h + l / 254) / 127 with
@@ -914,7 +938,9 @@ __global__ __launch_bounds__(kThreads) void bspline_mi_kernel(MIArgs a) {
 // the 256 registers a wave has at two waves per SIMD) leave no room for a step's A fragments, so
 // a K step's buffer stays in LDS until its last block row and NST - 1 steps are in flight.
 // EARLY: the next step's ten loads are issued together after the barrier (else two per block row).
-template <int NST, bool EARLY>
+// RE: planes in the two row orders and the register epilogue (10 bins); otherwise one row order
+// (a.q8[2], a.q8[3] repeat a.q8[0], a.q8[1]) and the LDS epilogue.
+template <int NST, bool EARLY, bool RE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NST == 2 ? 2 : 1, 2))) void bspline_mi_i8_kernel(MIArgs a) {
   extern __shared__ __align__(16) char smem[];
 
@@ -997,7 +1023,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NST ==
     for (int j = 0; j < 5; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[i][j][r] = (float)acc_h[i][j][r] + (float)acc_x[i][j][r] * (1.0f / 254.0f);
-  re_epilogue(a, acc, ti, tj);
+  if constexpr (RE) {
+    re_epilogue(a, acc, ti, tj);
+  } else {
+    __syncthreads();   // every wave is done with the stage buffers, which the epilogue reuses
+    lds_epilogue(a, acc, ti, tj, smem);
+  }
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -1332,10 +1363,10 @@ int prep_launch(mcpg_ctx* h, const float* dX, int64_t ldx, const Layout& L, cons
   mcpg::LaunchScope ls(h, "bspline_prep_kernel");
   if (o.whi_c && (L.bins != 10 || L.gw != 8 || (uint64_t)L.rows * (uint64_t)L.Sp * 2 >= (1ull << 32)))
     return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: two-order planes need 10 bins and < 4 GiB");
-  if (o.q8h && (L.bins != 10 || L.gw != 8 || L.ks != kKStep8))
-    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: int8 digit planes need 10 bins and 64-sample blocks");
-  // two-order and int8 planes: every gene slot of the planes, padding slots written as zeros
-  const unsigned grid = (unsigned)((o.whi_c || o.q8h) ? L.ntiles * L.gt : L.G);
+  if (o.q8h && (L.ks != kKStep8 || (o.q8h_c && (L.bins != 10 || L.gw != 8))))
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: int8 digit planes need 64-sample blocks (two orders: 10 bins)");
+  // two-order planes: every gene slot of the planes, padding slots written as zeros
+  const unsigned grid = (unsigned)((o.whi_c || o.q8h_c) ? L.ntiles * L.gt : L.G);
 #define PREP_GO(NB, NO, TWO, I8)                                                                                 \
   hipLaunchKernelGGL((bspline_prep_kernel<NB, NO, TWO, I8>), dim3(grid), dim3(kThreads), 0, h->stream, dX, \
                      ldx, L, sp, do_scale, o)
@@ -1404,6 +1435,11 @@ int mi_launch(mcpg_ctx* h, int precision, const WeightBufs& wb, const Layout& L,
   a.c0 = whole ? 0 : blk.c0; a.nc = whole ? L.G : blk.nc;
   const bool i8 = precision == MCPG_I8X3;
   const bool re = wb.whi_c != nullptr;
+  const bool re8 = i8 && wb.q8[2] != nullptr;   // int8 planes in two row orders
+  if (i8 && !re8) {   // one row order: operand B reads the operand-A planes
+    a.q8[2] = wb.q8[0];
+    a.q8[3] = wb.q8[1];
+  }
   if (re && (precision == MCPG_FP32 || L.bins != 10))
     return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: two-order planes need 10 bins and BF16");
   if (i8 && (!wb.q8[0] || L.ks != kKStep8))
@@ -1443,13 +1479,14 @@ int mi_launch(mcpg_ctx* h, int precision, const WeightBufs& wb, const Layout& L,
     MI_ATTR(kBF16X3, 2, true, false);
     MI_ATTR(kBF16X3, 2, true, true);
 #undef MI_ATTR
-#define MI8_ATTR(N, E)                                                                                        \
-  MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_i8_kernel<N, E>),                \
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, N * 4 * kPlaneB16))
-    MI8_ATTR(2, false);
-    MI8_ATTR(2, true);
-    MI8_ATTR(3, false);
-    MI8_ATTR(3, true);
+#define MI8_ATTR(N, E, R)                                                                                     \
+  MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi_i8_kernel<N, E, R>),             \
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, mi8_smem_bytes(N)))
+    MI8_ATTR(2, false, true);
+    MI8_ATTR(2, true, true);
+    MI8_ATTR(3, false, true);
+    MI8_ATTR(3, true, true);
+    MI8_ATTR(2, false, false);
 #undef MI8_ATTR
     lds_attr_set = true;
   }
@@ -1458,12 +1495,13 @@ int mi_launch(mcpg_ctx* h, int precision, const WeightBufs& wb, const Layout& L,
     // issued together; 2, 3 = the same with three stages (120 KB, one workgroup per CU)
     const int v = mcpg::env_group("MCPG_MI8", 0);
     mcpg::LaunchScope ls(h, "bspline_mi_i8_kernel");
-#define MI8_GO(N, E)                                                                                           \
-  hipLaunchKernelGGL((bspline_mi_i8_kernel<N, E>), dim3(grid), dim3(kThreads), N * 4 * kPlaneB16, h->stream, a)
-    if (v == 1) MI8_GO(2, true);
-    else if (v == 2) MI8_GO(3, false);
-    else if (v == 3) MI8_GO(3, true);
-    else MI8_GO(2, false);
+#define MI8_GO(N, E, R)                                                                                           \
+  hipLaunchKernelGGL((bspline_mi_i8_kernel<N, E, R>), dim3(grid), dim3(kThreads), mi8_smem_bytes(N), h->stream, a)
+    if (!re8) MI8_GO(2, false, false);
+    else if (v == 1) MI8_GO(2, true, true);
+    else if (v == 2) MI8_GO(3, false, true);
+    else if (v == 3) MI8_GO(3, true, true);
+    else MI8_GO(2, false, true);
 #undef MI8_GO
   } else {
     const char* name = precision == MCPG_FP32 ? "bspline_mi_kernel<fp32>"
@@ -1495,12 +1533,15 @@ bool two_orders(int precision, const Layout& L) {
          mcpg::env_group("MCPG_MI_RE", 1) != 0;
 }
 
+// int8 digit planes: two row orders and the register epilogue for 10 bins, else one row order
+bool two_orders_i8(const Layout& L) { return L.bins == 10 && L.gw == 8; }
+
 void size_weight_bufs(mcpg::Sizer& sz, const Layout& L, int precision) {
   const size_t n = (size_t)L.rows * L.Sp;
   const int copies = two_orders(precision, L) ? 2 : 1;
   if (precision == MCPG_FP32) sz.add<float>(n);
   else if (precision == MCPG_I8X3)
-    for (int q = 0; q < 4; ++q) sz.add<int8_t>(n);
+    for (int q = 0; q < (two_orders_i8(L) ? 4 : 2); ++q) sz.add<int8_t>(n);
   else
     for (int c = 0; c < copies; ++c) { sz.add<uint16_t>(n); if (precision == MCPG_BF16X3) sz.add<uint16_t>(n); }
   sz.add<double>(L.ntiles * L.gt);
@@ -1511,7 +1552,7 @@ WeightBufs take_weight_bufs(mcpg::Carver& cv, const Layout& L, int precision) {
   const size_t n = (size_t)L.rows * L.Sp;
   if (precision == MCPG_FP32) w.w32 = cv.take<float>(n);
   else if (precision == MCPG_I8X3) {
-    for (int q = 0; q < 4; ++q) w.q8[q] = cv.take<int8_t>(n);
+    for (int q = 0; q < (two_orders_i8(L) ? 4 : 2); ++q) w.q8[q] = cv.take<int8_t>(n);
   } else {
     w.whi = cv.take<uint16_t>(n);
     if (precision == MCPG_BF16X3) w.wlo = cv.take<uint16_t>(n);
@@ -1552,11 +1593,10 @@ Layout mi_layout(int64_t G, int64_t S, int bins, int precision) {
   return make_layout(G, S, bins, mi_groups_per_tile(precision), precision == MCPG_I8X3 ? kKStep8 : kKStep);
 }
 
-// I8X3 runs on the two-row-order planes of 10 bins only, and its int32 sums stay exact for
-// padded sample counts with 2 * 127^2 * Sp < 2^31 (the largest 64-multiple is 66560).
+// I8X3's int32 sums stay exact for padded sample counts with 2 * 127^2 * Sp < 2^31 (the largest
+// 64-multiple is 66560).
 int check_i8x3(mcpg_ctx* h, int precision, const Layout& L) {
   if (precision != MCPG_I8X3) return MCPG_OK;
-  if (L.bins != 10 || L.gw != 8) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "i8x3 precision needs 10 bins");
   if (L.Sp > 66560) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "i8x3 precision supports at most 66560 samples");
   return MCPG_OK;
 }
@@ -1608,8 +1648,8 @@ int bspline_mi_run(mcpg_ctx* h, int ptrs, const float* X, int64_t G, int64_t S, 
     blk.ntiles = (int64_t)h->block_order.size();
   }
   WeightBufs wb = take_weight_bufs(cv, L, precision);
-  // the prep kernel writes every element of the two-order and int8 planes, padding included
-  if (!wb.whi_c && !wb.q8[0] && (st = clear_weight_bufs(h, wb, L))) return st;
+  // the prep kernel writes every element of the two-order planes (BF16 and int8), padding included
+  if (!wb.whi_c && !wb.q8[2] && (st = clear_weight_bufs(h, wb, L))) return st;
   if ((st = mcpg::reset_flags(h))) return st;
   PrepOut o{};
   o.w32 = wb.w32; o.whi = wb.whi; o.wlo = wb.wlo; o.whi_c = wb.whi_c; o.wlo_c = wb.wlo_c;

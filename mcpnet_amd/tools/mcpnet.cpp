@@ -228,7 +228,7 @@ int main(int argc, char** argv) {
   p.option("--clamped", "output is clamped (no effect, as in the reference)", false);
   p.option("--diagonal", "MI diagonal value; negative keeps the MI diagonal", true, "0");
   p.option("--format", "output format: h5 (as the reference), exp, csv or tsv", true, "h5");
-  p.option("--precision-code", "B-spline contraction: 0 fp32, 1 bf16, 2 bf16x3, 3 i8x3 (10 bins)", true, "0");
+  p.option("--precision-code", "B-spline contraction: 0 fp32, 1 bf16, 2 bf16x3, 3 i8x3", true, "0");
   p.option("--noise-seed", "seed of the U[0,1e-5) rank jitter", true, "1296388676612");
   p.option("--tf-input", "transcription factor file, 1 per line.", true, "");
   try {

@@ -90,7 +90,7 @@ int main(int argc, char** argv) {
   p.option("-b,--bins", "No. of bins", true, "10");
   p.option("-l,--spline_order", "Spline Order", true, "3");
   p.option("-m,--method", "Algorithm: B-spline MI=0, Adaptive partitioning=1", true, "1");
-  p.option("--precision", "B-spline contraction arithmetic: fp32, bf16, bf16x3 or i8x3 (10 bins)", true, "fp32");
+  p.option("--precision", "B-spline contraction arithmetic: fp32, bf16, bf16x3 or i8x3", true, "fp32");
   p.option("--noise-seed", "seed of the U[0,1e-5) rank jitter", true, "1296388676612");
   try {
     p.parse(argc, argv);

@@ -66,7 +66,7 @@ def test_ap_mi_blocks_equal_whole_matrix(handle):
         assert same_bits(gd.cpu().numpy(), whole[r0:r0 + nr, c0:c0 + nc]), ("device", r0, nr, c0, nc)
 
 
-@pytest.mark.parametrize("bins,precision", [(7, "fp32"), (7, "bf16x3"), (16, "bf16")])
+@pytest.mark.parametrize("bins,precision", [(7, "fp32"), (7, "bf16x3"), (16, "bf16"), (7, "i8x3")])
 def test_bspline_mi_blocks_one_row_order(handle, bins, precision):
     # bins other than 10 take the one-row-order kernels and their LDS epilogue
     X = grn.grn_expression(70, 150, 31)[0]

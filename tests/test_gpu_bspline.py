@@ -169,14 +169,14 @@ def test_mi_20000_genes_2000_samples_sampled_pairs(handle, torch_cuda, precision
     assert err.max() <= 1e-4
 
 
-def i8x3_emulation(X):
+def i8x3_emulation(X, bins=10, order=3):
     """MI of the int8 two-digit split, emulated exactly: the FP32 weights of the oracle, the digits
     h = rint(127 w), l = rint(254 (127 w - h)) clipped to [-127, 127] from the same float32
     operations, the integer sums HH = sum h_i h_j and X = sum h_i l_j + l_i h_j in int64, and
     p = (HH + X / 254) / (127^2 S) with the entropy terms in FP64."""
     G, S = X.shape
-    Wf = oracle.bspline_weights_f32(oracle.minmax_scale_f32(X))
-    W = Wf[:, :-1].reshape(G * 10, S)
+    Wf = oracle.bspline_weights_f32(oracle.minmax_scale_f32(X), bins=bins, order=order)
+    W = Wf[:, :-1].reshape(G * bins, S)
     w127 = W * np.float32(127.0)
     h = np.rint(w127)
     l = np.clip(np.rint((w127 - h) * np.float32(254.0)), -127, 127)
@@ -184,9 +184,9 @@ def i8x3_emulation(X):
     HH = h64 @ h64.T
     XL = h64 @ l64.T
     J = (HH.astype(np.float64) + (XL + XL.T).astype(np.float64) / 254.0) / (16129.0 * S)
-    P = J.reshape(G, 10, G, 10).transpose(0, 2, 1, 3)
+    P = J.reshape(G, bins, G, bins).transpose(0, 2, 1, 3)
     t = np.where(P > 0, P * np.log2(np.where(P > 0, P, 1.0)), 0.0).sum(axis=(2, 3))
-    p = W.reshape(G, 10, S).astype(np.float64).sum(axis=2) / S
+    p = W.reshape(G, bins, S).astype(np.float64).sum(axis=2) / S
     H = -np.where(p > 0, p * np.log2(np.where(p > 0, p, 1.0)), 0.0).sum(axis=1)
     return H[:, None] + H[None, :] + t
 
@@ -204,13 +204,27 @@ def test_i8x3_matches_digit_emulation(handle, G, S):
     assert np.array_equal(got, got.T)
 
 
+@pytest.mark.parametrize("bins,order,G,S", [(7, 3, 70, 300), (16, 4, 50, 200), (5, 2, 90, 257)])
+def test_i8x3_one_row_order_matches_digit_emulation(handle, bins, order, G, S):
+    """Bin counts other than 10 take the int8 kernel with one row order and the LDS epilogue:
+    against the exact emulation of the split (5e-6: the FP32 sum of up to 256 entropy terms;
+    a dropped or mis-scaled digit product costs more than 1e-4), and through imported
+    reference-layout weights.  The split's error grows with the bin count (more bins, smaller
+    weights per bin): against FP64 it stays within 2e-4 here (16 bins of order 4: 0.9-1.4e-4 at
+    100-1000 samples; 5 and 7 bins: below 6e-5)."""
+    X = gen(G, S, seed=G + S + bins)
+    got = handle.bspline_mi(X, bins=bins, order=order, precision="i8x3").astype(np.float64)
+    assert np.abs(got - i8x3_emulation(X, bins, order)).max() <= 5e-6
+    assert np.abs(got - oracle.bspline_mi_f64(X, bins, order)).max() <= 2e-4
+    W = handle.bspline_weights(handle.minmax_scale(X), bins=bins, order=order)
+    imp = handle.bspline_mi_weights(W, S=S, bins=bins, precision="i8x3").astype(np.float64)
+    assert np.abs(imp - got).max() <= 2e-6   # the reference layout stores H as float
+
+
 def test_i8x3_limits(handle):
-    """I8x3 needs 10 bins; imported weights outside [0, 1] are refused."""
+    """Imported weights outside [0, 1] are refused by the int8 split."""
     import mcpnet_amd
     X = gen(20, 100)
-    with pytest.raises(mcpnet_amd.MCPGError) as e:
-        handle.bspline_mi(X, bins=7, order=3, precision="i8x3")
-    assert e.value.status == -7
     W = oracle.bspline_weights_f32(oracle.minmax_scale_f32(X))
     W[3, 17] = 1.5
     with pytest.raises(mcpnet_amd.MCPGError) as e:
