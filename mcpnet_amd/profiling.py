@@ -107,6 +107,10 @@ def _short(name: str) -> str:
                 return key + "<f32>"
             if key == "bspline_mi_kernel":
                 return key + {"0": "<fp32>", "1": "<bf16>", "2": "<bf16x3>"}.get(tmpl[1:2], "")
+            if key == "bspline_prep_kernel":   # <NB, NO, TWO, I8>: BF16 two-order or int8 planes
+                flags = tmpl.split("(")[0]
+                return key + ("<int8>" if flags.rstrip(">").endswith("true") else
+                              "<bf16 two orders>" if "true" in flags else "")
             return key
     if name.startswith("void at::") or name.startswith("at::"):
         return name[:60]
