@@ -240,16 +240,17 @@ struct PrepOut {
 // One workgroup per gene: min-max scaling (MinMaxScale, BSplineMI.hpp:81-115), basis weights
 // (BSplineWeightsKernel::operator(), :248-262) and the FP64 marginal entropy (entropy1d_2, :204-224).
 // TWO: BF16 planes in the row orders R and C (o.whi_c, 10 bins) instead of Layout::row_of.
-// I8: the int8 digit planes of the I8X3 kernel (o.q8h ..., row orders R and C, 10 bins) instead.
-// TWO and I8 launches cover every gene slot of the planes (L.ntiles * L.gt) and every sample up to
-// Sp: padding slots and samples are written as zeros, so those planes need no clearing.
+// I8: the int8 digit planes of the I8X3 kernel (o.q8h, o.q8l; with TWO also o.q8h_c, o.q8l_c in
+// the two row orders) instead of BF16 planes.  TWO launches cover every gene slot of the planes
+// (L.ntiles * L.gt) and every sample up to Sp: padding slots and samples are written as zeros, so
+// those planes need no clearing.
 template <int NB, int NO, bool TWO, bool I8>
 __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __restrict__ X, int64_t ldx, Layout L,
                                                                 Spline sp, int do_scale, PrepOut o) {
   const int64_t g = blockIdx.x;
   const int tid = threadIdx.x;
   const int64_t S = L.S;
-  if constexpr (TWO || I8) {
+  if constexpr (TWO) {
     if (g >= L.G) {   // a padding slot of the last gene tile
       const int gl = (int)(g % 8);
       const uint32_t grp_base = (uint32_t)((g / 8) * (L.Sp / L.ks));
@@ -334,15 +335,15 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
 #pragma unroll
   for (int b = 0; b < NB; ++b) acc[b] = 0.0;
   float w[NB];
-  // TWO: element (row, s) of a plane is sbase(s) + 32 * (row within the gene's 80-row group),
+  // TWO: element (row, s) of a plane is sbase(s) + ks * (row within the gene's 80-row group),
   // and that row depends on b and on the gene's place in its group only (Layout::row_of_r,
   // row_of_c); 32-bit offsets, as the planes are kept under 4 GiB.
   const int gl = (int)(g % 8);
   const uint32_t grp_base = (uint32_t)((g / 8) * (L.Sp / L.ks));
-  // TWO / I8: samples S .. Sp - 1 get zero weights
-  const int64_t s_end = (TWO || I8) ? L.Sp : S;
+  // TWO: samples S .. Sp - 1 get zero weights
+  const int64_t s_end = TWO ? L.Sp : S;
   for (int64_t s = tid; s < s_end; s += kThreads) {
-    if ((TWO || I8) && s >= S) {
+    if (TWO && s >= S) {
 #pragma unroll
       for (int b = 0; b < NB; ++b) w[b] = 0.0f;
     } else {
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
           const float w127 = w[b] * 127.0f;
           const float hq = rintf(w127);
           const float lq = fminf(fmaxf(rintf((w127 - hq) * 254.0f), -127.0f), 127.0f);
-          if (o.q8h_c) {   // two row orders (10 bins)
+          if constexpr (TWO) {   // two row orders (10 bins)
             const int m = 10 * (gl & 1) + b;
             const uint32_t er = sbase + (uint32_t)(16 * (m >> 2) + 4 * (gl >> 1) + (m & 3)) * kKStep8;
             const uint32_t ec = sbase + (uint32_t)(16 * (b % 5) + gl + 8 * (b / 5)) * kKStep8;
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(kThreads) void bspline_prep_kernel(const float* __r
             o.q8l[er] = (int8_t)lq;
             o.q8h_c[ec] = (int8_t)hq;
             o.q8l_c[ec] = (int8_t)lq;
-          } else {         // one row order (Layout::row_of)
+          } else {               // one row order (Layout::row_of)
             const int64_t e = L.at(L.row_of(g, b), s);
             o.q8h[e] = (int8_t)hq;
             o.q8l[e] = (int8_t)lq;
@@ -1371,7 +1372,10 @@ int prep_launch(mcpg_ctx* h, const float* dX, int64_t ldx, const Layout& L, cons
   hipLaunchKernelGGL((bspline_prep_kernel<NB, NO, TWO, I8>), dim3(grid), dim3(kThreads), 0, h->stream, dX, \
                      ldx, L, sp, do_scale, o)
   const bool small = sp.bins <= 16 && sp.order <= 4;
-  if (o.q8h) {
+  if (o.q8h && o.q8h_c) {
+    if (small) PREP_GO(16, 4, true, true);
+    else PREP_GO(kMaxBins, kMaxOrder, true, true);
+  } else if (o.q8h) {
     if (small) PREP_GO(16, 4, false, true);
     else PREP_GO(kMaxBins, kMaxOrder, false, true);
   } else if (o.whi_c) {
