@@ -53,7 +53,7 @@ struct mcpg_ctx {
   // max-min key kernel: key blocks of the split tail tiles, and resident workgroups per device
   int* mm_tail = nullptr;
   size_t mm_tail_bytes = 0;
-  int mm_slots[2] = {0, 0};   // K chunk 16, 32
+  int mm_slots[4] = {0, 0, 0, 0};   // K chunk 16, 32; plain, then widened outputs
   int mm64_slots[2] = {0, 0}; // FP64 products, K chunk 8, 16 doubles
   // device address of the static scratch of the allocation-free entry point on this handle's
   // device (set by mcpg_create), and the error if it could not be taken

@@ -265,6 +265,46 @@ __device__ __forceinline__ int f32_key(float x) {
   return b ^ ((b >> 31) & 0x7fffffff);
 }
 __device__ __forceinline__ float key_f32(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7fffffff)); }
+// A key product widened to the FP64 kernel's value.  Its running maximum starts at
+// lowest<double>() (simd_maxmin.hpp:32) where the key kernel's starts at -inf; with finite
+// operands a result of -inf means that every K term was masked by a mask value of -inf or
+// lowest<double>(), where the FP64 product gives lowest<double>().
+__device__ __forceinline__ double key_f64(int k) {
+  const float v = key_f32(k);
+  return v == -INFINITY ? -DBL_MAX : (double)v;
+}
+
+// Keys of an FP64 operand, with flags[0] |= 1 unless every entry is a finite double that a float
+// holds exactly and is not -0 (the one value on which the FP32 and FP64 kernels break ties
+// differently).  A K-mask value above FLT_MAX acts as +inf and one of -inf or lowest<double>() as
+// -inf (exact against finite operands: the first never wins a min, the second loses every max
+// to an unmasked term); any other mask value must be held exactly too.
+__global__ void key_prep_f64_kernel(const double* __restrict__ X, int64_t ldx, int64_t rows, int64_t K,
+                                    const double* __restrict__ kmask, int* __restrict__ out, int64_t ldk,
+                                    int64_t rows_pad, int* __restrict__ flags) {
+  const int64_t total = rows_pad * ldk;
+  bool bad = false;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = t / ldk, k = t % ldk;
+    int v = INT_MIN;
+    if (r < rows && k < K) {
+      const double x = X[r * ldx + k];
+      const float f = (float)x;
+      bad |= !((double)f == x) || isinf(f) || (x == 0.0 && signbit(x));
+      v = f32_key(f);
+      if (kmask) {
+        const double m = kmask[k];
+        float fm = (float)m;
+        if (m > (double)FLT_MAX) fm = INFINITY;
+        else if (m == -INFINITY || m == -DBL_MAX) fm = -INFINITY;
+        else bad |= !((double)fm == m) || isinf(fm) || (m == 0.0 && signbit(m));
+        v = min(v, f32_key(fm));
+      }
+    }
+    out[t] = v;
+  }
+  if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(flags, 1);
+}
 
 __global__ void key_prep_kernel(const float* __restrict__ X, int64_t ldx, int64_t rows, int64_t K,
                                 const float* __restrict__ kmask, int* __restrict__ out, int64_t ldk, int64_t rows_pad) {
@@ -313,6 +353,11 @@ struct MKArgs {
   int* Ck; int64_t ldck;       // key output (nullable)
   const float* R; int64_t ldr; // ratio numerator (nullable: no fused ratio)
   float* O; int64_t ldo;       // ratio output
+  // double forms of C, R and O (same leading dimensions; set instead of C / R / O): products of
+  // FP64 operands that every float holds exactly, widened, with the ratio taken in double
+  double* Cd;
+  const double* Rd;
+  double* Od;
   int tri, shared;
   int64_t nbm, nbn, ntiles;
   const uint32_t* order;       // tri mode: tile coordinates in launch order, (bm << 16) | bn
@@ -387,7 +432,9 @@ __device__ __forceinline__ void key_dma_issue(__amdgpu_buffer_rsrc_t rsrc, const
   }
 }
 
-template <int KC>
+// WIDE: the double outputs Cd / Od in place of C / O.  A separate instance: both epilogues in one
+// took 182 VGPRs against 166, and the product one wave per SIMD fewer (2 against 3).
+template <int KC, bool WIDE>
 __global__ __launch_bounds__(kThreads) void maxmin_key_kernel(MKArgs a) {
   using KCh = KeyChunk<KC>;
   extern __shared__ __align__(16) char smem[];
@@ -489,7 +536,7 @@ __global__ __launch_bounds__(kThreads) void maxmin_key_kernel(MKArgs a) {
       const int64_t j = col0 + tx + 16 * n;
       if (j >= a.N) continue;
       const float v = key_f32(acc[m][n]);
-      if (a.C) {
+      if (!WIDE && a.C) {
         a.C[i * a.ldc + j] = v;
         if (mirror) a.C[j * a.ldc + i] = v;
       }
@@ -497,9 +544,20 @@ __global__ __launch_bounds__(kThreads) void maxmin_key_kernel(MKArgs a) {
         a.Ck[i * a.ldck + j] = acc[m][n];
         if (mirror) a.Ck[j * a.ldck + i] = acc[m][n];
       }
-      if (a.O) {
+      if (!WIDE && a.O) {
         a.O[i * a.ldo + j] = ratio_of<float>(a.R[i * a.ldr + j], v);
         if (mirror) a.O[j * a.ldo + i] = ratio_of<float>(a.R[j * a.ldr + i], v);
+      }
+      if (WIDE) {
+        const double w = key_f64(acc[m][n]);
+        if (a.Cd) {
+          a.Cd[i * a.ldc + j] = w;
+          if (mirror) a.Cd[j * a.ldc + i] = w;
+        }
+        if (a.Od) {
+          a.Od[i * a.ldo + j] = ratio_of<double>(a.Rd[i * a.ldr + j], w);
+          if (mirror) a.Od[j * a.ldo + i] = ratio_of<double>(a.Rd[j * a.ldr + i], w);
+        }
       }
     }
   }
@@ -530,6 +588,17 @@ __global__ void mm_finish_tail_kernel(MKArgs a) {
     if (a.O) {
       a.O[i * a.ldo + j] = ratio_of<float>(a.R[i * a.ldr + j], v);
       if (mirror) a.O[j * a.ldo + i] = ratio_of<float>(a.R[j * a.ldr + i], v);
+    }
+    if (a.Cd || a.Od) {
+      const double w = key_f64(k);
+      if (a.Cd) {
+        a.Cd[i * a.ldc + j] = w;
+        if (mirror) a.Cd[j * a.ldc + i] = w;
+      }
+      if (a.Od) {
+        a.Od[i * a.ldo + j] = ratio_of<double>(a.Rd[i * a.ldr + j], w);
+        if (mirror) a.Od[j * a.ldo + i] = ratio_of<double>(a.Rd[j * a.ldr + i], w);
+      }
     }
   }
 }
@@ -960,9 +1029,10 @@ KeyMat key_geometry(int64_t rows, int64_t K) {
   return km;
 }
 
+bool key_fits(const KeyMat& km) { return (uint64_t)km.rows_pad * (uint64_t)km.ldk * 4 < (1ull << 32); }
+
 int key_check(mcpg_ctx* h, const KeyMat& km) {
-  if ((uint64_t)km.rows_pad * (uint64_t)km.ldk * 4 >= (1ull << 32))
-    return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min operand exceeds 4 GiB of keys");
+  if (!key_fits(km)) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min operand exceeds 4 GiB of keys");
   return MCPG_OK;
 }
 
@@ -1007,16 +1077,32 @@ __global__ void tri_order_kernel(uint32_t* __restrict__ out, int64_t nb, int gro
         for (int64_t tj = bj > ti ? bj : ti; tj < bj + group && tj < nb; ++tj) out[n++] = (uint32_t)((ti << 16) | tj);
 }
 
-// C / Ck / O over [0, M) x [0, N) of A (x) B, with K = A.ldk (== B.ldk).  With `sc`, the tile
-// order and the tail blocks live in the caller's buffers and nothing is allocated.
+// FP64 outputs of a key product (see MKArgs::Cd): the max-min matrix and/or the ratio R / C
+struct WideOut {
+  double* C = nullptr;
+  int64_t ldc = 0;
+  const double* R = nullptr;
+  int64_t ldr = 0;
+  double* O = nullptr;
+  int64_t ldo = 0;
+};
+
+// C / Ck / O over [0, M) x [0, N) of A (x) B, with K = A.ldk (== B.ldk); with `wide`, its double
+// outputs in place of C and O.  With `sc`, the tile order and the tail blocks live in the
+// caller's buffers and nothing is allocated.
 int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M, int64_t N, int tri, float* C,
                        int64_t ldc, int* Ck, int64_t ldck, const float* R, int64_t ldr, float* O, int64_t ldo,
-                       const MMScratch* sc = nullptr) {
+                       const MMScratch* sc = nullptr, const WideOut* wide = nullptr) {
   if (M <= 0 || N <= 0) return MCPG_OK;
   MKArgs a;
   a.A = A.d; a.lda = A.ldk; a.B = B.d; a.ldb = B.ldk; a.a_bytes = A.bytes(); a.b_bytes = B.bytes();
   a.M = M; a.N = N; a.Kp = A.ldk;
   a.C = C; a.ldc = ldc; a.Ck = Ck; a.ldck = ldck; a.R = R; a.ldr = ldr; a.O = O; a.ldo = ldo;
+  a.Cd = nullptr; a.Rd = nullptr; a.Od = nullptr;
+  if (wide) {
+    if (C || O) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "a key product writes float or double outputs, not both");
+    a.Cd = wide->C; a.ldc = wide->ldc; a.Rd = wide->R; a.ldr = wide->ldr; a.Od = wide->O; a.ldo = wide->ldo;
+  }
   a.tri = tri;
   a.shared = A.d == B.d ? 1 : 0;
   a.nbm = ceil_div(M, kBM);
@@ -1037,22 +1123,25 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
     if (!a.order) return st;
   }
   const int kc = mcpg::env_group("MCPG_MM_KC", 16) == 32 ? 32 : 16;
-  const void* kfn = kc == 16 ? reinterpret_cast<const void*>(&maxmin_key_kernel<16>)
-                             : reinterpret_cast<const void*>(&maxmin_key_kernel<32>);
+  const void* kfns[2][2] = {{reinterpret_cast<const void*>(&maxmin_key_kernel<16, false>),
+                             reinterpret_cast<const void*>(&maxmin_key_kernel<16, true>)},
+                            {reinterpret_cast<const void*>(&maxmin_key_kernel<32, false>),
+                             reinterpret_cast<const void*>(&maxmin_key_kernel<32, true>)}};
+  const void* kfn = kfns[kc == 16 ? 0 : 1][wide ? 1 : 0];
   const int smem = 2 * (kc == 16 ? KeyChunk<16>::kStageBytes : KeyChunk<32>::kStageBytes);
   static bool attr_set = false;
   if (!attr_set) {
-    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&maxmin_key_kernel<16>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 2 * KeyChunk<16>::kStageBytes));
-    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&maxmin_key_kernel<32>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 2 * KeyChunk<32>::kStageBytes));
+    for (int c = 0; c < 2; ++c)
+      for (int w = 0; w < 2; ++w)
+        MCPG_HIP(h, hipFuncSetAttribute(kfns[c][w], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        2 * (c == 0 ? KeyChunk<16>::kStageBytes : KeyChunk<32>::kStageBytes)));
     attr_set = true;
   }
   // Split tail: the tiles beyond the last full round of resident workgroups (all tiles, for a
   // grid smaller than one round) would leave most of the GPU idle for a whole tile time; they
   // run instead as several workgroups each over a range of K, folding exact partial maxima with
   // atomicMax on the keys.
-  int& slots = h->mm_slots[kc == 16 ? 0 : 1];
+  int& slots = h->mm_slots[(kc == 16 ? 0 : 1) + (wide ? 2 : 0)];
   if (slots == 0) {
     int per_cu = 0, cus = 0;
     MCPG_HIP(h, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kThreads, smem));
@@ -1091,11 +1180,11 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
   const int64_t grid = a.nfull + (a.ntiles - a.nfull) * a.splits;
   if (grid > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min grid too large");
   {
-    mcpg::LaunchScope ls(h, tri ? "maxmin_key_kernel<f32,tri>" : "maxmin_key_kernel<f32>");
-    if (kc == 16)
-      hipLaunchKernelGGL(maxmin_key_kernel<16>, dim3((unsigned)grid), dim3(kThreads), smem, h->stream, a);
-    else
-      hipLaunchKernelGGL(maxmin_key_kernel<32>, dim3((unsigned)grid), dim3(kThreads), smem, h->stream, a);
+    mcpg::LaunchScope ls(h, wide ? (tri ? "maxmin_key_kernel<f64 out,tri>" : "maxmin_key_kernel<f64 out>")
+                                 : (tri ? "maxmin_key_kernel<f32,tri>" : "maxmin_key_kernel<f32>"));
+    void* args[] = {&a};
+    const hipError_t le = hipLaunchKernel(kfn, dim3((unsigned)grid), dim3(kThreads), args, smem, h->stream);
+    if (le != hipSuccess) return mcpg::hip_error(h, le, "maxmin_key_kernel launch");
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return mcpg::hip_error(h, e, "maxmin_key_kernel launch");
@@ -1141,6 +1230,44 @@ int run_keys(mcpg_ctx* h, const KeyPlan& p, const float* A, int64_t lda, const f
   if ((st = prep_keys(h, A, lda, K, kmask, p.ka))) return st;
   if (!p.shareB && (st = prep_keys(h, B, ldb, K, nullptr, p.kb))) return st;
   return launch_maxmin_keys(h, p.ka, p.kb, p.ka.rows, p.kb.rows, symmetric, C, ldc, nullptr, 0, nullptr, 0, nullptr, 0);
+}
+
+// ---- FP64 operands on the key kernel
+// Max and min only select entries, so a product of FP64 operands whose entries a float holds
+// exactly selects the same entries as the key product of their float images (key_prep_f64_kernel
+// states the conditions); the key epilogue widens the result and takes the ratio in double.  The
+// key kernel issues 1.5 VALU instructions per triple against the FP64 kernel's 2 (FP64 rate),
+// and reads 4-byte keys.  MCPG_MM64_KEYS=0 keeps FP64 operands on the FP64 kernel.
+bool f64_keys_enabled() { return mcpg::env_group("MCPG_MM64_KEYS", 1) != 0; }
+
+int prep_keys(mcpg_ctx* h, const double* X, int64_t ldx, int64_t K, const double* kmask, const KeyMat& km) {
+  mcpg::LaunchScope ls(h, "key_prep_f64_kernel");
+  hipLaunchKernelGGL(key_prep_f64_kernel, dim3(grid_of(km.rows_pad * km.ldk)), dim3(256), 0, h->stream, X, ldx, km.rows,
+                     K, kmask, km.d, km.ldk, km.rows_pad, h->d_flags);
+  return MCPG_OK;
+}
+
+// after the FP64 operands' prep_keys calls (flags reset before them): whether every entry passed
+int keys_exact(mcpg_ctx* h, bool* exact) {
+  int flags[1];
+  const int st = mcpg::read_flags(h, flags, 1);
+  *exact = st == MCPG_OK && (flags[0] & 1) == 0;
+  return st;
+}
+
+// One key product into outputs of the operands' type (see launch_maxmin_keys).
+int key_product(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M, int64_t N, int tri, float* C, int64_t ldc,
+                int* Ck, int64_t ldck, const float* R, int64_t ldr, float* O, int64_t ldo,
+                const MMScratch* sc = nullptr) {
+  return launch_maxmin_keys(h, A, B, M, N, tri, C, ldc, Ck, ldck, R, ldr, O, ldo, sc);
+}
+int key_product(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M, int64_t N, int tri, double* C, int64_t ldc,
+                int* Ck, int64_t ldck, const double* R, int64_t ldr, double* O, int64_t ldo,
+                const MMScratch* sc = nullptr) {
+  if (!C && !O) return launch_maxmin_keys(h, A, B, M, N, tri, nullptr, 0, Ck, ldck, nullptr, 0, nullptr, 0, sc);
+  WideOut w;
+  w.C = C; w.ldc = ldc; w.R = R; w.ldr = ldr; w.O = O; w.ldo = ldo;
+  return launch_maxmin_keys(h, A, B, M, N, tri, nullptr, 0, Ck, ldck, nullptr, 0, nullptr, 0, sc, &w);
 }
 
 // ---------------------------------------------------------------- FP64 padded-value path, host side
@@ -1306,54 +1433,65 @@ int run_vals(mcpg_ctx* h, const ValPlan& p, const double* A, int64_t lda, const 
 
 // compute_maxmins (apps/pipeline/mcpnet_helper.hpp:436-460) on FP32 keys: M1 = A (x) A,
 // M2 = A (x) M1, M3 = M1 (x) M1, each written only when its output pointer is set.
-int maxmins_f32(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N, float* M1, int64_t ld1, float* M2,
-                int64_t ld2, float* M3, int64_t ld3) {
+// M1 = A (x) A, M2 = A (x) M1, M3 = M1 (x) M1 on keys.  FP64 operands: *done stays false when A
+// fails the key test, and the caller runs the FP64 kernel.
+template <typename T>
+int maxmins_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, T* M1, int64_t ld1, T* M2, int64_t ld2,
+                 T* M3, int64_t ld3, bool* done = nullptr) {
+  constexpr bool f64 = std::is_same<T, double>::value;
   const size_t NN = (size_t)N * N;
   KeyMat ka = key_geometry(N, N), k1 = key_geometry(N, N);
   int st;
+  if (f64 && !key_fits(ka)) return MCPG_OK;   // the FP64 kernel decides
   if ((st = key_check(h, ka))) return st;
   mcpg::Sizer sz;
-  if (host) { sz.add<float>(NN); for (float* p : {M1, M2, M3}) if (p) sz.add<float>(NN); }
+  if (host) { sz.add<T>(NN); for (T* p : {M1, M2, M3}) if (p) sz.add<T>(NN); }
   sz.add<int>(ka.rows_pad * ka.ldk);
   sz.add<int>(k1.rows_pad * k1.ldk);
   void* base = mcpg::workspace(h, sz.off, &st);
   if (!base) return st;
   mcpg::Carver cv(base);
-  const float* dA = A;
+  const T* dA = A;
   int64_t la = lda;
-  float* dM[3] = {M1, M2, M3};
+  T* dM[3] = {M1, M2, M3};
   int64_t ldM[3] = {ld1, ld2, ld3};
   if (host) {
-    float* a = cv.take<float>(NN);
-    if ((st = mcpg::copy_2d(h, a, N * 4, A, lda * 4, N * 4, N, hipMemcpyHostToDevice))) return st;
+    T* a = cv.take<T>(NN);
+    if ((st = mcpg::copy_2d(h, a, N * sizeof(T), A, lda * sizeof(T), N * sizeof(T), N, hipMemcpyHostToDevice))) return st;
     dA = a; la = N;
     for (int q = 0; q < 3; ++q)
-      if (dM[q]) { dM[q] = cv.take<float>(NN); ldM[q] = N; }
+      if (dM[q]) { dM[q] = cv.take<T>(NN); ldM[q] = N; }
   }
   ka.d = cv.take<int>(ka.rows_pad * ka.ldk);
   k1.d = cv.take<int>(k1.rows_pad * k1.ldk);
+  if (f64 && (st = mcpg::reset_flags(h))) return st;
   if ((st = prep_keys(h, dA, la, N, nullptr, ka))) return st;
+  if constexpr (f64) {
+    bool exact;
+    if ((st = keys_exact(h, &exact)) || !exact) return st;
+  }
   if ((st = pad_keys(h, k1, N))) return st;
-  if ((st = launch_maxmin_keys(h, ka, ka, N, N, 1, dM[0], ldM[0], k1.d, k1.ldk, nullptr, 0, nullptr, 0))) return st;
+  if ((st = key_product(h, ka, ka, N, N, 1, dM[0], ldM[0], k1.d, k1.ldk, nullptr, 0, nullptr, 0))) return st;
   if (dM[1]) {
     if ((st = mcpg::reset_flags(h))) return st;
     {
       mcpg::LaunchScope ls(h, "symmetry_check_kernel");
-      launch_symmetry_check<float>(h, dA, la, N);
+      launch_symmetry_check<T>(h, dA, la, N);
     }
     int flags[1];
     if ((st = mcpg::read_flags(h, flags, 1))) return st;
-    if ((st = launch_maxmin_keys(h, ka, k1, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0, nullptr, 0, nullptr, 0)))
-      return st;
+    if ((st = key_product(h, ka, k1, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0, nullptr, 0, nullptr, 0))) return st;
   }
-  if (dM[2] && (st = launch_maxmin_keys(h, k1, k1, N, N, 1, dM[2], ldM[2], nullptr, 0, nullptr, 0, nullptr, 0)))
-    return st;
+  if (dM[2] && (st = key_product(h, k1, k1, N, N, 1, dM[2], ldM[2], nullptr, 0, nullptr, 0, nullptr, 0))) return st;
   if (host) {
-    float* out[3] = {M1, M2, M3};
+    T* out[3] = {M1, M2, M3};
     const int64_t ldo[3] = {ld1, ld2, ld3};
     for (int q = 0; q < 3; ++q)
-      if (out[q] && (st = mcpg::copy_2d(h, out[q], ldo[q] * 4, dM[q], N * 4, N * 4, N, hipMemcpyDeviceToHost))) return st;
+      if (out[q] && (st = mcpg::copy_2d(h, out[q], ldo[q] * sizeof(T), dM[q], N * sizeof(T), N * sizeof(T), N,
+                                        hipMemcpyDeviceToHost)))
+        return st;
   }
+  if (done) *done = true;
   return mcpg::finish(h, "mcpg_maxmins");
 }
 
@@ -1598,8 +1736,16 @@ int maxmins_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T* 
     return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
-  if constexpr (std::is_same<T, float>::value) return maxmins_f32(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
-  else return maxmins_f64(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
+  if constexpr (std::is_same<T, float>::value) {
+    return maxmins_keys<float>(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
+  } else {
+    bool done = false;
+    if (f64_keys_enabled()) {
+      const int st = maxmins_keys<double>(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3, &done);
+      if (st || done) return st;
+    }
+    return maxmins_f64(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
+  }
 }
 
 template <typename T>
@@ -1608,6 +1754,52 @@ template <>
 struct TraitsOf<float> { using type = TrF32; };
 template <>
 struct TraitsOf<double> { using type = TrF64; };
+
+// mcpg_maxmin_f64 on the key kernel; *done stays false when an operand or the mask fails the key
+// test (or K is 0), and the caller runs the FP64 kernel.
+int maxmin_f64_keys(mcpg_ctx* h, bool host, const double* A, int64_t lda, const double* B, int64_t ldb,
+                    const double* kmask, int64_t M, int64_t N, int64_t K, int symmetric, double* C, int64_t ldc,
+                    bool* done) {
+  if (K == 0) return MCPG_OK;
+  const bool shareB = (A == B && M == N && lda == ldb);
+  KeyPlan kp = plan_keys(shareB, kmask != nullptr, M, N, K);
+  if (!key_fits(kp.ka) || !key_fits(kp.kb)) return MCPG_OK;
+  mcpg::Sizer sz;
+  if (host) {
+    sz.add<double>(M * K);
+    if (!shareB) sz.add<double>(N * K);
+    if (kmask) sz.add<double>(K);
+    sz.add<double>(M * N);
+  }
+  size_keys(sz, kp);
+  int st;
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const double *a = A, *b = B, *m = kmask;
+  int64_t la = lda, lb = ldb, lc = ldc;
+  double* c = C;
+  if (host) {
+    double* dA = cv.take<double>(M * K);
+    double* dB = shareB ? dA : cv.take<double>(N * K);
+    double* dm = kmask ? cv.take<double>(K) : nullptr;
+    c = cv.take<double>(M * N);
+    if ((st = mcpg::copy_2d(h, dA, K * 8, A, lda * 8, K * 8, M, hipMemcpyHostToDevice))) return st;
+    if (!shareB && (st = mcpg::copy_2d(h, dB, K * 8, B, ldb * 8, K * 8, N, hipMemcpyHostToDevice))) return st;
+    if (kmask) MCPG_HIP(h, hipMemcpyAsync(dm, kmask, K * 8, hipMemcpyHostToDevice, h->stream));
+    a = dA; b = dB; m = dm; la = lb = K; lc = N;
+  }
+  take_keys(cv, kp);
+  if ((st = mcpg::reset_flags(h))) return st;
+  if ((st = prep_keys(h, a, la, K, m, kp.ka))) return st;
+  if (!kp.shareB && (st = prep_keys(h, b, lb, K, nullptr, kp.kb))) return st;
+  bool exact;
+  if ((st = keys_exact(h, &exact)) || !exact) return st;
+  if ((st = key_product(h, kp.ka, kp.kb, M, N, symmetric, c, lc, nullptr, 0, nullptr, 0, nullptr, 0))) return st;
+  if (host && (st = mcpg::copy_2d(h, C, ldc * 8, c, N * 8, N * 8, M, hipMemcpyDeviceToHost))) return st;
+  *done = true;
+  return mcpg::finish(h, "mcpg_maxmin");
+}
 
 template <typename T>
 int maxmin_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* B, int64_t ldb, const T* kmask,
@@ -1625,6 +1817,13 @@ int maxmin_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* B, int
   MCPG_HIP(h, hipSetDevice(h->device));
   constexpr bool f32 = std::is_same<T, float>::value;
   const bool host = ptrs == MCPG_HOST;
+  if constexpr (!f32) {
+    bool done = false;
+    if (f64_keys_enabled()) {
+      const int st = maxmin_f64_keys(h, host, A, lda, B, ldb, kmask, M, N, K, symmetric, C, ldc, &done);
+      if (st || done) return st;
+    }
+  }
   const bool shareB = (A == B && M == N && lda == ldb);
   // FP32 runs on order keys, FP64 on padded values (both sized here with the host staging)
   KeyPlan kp;
@@ -1727,7 +1926,7 @@ int ratio_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, const T* Mm, int
 constexpr size_t kScratchBytes = size_t(64) << 20;
 static __device__ __align__(256) char scratch[kScratchBytes];
 
-// Scratch plan of mcp_f32_keys with device operands: keys of A, keys of M1 (k >= 3), one
+// Scratch plan of mcp_keys<float> with device operands: keys of A, keys of M1 (k >= 3), one
 // triangle tile order; the rest of `cap` bytes (if any) holds split-tail key blocks.
 size_t mcp_scratch_bytes(int64_t N, int k) {
   const KeyMat ka = key_geometry(N, N);
@@ -1743,14 +1942,17 @@ size_t mcp_scratch_bytes(int64_t N, int k) {
 // last product writes the max-min matrix only when asked for and fuses the ratio.  With
 // `scratch_base` (device operands only) every buffer is carved from those `scratch_cap` bytes and
 // nothing is allocated.
-int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N, int k, float* mm_out, int64_t ldm,
-                 float* out, int64_t ldo, void* scratch_base = nullptr, size_t scratch_cap = 0) {
+template <typename T>
+int mcp_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, int k, T* mm_out, int64_t ldm, T* out,
+             int64_t ldo, void* scratch_base = nullptr, size_t scratch_cap = 0, bool* done = nullptr) {
+  constexpr bool f64 = std::is_same<T, double>::value;
   const size_t NN = (size_t)N * N;
   KeyMat ka = key_geometry(N, N), k1 = key_geometry(N, N);
   int st;
+  if (f64 && !key_fits(ka)) return MCPG_OK;   // the FP64 kernel decides
   if ((st = key_check(h, ka))) return st;
   mcpg::Sizer sz;
-  if (host) { sz.add<float>(NN); sz.add<float>(NN); if (mm_out) sz.add<float>(NN); }
+  if (host) { sz.add<T>(NN); sz.add<T>(NN); if (mm_out) sz.add<T>(NN); }
   sz.add<int>(ka.rows_pad * ka.ldk);
   if (k >= 3) sz.add<int>(k1.rows_pad * k1.ldk);
   MMScratch msc;
@@ -1773,28 +1975,33 @@ int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N,
     if (!base) return st;
   }
   mcpg::Carver cv(base);
-  const float* dA = A;
+  const T* dA = A;
   int64_t la = lda;
-  float* dO = out;
+  T* dO = out;
   int64_t lo = ldo;
-  float* dM = mm_out;
+  T* dM = mm_out;
   int64_t lm = ldm;
   if (host) {
-    float* a = cv.take<float>(NN);
-    if ((st = mcpg::copy_2d(h, a, N * 4, A, lda * 4, N * 4, N, hipMemcpyHostToDevice))) return st;
+    T* a = cv.take<T>(NN);
+    if ((st = mcpg::copy_2d(h, a, N * sizeof(T), A, lda * sizeof(T), N * sizeof(T), N, hipMemcpyHostToDevice))) return st;
     dA = a; la = N;
-    dO = cv.take<float>(NN); lo = N;
-    if (mm_out) { dM = cv.take<float>(NN); lm = N; }
+    dO = cv.take<T>(NN); lo = N;
+    if (mm_out) { dM = cv.take<T>(NN); lm = N; }
   }
   ka.d = cv.take<int>(ka.rows_pad * ka.ldk);
+  if (f64 && (st = mcpg::reset_flags(h))) return st;
   if ((st = prep_keys(h, dA, la, N, nullptr, ka))) return st;
+  if constexpr (f64) {
+    bool exact;
+    if ((st = keys_exact(h, &exact)) || !exact) return st;
+  }
   if (k == 2) {
-    if ((st = launch_maxmin_keys(h, ka, ka, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
+    if ((st = key_product(h, ka, ka, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
   } else {
     k1.d = cv.take<int>(k1.rows_pad * k1.ldk);
     if ((st = pad_keys(h, k1, N))) return st;
     // the first product A (x) A is symmetric by construction
-    if ((st = launch_maxmin_keys(h, ka, ka, N, N, 1, nullptr, 0, k1.d, k1.ldk, nullptr, 0, nullptr, 0, sc))) return st;
+    if ((st = key_product(h, ka, ka, N, N, 1, (T*)nullptr, 0, k1.d, k1.ldk, nullptr, 0, nullptr, 0, sc))) return st;
     if (k == 3) {
       // A (x) M1 is exactly symmetric when A is (max/min never round); check A bitwise.  The
       // scratch path clears the flag with a kernel (no memset node for callers that capture).
@@ -1806,20 +2013,24 @@ int mcp_f32_keys(mcpg_ctx* h, bool host, const float* A, int64_t lda, int64_t N,
       }
       {
         mcpg::LaunchScope ls(h, "symmetry_check_kernel");
-        launch_symmetry_check<float>(h, dA, la, N);
+        launch_symmetry_check<T>(h, dA, la, N);
       }
       int flags[1];
       if ((st = mcpg::read_flags(h, flags, 1))) return st;
       const int sym = flags[0] == 0;
-      if ((st = launch_maxmin_keys(h, ka, k1, N, N, sym, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
+      if ((st = key_product(h, ka, k1, N, N, sym, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
     } else {
-      if ((st = launch_maxmin_keys(h, k1, k1, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
+      if ((st = key_product(h, k1, k1, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
     }
   }
   if (host) {
-    if ((st = mcpg::copy_2d(h, out, ldo * 4, dO, N * 4, N * 4, N, hipMemcpyDeviceToHost))) return st;
-    if (mm_out && (st = mcpg::copy_2d(h, mm_out, ldm * 4, dM, N * 4, N * 4, N, hipMemcpyDeviceToHost))) return st;
+    if ((st = mcpg::copy_2d(h, out, ldo * sizeof(T), dO, N * sizeof(T), N * sizeof(T), N, hipMemcpyDeviceToHost)))
+      return st;
+    if (mm_out && (st = mcpg::copy_2d(h, mm_out, ldm * sizeof(T), dM, N * sizeof(T), N * sizeof(T), N,
+                                      hipMemcpyDeviceToHost)))
+      return st;
   }
+  if (done) *done = true;
   return mcpg::finish(h, "mcpg_mcp");
 }
 
@@ -1895,8 +2106,16 @@ int mcp_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, int k, 
   if (lda < N || ldo < N || (mm_out && ldm < N)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
-  if constexpr (std::is_same<T, float>::value) return mcp_f32_keys(h, ptrs == MCPG_HOST, A, lda, N, k, mm_out, ldm, out, ldo);
-  else if (!f64_tile_kernel()) return mcp_f64_vals(h, ptrs == MCPG_HOST, A, lda, N, k, mm_out, ldm, out, ldo);
+  if constexpr (std::is_same<T, float>::value) {
+    return mcp_keys<float>(h, ptrs == MCPG_HOST, A, lda, N, k, mm_out, ldm, out, ldo);
+  } else {
+    bool done = false;
+    if (f64_keys_enabled()) {
+      const int st = mcp_keys<double>(h, ptrs == MCPG_HOST, A, lda, N, k, mm_out, ldm, out, ldo, nullptr, 0, &done);
+      if (st || done) return st;
+    }
+    if (!f64_tile_kernel()) return mcp_f64_vals(h, ptrs == MCPG_HOST, A, lda, N, k, mm_out, ldm, out, ldo);
+  }
   const size_t NN = (size_t)N * N;
   const bool host = ptrs == MCPG_HOST;
   mcpg::Sizer sz;
@@ -1955,7 +2174,7 @@ int mcp_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, int k, 
   return mcpg::finish(h, "mcpg_mcp");
 }
 
-// MCP2/3/4 of mcp_f32_keys on device operands with every buffer in `scratch`.
+// MCP2/3/4 of mcp_keys<float> on device operands with every buffer in `scratch`.
 int mcp_noalloc_entry(mcpg_ctx* h, const float* A, int64_t lda, int64_t N, int k, float* mm_out, int64_t ldm, float* out,
                       int64_t ldo) {
   MCPG_CHECK_HANDLE(h);
@@ -1978,7 +2197,7 @@ int mcp_noalloc_entry(mcpg_ctx* h, const float* A, int64_t lda, int64_t N, int k
   // launch timing would create events; this entry point records none
   const bool prof = h->profiling;
   h->profiling = false;
-  const int st = mcp_f32_keys(h, false, A, lda, N, k, mm_out, ldm, out, ldo, h->scratch, kScratchBytes);
+  const int st = mcp_keys<float>(h, false, A, lda, N, k, mm_out, ldm, out, ldo, h->scratch, kScratchBytes);
   h->profiling = prof;
   return st;
 }

@@ -11,7 +11,7 @@
                                          [--config N --update FILE]
       scaled HBM bytes per dispatch (FETCH + WRITE) per kernel, printed as JSON or merged into
       FILE under the key config<N> (bench.py reads profiles/traffic.json that way).
-  python -m mcpnet_amd.profiling maxmin --genes N [--dtype f32|f64] [--reps R] [--mcp K]
+  python -m mcpnet_amd.profiling maxmin --genes N [--dtype f32|f64] [--reps R] [--mcp K] [--float-exact]
       max-min products of a seeded symmetric N x N matrix with diagonal 0 on the GPU (the
       triangle launch, R times, or MCP_K): the workload for kernel traces of the product alone;
       prints each launch's duration from the library's own event timing.
@@ -41,7 +41,7 @@ def copy_main(mib: int) -> None:
     print(f"copied {mib} MiB x 3", flush=True)
 
 
-def maxmin_main(genes: int, dtype: str, reps: int, mcp: int) -> None:
+def maxmin_main(genes: int, dtype: str, reps: int, mcp: int, float_exact: bool = False) -> None:
     import numpy as np
     import torch
 
@@ -49,7 +49,10 @@ def maxmin_main(genes: int, dtype: str, reps: int, mcp: int) -> None:
 
     dt = np.float64 if dtype == "f64" else np.float32
     rng = np.random.default_rng(0x4D4D)
-    A = rng.random((genes, genes), dtype=np.float64).astype(dt)
+    A = rng.random((genes, genes), dtype=np.float64)
+    if float_exact:   # FP32 values held in doubles, as mcpnet's double chain hands its MI on
+        A = A.astype(np.float32)
+    A = A.astype(dt)
     A = np.maximum(A, A.T)
     np.fill_diagonal(A, 0.0)
     h = mcpnet_amd.Handle(0)
@@ -93,7 +96,8 @@ def _db(d: str):
 
 
 def _short(name: str) -> str:
-    for key in ("bspline_mi_kernel", "maxmin_key_kernel", "maxmin_tile_kernel", "key_prep_kernel", "key_pad_kernel",
+    for key in ("bspline_mi_kernel", "maxmin_key_kernel", "maxmin_tile_kernel", "key_prep_kernel",
+                "key_prep_f64_kernel", "key_pad_kernel",
                 "bspline_prep_kernel", "ap_mi_kernel", "ratio_kernel", "symmetry_check_kernel", "rank_kernel",
                 "cumhist_kernel", "noise_kernel", "weights_import_kernel", "bspline_pair_adjoint_kernel",
                 "bspline_gene_adjoint_kernel", "emax_kernel", "combo_kernel", "stouffer_kernel", "row_stats_kernel",
@@ -223,6 +227,7 @@ def main(argv=None) -> None:
     m.add_argument("--dtype", choices=["f32", "f64"], default="f64")
     m.add_argument("--reps", type=int, default=3)
     m.add_argument("--mcp", type=int, default=0, choices=[0, 2, 3, 4])
+    m.add_argument("--float-exact", action="store_true", help="f64: values a float holds exactly (the key route)")
     b = sub.add_parser("mi")
     b.add_argument("--genes", type=int, default=20000)
     b.add_argument("--samples", type=int, default=2000)
@@ -246,7 +251,7 @@ def main(argv=None) -> None:
     if a.cmd == "copy":
         copy_main(a.mib)
     elif a.cmd == "maxmin":
-        maxmin_main(a.genes, a.dtype, a.reps, a.mcp)
+        maxmin_main(a.genes, a.dtype, a.reps, a.mcp, a.float_exact)
     elif a.cmd == "mi":
         mi_main(a.genes, a.samples, a.precision, a.reps, a.seed)
     elif a.cmd == "summarize":

@@ -234,6 +234,7 @@ def test_f64_kernels_agree(handle, monkeypatch, env):
     B = rng.standard_normal((260, 333))
     km = np.where(rng.random(333) < 0.5, np.finfo(np.float64).max, np.finfo(np.float64).min)
     Y = mi_like(300, 8).astype(np.float64)
+    monkeypatch.setenv("MCPG_MM64_KEYS", "0")         # Y is float-exact: keep it on the FP64 kernels
     ref = (handle.maxmin(A, B, kmask=km), handle.maxmin(Y, Y, symmetric=True), handle.mcp(Y, 3, want_maxmin=True))
     assert bits_equal(ref[0], oracle.maxmin(A, B, km))
     for k, v in env.items():
@@ -242,3 +243,141 @@ def test_f64_kernels_agree(handle, monkeypatch, env):
     assert bits_equal(got[0], ref[0])
     assert bits_equal(got[1], ref[1])
     assert bits_equal(got[2][0], ref[2][0]) and bits_equal(got[2][1], ref[2][1])
+
+
+# ---- FP64 operands that every float holds exactly run on the key kernel
+
+F64_MAX = np.finfo(np.float64).max
+F64_LOW = np.finfo(np.float64).min
+
+
+def float_exact(rng, shape):
+    return rng.standard_normal(shape).astype(np.float32).astype(np.float64)
+
+
+def _on_fp64_kernel(handle, monkeypatch, fn):
+    monkeypatch.setenv("MCPG_MM64_KEYS", "0")
+    try:
+        return fn()
+    finally:
+        monkeypatch.delenv("MCPG_MM64_KEYS")
+
+
+def _route(names):
+    keys = "key_prep_f64_kernel" in names and any(n.startswith("maxmin_key_kernel") for n in names)
+    vals = any(n.startswith("maxmin_f64_kernel") for n in names)
+    assert keys != vals, names
+    return "keys" if keys else "fp64"
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 5, 7), (129, 127, 131), (257, 130, 1000), (400, 400, 300)])
+def test_f64_key_route_matches_fp64_kernel(handle, monkeypatch, shape):
+    """Float-exact FP64 operands: the key kernel with widened outputs gives the FP64 kernel's and
+    the oracle's bits, unmasked, with max/lowest<double> masks (the mcpnet TF mask), with every K
+    masked (lowest<double> everywhere), with finite float-exact mask values, and for a symmetric
+    triangle."""
+    M, N, K = shape
+    rng = np.random.default_rng(M + 7 * K)
+    A, B = float_exact(rng, (M, K)), float_exact(rng, (N, K))
+    S = float_exact(rng, (M, M))
+    S = np.maximum(S, S.T)
+    cases = [(A, B, None, False), (A, B, np.where(rng.random(K) < 0.4, F64_MAX, F64_LOW), False),
+             (A, B, np.full(K, F64_LOW), False), (A, B, np.full(K, -np.inf), False),
+             (A, B, float_exact(rng, K), False), (S, S, None, True)]
+    for a, b, km, sym in cases:
+        got, names = _names(handle, lambda: handle.maxmin(a, b, kmask=km, symmetric=sym))
+        assert _route(names) == "keys"
+        ref = _on_fp64_kernel(handle, monkeypatch, lambda: handle.maxmin(a, b, kmask=km, symmetric=sym))
+        assert bits_equal(got, ref)
+        rows = slice(0, min(M, 40))
+        assert bits_equal(got[rows], oracle.maxmin(np.ascontiguousarray(a[rows]), b, km))
+    assert np.all(handle.maxmin(A, B, kmask=np.full(K, F64_LOW)) == F64_LOW)
+
+
+@pytest.mark.parametrize("case", ["inexact", "minus_zero", "inf", "nan", "mask_inexact", "mask_huge_negative"])
+def test_f64_key_route_falls_back(handle, monkeypatch, case):
+    """One entry a float does not hold exactly (or -0, inf, NaN; or a mask value that is neither
+    float-exact nor beyond the float range in a way the key kernel reproduces) sends the product
+    to the FP64 kernel, whose bits it returns."""
+    rng = np.random.default_rng(5)
+    A, B = float_exact(rng, (70, 90)), float_exact(rng, (60, 90))
+    km = None
+    if case == "inexact":
+        A[33, 17] = 0.1
+    elif case == "minus_zero":
+        B[59, 89] = -0.0
+    elif case == "inf":
+        A[0, 0] = np.inf
+    elif case == "nan":
+        B[5, 5] = np.nan
+    elif case == "mask_inexact":
+        km = np.full(90, F64_MAX)
+        km[3] = 0.1
+    else:
+        km = np.full(90, F64_MAX)
+        km[3] = -1e300        # all-masked results would be -1e300, not lowest<double>
+    got, names = _names(handle, lambda: handle.maxmin(A, B, kmask=km))
+    assert _route(names) == "fp64"
+    assert bits_equal(got, _on_fp64_kernel(handle, monkeypatch, lambda: handle.maxmin(A, B, kmask=km)))
+    if case != "nan":
+        assert bits_equal(got, oracle.maxmin(A, B, km))
+
+
+def test_f64_key_route_mask_beyond_float_range(handle, monkeypatch):
+    """Mask values above FLT_MAX act as +inf on the key kernel: min(a, m) = a for every finite
+    float a, as in FP64."""
+    rng = np.random.default_rng(6)
+    A, B = float_exact(rng, (80, 64)), float_exact(rng, (75, 64))
+    km = np.where(rng.random(64) < 0.5, 1e300, F64_LOW)
+    km[:2] = [np.inf, 3.5e38]
+    got, names = _names(handle, lambda: handle.maxmin(A, B, kmask=km))
+    assert _route(names) == "keys"
+    assert bits_equal(got, _on_fp64_kernel(handle, monkeypatch, lambda: handle.maxmin(A, B, kmask=km)))
+    assert bits_equal(got, oracle.maxmin(A, B, km))
+
+
+@pytest.mark.parametrize("where", ["host", "device"])
+def test_f64_key_route_mcp_and_maxmins(handle, torch_cuda, monkeypatch, where):
+    """MCP2/3/4 (with and without the max-min output) and the three max-min chains of a
+    float-exact FP64 MI matrix run on the key kernel, the ratio taken in double: the FP64
+    kernels' and the oracle's bits.  An asymmetric float-exact input takes MCP3's full product."""
+    Y = mi_like(300, 12).astype(np.float64)
+    rng = np.random.default_rng(13)
+    Z = float_exact(rng, (200, 200))
+    put = (lambda x: torch_cuda.from_numpy(x).cuda()) if where == "device" else (lambda x: x)
+    get = (lambda x: x.cpu().numpy()) if where == "device" else (lambda x: x)
+    for A, ks in ((Y, (2, 3, 4)), (Z, (3,))):
+        for k in ks:
+            (out, mm), names = _names(handle, lambda: handle.mcp(put(A), k, want_maxmin=True))
+            assert _route(names) == "keys"
+            ref_out, ref_mm = oracle.mcp(A, k)
+            assert bits_equal(get(mm), ref_mm), k
+            assert bits_equal(get(out), ref_out), k
+            only = handle.mcp(put(A), k)[0]
+            assert bits_equal(get(only), ref_out), k
+    (m1, m2, m3), names = _names(handle, lambda: handle.maxmins(put(Y)))
+    assert _route(names) == "keys"
+    ref = _on_fp64_kernel(handle, monkeypatch, lambda: handle.maxmins(put(Y)))
+    for g, r in zip((m1, m2, m3), ref):
+        assert bits_equal(get(g), get(r))
+    assert bits_equal(get(m1), oracle.maxmin(Y, Y))
+
+
+def test_f64_key_route_tf_chain(handle):
+    """mcpnet on a float-exact FP64 MI with a TF mask of max/lowest<double> (mcpnet's load_tfs):
+    every masked product runs on the key kernel; the scores equal the oracle's."""
+    X = grn.grn_expression(48, 256, 34)[0]
+    A = oracle.bspline_mi_f64(X).astype(np.float32).astype(np.float64)
+    tf_idx = [1, 4, 9, 22, 40]
+    tfs = np.full(48, F64_LOW)
+    tfs[tf_idx] = F64_MAX
+    res, names = _names(handle, lambda: handle.mcpnet(A, mi_method=-1, tfs=tfs))
+    assert "maxmin_f64_kernel" not in names and "key_prep_f64_kernel" in names
+    Ad = A.copy()
+    np.fill_diagonal(Ad, 0)
+    T = np.ascontiguousarray(Ad[tf_idx])
+    mm = oracle.maxmin
+    G1 = mm(Ad, Ad, tfs)
+    want = {2: mm(T, Ad, tfs), 3: mm(T, G1, tfs), 4: mm(mm(T, Ad, tfs), G1, tfs)}
+    for k in (2, 3, 4):
+        assert bits_equal(res[k], oracle.ratio(T, want[k])), k
