@@ -333,15 +333,24 @@ __global__ void key_pad_kernel(int* __restrict__ out, int64_t ldk, int64_t rows,
 // DPI operand keys (include/mcp/filter/dpi.hpp:56-66): the diagonal is excluded (a triangle
 // x-y-z needs y != x, z), and with a TF list the columns of non-TF genes too (for the TF
 // variant's pairs with exactly one TF endpoint, dpi.hpp:90-101).  Excluded entries are INT_MIN.
-__global__ void dpi_key_kernel(const float* __restrict__ A, int64_t lda, int64_t N, const float* __restrict__ tfs,
-                               int* __restrict__ out, int64_t ldk, int64_t rows_pad) {
+template <typename T>
+__global__ void dpi_key_kernel(const T* __restrict__ A, int64_t lda, int64_t N, const T* __restrict__ tfs,
+                               int* __restrict__ out, int64_t ldk, int64_t rows_pad, int* __restrict__ flags) {
   const int64_t total = rows_pad * ldk;
+  bool bad = false;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t r = t / ldk, k = t % ldk;
     int v = INT_MIN;
-    if (r < N && k < N && r != k && !(tfs && !(tfs[k] > 0.0f))) v = f32_key(A[r * lda + k]);
+    if (r < N && k < N && r != k && !(tfs && !(tfs[k] > (T)0))) {
+      const T x = A[r * lda + k];
+      const float f = (float)x;
+      if constexpr (sizeof(T) == 8)   // FP64: the key test of key_prep_f64_kernel on the entries used
+        bad |= !((double)f == x) || isinf(f) || (x == 0.0 && signbit(x));
+      v = f32_key(f);
+    }
     out[t] = v;
   }
+  if (flags && __ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(flags, 1);
 }
 
 struct MKArgs {
@@ -1517,6 +1526,7 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
   KeyMat kx = key_geometry(N, N), kt = key_geometry(N, N);
   int st;
   if (f32 && (st = key_check(h, kx))) return st;
+  const bool try_keys = !f32 && f64_keys_enabled() && key_fits(kx);   // FP64 on keys when float-exact
   mcpg::Sizer sz;
   if (host) { sz.add<T>(NN); if (out) sz.add<T>(NN); if (mask) sz.add<uint8_t>(NN); }
   if (tfs) sz.add<T>(N);
@@ -1525,8 +1535,8 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
   const bool vals = !f32 && !f64_tile_kernel();
   ValMat vx = val_geometry(N, N), vt = val_geometry(N, N);
   if (vals && (st = val_check(h, vx))) return st;
-  if (f32) { sz.add<int>(kx.rows_pad * kx.ldk); if (tfs) sz.add<int>(kt.rows_pad * kt.ldk); }
-  else { sz.add<T>(NN); if (tfs) sz.add<T>(N); }  // diagonal-excluded copy, TF column mask
+  if (f32 || try_keys) { sz.add<int>(kx.rows_pad * kx.ldk); if (tfs) sz.add<int>(kt.rows_pad * kt.ldk); }
+  if (!f32) { sz.add<T>(NN); if (tfs) sz.add<T>(N); }  // diagonal-excluded copy, TF column mask
   if (vals) { sz.add<double>(vx.rows_pad * vx.ld); if (tfs) sz.add<double>(vt.rows_pad * vt.ld); }
   sz.add<unsigned long long>(1);
   void* base = mcpg::workspace(h, sz.off, &st);
@@ -1552,24 +1562,29 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
   }
   T* W = cv.take<T>(NN);
   T* WT = tfs ? cv.take<T>(NN) : nullptr;
-  if constexpr (f32) {
+  bool keyed = f32;
+  if (f32 || try_keys) {
     kx.d = cv.take<int>(kx.rows_pad * kx.ldk);
+    if (tfs) kt.d = cv.take<int>(kt.rows_pad * kt.ldk);
+    int* flags = nullptr;
+    if (!f32) {
+      if ((st = mcpg::reset_flags(h))) return st;
+      flags = h->d_flags;
+    }
     {
       mcpg::LaunchScope ls(h, "dpi_key_kernel");
-      hipLaunchKernelGGL(dpi_key_kernel, dim3(grid_of(kx.rows_pad * kx.ldk)), dim3(256), 0, h->stream, dA, la, N,
-                         (const float*)nullptr, kx.d, kx.ldk, kx.rows_pad);
+      hipLaunchKernelGGL(dpi_key_kernel<T>, dim3(grid_of(kx.rows_pad * kx.ldk)), dim3(256), 0, h->stream, dA, la, N,
+                         (const T*)nullptr, kx.d, kx.ldk, kx.rows_pad, flags);
+      if (tfs)
+        hipLaunchKernelGGL(dpi_key_kernel<T>, dim3(grid_of(kt.rows_pad * kt.ldk)), dim3(256), 0, h->stream, dA, la, N,
+                           dtf, kt.d, kt.ldk, kt.rows_pad, flags);
     }
-    if ((st = launch_maxmin_keys(h, kx, kx, N, N, 1, W, N, nullptr, 0, nullptr, 0, nullptr, 0))) return st;
-    if (tfs) {
-      kt.d = cv.take<int>(kt.rows_pad * kt.ldk);
-      {
-        mcpg::LaunchScope ls(h, "dpi_key_kernel");
-        hipLaunchKernelGGL(dpi_key_kernel, dim3(grid_of(kt.rows_pad * kt.ldk)), dim3(256), 0, h->stream, dA, la, N, dtf,
-                           kt.d, kt.ldk, kt.rows_pad);
-      }
-      // max over TF y of min(A[x][y], A[z][y]): the mask on one operand suffices (min is associative)
-      if ((st = launch_maxmin_keys(h, kt, kx, N, N, 1, WT, N, nullptr, 0, nullptr, 0, nullptr, 0))) return st;
-    }
+    if (!f32 && (st = keys_exact(h, &keyed))) return st;
+  }
+  if (keyed) {
+    if ((st = key_product(h, kx, kx, N, N, 1, W, N, nullptr, 0, nullptr, 0, nullptr, 0))) return st;
+    // max over TF y of min(A[x][y], A[z][y]): the mask on one operand suffices (min is associative)
+    if (tfs && (st = key_product(h, kt, kx, N, N, 1, WT, N, nullptr, 0, nullptr, 0, nullptr, 0))) return st;
   } else {
     T* Ad = cv.take<T>(NN);
     {

@@ -118,6 +118,32 @@ def test_dpi_bit_exact(handle, dt, with_tfs, tol):
     assert m > 0
 
 
+@pytest.mark.parametrize("with_tfs", [False, True])
+@pytest.mark.parametrize("N", [2, 3, 200])
+def test_dpi_f64_float_exact_on_keys(handle, monkeypatch, with_tfs, N):
+    """A float-exact FP64 MI matrix (the FP32 MI widened, as the double tools hand it on): the
+    DPI products run on the key kernel with widened outputs and give the FP64 kernel's and the
+    oracle's bits; N = 2 leaves every triangle empty (the products stay at lowest<double>)."""
+    A = mi_matrix(N, 300, 11).astype(np.float64)
+    A[0, 0] = 0.1                                    # the diagonal is excluded: not tested for exactness
+    rng = np.random.default_rng(N)
+    tfs = np.where(rng.random(N) < 0.3, np.finfo(np.float64).max, np.finfo(np.float64).min) if with_tfs else None
+    if with_tfs:
+        tfs[0] = np.finfo(np.float64).max
+    handle.set_profiling(True)
+    try:
+        got, n = handle.dpi(A, 0.1, tfs)
+        names = [k for k, _ in handle.last_timings()]
+    finally:
+        handle.set_profiling(False)
+    assert "dpi_key_kernel" in names and not any(k.startswith("maxmin_f64_kernel") for k in names)
+    monkeypatch.setenv("MCPG_MM64_KEYS", "0")
+    ref, r = handle.dpi(A, 0.1, tfs)
+    want, m = oracle.dpi(A, 0.1, tfs)
+    assert bits_equal(got, ref) and n == r
+    assert bits_equal(got, want) and n == m
+
+
 def test_dpi_asymmetric_and_device(handle, torch_cuda):
     rng = np.random.default_rng(4)
     A = (rng.random((257, 257)) ** 3).astype(np.float32)     # asymmetric: i_xz uses row x
