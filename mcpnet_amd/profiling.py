@@ -107,8 +107,8 @@ def _short(name: str) -> str:
             tmpl = name[name.find(key) + len(key):]
             if key == "maxmin_tile_kernel":
                 return key + ("<f64>" if "TrF64" in tmpl else "<f32>")
-            if key == "maxmin_key_kernel":
-                return key + "<f32>"
+            if key == "maxmin_key_kernel":   # <KC, WIDE>: float or widened double outputs
+                return key + ("<f64 out>" if tmpl.split("(")[0].rstrip(">").endswith("true") else "<f32>")
             if key == "bspline_mi_kernel":
                 return key + {"0": "<fp32>", "1": "<bf16>", "2": "<bf16x3>"}.get(tmpl[1:2], "")
             if key == "bspline_prep_kernel":   # <NB, NO, TWO, I8>: BF16 two-order or int8 planes
