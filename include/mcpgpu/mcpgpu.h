@@ -48,9 +48,12 @@ enum mcpg_precision {
   MCPG_FP32 = 0,   /* FP32 weights, FP32 MFMA (v_mfma_f32_16x16x4_f32) */
   MCPG_BF16 = 1,   /* BF16 weights, BF16 MFMA, FP32 accumulation */
   MCPG_BF16X3 = 2, /* hi+lo BF16 split, 3 MFMA products (hi*hi + hi*lo + lo*hi), FP32 accumulation */
-  MCPG_I8X3 = 3    /* two int8 digits per weight, w ~= (h + l/254)/127; products h*h and h*l + l*h
+  MCPG_I8X3 = 3,   /* two int8 digits per weight, w ~= (h + l/254)/127; products h*h and h*l + l*h
                       on the int8 MFMA (v_mfma_i32_16x16x64_i8) with exact int32 sums;
                       S <= 66560.  Not in the reference (its MI is FP32/FP64 on the CPU). */
+  MCPG_FP64 = 4    /* the reference's double DataType: scaling, basis, marginals and joint tables
+                      in double (v_mfma_f64_16x16x4_f64).  Taken by the *_f64 MI entry points
+                      below and by mcpg_mcpnet_*; the *_f32 MI entry points reject it. */
 };
 
 /* ------------------------------------------------------------------ handle */
@@ -111,6 +114,23 @@ int mcpg_bspline_mi_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int6
 int mcpg_bspline_mi_block_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx,
                               int bins, int order, int precision, int64_t row0, int64_t nrows, int64_t col0,
                               int64_t ncols, float* MI, int64_t ldmi);
+
+/* FP64 forms of the four B-spline entry points above, for the reference's double DataType
+ * (mi.cpp:300-304 and mcpnet.cpp run<double> unless --single): MinMaxScale<double>,
+ * BSplineWeightsKernel<double> and BSplineMIKernel<double> (BSplineMI.hpp:75-116, :119-274,
+ * :339-500).  Scale 1.0/(max-min) and x' = scale*(x-min) in double, knots of
+ * buildKnotVector<double>, the recursive basis in double, p = (1/S)<W_i,b , W_j,c> with the dot
+ * products on the FP64 MFMA, MI = H_i + H_j - H_ij.  The weight rows of mcpg_bspline_weights_f64
+ * hold H in double at [bins*S].  MI results agree with the reference's double arithmetic up to
+ * the summation order of the dot products. */
+int mcpg_minmax_scale_f64(mcpg_handle h, int ptrs, const double* X, int64_t G, int64_t S, int64_t ldx,
+                          double* Y, int64_t ldy);
+int mcpg_bspline_weights_f64(mcpg_handle h, int ptrs, const double* Xs, int64_t G, int64_t S, int64_t ldx,
+                             int bins, int order, double* W, int64_t ldw);
+int mcpg_bspline_mi_weights_f64(mcpg_handle h, int ptrs, const double* W, int64_t G, int64_t S, int64_t ldw,
+                                int bins, double* MI, int64_t ldmi);
+int mcpg_bspline_mi_f64(mcpg_handle h, int ptrs, const double* X, int64_t G, int64_t S, int64_t ldx,
+                        int bins, int order, double* MI, int64_t ldmi);
 
 /* Adjoint of the B-spline MI (no counterpart in the reference, which has no gradients):
  * for L = sum_n m[n] * MI(pi[n], pj[n]) (pairs with pi != pj; the MI is the MCPG_FP32 forward),
@@ -221,7 +241,8 @@ int mcpg_mcp_f32_noalloc(mcpg_handle h, const float* A, int64_t lda, int64_t N, 
  * diagonal, filter_local_mat_rows_by_tfs and compute_mcp{2,3,4}[_tfs], mcpnet_helper.hpp:56-435)
  * in one call, every intermediate in device memory: with MCPG_HOST pointers the only transfers
  * are X in and the requested outputs out.
- *   mi_method: 0 = B-spline MI (bins, order, precision as mcpg_bspline_mi_f32), 1 = adaptive
+ *   mi_method: 0 = B-spline MI (bins, order, precision as mcpg_bspline_mi_f32, or MCPG_FP64 as
+ *     mcpg_bspline_mi_f64 on double values of X), 1 = adaptive
  *     partitioning (noise_seed, as mcpg_add_noise_f32 -> mcpg_rank_f32 -> mcpg_ap_mi_i32),
  *     -1 = X already is the G x G MI matrix (S ignored, ldx >= G).
  *   diagonal >= 0 sets the MI diagonal (mcpnet.cpp:299-308); a negative value keeps it.
@@ -231,8 +252,10 @@ int mcpg_mcp_f32_noalloc(mcpg_handle h, const float* A, int64_t lda, int64_t N, 
  *   mi (nullable): the whole G x G MI after the diagonal is set (the ensemble stage needs all of
  *     it; its TF rows are what mcpnet writes in TF mode).
  *   mcp2 / mcp3 / mcp4 (nullable): the MCP scores of that order, G or T rows by G columns.
- * The f64 form computes the MI in FP32-class arithmetic from FP32 values of X and widens it
- * before the diagonal; its max-min products and ratios run in FP64, as the stage entry points'. */
+ * The f64 form with precision MCPG_FP64 computes the MI in double (the reference's double
+ * DataType); with the other precisions in FP32-class arithmetic from FP32 values of X, widened
+ * before the diagonal.  Its max-min products and ratios run in FP64, as the stage entry points'.
+ * The f32 form with MCPG_FP64 rounds the double MI to float. */
 int mcpg_mcpnet_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int mi_method, int bins,
                     int order, int precision, uint64_t noise_seed, double diagonal, const float* tfs, float* mi,
                     int64_t ldmi, float* mcp2, int64_t ld2, float* mcp3, int64_t ld3, float* mcp4, int64_t ld4);

@@ -16,7 +16,7 @@ import numpy as np
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libmcpgpu.so"
 
 MCPG_HOST, MCPG_DEVICE = 0, 1
-PRECISIONS = {"fp32": 0, "bf16": 1, "bf16x3": 2, "i8x3": 3}
+PRECISIONS = {"fp32": 0, "bf16": 1, "bf16x3": 2, "i8x3": 3, "fp64": 4}
 STATUS = {
     0: "ok", -1: "invalid argument", -2: "shape mismatch", -3: "non-finite input", -4: "out of device memory",
     -5: "HIP runtime error", -6: "no usable HIP device", -7: "unsupported size",
@@ -48,6 +48,10 @@ _SIGS = {
     "mcpg_bspline_weights_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
     "mcpg_bspline_mi_weights_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
     "mcpg_bspline_mi_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, c_int, _P, c_int64],
+    "mcpg_minmax_scale_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_bspline_weights_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
+    "mcpg_bspline_mi_weights_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, _P, c_int64],
+    "mcpg_bspline_mi_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
     "mcpg_bspline_mi_block_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, c_int, c_int64, c_int64, c_int64,
                                   c_int64, _P, c_int64],
     "mcpg_bspline_mi_backward_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, _P, _P, c_int64,
@@ -189,6 +193,10 @@ def _check_block(G, row0, nrows, col0, ncols):
         raise ValueError(f"block rows {row0}+{nrows}, columns {col0}+{ncols} exceed the {G} x {G} matrix")
 
 
+def _dtype_of(x):
+    return np.float64 if str(getattr(x, "dtype", "")).endswith("float64") else np.float32
+
+
 def _same_kind(*mats):
     kinds = {m.kind for m in mats if m is not None}
     if len(kinds) > 1:
@@ -265,40 +273,60 @@ class Handle:
 
     # ------------------------------------------------------------------ B-spline MI
     def minmax_scale(self, X, out=None):
-        x = _Mat(X, np.float32, "X")
+        """MinMaxScale per row; float64 rows run in double (the reference's double DataType)."""
+        dt = _dtype_of(X)
+        x = _Mat(X, dt, "X")
         G, S = x.shape
-        out, y = _out_mat(out, x.kind, (G, S), np.float32, X, "out")
+        out, y = _out_mat(out, x.kind, (G, S), dt, X, "out")
+        fn, name = ((self.lib.mcpg_minmax_scale_f64, "mcpg_minmax_scale_f64") if dt == np.float64 else
+                    (self.lib.mcpg_minmax_scale_f32, "mcpg_minmax_scale_f32"))
         self._pre(x.kind)
-        self._check(self.lib.mcpg_minmax_scale_f32(self.h, x.kind, x.ptr, G, S, x.ld, y.ptr, y.ld), "mcpg_minmax_scale_f32")
+        self._check(fn(self.h, x.kind, x.ptr, G, S, x.ld, y.ptr, y.ld), name)
         return out
 
     def bspline_weights(self, Xs, bins=10, order=3, out=None):
-        x = _Mat(Xs, np.float32, "Xs")
+        """Reference-layout weight rows (bins*S weights, then H); float64 rows run in double."""
+        dt = _dtype_of(Xs)
+        x = _Mat(Xs, dt, "Xs")
         G, S = x.shape
-        out, w = _out_mat(out, x.kind, (G, bins * S + 1), np.float32, Xs, "out")
+        out, w = _out_mat(out, x.kind, (G, bins * S + 1), dt, Xs, "out")
+        fn, name = ((self.lib.mcpg_bspline_weights_f64, "mcpg_bspline_weights_f64") if dt == np.float64 else
+                    (self.lib.mcpg_bspline_weights_f32, "mcpg_bspline_weights_f32"))
         self._pre(x.kind)
-        self._check(self.lib.mcpg_bspline_weights_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order, w.ptr, w.ld),
-                    "mcpg_bspline_weights_f32")
+        self._check(fn(self.h, x.kind, x.ptr, G, S, x.ld, bins, order, w.ptr, w.ld), name)
         return out
 
     def bspline_mi_weights(self, W, S, bins=10, precision="fp32", out=None):
-        w = _Mat(W, np.float32, "W")
+        """MI from reference-layout weight rows; precision "fp64" takes float64 rows."""
+        dt = np.float64 if precision == "fp64" else np.float32
+        w = _Mat(W, dt, "W")
         G = w.shape[0]
-        out, m = _out_mat(out, w.kind, (G, G), np.float32, W, "out")
+        out, m = _out_mat(out, w.kind, (G, G), dt, W, "out")
         self._pre(w.kind)
-        self._check(self.lib.mcpg_bspline_mi_weights_f32(self.h, w.kind, w.ptr, G, S, w.ld, bins,
-                                                         PRECISIONS[precision], m.ptr, m.ld),
-                    "mcpg_bspline_mi_weights_f32")
+        if dt == np.float64:
+            self._check(self.lib.mcpg_bspline_mi_weights_f64(self.h, w.kind, w.ptr, G, S, w.ld, bins, m.ptr, m.ld),
+                        "mcpg_bspline_mi_weights_f64")
+        else:
+            self._check(self.lib.mcpg_bspline_mi_weights_f32(self.h, w.kind, w.ptr, G, S, w.ld, bins,
+                                                             PRECISIONS[precision], m.ptr, m.ld),
+                        "mcpg_bspline_mi_weights_f32")
         return out
 
     def bspline_mi(self, X, bins=10, order=3, precision="fp32", out=None):
-        x = _Mat(X, np.float32, "X")
+        """All-pairs B-spline MI.  precision "fp64" takes float64 X and returns float64 MI (the
+        reference's double DataType); the others take and return float32."""
+        dt = np.float64 if precision == "fp64" else np.float32
+        x = _Mat(X, dt, "X")
         G, S = x.shape
-        out, m = _out_mat(out, x.kind, (G, G), np.float32, X, "out")
+        out, m = _out_mat(out, x.kind, (G, G), dt, X, "out")
         _same_kind(x, m)
         self._pre(x.kind)
-        self._check(self.lib.mcpg_bspline_mi_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order,
-                                                 PRECISIONS[precision], m.ptr, m.ld), "mcpg_bspline_mi_f32")
+        if dt == np.float64:
+            self._check(self.lib.mcpg_bspline_mi_f64(self.h, x.kind, x.ptr, G, S, x.ld, bins, order, m.ptr, m.ld),
+                        "mcpg_bspline_mi_f64")
+        else:
+            self._check(self.lib.mcpg_bspline_mi_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order,
+                                                     PRECISIONS[precision], m.ptr, m.ld), "mcpg_bspline_mi_f32")
         return out
 
     def bspline_mi_block(self, X, row0, nrows, col0, ncols, bins=10, order=3, precision="fp32", out=None):

@@ -62,28 +62,30 @@ Spline make_spline(int bins, int order) {
 }
 
 // Cox-de Boor basis values N_{b,order}(t) for b < bins, evaluated bottom-up with the same
-// float operations, degenerate-span guards (d < 1e-10) and negative truncation as the
-// reference's recursive basisFunction (BSplineMI.hpp:130-164), including the right-end rule
-// N_{bins-1,1}(t) = 1 when |t - knot[bins]| < 1e-10.  `lower` (nullable) receives the order-1
-// level N_{b,order-1}(t) for b < bins+1, used by the derivative.  NB / NO are compile-time
-// capacities (bins <= NB, order <= NO) so that the tables stay in registers.
-template <int NB, int NO>
-__device__ __forceinline__ void basis_values(const Spline& sp, float t, float* out, float* lower) {
-  float N[NB + NO];
-  const int bins = sp.bins, order = sp.order;
-  const float* k = sp.knots;
+// operations in T (float for the reference's float DataType, double for its double), degenerate-
+// span guards (d < 1e-10) and negative truncation as the reference's recursive basisFunction
+// (BSplineMI.hpp:130-164), including the right-end rule N_{bins-1,1}(t) = 1 when
+// |t - knot[bins]| < 1e-10.  `lower` (nullable) receives the order-1 level N_{b,order-1}(t) for
+// b < bins+1, used by the derivative.  NB / NO are compile-time capacities (bins <= NB,
+// order <= NO) so that the tables stay in registers.
+__device__ __forceinline__ double abs_of(float v) { return (double)fabsf(v); }
+__device__ __forceinline__ double abs_of(double v) { return fabs(v); }
+
+template <int NB, int NO, typename T>
+__device__ __forceinline__ void basis_values_t(const T* k, int bins, int order, T t, T* out, T* lower) {
+  T N[NB + NO];
   const int n1 = bins + order - 1;
 #pragma unroll
   for (int i = 0; i < NB + NO - 1; ++i) {
     if (i < n1) {
       const bool in = (t >= k[i] && t < k[i + 1] && k[i] < k[i + 1]);
-      const bool endpt = ((double)fabsf(t - k[i + 1]) < 1e-10) && (i + 1 == bins);
-      N[i] = (in || endpt) ? 1.0f : 0.0f;
+      const bool endpt = (abs_of(t - k[i + 1]) < 1e-10) && (i + 1 == bins);
+      N[i] = (in || endpt) ? (T)1 : (T)0;
     } else {
-      N[i] = 0.0f;
+      N[i] = (T)0;
     }
   }
-  N[NB + NO - 1] = 0.0f;
+  N[NB + NO - 1] = (T)0;
 #pragma unroll
   for (int p = 2; p <= NO; ++p) {
     if (p <= order) {
@@ -95,25 +97,25 @@ __device__ __forceinline__ void basis_values(const Spline& sp, float t, float* o
 #pragma unroll
       for (int i = 0; i < NB + NO - 2; ++i) {
         if (i < n) {
-          const float d1 = k[i + p - 1] - k[i];
-          const float nn1 = t - k[i];
-          const float d2 = k[i + p] - k[i + 1];
-          const float nn2 = k[i + p] - t;
-          float e1, e2, v;
+          const T d1 = k[i + p - 1] - k[i];
+          const T nn1 = t - k[i];
+          const T d2 = k[i + p] - k[i + 1];
+          const T nn2 = k[i + p] - t;
+          T e1, e2, v;
           if ((double)d1 < 1e-10 && (double)d2 < 1e-10) {
-            v = 0.0f;
+            v = (T)0;
           } else {
             if ((double)d1 < 1e-10) {
-              e1 = 0.0f;
+              e1 = (T)0;
               e2 = nn2 / d2 * N[i + 1];
             } else if ((double)d2 < 1e-10) {
-              e2 = 0.0f;
+              e2 = (T)0;
               e1 = nn1 / d1 * N[i];
             } else {
               e1 = nn1 / d1 * N[i];
               e2 = nn2 / d2 * N[i + 1];
             }
-            v = (e1 + e2 < 0.0f) ? 0.0f : e1 + e2;
+            v = (e1 + e2 < (T)0) ? (T)0 : e1 + e2;
           }
           N[i] = v;  // in place: N[i+1] of the previous level is read before it is overwritten
         }
@@ -122,10 +124,15 @@ __device__ __forceinline__ void basis_values(const Spline& sp, float t, float* o
   }
   if (lower && order == 1) {
 #pragma unroll
-    for (int i = 0; i < NB + 1; ++i) lower[i] = 0.0f;
+    for (int i = 0; i < NB + 1; ++i) lower[i] = (T)0;
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) out[b] = N[b];
+}
+
+template <int NB, int NO>
+__device__ __forceinline__ void basis_values(const Spline& sp, float t, float* out, float* lower) {
+  basis_values_t<NB, NO, float>(sp.knots, sp.bins, sp.order, t, out, lower);
 }
 
 // d N_{b,order} / dt from the order-1 level (exact derivative of the piecewise polynomial).
@@ -1032,6 +1039,254 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(NST ==
   }
 }
 
+// ---------------------------------------------------------------- FP64 (the reference's double DataType)
+// MinMaxScale<double>, BSplineWeightsKernel<double> and BSplineMIKernel<double>
+// (BSplineMI.hpp:75-116, :119-274, :339-500): scaling, basis, marginals and joint tables in
+// double.  The joint tables W_i W_j^T run on v_mfma_f64_16x16x4_f64.
+constexpr int kKStep64 = 8;   // samples per K block of the FP64 weight panel (64-byte rows)
+typedef double doublex2 __attribute__((ext_vector_type(2)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// buildKnotVector<double> (BSplineMI.hpp:35-71)
+struct Spline64 {
+  int bins, order;
+  double knots[kMaxBins + kMaxOrder];
+};
+
+Spline64 make_spline64(int bins, int order) {
+  Spline64 sp;
+  std::memset(&sp, 0, sizeof sp);
+  sp.bins = bins;
+  sp.order = order;
+  const int n_int = bins - order;
+  const double norm = 1.0 / (double)(n_int + 1);
+  for (int i = 0; i < bins + order; ++i) {
+    if (i < order) sp.knots[i] = 0.0;
+    else if (i < order + n_int) sp.knots[i] = (double)(i - order + 1) * norm;
+    else sp.knots[i] = 1.0;
+  }
+  return sp;
+}
+
+struct Prep64Out {
+  double* w;        // group-tiled weights, Layout with ks = kKStep64 (nullable)
+  double* wref;     // reference layout: G rows of bins*S weights (bin-major) + H (nullable)
+  int64_t ldw;
+  double* xs;       // scaled rows (nullable)
+  int64_t ldxs;
+  double* H64;      // marginal entropy per gene (nullable)
+  int* flags;       // flags[0] |= 1 on non-finite input
+};
+
+// One workgroup per gene: MinMaxScale<double> (scale = 1.0 / (max - min), x' = scale (x - min);
+// a constant row keeps its values), the basis in double and entropy1d_2's marginal entropy
+// (ex += w * (1.0 / S), H = 0 - sum ex log2 ex).
+template <int NB, int NO>
+__global__ __launch_bounds__(kThreads) void bspline_prep64_kernel(const double* __restrict__ X, int64_t ldx, Layout L,
+                                                                  Spline64 sp, int do_scale, Prep64Out o) {
+  const int64_t g = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t S = L.S;
+  const double* x = X + g * ldx;
+  __shared__ double shd[4];
+  __shared__ double psum[kMaxBins];
+
+  double mn = S > 0 ? x[0] : 0.0, mx = mn;
+  int bad = 0;
+  for (int64_t s = tid; s < S; s += kThreads) {
+    const double v = x[s];
+    if (!isfinite(v)) bad = 1;
+    mn = fmin(mn, v);
+    mx = fmax(mx, v);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    mn = fmin(mn, __shfl_xor(mn, off));
+    mx = fmax(mx, __shfl_xor(mx, off));
+  }
+  __syncthreads();
+  if ((tid & 63) == 0) shd[tid >> 6] = mn;
+  __syncthreads();
+  mn = fmin(fmin(shd[0], shd[1]), fmin(shd[2], shd[3]));
+  __syncthreads();
+  if ((tid & 63) == 0) shd[tid >> 6] = mx;
+  __syncthreads();
+  mx = fmax(fmax(shd[0], shd[1]), fmax(shd[2], shd[3]));
+  if (bad) atomicOr(o.flags, 1);
+
+  double scale = 1.0, offset = 0.0;
+  if (do_scale && mx > mn) {
+    scale = 1.0 / (mx - mn);
+    offset = mn;
+  }
+  const int bins = sp.bins;
+  const double norm = 1.0 / (double)S;
+  double acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) acc[b] = 0.0;
+  double w[NB];
+  for (int64_t s = tid; s < S; s += kThreads) {
+    const double t = do_scale ? scale * (x[s] - offset) : x[s];
+    if (o.xs) o.xs[g * o.ldxs + s] = t;
+    basis_values_t<NB, NO, double>(sp.knots, bins, sp.order, t, w, nullptr);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (b < bins) {
+        acc[b] += w[b] * norm;
+        if (o.w) o.w[L.at(L.row_of(g, b), s)] = w[b];
+        if (o.wref) o.wref[g * o.ldw + (int64_t)b * S + s] = w[b];
+      }
+    }
+  }
+  for (int b = 0; b < bins; ++b) {
+    double v = 0.0;
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (c == b) v = acc[c];
+    const double t = block_sum<double>(v, shd);
+    if (tid == 0) psum[b] = t;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double H = 0.0;
+    for (int b = 0; b < bins; ++b)
+      if (psum[b] > 0.0) H += psum[b] * log2(psum[b]);
+    H = 0.0 - H;
+    if (o.H64) o.H64[g] = H;
+    if (o.wref) o.wref[g * o.ldw + (int64_t)bins * S] = H;
+  }
+}
+
+// Reference-layout FP64 weight rows (bins*S weights, then H) -> group-tiled panel + H64.
+__global__ void weights_import64_kernel(const double* __restrict__ W, int64_t ldw, Layout L, double* __restrict__ w,
+                                        double* __restrict__ H64, int* __restrict__ flags) {
+  const int64_t n = (int64_t)L.bins * L.S;
+  bool bad = false;
+  for (int64_t g = blockIdx.y; g < L.G; g += gridDim.y) {
+    const double* row = W + g * ldw;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= n; e += (int64_t)gridDim.x * blockDim.x) {
+      const double v = row[e];
+      bad |= !isfinite(v);
+      if (e == n) H64[g] = v;
+      else w[L.at(L.row_of(g, (int)(e / L.S)), e % L.S)] = v;
+    }
+  }
+  if (__ballot(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(flags, 1);
+}
+
+struct MI64Args {
+  const double* w;        // group-tiled weights (Layout with ks = kKStep64, zero padding)
+  const double* H64;
+  double* MI;
+  int64_t ldmi;
+  Layout L;
+  const uint32_t* order;  // tile coordinates in launch order, (ti << 16) | tj
+  double norm;            // 1.0 / S: BSplineEntropyKernel<double>::norm_factor (BSplineMI.hpp:319)
+};
+
+constexpr int kEpi64Stride = 81;                          // doubles per row of a wave's t tile
+constexpr int kEpi64Wave = kGroupRows * kEpi64Stride;     // doubles per wave t tile
+constexpr int kEpi64Bytes = 2 * kEpi64Wave * 8;           // two waves' t tiles at a time (101.25 KB)
+
+// one 8-sample K block of a wave's 80 x 80 block of MFMA outputs: the .x halves of the fragments
+// carry samples 2q, the .y halves samples 2q + 1 (q = lane >> 4, the MFMA's K index)
+__device__ __forceinline__ void mi64_step(const doublex2 (&fa)[5], const doublex2 (&fb)[5], doublex4 (&acc)[5][5]) {
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].x, fb[j].x, acc[i][j], 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i].y, fb[j].y, acc[i][j], 0, 0, 0);
+}
+
+// fragments of K block kb: block row i of the wave's A (B) group, lane (lr, lq) -> row 16 i + lr,
+// samples 2 lq and 2 lq + 1 (one 16-byte load; 64 lanes read one contiguous 1 KB row block)
+__device__ __forceinline__ void mi64_load(const double* pa, const double* pb, int64_t kb, doublex2 (&fa)[5],
+                                          doublex2 (&fb)[5]) {
+  const int64_t off = kb * (kGroupRows * kKStep64);
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    fa[i] = *reinterpret_cast<const doublex2*>(pa + off + i * 16 * kKStep64);
+    fb[i] = *reinterpret_cast<const doublex2*>(pb + off + i * 16 * kKStep64);
+  }
+}
+
+// All-pairs FP64 MI: the 160 x 160 workgroup tile of the other MI kernels, four waves of 80 x 80
+// (5 x 5 blocks of 16 x 16 FP64 MFMA outputs, 200 registers).  The FP64 MFMA takes 64 cycles
+// per 16 x 16 x 4 product, so operands come straight from memory into registers (two K blocks in
+// flight, one wave per SIMD) rather than through LDS.  Epilogue: t = p log2 p in double with
+// p = dot * (1 / S), summed per gene pair over bins x bins in the reference's (x bin, y bin)
+// order through LDS (two waves at a time), and MI = (H_i + H_j) - (0 - sum t)
+// (BSplineMIKernel::operator(), BSplineMI.hpp:492-499; entropy2d_simd :264-286).
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void bspline_mi64_kernel(MI64Args a) {
+  extern __shared__ __align__(16) char smem[];
+  const uint32_t code = a.order[blockIdx.x];
+  const int64_t ti = code >> 16, tj = code & 0xffffu;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1, lr = lane & 15, lq = lane >> 4;
+  const int64_t nk = a.L.Sp / kKStep64;
+  const int64_t group_elems = nk * (kGroupRows * kKStep64);
+  const double* pa = a.w + (ti * 2 + wr) * group_elems + lr * kKStep64 + 2 * lq;
+  const double* pb = a.w + (tj * 2 + wc) * group_elems + lr * kKStep64 + 2 * lq;
+
+  doublex4 acc[5][5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+  doublex2 a0[5], b0[5], a1[5], b1[5];
+  mi64_load(pa, pb, 0, a0, b0);
+  if (nk > 1) mi64_load(pa, pb, 1, a1, b1);
+  for (int64_t kb = 0; kb < nk; kb += 2) {
+    mi64_step(a0, b0, acc);
+    if (kb + 2 < nk) mi64_load(pa, pb, kb + 2, a0, b0);
+    if (kb + 1 < nk) {
+      mi64_step(a1, b1, acc);
+      if (kb + 3 < nk) mi64_load(pa, pb, kb + 3, a1, b1);
+    }
+  }
+
+  double* E = reinterpret_cast<double*>(smem);
+  const bool diag = ti == tj;
+  const int bins = a.L.bins, gw = a.L.gw;
+  for (int phase = 0; phase < 2; ++phase) {
+    if (wr == phase) {
+      double* Ew = E + wc * kEpi64Wave;
+#pragma unroll
+      for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            // v_mfma_f64_16x16x4_f64 output: register r of lane (lr, lq) is D[4 r + lq][lr]
+            // (the FP32 16 x 16 forms hold D[4 lq + r][lr]; measured on MI355X)
+            const double p = acc[i][j][r] * a.norm;
+            Ew[(i * 16 + r * 4 + lq) * kEpi64Stride + j * 16 + lr] = p > 0.0 ? p * log2(p) : 0.0;
+          }
+    }
+    __syncthreads();
+    if (wr == phase) {
+      const double* Ew = E + wc * kEpi64Wave;
+      for (int pr = lane; pr < gw * gw; pr += 64) {
+        const int pi = pr / gw, pj = pr % gw;
+        const int64_t gi = ti * a.L.gt + wr * gw + pi;
+        const int64_t gj = tj * a.L.gt + wc * gw + pj;
+        if (gi >= a.L.G || gj >= a.L.G || (diag && gi > gj)) continue;
+        double sum = 0.0;
+        for (int r = 0; r < bins; ++r) {
+          const double* row = Ew + (pi * bins + r) * kEpi64Stride + pj * bins;
+          for (int c = 0; c < bins; ++c) sum += row[c];
+        }
+        const double mi = (a.H64[gi] + a.H64[gj]) - (0.0 - sum);
+        a.MI[gi * a.ldmi + gj] = mi;
+        a.MI[gj * a.ldmi + gi] = mi;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- adjoint
 // Per masked pair: joint table p = W_i W_j^T / S (FP32, one wave per pair), its entropy term and
 // Gm[b][c] = m * phi(p_bc), phi(p) = -(log2 p + 1/ln 2) = dH/dp for p > 0.
@@ -1667,6 +1922,93 @@ int bspline_mi_run(mcpg_ctx* h, int ptrs, const float* X, int64_t G, int64_t S, 
   return check_flags_nonfinite(h, where);
 }
 
+// ---- FP64 host side
+Layout mi64_layout(int64_t G, int64_t S, int bins) { return make_layout(G, S, bins, 2, kKStep64); }
+
+int prep64_launch(mcpg_ctx* h, const double* dX, int64_t ldx, const Layout& L, const Spline64& sp, int do_scale,
+                  const Prep64Out& o) {
+  mcpg::LaunchScope ls(h, "bspline_prep64_kernel");
+  if (sp.bins <= 16 && sp.order <= 4)
+    hipLaunchKernelGGL((bspline_prep64_kernel<16, 4>), dim3((unsigned)L.G), dim3(kThreads), 0, h->stream, dX, ldx, L,
+                       sp, do_scale, o);
+  else
+    hipLaunchKernelGGL((bspline_prep64_kernel<kMaxBins, kMaxOrder>), dim3((unsigned)L.G), dim3(kThreads), 0,
+                       h->stream, dX, ldx, L, sp, do_scale, o);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mcpg::hip_error(h, e, "bspline_prep64_kernel launch");
+  return MCPG_OK;
+}
+
+int mi64_launch(mcpg_ctx* h, const double* w, const double* H64, const Layout& L, double* MI, int64_t ldmi) {
+  if (L.tr != kTileRows || L.ks != kKStep64)
+    return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: weight layout does not match the FP64 MI kernel");
+  const int64_t ntri = L.ntiles * (L.ntiles + 1) / 2;
+  if (ntri > (int64_t)INT32_MAX || L.ntiles > 65535) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "MI grid too large");
+  MI64Args a;
+  a.w = w; a.H64 = H64; a.MI = MI; a.ldmi = ldmi; a.L = L;
+  a.norm = 1.0 / (double)L.S;
+  int st;
+  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 32), mcpg::env_group("MCPG_MI_XCD", 0), &st);
+  if (!a.order) return st;
+  static bool attr_set = false;
+  if (!attr_set) {
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi64_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kEpi64Bytes));
+    attr_set = true;
+  }
+  mcpg::LaunchScope ls(h, "bspline_mi64_kernel");
+  hipLaunchKernelGGL(bspline_mi64_kernel, dim3((unsigned)ntri), dim3(kThreads), kEpi64Bytes, h->stream, a);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return mcpg::hip_error(h, e, "bspline_mi64_kernel launch");
+  return MCPG_OK;
+}
+
+// compute_mi on the reference's double DataType: scaling, weights and marginals in double, then
+// the FP64 MI of every pair.
+int bspline_mi64_run(mcpg_ctx* h, int ptrs, const double* X, int64_t G, int64_t S, int64_t ldx, int bins, int order,
+                     double* MI, int64_t ldmi, const char* where) {
+  int st;
+  if ((st = check_spline_args(h, bins, order))) return st;
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (!X || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldx < S || ldmi < G) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const Layout L = mi64_layout(G, S, bins);
+  const Spline64 sp = make_spline64(bins, order);
+  const size_t n = (size_t)L.rows * L.Sp;
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) { sz.add<double>(G * S); sz.add<double>(G * G); }
+  sz.add<double>(n);
+  sz.add<double>(L.ntiles * L.gt);
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const double* dX = X;
+  int64_t lx = ldx;
+  double* dMI = MI;
+  int64_t lm = ldmi;
+  if (ptrs == MCPG_HOST) {
+    double* t = cv.take<double>(G * S);
+    if ((st = mcpg::copy_2d(h, t, S * 8, X, ldx * 8, S * 8, G, hipMemcpyHostToDevice))) return st;
+    dX = t; lx = S;
+    dMI = cv.take<double>(G * G); lm = G;
+  }
+  double* w = cv.take<double>(n);
+  double* H64 = cv.take<double>(L.ntiles * L.gt);
+  MCPG_HIP(h, hipMemsetAsync(w, 0, n * sizeof(double), h->stream));   // padding rows and samples
+  if ((st = mcpg::reset_flags(h))) return st;
+  Prep64Out o{};
+  o.w = w; o.H64 = H64; o.flags = h->d_flags;
+  if ((st = prep64_launch(h, dX, lx, L, sp, 1, o))) return st;
+  if ((st = mi64_launch(h, w, H64, L, dMI, lm))) return st;
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 8, dMI, G * 8, G * 8, G, hipMemcpyDeviceToHost))) return st;
+  if ((st = mcpg::finish(h, where))) return st;
+  return check_flags_nonfinite(h, where);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1823,6 +2165,140 @@ int mcpg_bspline_mi_block_f32(mcpg_handle h, int ptrs, const float* X, int64_t G
     return mcpg::set_error(h, MCPG_ERR_SHAPE, "block outside the G x G matrix");
   return bspline_mi_run(h, ptrs, X, G, S, ldx, bins, order, precision, row0, nrows, col0, ncols, MI, ldmi,
                         "mcpg_bspline_mi_block_f32");
+}
+
+int mcpg_minmax_scale_f64(mcpg_handle h, int ptrs, const double* X, int64_t G, int64_t S, int64_t ldx, double* Y,
+                          int64_t ldy) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0 || S == 0) return MCPG_OK;
+  if (!X || !Y) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldx < S || ldy < S) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const Layout L = mi64_layout(G, S, 10);
+  const Spline64 sp = make_spline64(10, 3);
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) { sz.add<double>(G * S); sz.add<double>(G * S); }
+  int st;
+  void* base = sz.off ? mcpg::workspace(h, sz.off, &st) : h->ws;
+  if (sz.off && !base) return st;
+  mcpg::Carver cv(base);
+  const double* dX = X;
+  int64_t lx = ldx;
+  double* dY = Y;
+  int64_t ly = ldy;
+  if (ptrs == MCPG_HOST) {
+    double* t = cv.take<double>(G * S);
+    if ((st = mcpg::copy_2d(h, t, S * 8, X, ldx * 8, S * 8, G, hipMemcpyHostToDevice))) return st;
+    dX = t; lx = S;
+    dY = cv.take<double>(G * S); ly = S;
+  }
+  if ((st = mcpg::reset_flags(h))) return st;
+  Prep64Out o{};
+  o.xs = dY; o.ldxs = ly; o.flags = h->d_flags;
+  if ((st = prep64_launch(h, dX, lx, L, sp, 1, o))) return st;
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, Y, ldy * 8, dY, S * 8, S * 8, G, hipMemcpyDeviceToHost))) return st;
+  if ((st = mcpg::finish(h, "mcpg_minmax_scale_f64"))) return st;
+  return check_flags_nonfinite(h, "mcpg_minmax_scale_f64");
+}
+
+int mcpg_bspline_weights_f64(mcpg_handle h, int ptrs, const double* Xs, int64_t G, int64_t S, int64_t ldx, int bins,
+                             int order, double* W, int64_t ldw) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  int st;
+  if ((st = check_spline_args(h, bins, order))) return st;
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (!Xs || !W) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldx < S || ldw < bins * S + 1) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const Layout L = mi64_layout(G, S, bins);
+  const Spline64 sp = make_spline64(bins, order);
+  const int64_t wrow = (int64_t)bins * S + 1;
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) { sz.add<double>(G * S); sz.add<double>(G * wrow); }
+  void* base = sz.off ? mcpg::workspace(h, sz.off, &st) : h->ws;
+  if (sz.off && !base) return st;
+  mcpg::Carver cv(base);
+  const double* dX = Xs;
+  int64_t lx = ldx;
+  double* dW = W;
+  int64_t lw = ldw;
+  if (ptrs == MCPG_HOST) {
+    double* t = cv.take<double>(G * S);
+    if ((st = mcpg::copy_2d(h, t, S * 8, Xs, ldx * 8, S * 8, G, hipMemcpyHostToDevice))) return st;
+    dX = t; lx = S;
+    dW = cv.take<double>(G * wrow); lw = wrow;
+  }
+  if ((st = mcpg::reset_flags(h))) return st;
+  Prep64Out o{};
+  o.wref = dW; o.ldw = lw; o.flags = h->d_flags;
+  if ((st = prep64_launch(h, dX, lx, L, sp, 0, o))) return st;
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, W, ldw * 8, dW, wrow * 8, wrow * 8, G, hipMemcpyDeviceToHost)))
+    return st;
+  if ((st = mcpg::finish(h, "mcpg_bspline_weights_f64"))) return st;
+  return check_flags_nonfinite(h, "mcpg_bspline_weights_f64");
+}
+
+int mcpg_bspline_mi_weights_f64(mcpg_handle h, int ptrs, const double* W, int64_t G, int64_t S, int64_t ldw, int bins,
+                                double* MI, int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (bins < 1 || bins > kMaxBins) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bins must be in [1, 80]");
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (!W || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldw < bins * S + 1 || ldmi < G) return mcpg::set_error(h, MCPG_ERR_SHAPE, "weight rows must hold bins*S+1 values");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  MCPG_HIP(h, hipSetDevice(h->device));
+  const Layout L = mi64_layout(G, S, bins);
+  const int64_t wrow = (int64_t)bins * S + 1;
+  const size_t n = (size_t)L.rows * L.Sp;
+  int st;
+  mcpg::Sizer sz;
+  if (ptrs == MCPG_HOST) { sz.add<double>(G * wrow); sz.add<double>(G * G); }
+  sz.add<double>(n);
+  sz.add<double>(L.ntiles * L.gt);
+  void* base = mcpg::workspace(h, sz.off, &st);
+  if (!base) return st;
+  mcpg::Carver cv(base);
+  const double* dW = W;
+  int64_t lw = ldw;
+  double* dMI = MI;
+  int64_t lm = ldmi;
+  if (ptrs == MCPG_HOST) {
+    double* t = cv.take<double>(G * wrow);
+    if ((st = mcpg::copy_2d(h, t, wrow * 8, W, ldw * 8, wrow * 8, G, hipMemcpyHostToDevice))) return st;
+    dW = t; lw = wrow;
+    dMI = cv.take<double>(G * G); lm = G;
+  }
+  double* w = cv.take<double>(n);
+  double* H64 = cv.take<double>(L.ntiles * L.gt);
+  MCPG_HIP(h, hipMemsetAsync(w, 0, n * sizeof(double), h->stream));
+  if ((st = mcpg::reset_flags(h))) return st;
+  {
+    mcpg::LaunchScope ls(h, "weights_import64_kernel");
+    hipLaunchKernelGGL(weights_import64_kernel,
+                       dim3((unsigned)std::min<int64_t>(ceil_div(wrow, 256), 64), (unsigned)std::min<int64_t>(G, 65535)),
+                       dim3(256), 0, h->stream, dW, lw, L, w, H64, h->d_flags);
+  }
+  if ((st = mi64_launch(h, w, H64, L, dMI, lm))) return st;
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 8, dMI, G * 8, G * 8, G, hipMemcpyDeviceToHost))) return st;
+  if ((st = mcpg::finish(h, "mcpg_bspline_mi_weights_f64"))) return st;
+  return check_flags_nonfinite(h, "mcpg_bspline_mi_weights_f64");
+}
+
+int mcpg_bspline_mi_f64(mcpg_handle h, int ptrs, const double* X, int64_t G, int64_t S, int64_t ldx, int bins, int order,
+                        double* MI, int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  return bspline_mi64_run(h, ptrs, X, G, S, ldx, bins, order, MI, ldmi, "mcpg_bspline_mi_f64");
 }
 
 int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int bins,

@@ -160,6 +160,38 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
   if ((st = alloc(h, bmi, (size_t)G * G, &dmi))) return st;
   if (mi_method < 0) {
     if ((st = mcpg::copy_2d(h, dmi, G * sizeof(T), X, ldx * sizeof(T), G * sizeof(T), G, in_kind))) return st;
+  } else if (mi_method == 0 && precision == MCPG_FP64) {
+    // B-spline MI of double rows in double (the reference's double DataType); the FP32 chain
+    // rounds it to float
+    Buf bxd, bmid;
+    const double* xd = reinterpret_cast<const double*>(X);
+    int64_t ldxd = ldx;
+    if (f32 || host) {
+      double* dxd;
+      if ((st = alloc(h, bxd, (size_t)G * S, &dxd))) return st;
+      if (!f32) {
+        if ((st = mcpg::copy_2d(h, dxd, S * 8, X, ldx * 8, S * 8, G, in_kind))) return st;
+      } else if (host) {
+        std::vector<double> tmp((size_t)G * S);
+        for (int64_t i = 0; i < G; ++i)
+          for (int64_t j = 0; j < S; ++j) tmp[i * S + j] = static_cast<double>(X[i * ldx + j]);
+        MCPG_HIP(h, hipMemcpy(dxd, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+      } else {
+        mcpg::LaunchScope ls(h, "convert_kernel");
+        hipLaunchKernelGGL((convert_kernel<T, double>), dim3(grid_of(G * S)), dim3(256), 0, h->stream, X, ldx, G, S,
+                           dxd, S);
+      }
+      xd = dxd;
+      ldxd = S;
+    }
+    double* dmid = reinterpret_cast<double*>(dmi);
+    if (f32 && (st = alloc(h, bmid, (size_t)G * G, &dmid))) return st;
+    if ((st = mcpg_bspline_mi_f64(h, MCPG_DEVICE, xd, G, S, ldxd, bins, order, dmid, G))) return st;
+    if (f32) {
+      mcpg::LaunchScope ls(h, "convert_kernel");
+      hipLaunchKernelGGL((convert_kernel<double, T>), dim3(grid_of(G * G)), dim3(256), 0, h->stream, dmid, G, G, G,
+                         dmi, G);
+    }
   } else {
     // the MI kernels take FP32 expression values
     Buf bx, bmif;

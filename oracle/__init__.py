@@ -30,6 +30,8 @@ _SIGS = {
     "orc_bspline_weights_f32": (None, [_P, c_int64, c_int64, c_int, c_int, _P]),
     "orc_bspline_weights_f64": (None, [_P, c_int64, c_int64, c_int, c_int, c_int, _P, _P]),
     "orc_bspline_mi_f64": (None, [_P, c_int64, c_int64, c_int, c_int, _P]),
+    "orc_bspline_weights_x64": (None, [_P, c_int64, c_int64, c_int, c_int, c_int, _P, _P]),
+    "orc_bspline_mi_x64": (None, [_P, c_int64, c_int64, c_int, c_int, _P]),
     "orc_bspline_mi_pairs_f64": (None, [_P, c_int64, c_int64, c_int, c_int, _P, _P, c_int64, _P]),
     "orc_bspline_mi_loss_f64": (c_double, [_P, c_int64, c_int64, c_int, c_int, _P, _P, _P, c_int64]),
     "orc_bspline_mi_loss_x64": (c_double, [_P, c_int64, c_int64, c_int, c_int, _P, _P, _P, c_int64]),
@@ -174,6 +176,25 @@ def bspline_mi_f64(X, bins=10, order=3):
     G, S = X.shape
     MI = np.empty((G, G), dtype=np.float64)
     lib().orc_bspline_mi_f64(X.ctypes.data, G, S, bins, order, MI.ctypes.data)
+    return MI
+
+
+def bspline_weights_x64(X, bins=10, order=3, scale=True):
+    """bspline_weights_f64 on double rows (the reference's double DataType)."""
+    X = _c(X, np.float64)
+    G, S = X.shape
+    W = np.empty((G, bins, S), dtype=np.float64)
+    H = np.empty(G, dtype=np.float64)
+    lib().orc_bspline_weights_x64(X.ctypes.data, G, S, bins, order, int(scale), W.ctypes.data, H.ctypes.data)
+    return W, H
+
+
+def bspline_mi_x64(X, bins=10, order=3):
+    """bspline_mi_f64 on double rows (the reference's double DataType)."""
+    X = _c(X, np.float64)
+    G, S = X.shape
+    MI = np.empty((G, G), dtype=np.float64)
+    lib().orc_bspline_mi_x64(X.ctypes.data, G, S, bins, order, MI.ctypes.data)
     return MI
 
 

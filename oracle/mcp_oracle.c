@@ -154,17 +154,18 @@ void orc_bspline_weights_f32(const float* Xs, int64_t G, int64_t S, int bins, in
 
 /* FP64 weights of raw rows: x scaled to [0,1] in double (constant rows kept), dense
  * W[g][b][s] and marginal entropy H[g]. */
-void orc_bspline_weights_f64(const float* X, int64_t G, int64_t S, int bins, int order, int do_scale, double* W,
-                             double* H) {
+/* Rows of float (Xf) or double (Xd) values; the scaling and everything after it in double. */
+static void weights_f64_rows(const float* Xf, const double* Xd, int64_t G, int64_t S, int bins, int order, int do_scale,
+                             double* W, double* H) {
   double k[MAXB + MAXO];
   orc_knots_f64(bins, order, k);
 #pragma omp parallel for schedule(static)
   for (int64_t g = 0; g < G; ++g) {
-    const float* x = X + g * S;
-    double mn = S > 0 ? x[0] : 0.0, mx = mn;
+#define XV(j) (Xf ? (double)Xf[g * S + (j)] : Xd[g * S + (j)])
+    double mn = S > 0 ? XV(0) : 0.0, mx = mn;
     for (int64_t j = 1; j < S; ++j) {
-      mn = x[j] < mn ? x[j] : mn;
-      mx = x[j] > mx ? x[j] : mx;
+      mn = XV(j) < mn ? XV(j) : mn;
+      mx = XV(j) > mx ? XV(j) : mx;
     }
     double scale = (do_scale && mx > mn) ? 1.0 / (mx - mn) : 1.0;
     double offset = (do_scale && mx > mn) ? mn : 0.0;
@@ -173,7 +174,7 @@ void orc_bspline_weights_f64(const float* X, int64_t G, int64_t S, int bins, int
     for (int b = 0; b < bins; ++b) {
       double acc = 0.0;
       for (int64_t s = 0; s < S; ++s) {
-        double t = ((double)x[s] - offset) * scale;
+        double t = (XV(s) - offset) * scale;
         double v = orc_basis_f64(k, bins, b, order, t);
         w[b * S + s] = v;
         acc += v;
@@ -182,7 +183,20 @@ void orc_bspline_weights_f64(const float* X, int64_t G, int64_t S, int bins, int
       if (p > 0.0) Hg += p * log2(p);
     }
     H[g] = -Hg;
+#undef XV
   }
+}
+
+void orc_bspline_weights_f64(const float* X, int64_t G, int64_t S, int bins, int order, int do_scale, double* W,
+                             double* H) {
+  weights_f64_rows(X, NULL, G, S, bins, order, do_scale, W, H);
+}
+
+/* The same on double rows: MinMaxScale<double> + BSplineWeightsKernel<double>, the reference's
+ * double DataType (BSplineMI.hpp:75-116, :119-274). */
+void orc_bspline_weights_x64(const double* X, int64_t G, int64_t S, int bins, int order, int do_scale, double* W,
+                             double* H) {
+  weights_f64_rows(NULL, X, G, S, bins, order, do_scale, W, H);
 }
 
 /* Sparse form of FP64 weight rows: per sample the first bin with a non-zero weight (or 0) and
@@ -227,6 +241,24 @@ void orc_bspline_mi_f64(const float* X, int64_t G, int64_t S, int bins, int orde
   int* F = (int*)malloc(sizeof(int) * G * S);
   double* V = (double*)malloc(sizeof(double) * G * S * order);
   orc_bspline_weights_f64(X, G, S, bins, order, 1, W, H);
+  sparse_rows(W, G, S, bins, order, F, V);
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int64_t i = 0; i < G; ++i)
+    for (int64_t j = i; j < G; ++j) {
+      double v = pair_mi_sparse(F, V, H, S, bins, order, i, j);
+      MI[i * G + j] = v;
+      MI[j * G + i] = v;
+    }
+  free(W); free(H); free(F); free(V);
+}
+
+/* compute_mi on double rows (the reference's double DataType): all pairs. */
+void orc_bspline_mi_x64(const double* X, int64_t G, int64_t S, int bins, int order, double* MI) {
+  double* W = (double*)malloc(sizeof(double) * G * bins * S);
+  double* H = (double*)malloc(sizeof(double) * G);
+  int* F = (int*)malloc(sizeof(int) * G * S);
+  double* V = (double*)malloc(sizeof(double) * G * S * order);
+  orc_bspline_weights_x64(X, G, S, bins, order, 1, W, H);
   sparse_rows(W, G, S, bins, order, F, V);
 #pragma omp parallel for schedule(dynamic, 4)
   for (int64_t i = 0; i < G; ++i)

@@ -100,6 +100,32 @@ def test_mi_independent_evaluation_from_scipy_bases():
             assert abs(MI[i, j] - ref) < 1e-9
 
 
+def test_mi_x64_independent_evaluation_from_scipy_bases():
+    """The double-row oracle (the reference's double DataType) against numpy + scipy in double, on
+    rows that no float holds exactly; on float-exact rows it is the float-row oracle, bitwise."""
+    rng = np.random.default_rng(5)
+    X = grn.grn_expression(10, 300, 5)[0].astype(np.float64) * (1.0 + 1e-9 * rng.standard_normal((10, 300)))
+    assert not np.array_equal(X.astype(np.float32).astype(np.float64), X)
+    MI = oracle.bspline_mi_x64(X)
+    mn, mx = X.min(1, keepdims=True), X.max(1, keepdims=True)
+    Xs = (1.0 / (mx - mn)) * (X - mn)
+    W = [scipy_basis(Xs[g], 10, 3) for g in range(10)]
+    S = X.shape[1]
+
+    def H(p):
+        p = p[p > 0]
+        return -(p * np.log2(p)).sum()
+
+    for i in range(10):
+        for j in range(10):
+            ref = H(W[i].mean(0)) + H(W[j].mean(0)) - H((W[i].T @ W[j]) / S)
+            assert abs(MI[i, j] - ref) < 1e-12
+    Wx, Hx = oracle.bspline_weights_x64(X)
+    np.testing.assert_allclose(Wx[4].T, W[4], rtol=0, atol=1e-14)
+    Xf = grn.grn_expression(8, 200, 6)[0]
+    assert np.array_equal(oracle.bspline_mi_x64(Xf.astype(np.float64)), oracle.bspline_mi_f64(Xf))
+
+
 def test_mi_invariances():
     X = grn.grn_expression(10, 400, 4)[0]
     MI = oracle.bspline_mi_f64(X)
