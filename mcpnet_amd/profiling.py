@@ -81,8 +81,11 @@ def mi_main(genes: int, samples: int, precision: str, reps: int, seed: int = 0x4
     Xd = torch.from_numpy(X).cuda()
     MI = torch.empty((genes, genes), dtype=torch.float32, device="cuda")
     for prec in precision.split(","):
+        x, out = Xd, MI
+        if prec == "fp64":   # double rows and MI (the values of the same expression matrix)
+            x, out = Xd.double(), torch.empty((genes, genes), dtype=torch.float64, device="cuda")
         for r in range(reps):
-            h.bspline_mi(Xd, precision=prec, out=MI)
+            h.bspline_mi(x, precision=prec, out=out)
             for name, ms in h.last_timings():
                 print(f"{genes} x {samples} {prec} rep {r}: {name} {ms:.3f} ms", flush=True)
     h.close()
@@ -96,7 +99,8 @@ def _db(d: str):
 
 
 def _short(name: str) -> str:
-    for key in ("bspline_mi_kernel", "maxmin_key_kernel", "maxmin_tile_kernel", "key_prep_kernel",
+    for key in ("bspline_mi_kernel", "bspline_mi64_kernel", "bspline_prep64_kernel", "weights_import64_kernel",
+                "maxmin_key_kernel", "maxmin_tile_kernel", "key_prep_kernel",
                 "key_prep_f64_kernel", "key_pad_kernel",
                 "bspline_prep_kernel", "ap_mi_kernel", "ratio_kernel", "symmetry_check_kernel", "rank_kernel",
                 "cumhist_kernel", "noise_kernel", "weights_import_kernel", "bspline_pair_adjoint_kernel",
@@ -231,7 +235,7 @@ def main(argv=None) -> None:
     b = sub.add_parser("mi")
     b.add_argument("--genes", type=int, default=20000)
     b.add_argument("--samples", type=int, default=2000)
-    b.add_argument("--precision", default="bf16x3", help="fp32, bf16, bf16x3 or i8x3, or a comma-separated list")
+    b.add_argument("--precision", default="bf16x3", help="fp32, bf16, bf16x3, i8x3 or fp64, or a comma-separated list")
     b.add_argument("--seed", type=lambda v: int(v, 0), default=0x4D49, help="GRN generator seed")
     b.add_argument("--reps", type=int, default=3)
     for name in ("summarize", "traffic"):

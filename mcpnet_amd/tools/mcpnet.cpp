@@ -15,6 +15,8 @@
 #include <iomanip>
 #include <set>
 #include <sstream>
+#include <stdexcept>
+#include <type_traits>
 
 #include "common.hpp"
 #include "mcp/filter/mcp.hpp"
@@ -119,6 +121,18 @@ void compute_stouffer_tf(const splash::ds::aligned_matrix<T>& in, splash::ds::al
                 "compute_stouffer_tf");
 }
 
+// -1 selects the DataType's own arithmetic: MCPG_FP64 for the default double run, MCPG_FP32 with
+// --single (which cannot take MCPG_FP64)
+template <typename T>
+int precision_code(const cli::Parser& p) {
+  const int c = (int)p.integer("--precision-code");
+  const bool dbl = std::is_same<T, double>::value;
+  if (c == -1) return dbl ? MCPG_FP64 : MCPG_FP32;
+  if (c < 0 || c > MCPG_FP64) throw std::runtime_error("--precision-code must be 0, 1, 2, 3 or 4");
+  if (c == MCPG_FP64 && !dbl) throw std::runtime_error("--precision-code 4 (fp64) needs the double DataType");
+  return c;
+}
+
 template <typename T>
 int run(const cli::Parser& p) {
   using Mat = splash::ds::aligned_matrix<T>;
@@ -167,7 +181,7 @@ int run(const cli::Parser& p) {
       ldo[m - 1] = dratio[m - 1].ld();
     }
   mcpgpu::check(mcpnet_call(mcpgpu::thread_handle(), input.data(), G, mi_input ? 0 : input.columns(), input.ld(),
-                            mi_input ? -1 : mim, (int)p.integer("--precision-code"), (uint64_t)p.integer("--noise-seed"),
+                            mi_input ? -1 : mim, precision_code<T>(p), (uint64_t)p.integer("--noise-seed"),
                             p.real("--diagonal"), tfonly ? tfs.data() : nullptr, mi.data(), mi.ld(), outs, ldo),
                 "mcpnet pipeline");
   printf("Computed,MI and MCP,,%f,sec\n", t.seconds());
@@ -228,7 +242,8 @@ int main(int argc, char** argv) {
   p.option("--clamped", "output is clamped (no effect, as in the reference)", false);
   p.option("--diagonal", "MI diagonal value; negative keeps the MI diagonal", true, "0");
   p.option("--format", "output format: h5 (as the reference), exp, csv or tsv", true, "h5");
-  p.option("--precision-code", "B-spline contraction: 0 fp32, 1 bf16, 2 bf16x3, 3 i8x3", true, "0");
+  p.option("--precision-code",
+           "B-spline contraction: 0 fp32, 1 bf16, 2 bf16x3, 3 i8x3, 4 fp64 (default: 4, 0 with --single)", true, "-1");
   p.option("--noise-seed", "seed of the U[0,1e-5) rank jitter", true, "1296388676612");
   p.option("--tf-input", "transcription factor file, 1 per line.", true, "");
   try {

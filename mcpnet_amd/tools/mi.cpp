@@ -4,6 +4,8 @@
 // -m 0 computes the MI.
 #include <cstdio>
 #include <iostream>
+#include <stdexcept>
+#include <type_traits>
 
 #include "common.hpp"
 #include "mcp/correlation/AdaptivePartitioningMI.hpp"
@@ -12,12 +14,20 @@
 
 namespace {
 
+// "" selects the DataType's own arithmetic: fp64 for the default double run, fp32 with --single
+template <typename DataType>
 int precision_of(const std::string& s) {
+  const bool dbl = std::is_same<DataType, double>::value;
+  if (s.empty()) return dbl ? MCPG_FP64 : MCPG_FP32;
   if (s == "fp32") return MCPG_FP32;
   if (s == "bf16") return MCPG_BF16;
   if (s == "bf16x3") return MCPG_BF16X3;
   if (s == "i8x3") return MCPG_I8X3;
-  throw std::runtime_error("--precision must be fp32, bf16, bf16x3 or i8x3");
+  if (s == "fp64") {
+    if (!dbl) throw std::runtime_error("--precision fp64 needs the double DataType (no --single)");
+    return MCPG_FP64;
+  }
+  throw std::runtime_error("--precision must be fp32, bf16, bf16x3, i8x3 or fp64");
 }
 
 template <typename DataType>
@@ -70,7 +80,8 @@ int run(const cli::Parser& p) {
     t = tool::Timer();
     splash::pattern::InnerProduct<Mat, mcp::kernel::BSplineMIKernel<DataType>, Mat, true> mi_proc;
     output.resize(input.rows(), input.rows());
-    mi_proc(weighted, weighted, mcp::kernel::BSplineMIKernel<DataType>(bins, S, precision_of(p.str("--precision"))),
+    mi_proc(weighted, weighted,
+            mcp::kernel::BSplineMIKernel<DataType>(bins, S, precision_of<DataType>(p.str("--precision"))),
             output);
     printf("Compute MI in %f sec\n", t.seconds());
   } else {
@@ -90,7 +101,9 @@ int main(int argc, char** argv) {
   p.option("-b,--bins", "No. of bins", true, "10");
   p.option("-l,--spline_order", "Spline Order", true, "3");
   p.option("-m,--method", "Algorithm: B-spline MI=0, Adaptive partitioning=1", true, "1");
-  p.option("--precision", "B-spline contraction arithmetic: fp32, bf16, bf16x3 or i8x3", true, "fp32");
+  p.option("--precision",
+           "B-spline contraction arithmetic: fp32, bf16, bf16x3, i8x3 or fp64 (default: fp64, fp32 with --single)",
+           true, "");
   p.option("--noise-seed", "seed of the U[0,1e-5) rank jitter", true, "1296388676612");
   try {
     p.parse(argc, argv);

@@ -59,6 +59,28 @@ def test_mi_bspline_on_reference_data(tmp_path, src):
     assert np.abs(mi - ref).max() <= 1e-5
 
 
+@pytest.mark.parametrize("src", [R10, R10_H5])
+def test_mi_bspline_double_default(tmp_path, src):
+    """mi -m 0 without --single runs the reference's double DataType: scaling, weights and MI in
+    double (the *_f64 entry points), within 1e-12 of the double-row oracle on the reference's data;
+    --precision bf16x3 in the double run contracts FP32 images of the double weights."""
+    out = tmp_path / "r10c10.mi.exp"
+    run("mi", "-i", src, "-o", out, "-m", "0")
+    _, mi = read_exp(out)
+    _, X = read_exp(R10)
+    ref = oracle.bspline_mi_x64(np.ascontiguousarray(X.T))
+    assert np.abs(mi - ref).max() <= 1e-12
+    run("mi", "-i", src, "-o", out, "-m", "0", "--precision", "bf16x3")
+    assert np.abs(read_exp(out)[1] - ref).max() <= 1e-4
+
+
+def test_mi_tool_rejects_fp64_with_single(tmp_path):
+    exe = BIN / "mi"
+    res = subprocess.run([str(exe), "-i", str(R10), "-o", str(tmp_path / "o.exp"), "-m", "0", "--single",
+                          "--precision", "fp64"], capture_output=True, text=True, timeout=120)
+    assert res.returncode != 0 and "double" in res.stderr
+
+
 @pytest.mark.parametrize("precision", ["bf16x3", "i8x3"])
 def test_mi_tool_split_precisions(tmp_path, precision):
     """mi --precision bf16x3 / i8x3 on 60 genes x 500 samples of GRN data, against the FP64 oracle."""
@@ -195,6 +217,24 @@ def test_mcpnet_pipeline(tmp_path):
     for k in (2, 3):
         got = read_exp(str(prefix) + f"_bs.mcp{k}.exp")[1].astype(np.float32)
         assert np.array_equal(got, oracle.mcp(A, k)[0])
+
+
+def test_mcpnet_pipeline_double(tmp_path):
+    """mcpnet --mi-method 0 in the default double run: the MI in double (within 1e-12 of the
+    double-row oracle, diagonal 0) and the MCP scores of that MI, bit-exact against the oracle."""
+    rng = np.random.default_rng(13)
+    X = grn.grn_expression(40, 300, 13)[0].astype(np.float64) * (1.0 + 1e-9 * rng.standard_normal((40, 300)))
+    src = tmp_path / "x.exp"
+    write_exp(src, X)
+    prefix = tmp_path / "net"
+    run("mcpnet", "-i", src, "-o", prefix, "--mi-method", "0", "-m", "1", "2", "--format", "exp")
+    _, mi = read_exp(str(prefix) + "_bs.mi.exp")
+    ref = oracle.bspline_mi_x64(read_exp(src)[1])
+    np.fill_diagonal(ref, 0.0)
+    assert np.abs(mi - ref).max() <= 1e-12
+    for k in (2, 3):
+        got = read_exp(str(prefix) + f"_bs.mcp{k}.exp")[1]
+        assert np.array_equal(got, oracle.mcp(mi, k)[0])
 
 
 def test_tools_report_errors(tmp_path):
