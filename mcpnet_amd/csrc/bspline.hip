@@ -1212,41 +1212,14 @@ __device__ __forceinline__ void mi64_load(const double* pa, const double* pb, in
   }
 }
 
-// All-pairs FP64 MI: the 160 x 160 workgroup tile of the other MI kernels, four waves of 80 x 80
-// (5 x 5 blocks of 16 x 16 FP64 MFMA outputs, 200 registers).  The FP64 MFMA takes 64 cycles
-// per 16 x 16 x 4 product, so operands come straight from memory into registers (two K blocks in
-// flight, one wave per SIMD) rather than through LDS.  Epilogue: t = p log2 p in double with
-// p = dot * (1 / S), summed per gene pair over bins x bins in the reference's (x bin, y bin)
+// t = p log2 p in double with p = dot * (1 / S) (BSplineEntropyKernel::entropy2d_simd's
+// hx *= norm_factor), summed per gene pair over bins x bins in the reference's (x bin, y bin)
 // order through LDS (two waves at a time), and MI = (H_i + H_j) - (0 - sum t)
-// (BSplineMIKernel::operator(), BSplineMI.hpp:492-499; entropy2d_simd :264-286).
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void bspline_mi64_kernel(MI64Args a) {
-  extern __shared__ __align__(16) char smem[];
-  const uint32_t code = a.order[blockIdx.x];
-  const int64_t ti = code >> 16, tj = code & 0xffffu;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+// (BSplineMIKernel::operator(), BSplineMI.hpp:492-499).  Every thread of the block calls it.
+__device__ __forceinline__ void mi64_epilogue(const MI64Args& a, const doublex4 (&acc)[5][5], int64_t ti, int64_t tj,
+                                              char* smem) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = wave >> 1, wc = wave & 1, lr = lane & 15, lq = lane >> 4;
-  const int64_t nk = a.L.Sp / kKStep64;
-  const int64_t group_elems = nk * (kGroupRows * kKStep64);
-  const double* pa = a.w + (ti * 2 + wr) * group_elems + lr * kKStep64 + 2 * lq;
-  const double* pb = a.w + (tj * 2 + wc) * group_elems + lr * kKStep64 + 2 * lq;
-
-  doublex4 acc[5][5];
-#pragma unroll
-  for (int i = 0; i < 5; ++i)
-#pragma unroll
-    for (int j = 0; j < 5; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
-  doublex2 a0[5], b0[5], a1[5], b1[5];
-  mi64_load(pa, pb, 0, a0, b0);
-  if (nk > 1) mi64_load(pa, pb, 1, a1, b1);
-  for (int64_t kb = 0; kb < nk; kb += 2) {
-    mi64_step(a0, b0, acc);
-    if (kb + 2 < nk) mi64_load(pa, pb, kb + 2, a0, b0);
-    if (kb + 1 < nk) {
-      mi64_step(a1, b1, acc);
-      if (kb + 3 < nk) mi64_load(pa, pb, kb + 3, a1, b1);
-    }
-  }
-
   double* E = reinterpret_cast<double*>(smem);
   const bool diag = ti == tj;
   const int bins = a.L.bins, gw = a.L.gw;
@@ -1285,6 +1258,47 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))
     }
     __syncthreads();
   }
+}
+
+// All-pairs FP64 MI: the 160 x 160 workgroup tile of the other MI kernels, four waves of 80 x 80
+// (5 x 5 blocks of 16 x 16 FP64 MFMA outputs, 200 registers).  The FP64 MFMA takes 64 cycles
+// per 16 x 16 x 4 product, so operands come straight from memory into a ring of NBUF register
+// sets (NBUF K blocks in flight, one wave per SIMD).  Staging them through an LDS ring of direct
+// loads instead, with a barrier per K block, measured 1.5-1.6x slower at one wave per SIMD.
+// Epilogue: mi64_epilogue.
+template <int NBUF>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) void bspline_mi64_kernel(MI64Args a) {
+  extern __shared__ __align__(16) char smem[];
+  const uint32_t code = a.order[blockIdx.x];
+  const int64_t ti = code >> 16, tj = code & 0xffffu;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1, lr = lane & 15, lq = lane >> 4;
+  const int64_t nk = a.L.Sp / kKStep64;
+  const int64_t group_elems = nk * (kGroupRows * kKStep64);
+  const double* pa = a.w + (ti * 2 + wr) * group_elems + lr * kKStep64 + 2 * lq;
+  const double* pb = a.w + (tj * 2 + wc) * group_elems + lr * kKStep64 + 2 * lq;
+
+  doublex4 acc[5][5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 5; ++j) acc[i][j] = doublex4{0.0, 0.0, 0.0, 0.0};
+  // a ring of NBUF register sets: K block kb + NBUF loads into the set K block kb just used
+  doublex2 fa[NBUF][5], fb[NBUF][5];
+#pragma unroll
+  for (int u = 0; u < NBUF; ++u)
+    if (u < nk) mi64_load(pa, pb, u, fa[u], fb[u]);
+  for (int64_t kb = 0; kb < nk; kb += NBUF) {
+#pragma unroll
+    for (int u = 0; u < NBUF; ++u) {
+      if (kb + u < nk) {
+        mi64_step(fa[u], fb[u], acc);
+        if (kb + u + NBUF < nk) mi64_load(pa, pb, kb + u + NBUF, fa[u], fb[u]);
+      }
+    }
+  }
+
+  mi64_epilogue(a, acc, ti, tj, smem);
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -1952,12 +1966,23 @@ int mi64_launch(mcpg_ctx* h, const double* w, const double* H64, const Layout& L
   if (!a.order) return st;
   static bool attr_set = false;
   if (!attr_set) {
-    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi64_kernel),
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi64_kernel<2>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kEpi64Bytes));
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi64_kernel<3>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, kEpi64Bytes));
+    MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi64_kernel<4>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kEpi64Bytes));
     attr_set = true;
   }
+  // MCPG_MI64_BUF: register sets in the load ring (2 to 4; 3 by default)
+  const int nbuf = std::min(std::max(mcpg::env_group("MCPG_MI64_BUF", 3), 2), 4);
   mcpg::LaunchScope ls(h, "bspline_mi64_kernel");
-  hipLaunchKernelGGL(bspline_mi64_kernel, dim3((unsigned)ntri), dim3(kThreads), kEpi64Bytes, h->stream, a);
+  if (nbuf == 2)
+    hipLaunchKernelGGL(bspline_mi64_kernel<2>, dim3((unsigned)ntri), dim3(kThreads), kEpi64Bytes, h->stream, a);
+  else if (nbuf == 4)
+    hipLaunchKernelGGL(bspline_mi64_kernel<4>, dim3((unsigned)ntri), dim3(kThreads), kEpi64Bytes, h->stream, a);
+  else
+    hipLaunchKernelGGL(bspline_mi64_kernel<3>, dim3((unsigned)ntri), dim3(kThreads), kEpi64Bytes, h->stream, a);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return mcpg::hip_error(h, e, "bspline_mi64_kernel launch");
   return MCPG_OK;

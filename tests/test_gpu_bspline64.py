@@ -65,6 +65,21 @@ def test_constant_gene_and_float_rows(handle):
     assert np.abs(got - oracle.bspline_mi_f64(X.astype(np.float32))).max() <= TOL
 
 
+@pytest.mark.parametrize("env", [{"MCPG_MI64_BUF": "2"}, {"MCPG_MI64_BUF": "4"}])
+def test_kernel_forms_bit_identical(handle, monkeypatch, env):
+    """Register rings of 3 K blocks (default), 2 or 4 run the same MFMA sequence per output:
+    bit-identical MI, including sample counts with one and with two K blocks."""
+    for G, S in ((200, 777), (35, 8), (50, 3), (40, 13)):
+        X = double_rows(G, S, G * S)
+        ref = handle.bspline_mi(X, precision="fp64")
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        got = handle.bspline_mi(X, precision="fp64")
+        for k in env:
+            monkeypatch.delenv(k)
+        assert np.array_equal(got, ref), (G, S)
+
+
 def test_mi_from_weights_equals_fused(handle):
     X = double_rows(50, 180, 3)
     W = handle.bspline_weights(handle.minmax_scale(X))
