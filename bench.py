@@ -34,6 +34,7 @@ PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_INT8_TOPS = 5000.0
+PEAK_FP64_TFLOPS = 78.6
 # VALU issue: 256 CUs x 64 lanes x 2.4 GHz; the max-min inner loop spends 1.5 instructions per
 # (i, k, j) triple (v_min_i32 per triple + v_max3_i32 per two triples).
 CUS, LANES, CLOCK_GHZ = 256, 64, 2.4
@@ -347,6 +348,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of each bounded CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 MI comparison pass")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 (double DataType) MI pass")
     ap.add_argument("--extras", action="store_true", help="also time the ensemble MCP and the DPI filter "
                                                           "(always on for config 2)")
     ap.add_argument("--no-secondary", action="store_true",
@@ -367,7 +369,7 @@ def main():
         for cid in (1, 2, 4, 5):
             sub = argparse.Namespace(**vars(args))
             sub.steps, sub.warmup, sub.precision = 3, 1, None
-            sub.no_cpu, sub.no_fp32, sub.extras = True, True, False
+            sub.no_cpu, sub.no_fp32, sub.no_fp64, sub.extras = True, True, True, False
             log(f"bench: secondary config {cid}")
             r = run_config(sub, cid)
             sec[f"config{cid}"] = {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "dtype", "config",
@@ -384,8 +386,8 @@ def mi_kernel_name(precision):
 def mi_roofline(precision, flops, ms):
     """Algorithmic work (2*B^2*S per unordered pair; the split products are not credited) per
     launch over the launch's mean duration, against the dense MFMA peak of the arithmetic used."""
-    peak, unit = {"fp32": (PEAK_FP32_TFLOPS, "TFLOP/s"), "i8x3": (PEAK_INT8_TOPS, "TOP/s")}.get(
-        precision, (PEAK_BF16_TFLOPS, "TFLOP/s"))
+    peak, unit = {"fp32": (PEAK_FP32_TFLOPS, "TFLOP/s"), "i8x3": (PEAK_INT8_TOPS, "TOP/s"),
+                  "fp64": (PEAK_FP64_TFLOPS, "TFLOP/s")}.get(precision, (PEAK_BF16_TFLOPS, "TFLOP/s"))
     ach = flops / (ms / 1e3) / 1e12
     return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": unit, "frac": ach / peak, "traffic": None,
             "mean_ms": ms, "work_per_launch": flops,
@@ -554,6 +556,29 @@ def run_config(args, cid):
             attach_traffic(mi_kernel_name("i8x3"), i8["roofline"])
         extra["mi_i8x3"] = i8
         del MI8
+    f64 = None
+    if args.precision == "bf16x3" and not args.no_fp64:
+        # the reference's double DataType on the same matrix (its values as doubles): scaling,
+        # basis, marginals and joint tables in double on the FP64 MFMA
+        X64 = Xd.double()
+        MI64 = torch.empty((G, G), dtype=torch.float64, device=dev)
+        h.bspline_mi(X64, bins=bins, precision="fp64", out=MI64)
+        ms64, t64 = [], []
+        for _ in range(1 if G > 5000 else 3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            h.bspline_mi(X64, bins=bins, precision="fp64", out=MI64)
+            e1.record(stream)
+            ms64 += [ms for n, ms in h.last_timings() if n == "bspline_mi64_kernel"]
+            torch.cuda.synchronize()
+            t64.append(e0.elapsed_time(e1) / 1e3)
+        t64 = float(np.mean(t64))
+        f64 = {"pairs_per_s": P / t64, "stage_ms": t64 * 1e3, "bf16x3_speedup": t64 / t_mi,
+               "dtype": "double throughout (the reference's default double DataType): v_mfma_f64_16x16x4_f64",
+               "roofline": mi_roofline("fp64", mi_flops, float(np.mean(ms64))) if ms64 else None}
+        if f64["roofline"]:
+            attach_traffic("bspline_mi64_kernel", f64["roofline"])
+        extra["mi_fp64"] = f64
 
     cpu = {}
     if not args.no_cpu:
@@ -578,6 +603,12 @@ def run_config(args, cid):
             e8 = np.abs(got8.astype(np.float64) - ref)
             i8["err_vs_fp64"] = {"max": float(e8.max()), "mean": float(e8.mean()), "pairs": int(len(li)),
                                  "sample": "the same pairs"}
+        if f64 is not None:
+            got64 = MI64[torch.from_numpy(genes[li]).long().to(dev),
+                         torch.from_numpy(genes[lj]).long().to(dev)].cpu().numpy()
+            e64 = np.abs(got64 - ref)
+            f64["err_vs_fp64"] = {"max": float(e64.max()), "mean": float(e64.mean()), "pairs": int(len(li)),
+                                  "sample": "the same pairs"}
         if G <= 2000:   # small configs: every pair, every precision
             full = oracle.bspline_mi_f64(X)
             errs = {}
