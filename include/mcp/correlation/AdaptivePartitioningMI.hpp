@@ -3,7 +3,7 @@
 // apply them specialised to the GPU:
 //   add_noise<T, Distribution, Generator>::operator()(in, out)          -> mcpg_add_noise_f32
 //   Transform<M<IT>, IntegralCumulativeHistogramKernel<IT,0>, M<IT>>    -> mcpg_cumhist_i32
-//   InnerProduct<M<IT>, AdaptivePartitionRankMIKernel2<IT,OT,0>, M<OT>> -> mcpg_ap_mi_i32
+//   InnerProduct<M<IT>, AdaptivePartitionRankMIKernel2<IT,OT,0>, M<OT>> -> mcpg_ap_mi_i32 / _f64 (OT double)
 // The reference's add_noise draws from per-thread std::default_random_engine streams of splash
 // (absent); here the jitter is a seeded counter-based stream, identical for any thread count.
 // Like the reference header, it brings in splash/kernel/random.hpp for the
@@ -82,6 +82,25 @@ class AdaptivePartitionRankMIKernel2 {
 }  // namespace kernel
 }  // namespace mcp
 
+namespace mcpgpu {
+// The output arithmetic of AdaptivePartitionRankMIKernel2<IT, OT>: its result mi/S + log2 S is a
+// double (AdaptivePartitioningMI.hpp:644), kept for OT = double, rounded to float otherwise.
+template <typename OT>
+struct ap_value {
+  typedef float type;
+};
+template <>
+struct ap_value<double> {
+  typedef double type;
+};
+inline int ap_mi(const int32_t* r, size_t G, size_t S, float* mi) {
+  return mcpg_ap_mi_i32(thread_handle(), MCPG_HOST, r, G, S, S, mi, G);
+}
+inline int ap_mi(const int32_t* r, size_t G, size_t S, double* mi) {
+  return mcpg_ap_mi_f64(thread_handle(), MCPG_HOST, r, G, S, S, mi, G);
+}
+}  // namespace mcpgpu
+
 namespace splash {
 namespace pattern {
 
@@ -108,9 +127,9 @@ class InnerProduct<ds::aligned_matrix<IT>, mcp::kernel::AdaptivePartitionRankMIK
       throw mcpgpu::error(MCPG_ERR_INVALID_ARG, "AdaptivePartitionRankMIKernel2: the GPU path pairs rows of one rank matrix");
     const size_t G = first.rows(), S = first.columns();
     std::vector<int32_t> r = mcpgpu::packed<int32_t>(first);
-    std::vector<float> mi(G * G);
-    mcpgpu::check(mcpg_ap_mi_i32(mcpgpu::thread_handle(), MCPG_HOST, r.data(), G, S, S, mi.data(), G),
-                  "AdaptivePartitionRankMIKernel2");
+    typedef typename mcpgpu::ap_value<OT>::type V;
+    std::vector<V> mi(G * G);
+    mcpgpu::check(mcpgpu::ap_mi(r.data(), G, S, mi.data()), "AdaptivePartitionRankMIKernel2");
     mcpgpu::unpack(mi, G, G, out);
   }
 };

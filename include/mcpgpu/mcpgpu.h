@@ -170,6 +170,12 @@ int mcpg_cumhist_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64
 int mcpg_ap_mi_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,
                    float* MI, int64_t ldmi);
 
+/* The same with the MI kept in double: AdaptivePartitionRankMIKernel2<IT, double, 0>, the
+ * reference's double DataType, whose result mi/S + log2 S (AdaptivePartitioningMI.hpp:644) is not
+ * rounded to float.  Bit-identical to mcpg_ap_mi_i32 before that rounding. */
+int mcpg_ap_mi_f64(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,
+                   double* MI, int64_t ldmi);
+
 /* One block of the same matrix (rows row0 .. row0+nrows-1, columns col0 .. col0+ncols-1 into MI,
  * ldmi >= ncols), bit-identical to those entries of mcpg_ap_mi_i32: each entry (i, j) is
  * kernel(x=min(i,j), y=max(i,j)).  The row-partition unit of mi.cpp:170-177 and

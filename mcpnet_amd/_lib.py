@@ -52,6 +52,7 @@ _SIGS = {
     "mcpg_bspline_weights_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
     "mcpg_bspline_mi_weights_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, _P, c_int64],
     "mcpg_bspline_mi_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
+    "mcpg_ap_mi_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
     "mcpg_bspline_mi_block_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, c_int, c_int64, c_int64, c_int64,
                                   c_int64, _P, c_int64],
     "mcpg_bspline_mi_backward_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, _P, _P, c_int64,
@@ -386,12 +387,16 @@ class Handle:
         self._check(self.lib.mcpg_cumhist_i32(self.h, r.kind, r.ptr, G, S, r.ld, hm.ptr, hm.ld), "mcpg_cumhist_i32")
         return out
 
-    def ap_mi(self, R, out=None):
+    def ap_mi(self, R, out=None, dtype=np.float32):
+        """All-pairs AP-MI of rank rows; dtype float64 keeps the MI in double (mcpg_ap_mi_f64)."""
+        dt = np.float64 if np.dtype(dtype) == np.float64 else np.float32
         r = _Mat(R, np.int32, "R")
         G, S = r.shape
-        out, m = _out_mat(out, r.kind, (G, G), np.float32, R, "out")
+        out, m = _out_mat(out, r.kind, (G, G), dt, R, "out")
         self._pre(r.kind)
-        self._check(self.lib.mcpg_ap_mi_i32(self.h, r.kind, r.ptr, G, S, r.ld, m.ptr, m.ld), "mcpg_ap_mi_i32")
+        fn, name = ((self.lib.mcpg_ap_mi_f64, "mcpg_ap_mi_f64") if dt == np.float64 else
+                    (self.lib.mcpg_ap_mi_i32, "mcpg_ap_mi_i32"))
+        self._check(fn(self.h, r.kind, r.ptr, G, S, r.ld, m.ptr, m.ld), name)
         return out
 
     def ap_mi_block(self, R, row0, nrows, col0, ncols, out=None):

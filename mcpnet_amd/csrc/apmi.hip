@@ -249,6 +249,7 @@ struct APArgs {
   const int32_t* P;   // samples grouped by rank, G x S (ybucket_kernel)
   int64_t G, S;
   float* MI;
+  double* MI64;       // non-null: the MI kept in double (AdaptivePartitionRankMIKernel2<IT, double>)
   int64_t ldmi;
   // output block: MI holds rows r0 .. r0 + nr - 1 and columns c0 .. c0 + nc - 1 of the G x G
   // matrix; block = false: the whole matrix, its pairs enumerated over the upper triangle
@@ -722,9 +723,16 @@ void ap_mi_kernel(APArgs a, const int32_t* __restrict__ H) {
       if (lane == 0) {
         const float f = (float)v;
         const uint64_t ri = (uint64_t)(i - a.r0), cj = (uint64_t)(j - a.c0);
-        if (ri < (uint64_t)a.nr && cj < (uint64_t)a.nc) a.MI[(int64_t)ri * a.ldmi + (int64_t)cj] = f;
         const uint64_t rj = (uint64_t)(j - a.r0), ci = (uint64_t)(i - a.c0);
-        if (i != j && rj < (uint64_t)a.nr && ci < (uint64_t)a.nc) a.MI[(int64_t)rj * a.ldmi + (int64_t)ci] = f;
+        const bool at_ij = ri < (uint64_t)a.nr && cj < (uint64_t)a.nc;
+        const bool at_ji = i != j && rj < (uint64_t)a.nr && ci < (uint64_t)a.nc;
+        if (a.MI64) {
+          if (at_ij) a.MI64[(int64_t)ri * a.ldmi + (int64_t)cj] = v;
+          if (at_ji) a.MI64[(int64_t)rj * a.ldmi + (int64_t)ci] = v;
+        } else {
+          if (at_ij) a.MI[(int64_t)ri * a.ldmi + (int64_t)cj] = f;
+          if (at_ji) a.MI[(int64_t)rj * a.ldmi + (int64_t)ci] = f;
+        }
       }
     };
     // All pairs (i <= j, row-major over the upper triangle; for a block, its positions row-major),
@@ -900,9 +908,10 @@ int mcpg_cumhist_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64
 
 // The ranks' histograms, buckets and cell codes, then the pair kernel over the whole matrix
 // (blk == nullptr), a block of it (blk = {row0, nrows, col0, ncols}) or the listed pairs (cells).
+// MI64 non-null: the MI is written in double there instead of MI.
 static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, float* MI, int64_t ldmi,
                   const int32_t* px, const int32_t* py, int64_t npairs, int32_t max_leaves, int32_t* cells,
-                  int32_t* nleaves, double* miv, const int64_t* blk = nullptr) {
+                  int32_t* nleaves, double* miv, const int64_t* blk = nullptr, double* MI64 = nullptr) {
   MCPG_HIP(h, hipSetDevice(h->device));
   int st;
   const bool cells_mode = npairs > 0;
@@ -913,7 +922,10 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   sz.add<int32_t>(G);
   sz.add<uint16_t>(G * ap_ldz(S));
   const int64_t r0 = blk ? blk[0] : 0, nr = blk ? blk[1] : G, c0 = blk ? blk[2] : 0, nc = blk ? blk[3] : G;
-  if (!cells_mode && ptrs == MCPG_HOST) sz.add<float>(nr * nc);
+  if (!cells_mode && ptrs == MCPG_HOST) {
+    if (MI64) sz.add<double>(nr * nc);
+    else sz.add<float>(nr * nc);
+  }
   if (cells_mode) {
     sz.add<int32_t>(npairs); sz.add<int32_t>(npairs);
     sz.add<int32_t>(npairs * 3 * (int64_t)max_leaves); sz.add<int32_t>(npairs); sz.add<double>(npairs);
@@ -933,8 +945,13 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
   int32_t* dId = cv.take<int32_t>(G);
   uint16_t* dZ = cv.take<uint16_t>(G * ap_ldz(S));
   float* dMI = MI;
+  double* dMI64 = MI64;
   int64_t lm = ldmi;
-  if (!cells_mode && ptrs == MCPG_HOST) { dMI = cv.take<float>(nr * nc); lm = nc; }
+  if (!cells_mode && ptrs == MCPG_HOST) {
+    if (MI64) dMI64 = cv.take<double>(nr * nc);
+    else dMI = cv.take<float>(nr * nc);
+    lm = nc;
+  }
   if ((st = mcpg::reset_flags(h))) return st;
   {
     mcpg::LaunchScope ls(h, "cumhist_kernel");
@@ -954,7 +971,7 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
                        0, h->stream, dR, lr, G, S, dZ, ap_ldz(S));
   }
   APArgs a{};
-  a.R = dR; a.ldr = lr; a.H = dH; a.ldh = S + 1; a.P = dP; a.G = G; a.S = S; a.MI = dMI; a.ldmi = lm;
+  a.R = dR; a.ldr = lr; a.H = dH; a.ldh = S + 1; a.P = dP; a.G = G; a.S = S; a.MI = dMI; a.MI64 = dMI64; a.ldmi = lm;
   a.r0 = r0; a.nr = nr; a.c0 = c0; a.nc = nc; a.block = blk != nullptr;
   a.flags = h->d_flags;
   a.queue = reinterpret_cast<unsigned long long*>(h->d_flags + 8);   // zeroed by reset_flags
@@ -1000,6 +1017,8 @@ static int ap_run(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S,
     MCPG_HIP(h, hipMemcpyAsync(cells, d_cells, npairs * 3 * (int64_t)max_leaves * 4, hipMemcpyDeviceToHost, h->stream));
     MCPG_HIP(h, hipMemcpyAsync(nleaves, d_nl, npairs * 4, hipMemcpyDeviceToHost, h->stream));
     MCPG_HIP(h, hipMemcpyAsync(miv, d_mi, npairs * 8, hipMemcpyDeviceToHost, h->stream));
+  } else if (ptrs == MCPG_HOST && MI64) {
+    if ((st = mcpg::copy_2d(h, MI64, ldmi * 8, dMI64, nc * 8, nc * 8, nr, hipMemcpyDeviceToHost))) return st;
   } else if (ptrs == MCPG_HOST) {
     if ((st = mcpg::copy_2d(h, MI, ldmi * 4, dMI, nc * 4, nc * 4, nr, hipMemcpyDeviceToHost))) return st;
   }
@@ -1019,6 +1038,20 @@ int mcpg_ap_mi_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t
   if (ldr < S || ldmi < G) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   return ap_run(h, ptrs, R, G, S, ldr, MI, ldmi, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr);
+}
+
+int mcpg_ap_mi_f64(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, double* MI,
+                   int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
+  if (G == 0) return MCPG_OK;
+  if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
+  if (S > kMaxS) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "adaptive-partitioning MI supports S <= 8191");
+  if (!R || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (ldr < S || ldmi < G) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
+  return ap_run(h, ptrs, R, G, S, ldr, nullptr, ldmi, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, MI);
 }
 
 int mcpg_ap_mi_block_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, int64_t row0,

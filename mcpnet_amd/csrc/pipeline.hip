@@ -101,8 +101,9 @@ struct Stage<double> {
 };
 
 // The G x G MI matrix (FP32-class arithmetic) of the G x S expression matrix X in device memory.
+// With MI64 (adaptive partitioning only) the MI is written there in double instead.
 int mi_f32(mcpg_ctx* h, const float* X, int64_t G, int64_t S, int64_t ldx, int method, int bins, int order,
-           int precision, uint64_t seed, float* MI) {
+           int precision, uint64_t seed, float* MI, double* MI64 = nullptr) {
   if (method == 0) return mcpg_bspline_mi_f32(h, MCPG_DEVICE, X, G, S, ldx, bins, order, precision, MI, G);
   // compute_ap_mi (mcpnet_helper.hpp:102-167): noise, descending ranks, AP-MI on the ranks
   Buf by, br;
@@ -112,6 +113,7 @@ int mi_f32(mcpg_ctx* h, const float* X, int64_t G, int64_t S, int64_t ldx, int m
   if ((st = alloc(h, by, (size_t)G * S, &y)) || (st = alloc(h, br, (size_t)G * S, &r))) return st;
   if ((st = mcpg_add_noise_f32(h, MCPG_DEVICE, X, G, S, ldx, seed, y, S))) return st;
   if ((st = mcpg_rank_f32(h, MCPG_DEVICE, y, G, S, S, r, S))) return st;
+  if (MI64) return mcpg_ap_mi_f64(h, MCPG_DEVICE, r, G, S, S, MI64, G);
   return mcpg_ap_mi_i32(h, MCPG_DEVICE, r, G, S, S, MI, G);
 }
 
@@ -217,10 +219,14 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
       }
       xf = dx;
     }
+    // the double chain keeps the AP-MI in double (AdaptivePartitionRankMIKernel2<IT, double>)
+    const bool ap64 = !f32 && mi_method == 1;
     float* dmif = reinterpret_cast<float*>(dmi);
-    if (!f32 && (st = alloc(h, bmif, (size_t)G * G, &dmif))) return st;
-    if ((st = mi_f32(h, xf, G, S, ldxf, mi_method, bins, order, precision, seed, dmif))) return st;
-    if (!f32) {
+    if (!f32 && !ap64 && (st = alloc(h, bmif, (size_t)G * G, &dmif))) return st;
+    if ((st = mi_f32(h, xf, G, S, ldxf, mi_method, bins, order, precision, seed, dmif,
+                     ap64 ? reinterpret_cast<double*>(dmi) : nullptr)))
+      return st;
+    if (!f32 && !ap64) {
       mcpg::LaunchScope ls(h, "convert_kernel");
       hipLaunchKernelGGL((convert_kernel<float, T>), dim3(grid_of(G * G)), dim3(256), 0, h->stream, dmif, G, G, G, dmi,
                          G);

@@ -66,6 +66,21 @@ def test_ap_mi_matrix_vs_oracle(handle, G, S):
     assert np.array_equal(got, got.T)
 
 
+def test_ap_mi_double_output(handle, torch_cuda):
+    """AdaptivePartitionRankMIKernel2<IT, double>: the same per-pair double as the float entry,
+    not rounded (its float rounding is the float entry bitwise), within 1e-12 of the oracle's
+    double per pair; host and device outputs agree."""
+    R = ranks(40, 301, 6)
+    mi64 = handle.ap_mi(R, dtype=np.float64)
+    mi32 = handle.ap_mi(R)
+    assert mi64.dtype == np.float64 and np.array_equal(mi64.astype(np.float32), mi32)
+    assert np.array_equal(mi64, mi64.T)
+    for i, j in [(0, 0), (0, 1), (3, 17), (39, 2), (20, 20), (11, 38)]:
+        assert abs(mi64[i, j] - oracle.ap_mi_cells(R, min(i, j), max(i, j))[1]) <= 1e-12
+    dev = handle.ap_mi(torch_cuda.from_numpy(R).cuda(), dtype=np.float64)
+    assert np.array_equal(dev.cpu().numpy(), mi64)
+
+
 def test_cells_bit_exact(handle):
     R = ranks(30, 500, 8)
     rng = np.random.default_rng(1)

@@ -51,6 +51,15 @@ def test_ap_chain_matches_stages(handle):
     assert 2 not in res and 4 not in res
 
 
+def test_double_ap_chain_keeps_the_mi_in_double(handle):
+    X = grn.grn_expression(50, 256, 31)[0]
+    res = handle.mcpnet(X.astype(np.float64), mi_method=1, noise_seed=99, orders=(3,), diagonal=-1.0)
+    R = handle.rank(handle.add_noise(X, 99))
+    MI = handle.ap_mi(R, dtype=np.float64)
+    assert res["mi"].dtype == np.float64 and bits_equal(res["mi"], MI)
+    assert bits_equal(res[3], handle.mcp(MI, 3)[0])
+
+
 def test_double_chain_widens_the_mi(handle):
     X = grn.grn_expression(60, 200, 33)[0].astype(np.float64)
     res = handle.mcpnet(X, mi_method=0, diagonal=0.5)

@@ -135,6 +135,10 @@ def test_mi_adaptive_partitioning_double(tmp_path):
     Xin = read_exp(src)[1].astype(np.float32)
     R = grn.rank_desc(grn.jitter(Xin, 77))
     assert np.abs(mi - oracle.ap_mi(R)).max() <= 1e-6
+    # the double run keeps each pair's double (AdaptivePartitionRankMIKernel2<IT, double>)
+    for i in range(24):
+        for j in range(i, 24):
+            assert abs(mi[i, j] - oracle.ap_mi_cells(R, i, j)[1]) <= 1e-12
 
 
 @pytest.mark.parametrize("method,k", [(1, 2), (2, 3), (3, 4)])
