@@ -131,6 +131,11 @@ int mcpg_bspline_mi_weights_f64(mcpg_handle h, int ptrs, const double* W, int64_
                                 int bins, double* MI, int64_t ldmi);
 int mcpg_bspline_mi_f64(mcpg_handle h, int ptrs, const double* X, int64_t G, int64_t S, int64_t ldx,
                         int bins, int order, double* MI, int64_t ldmi);
+/* The block of mcpg_bspline_mi_f64, as mcpg_bspline_mi_block_f32 is of mcpg_bspline_mi_f32 (the
+ * double DataType's row partition of mcpnet_helper.hpp:90-98), bit-identical to those entries. */
+int mcpg_bspline_mi_block_f64(mcpg_handle h, int ptrs, const double* X, int64_t G, int64_t S, int64_t ldx,
+                              int bins, int order, int64_t row0, int64_t nrows, int64_t col0, int64_t ncols,
+                              double* MI, int64_t ldmi);
 
 /* Adjoint of the B-spline MI (no counterpart in the reference, which has no gradients):
  * for L = sum_n m[n] * MI(pi[n], pj[n]) (pairs with pi != pj; the MI is the MCPG_FP32 forward),
@@ -182,6 +187,9 @@ int mcpg_ap_mi_f64(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t
  * mcpnet_helper.hpp:156-162, where each rank's InnerProduct fills its rows of the distributed MI. */
 int mcpg_ap_mi_block_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,
                          int64_t row0, int64_t nrows, int64_t col0, int64_t ncols, float* MI, int64_t ldmi);
+/* The block of mcpg_ap_mi_f64 (the double DataType's row partition). */
+int mcpg_ap_mi_block_f64(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,
+                         int64_t row0, int64_t nrows, int64_t col0, int64_t ncols, double* MI, int64_t ldmi);
 
 /* Debug/parity view of the same kernel: for each requested pair (px[n], py[n]) the leaf windows
  * in the order their terms enter the FP64 sum, as (nxy, nx, ny) int32 triples, at most

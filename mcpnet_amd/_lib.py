@@ -53,6 +53,10 @@ _SIGS = {
     "mcpg_bspline_mi_weights_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, _P, c_int64],
     "mcpg_bspline_mi_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, c_int64],
     "mcpg_ap_mi_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, _P, c_int64],
+    "mcpg_ap_mi_block_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, _P,
+                             c_int64],
+    "mcpg_bspline_mi_block_f64": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, c_int64, c_int64, c_int64,
+                                  c_int64, _P, c_int64],
     "mcpg_bspline_mi_block_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, c_int, c_int64, c_int64, c_int64,
                                   c_int64, _P, c_int64],
     "mcpg_bspline_mi_backward_f32": [_P, c_int, _P, c_int64, c_int64, c_int64, c_int, c_int, _P, _P, _P, c_int64,
@@ -332,15 +336,20 @@ class Handle:
 
     def bspline_mi_block(self, X, row0, nrows, col0, ncols, bins=10, order=3, precision="fp32", out=None):
         """Rows row0 .. row0+nrows-1, columns col0 .. col0+ncols-1 of bspline_mi(X)."""
-        x = _Mat(X, np.float32, "X")
+        dt = np.float64 if precision == "fp64" else np.float32
+        x = _Mat(X, dt, "X")
         G, S = x.shape
         _check_block(G, row0, nrows, col0, ncols)
-        out, m = _out_mat(out, x.kind, (nrows, ncols), np.float32, X, "out")
+        out, m = _out_mat(out, x.kind, (nrows, ncols), dt, X, "out")
         _same_kind(x, m)
         self._pre(x.kind)
-        self._check(self.lib.mcpg_bspline_mi_block_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order,
-                                                       PRECISIONS[precision], row0, nrows, col0, ncols, m.ptr,
-                                                       max(m.ld, 1)), "mcpg_bspline_mi_block_f32")
+        if dt == np.float64:
+            self._check(self.lib.mcpg_bspline_mi_block_f64(self.h, x.kind, x.ptr, G, S, x.ld, bins, order, row0, nrows,
+                                                           col0, ncols, m.ptr, max(m.ld, 1)), "mcpg_bspline_mi_block_f64")
+        else:
+            self._check(self.lib.mcpg_bspline_mi_block_f32(self.h, x.kind, x.ptr, G, S, x.ld, bins, order,
+                                                           PRECISIONS[precision], row0, nrows, col0, ncols, m.ptr,
+                                                           max(m.ld, 1)), "mcpg_bspline_mi_block_f32")
         return out
 
     def bspline_mi_backward(self, X, pi, pj, m, bins=10, order=3, out=None):
@@ -399,15 +408,17 @@ class Handle:
         self._check(fn(self.h, r.kind, r.ptr, G, S, r.ld, m.ptr, m.ld), name)
         return out
 
-    def ap_mi_block(self, R, row0, nrows, col0, ncols, out=None):
-        """Rows row0 .. row0+nrows-1, columns col0 .. col0+ncols-1 of ap_mi(R)."""
+    def ap_mi_block(self, R, row0, nrows, col0, ncols, out=None, dtype=np.float32):
+        """Rows row0 .. row0+nrows-1, columns col0 .. col0+ncols-1 of ap_mi(R, dtype=dtype)."""
+        dt = np.float64 if np.dtype(dtype) == np.float64 else np.float32
         r = _Mat(R, np.int32, "R")
         G, S = r.shape
         _check_block(G, row0, nrows, col0, ncols)
-        out, m = _out_mat(out, r.kind, (nrows, ncols), np.float32, R, "out")
+        out, m = _out_mat(out, r.kind, (nrows, ncols), dt, R, "out")
         self._pre(r.kind)
-        self._check(self.lib.mcpg_ap_mi_block_i32(self.h, r.kind, r.ptr, G, S, r.ld, row0, nrows, col0, ncols, m.ptr,
-                                                  max(m.ld, 1)), "mcpg_ap_mi_block_i32")
+        fn, name = ((self.lib.mcpg_ap_mi_block_f64, "mcpg_ap_mi_block_f64") if dt == np.float64 else
+                    (self.lib.mcpg_ap_mi_block_i32, "mcpg_ap_mi_block_i32"))
+        self._check(fn(self.h, r.kind, r.ptr, G, S, r.ld, row0, nrows, col0, ncols, m.ptr, max(m.ld, 1)), name)
         return out
 
     def ap_mi_cells(self, R, px, py, max_leaves=4096):

@@ -1054,21 +1054,34 @@ int mcpg_ap_mi_f64(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t
   return ap_run(h, ptrs, R, G, S, ldr, nullptr, ldmi, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, MI);
 }
 
-int mcpg_ap_mi_block_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, int64_t row0,
-                         int64_t nrows, int64_t col0, int64_t ncols, float* MI, int64_t ldmi) {
-  MCPG_CHECK_HANDLE(h);
-  mcpg::begin_call(h);
+// the argument checks and run of both block entries (MI64 non-null: the double output)
+static int ap_block_entry(mcpg_ctx* h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, int64_t row0,
+                          int64_t nrows, int64_t col0, int64_t ncols, float* MI, double* MI64, int64_t ldmi) {
   if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
   if (row0 < 0 || nrows < 0 || col0 < 0 || ncols < 0 || row0 > G - nrows || col0 > G - ncols)
     return mcpg::set_error(h, MCPG_ERR_SHAPE, "block outside the G x G matrix");
   if (G == 0 || nrows == 0 || ncols == 0) return MCPG_OK;
   if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
   if (S > kMaxS) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "adaptive-partitioning MI supports S <= 8191");
-  if (!R || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
+  if (!R || (!MI && !MI64)) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
   if (ldr < S || ldmi < ncols) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   const int64_t blk[4] = {row0, nrows, col0, ncols};
-  return ap_run(h, ptrs, R, G, S, ldr, MI, ldmi, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, blk);
+  return ap_run(h, ptrs, R, G, S, ldr, MI, ldmi, nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, blk, MI64);
+}
+
+int mcpg_ap_mi_block_f64(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, int64_t row0,
+                         int64_t nrows, int64_t col0, int64_t ncols, double* MI, int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  return ap_block_entry(h, ptrs, R, G, S, ldr, row0, nrows, col0, ncols, nullptr, MI, ldmi);
+}
+
+int mcpg_ap_mi_block_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr, int64_t row0,
+                         int64_t nrows, int64_t col0, int64_t ncols, float* MI, int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  return ap_block_entry(h, ptrs, R, G, S, ldr, row0, nrows, col0, ncols, MI, nullptr, ldmi);
 }
 
 int mcpg_ap_mi_cells_i32(mcpg_handle h, int ptrs, const int32_t* R, int64_t G, int64_t S, int64_t ldr,

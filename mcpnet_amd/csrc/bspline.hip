@@ -1181,6 +1181,9 @@ struct MI64Args {
   Layout L;
   const uint32_t* order;  // tile coordinates in launch order, (ti << 16) | tj
   double norm;            // 1.0 / S: BSplineEntropyKernel<double>::norm_factor (BSplineMI.hpp:319)
+  // output block: MI holds rows r0 .. r0 + nr - 1 and columns c0 .. c0 + nc - 1 of the G x G
+  // matrix (0, G, 0, G for the whole matrix)
+  int64_t r0, nr, c0, nc;
 };
 
 constexpr int kEpi64Stride = 81;                          // doubles per row of a wave's t tile
@@ -1252,8 +1255,11 @@ __device__ __forceinline__ void mi64_epilogue(const MI64Args& a, const doublex4 
           for (int c = 0; c < bins; ++c) sum += row[c];
         }
         const double mi = (a.H64[gi] + a.H64[gj]) - (0.0 - sum);
-        a.MI[gi * a.ldmi + gj] = mi;
-        a.MI[gj * a.ldmi + gi] = mi;
+        // (gi, gj) and (gj, gi) where they fall inside the output block
+        const uint64_t ri = (uint64_t)(gi - a.r0), cj = (uint64_t)(gj - a.c0);
+        if (ri < (uint64_t)a.nr && cj < (uint64_t)a.nc) a.MI[(int64_t)ri * a.ldmi + (int64_t)cj] = mi;
+        const uint64_t rj = (uint64_t)(gj - a.r0), ci = (uint64_t)(gi - a.c0);
+        if (gi != gj && rj < (uint64_t)a.nr && ci < (uint64_t)a.nc) a.MI[(int64_t)rj * a.ldmi + (int64_t)ci] = mi;
       }
     }
     __syncthreads();
@@ -1953,17 +1959,27 @@ int prep64_launch(mcpg_ctx* h, const double* dX, int64_t ldx, const Layout& L, c
   return MCPG_OK;
 }
 
-int mi64_launch(mcpg_ctx* h, const double* w, const double* H64, const Layout& L, double* MI, int64_t ldmi) {
+int mi64_launch(mcpg_ctx* h, const double* w, const double* H64, const Layout& L, double* MI, int64_t ldmi,
+                const MIBlock& blk = MIBlock()) {
   if (L.tr != kTileRows || L.ks != kKStep64)
     return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "internal: weight layout does not match the FP64 MI kernel");
-  const int64_t ntri = L.ntiles * (L.ntiles + 1) / 2;
+  const bool whole = blk.nr < 0;
+  int64_t ntri = L.ntiles * (L.ntiles + 1) / 2;
   if (ntri > (int64_t)INT32_MAX || L.ntiles > 65535) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "MI grid too large");
   MI64Args a;
   a.w = w; a.H64 = H64; a.MI = MI; a.ldmi = ldmi; a.L = L;
   a.norm = 1.0 / (double)L.S;
+  a.r0 = whole ? 0 : blk.r0; a.nr = whole ? L.G : blk.nr;
+  a.c0 = whole ? 0 : blk.c0; a.nc = whole ? L.G : blk.nc;
   int st;
-  a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 32), mcpg::env_group("MCPG_MI_XCD", 0), &st);
-  if (!a.order) return st;
+  if (whole) {
+    a.order = mcpg::tri_order(h, L.ntiles, mcpg::env_group("MCPG_MI_GROUP", 32), mcpg::env_group("MCPG_MI_XCD", 0), &st);
+    if (!a.order) return st;
+  } else {
+    a.order = blk.order;
+    ntri = blk.ntiles;
+  }
+  if (ntri == 0) return MCPG_OK;
   static bool attr_set = false;
   if (!attr_set) {
     MCPG_HIP(h, hipFuncSetAttribute(reinterpret_cast<const void*>(&bspline_mi64_kernel<2>),
@@ -1988,24 +2004,28 @@ int mi64_launch(mcpg_ctx* h, const double* w, const double* H64, const Layout& L
   return MCPG_OK;
 }
 
-// compute_mi on the reference's double DataType: scaling, weights and marginals in double, then
-// the FP64 MI of every pair.
+// compute_mi on the reference's double DataType for one block of the output (the whole matrix:
+// 0, G, 0, G): scaling, weights and marginals of every gene in double, then the FP64 MI of the
+// tiles that cover the block.
 int bspline_mi64_run(mcpg_ctx* h, int ptrs, const double* X, int64_t G, int64_t S, int64_t ldx, int bins, int order,
-                     double* MI, int64_t ldmi, const char* where) {
+                     int64_t r0, int64_t nr, int64_t c0, int64_t nc, double* MI, int64_t ldmi, const char* where) {
   int st;
   if ((st = check_spline_args(h, bins, order))) return st;
   if (G < 0 || S < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
-  if (G == 0) return MCPG_OK;
+  if (G == 0 || nr == 0 || nc == 0) return MCPG_OK;
   if (S < 1) return mcpg::set_error(h, MCPG_ERR_SHAPE, "need at least one sample");
   if (!X || !MI) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "null matrix pointer");
-  if (ldx < S || ldmi < G) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
+  if (ldx < S || ldmi < nc) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "leading dimension too small");
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
   const Layout L = mi64_layout(G, S, bins);
   const Spline64 sp = make_spline64(bins, order);
+  const bool whole = r0 == 0 && c0 == 0 && nr == G && nc == G;
+  if (!whole) block_tiles(L, r0, nr, c0, nc, h->block_order);
   const size_t n = (size_t)L.rows * L.Sp;
   mcpg::Sizer sz;
-  if (ptrs == MCPG_HOST) { sz.add<double>(G * S); sz.add<double>(G * G); }
+  if (ptrs == MCPG_HOST) { sz.add<double>(G * S); sz.add<double>(nr * nc); }
+  if (!whole) sz.add<uint32_t>(h->block_order.size());
   sz.add<double>(n);
   sz.add<double>(L.ntiles * L.gt);
   void* base = mcpg::workspace(h, sz.off, &st);
@@ -2019,7 +2039,16 @@ int bspline_mi64_run(mcpg_ctx* h, int ptrs, const double* X, int64_t G, int64_t 
     double* t = cv.take<double>(G * S);
     if ((st = mcpg::copy_2d(h, t, S * 8, X, ldx * 8, S * 8, G, hipMemcpyHostToDevice))) return st;
     dX = t; lx = S;
-    dMI = cv.take<double>(G * G); lm = G;
+    dMI = cv.take<double>(nr * nc); lm = nc;
+  }
+  MIBlock blk;
+  if (!whole) {
+    uint32_t* d_order = cv.take<uint32_t>(h->block_order.size());
+    MCPG_HIP(h, hipMemcpyAsync(d_order, h->block_order.data(), h->block_order.size() * 4, hipMemcpyHostToDevice,
+                               h->stream));
+    blk.r0 = r0; blk.nr = nr; blk.c0 = c0; blk.nc = nc;
+    blk.order = d_order;
+    blk.ntiles = (int64_t)h->block_order.size();
   }
   double* w = cv.take<double>(n);
   double* H64 = cv.take<double>(L.ntiles * L.gt);
@@ -2028,8 +2057,9 @@ int bspline_mi64_run(mcpg_ctx* h, int ptrs, const double* X, int64_t G, int64_t 
   Prep64Out o{};
   o.w = w; o.H64 = H64; o.flags = h->d_flags;
   if ((st = prep64_launch(h, dX, lx, L, sp, 1, o))) return st;
-  if ((st = mi64_launch(h, w, H64, L, dMI, lm))) return st;
-  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 8, dMI, G * 8, G * 8, G, hipMemcpyDeviceToHost))) return st;
+  if ((st = mi64_launch(h, w, H64, L, dMI, lm, blk))) return st;
+  if (ptrs == MCPG_HOST && (st = mcpg::copy_2d(h, MI, ldmi * 8, dMI, nc * 8, nc * 8, nr, hipMemcpyDeviceToHost)))
+    return st;
   if ((st = mcpg::finish(h, where))) return st;
   return check_flags_nonfinite(h, where);
 }
@@ -2323,7 +2353,18 @@ int mcpg_bspline_mi_f64(mcpg_handle h, int ptrs, const double* X, int64_t G, int
                         double* MI, int64_t ldmi) {
   MCPG_CHECK_HANDLE(h);
   mcpg::begin_call(h);
-  return bspline_mi64_run(h, ptrs, X, G, S, ldx, bins, order, MI, ldmi, "mcpg_bspline_mi_f64");
+  return bspline_mi64_run(h, ptrs, X, G, S, ldx, bins, order, 0, G, 0, G, MI, ldmi, "mcpg_bspline_mi_f64");
+}
+
+int mcpg_bspline_mi_block_f64(mcpg_handle h, int ptrs, const double* X, int64_t G, int64_t S, int64_t ldx, int bins,
+                              int order, int64_t row0, int64_t nrows, int64_t col0, int64_t ncols, double* MI,
+                              int64_t ldmi) {
+  MCPG_CHECK_HANDLE(h);
+  mcpg::begin_call(h);
+  if (G < 0 || row0 < 0 || nrows < 0 || col0 < 0 || ncols < 0 || row0 > G - nrows || col0 > G - ncols)
+    return mcpg::set_error(h, MCPG_ERR_SHAPE, "block outside the G x G matrix");
+  return bspline_mi64_run(h, ptrs, X, G, S, ldx, bins, order, row0, nrows, col0, ncols, MI, ldmi,
+                          "mcpg_bspline_mi_block_f64");
 }
 
 int mcpg_bspline_mi_backward_f32(mcpg_handle h, int ptrs, const float* X, int64_t G, int64_t S, int64_t ldx, int bins,

@@ -84,3 +84,31 @@ def test_ap_mi_blocks_with_tied_ranks(handle):
     for r0, nr, c0, nc in [(0, 40, 0, 40), (5, 10, 20, 20), (30, 10, 0, 35)]:
         got = handle.ap_mi_block(R, r0, nr, c0, nc)
         assert same_bits(got, whole[r0:r0 + nr, c0:c0 + nc]), (r0, nr, c0, nc)
+
+
+def same_bits64(a, b):
+    return a.shape == b.shape and np.array_equal(np.ascontiguousarray(a).view(np.uint64),
+                                                  np.ascontiguousarray(b).view(np.uint64))
+
+
+def test_bspline_mi_blocks_fp64(handle):
+    """The double DataType's row partitions: FP64 MI blocks equal the whole FP64 matrix bitwise."""
+    rng = np.random.default_rng(3)
+    X = grn.grn_expression(G, 240, 4343)[0].astype(np.float64) * (1.0 + 1e-9 * rng.standard_normal((G, 240)))
+    whole = handle.bspline_mi(X, precision="fp64")
+    Xd = torch.from_numpy(X).cuda()
+    for r0, nr, c0, nc in BLOCKS:
+        got = handle.bspline_mi_block(X, r0, nr, c0, nc, precision="fp64")
+        assert got.dtype == np.float64 and same_bits64(got, whole[r0:r0 + nr, c0:c0 + nc]), (r0, nr, c0, nc)
+        gd = handle.bspline_mi_block(Xd, r0, nr, c0, nc, precision="fp64")
+        torch.cuda.synchronize()
+        assert same_bits64(gd.cpu().numpy(), whole[r0:r0 + nr, c0:c0 + nc]), ("device", r0, nr, c0, nc)
+
+
+def test_ap_mi_blocks_fp64(handle):
+    X = grn.grn_expression(90, 250, 778)[0]
+    R = grn.rank_desc(grn.jitter(X, 6))
+    whole = handle.ap_mi(R, dtype=np.float64)
+    for r0, nr, c0, nc in [(0, 90, 0, 90), (0, 30, 0, 90), (10, 50, 5, 70), (64, 1, 64, 1), (7, 0, 0, 3)]:
+        got = handle.ap_mi_block(R, r0, nr, c0, nc, dtype=np.float64)
+        assert got.dtype == np.float64 and same_bits64(got, whole[r0:r0 + nr, c0:c0 + nc]), (r0, nr, c0, nc)
