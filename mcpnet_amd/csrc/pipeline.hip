@@ -260,12 +260,17 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
     return st;
 
   // ---- MCP2/3/4 (compute_mcp{2,3,4}, or compute_mcp{2,3,4}_tfs with tf_gene_transition k - 1)
+  // With host outputs each order's scores come back on the side stream while the next order is
+  // computed; the downloads run one after another (they share the side stream's pinned buffers),
+  // and each order's device copy lives until the end of the call.
   Buf bm1;
   T* dm1 = nullptr;   // masked A (x) A (gene-TF-gene), shared by the TF orders 3 and 4
+  Buf bouts[3], btmp[3][2];
+  mcpg::Download out_dl[3];
   for (int k = 2; k <= 4; ++k) {
     T* out = outs[k - 1];
     if (!out) continue;
-    Buf bout;
+    Buf& bout = bouts[k - 2];
     T* dout = out;
     int64_t ldo = lds[k - 1];
     if (host) {
@@ -279,7 +284,8 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
         if ((st = alloc(h, bm1, (size_t)G * G, &dm1))) return st;
         if ((st = Stage<T>::maxmin(h, dmi, G, dmi, G, dmask, G, G, G, 1, dm1, G))) return st;
       }
-      Buf bmm, bm2;
+      Buf& bmm = btmp[k - 2][0];   // kept to the end of the call: a free would wait for the download
+      Buf& bm2 = btmp[k - 2][1];
       T* dmm;
       if ((st = alloc(h, bmm, (size_t)R * G, &dmm))) return st;
       if (k == 2) {          // tfs-tfs-genes
@@ -294,11 +300,17 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
       }
       if (st || (st = Stage<T>::ratio(h, dmit, G, dmm, G, R, G, dout, ldo))) return st;
     }
-    if (host && (st = mcpg::copy_2d(h, out, lds[k - 1] * sizeof(T), dout, G * sizeof(T), G * sizeof(T), R, out_kind)))
-      return st;
-    if ((st = mcpg::finish(h, "mcpg_mcpnet"))) return st;   // the output copy completes before bout is freed
+    if (host) {
+      if ((st = mi_dl.wait(h))) return st;
+      for (int q = 0; q < k - 2; ++q)
+        if ((st = out_dl[q].wait(h))) return st;
+      if ((st = out_dl[k - 2].start(h, out, lds[k - 1] * sizeof(T), dout, G * sizeof(T), G * sizeof(T), R))) return st;
+    }
+    if ((st = mcpg::finish(h, "mcpg_mcpnet"))) return st;
   }
   if ((st = mi_dl.wait(h))) return st;
+  for (int q = 0; q < 3; ++q)
+    if ((st = out_dl[q].wait(h))) return st;
   return mcpg::finish(h, "mcpg_mcpnet");
 }
 
