@@ -84,6 +84,9 @@ struct Stage<float> {
   static int mcp(mcpg_ctx* h, const float* A, int64_t lda, int64_t N, int k, float* out, int64_t ldo) {
     return mcpg_mcp_f32(h, MCPG_DEVICE, A, lda, N, k, nullptr, 0, out, ldo);
   }
+  static int maxmins(mcpg_ctx* h, const float* A, int64_t N, float* M1, float* M2, float* M3) {
+    return mcpg_maxmins_f32(h, MCPG_DEVICE, A, N, N, M1, N, M2, N, M3, N);
+  }
 };
 template <>
 struct Stage<double> {
@@ -97,6 +100,9 @@ struct Stage<double> {
   }
   static int mcp(mcpg_ctx* h, const double* A, int64_t lda, int64_t N, int k, double* out, int64_t ldo) {
     return mcpg_mcp_f64(h, MCPG_DEVICE, A, lda, N, k, nullptr, 0, out, ldo);
+  }
+  static int maxmins(mcpg_ctx* h, const double* A, int64_t N, double* M1, double* M2, double* M3) {
+    return mcpg_maxmins_f64(h, MCPG_DEVICE, A, N, N, M1, N, M2, N, M3, N);
   }
 };
 
@@ -267,6 +273,16 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
   T* dm1 = nullptr;   // masked A (x) A (gene-TF-gene), shared by the TF orders 3 and 4
   Buf bouts[3], btmp[3][2];
   mcpg::Download out_dl[3];
+  // Without TFs and with two or more orders the max-min matrices are shared: M1 = A (x) A once,
+  // M2 = A (x) M1 and M3 = M1 (x) M1 as asked (compute_maxmins, mcpnet_helper.hpp:436-460), and
+  // MCP_k = ratio(A, M_{k-1}); each compute_mcp{k} alone would form A (x) A again.
+  Buf bmx[3];
+  T* dmx[3] = {nullptr, nullptr, nullptr};
+  if (!tfs && (mcp2 ? 1 : 0) + (mcp3 ? 1 : 0) + (mcp4 ? 1 : 0) >= 2) {
+    for (int q = 0; q < 3; ++q)
+      if ((q == 0 || outs[q + 1]) && (st = alloc(h, bmx[q], (size_t)G * G, &dmx[q]))) return st;
+    if ((st = Stage<T>::maxmins(h, dmi, G, dmx[0], dmx[1], dmx[2]))) return st;
+  }
   for (int k = 2; k <= 4; ++k) {
     T* out = outs[k - 1];
     if (!out) continue;
@@ -277,7 +293,9 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
       if ((st = alloc(h, bout, (size_t)R * G, &dout))) return st;
       ldo = G;
     }
-    if (!tfs) {
+    if (!tfs && dmx[0]) {
+      if ((st = Stage<T>::ratio(h, dmi, G, dmx[k - 2], G, G, G, dout, ldo))) return st;
+    } else if (!tfs) {
       if ((st = Stage<T>::mcp(h, dmi, G, G, k, dout, ldo))) return st;
     } else {
       if (k >= 3 && !dm1) {
