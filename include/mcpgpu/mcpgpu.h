@@ -265,7 +265,9 @@ int mcpg_mcp_f32_noalloc(mcpg_handle h, const float* A, int64_t lda, int64_t N, 
  *     tf_gene_transition k - 1; the tfs values themselves are the masked kernel's kmask.
  *   mi (nullable): the whole G x G MI after the diagonal is set (the ensemble stage needs all of
  *     it; its TF rows are what mcpnet writes in TF mode).
- *   mcp2 / mcp3 / mcp4 (nullable): the MCP scores of that order, G or T rows by G columns.
+ *   mcp2 / mcp3 / mcp4 (nullable): the MCP scores of that order, G or T rows by G columns.  Without
+ *     TFs, two or more orders share their max-min matrices (A (x) A once, as compute_maxmins,
+ *     mcpnet_helper.hpp:436-460); the scores equal the per-order mcpg_mcp_* results bitwise.
  * The f64 form with precision MCPG_FP64 computes the MI in double (the reference's double
  * DataType); with the other precisions in FP32-class arithmetic from FP32 values of X, widened
  * before the diagonal.  Its max-min products and ratios run in FP64, as the stage entry points'.
