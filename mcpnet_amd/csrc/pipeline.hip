@@ -310,10 +310,12 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
         st = Stage<T>::maxmin(h, dmit, G, dmi, G, dmask, R, G, G, 0, dmm, G);
       } else if (k == 3) {   // tf-(tf-tf-gene)
         st = Stage<T>::maxmin(h, dmit, G, dm1, G, dmask, R, G, G, 0, dmm, G);
-      } else {               // (tf-tf-gene)-(tf-gene)
-        T* dm2;
-        if ((st = alloc(h, bm2, (size_t)R * G, &dm2))) return st;
-        if ((st = Stage<T>::maxmin(h, dmit, G, dmi, G, dmask, R, G, G, 0, dm2, G))) return st;
+      } else {               // (tf-tf-gene)-(tf-gene); its first factor is MCP2's max-min
+        T* dm2 = mcp2 ? reinterpret_cast<T*>(btmp[0][0].p) : nullptr;
+        if (!dm2) {
+          if ((st = alloc(h, bm2, (size_t)R * G, &dm2))) return st;
+          if ((st = Stage<T>::maxmin(h, dmit, G, dmi, G, dmask, R, G, G, 0, dm2, G))) return st;
+        }
         st = Stage<T>::maxmin(h, dm2, G, dm1, G, dmask, R, G, G, 0, dmm, G);
       }
       if (st || (st = Stage<T>::ratio(h, dmit, G, dmm, G, R, G, dout, ldo))) return st;
