@@ -27,6 +27,24 @@ def test_bspline_chain_matches_stages(handle, precision):
         assert bits_equal(res[k], handle.mcp(MI, k)[0]), k
 
 
+@pytest.mark.parametrize("orders", [(2, 3), (3, 4), (2, 4), (4,), (2, 3, 4)])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_order_subsets_share_maxmin(handle, torch_cuda, orders, dt):
+    """Two or more orders share A (x) A (and M1 (x) M1, A (x) M1 as asked); every subset, with host
+    and device buffers, gives the per-order entry points' bits; each order's host download
+    overlaps the next order."""
+    X = grn.grn_expression(130, 200, 36)[0].astype(dt)
+    MI = handle.bspline_mi(X.astype(np.float32), precision="bf16x3").astype(dt)
+    np.fill_diagonal(MI, 0.0)
+    want = {k: handle.mcp(MI, k)[0] for k in orders}
+    host = handle.mcpnet(X, mi_method=0, orders=orders, precision="bf16x3")
+    dev = handle.mcpnet(torch_cuda.from_numpy(X).cuda(), mi_method=0, orders=orders, precision="bf16x3")
+    for k in orders:
+        assert bits_equal(host[k], want[k]), k
+        assert bits_equal(dev[k].cpu().numpy(), want[k]), k
+    assert set(host) == {"mi", *orders}
+
+
 @pytest.mark.slow
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 def test_host_chain_with_overlapped_mi_download(handle, dt):
