@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <thread>
 #include <vector>
@@ -139,6 +140,16 @@ const uint32_t* tile_order(mcpg_ctx* h, int64_t ni, int64_t nj, int ratio, int g
 // The upper triangle (ti <= tj) of an nb x nb grid in group x group blocks.  Both kernels that use
 // it measured faster without the XCD deal at 20000 genes.
 const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int xcd_deal, int* status);
+
+// compute_maxmins on device matrices (M1 = A (x) A, M2 = A (x) M1, M3 = M1 (x) M1; M1 is
+// required, M2 and M3 are formed when set).  `after(q)` runs once each product q (0: M1, 1: M2,
+// 2: M3) is on the handle's stream, before the next one is launched, so that its consumers (a
+// ratio, a download on the side stream) overlap the next product; a nonzero return ends the
+// chain with that status.  It must not use the handle's workspace.
+int maxmins_chain(mcpg_ctx* h, const float* A, int64_t N, float* M1, float* M2, float* M3,
+                  const std::function<int(int)>* after);
+int maxmins_chain(mcpg_ctx* h, const double* A, int64_t N, double* M1, double* M2, double* M3,
+                  const std::function<int(int)>* after);
 
 // Tile-group size: the value of environment variable `name` if set (1..256), else `def`.
 int env_group(const char* name, int def);

@@ -1446,7 +1446,7 @@ int run_vals(mcpg_ctx* h, const ValPlan& p, const double* A, int64_t lda, const 
 // fails the key test, and the caller runs the FP64 kernel.
 template <typename T>
 int maxmins_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, T* M1, int64_t ld1, T* M2, int64_t ld2,
-                 T* M3, int64_t ld3, bool* done = nullptr) {
+                 T* M3, int64_t ld3, bool* done = nullptr, const std::function<int(int)>* after = nullptr) {
   constexpr bool f64 = std::is_same<T, double>::value;
   const size_t NN = (size_t)N * N;
   KeyMat ka = key_geometry(N, N), k1 = key_geometry(N, N);
@@ -1481,6 +1481,7 @@ int maxmins_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, T* 
   }
   if ((st = pad_keys(h, k1, N))) return st;
   if ((st = key_product(h, ka, ka, N, N, 1, dM[0], ldM[0], k1.d, k1.ldk, nullptr, 0, nullptr, 0))) return st;
+  if (after && (st = (*after)(0))) return st;
   if (dM[1]) {
     if ((st = mcpg::reset_flags(h))) return st;
     {
@@ -1490,8 +1491,12 @@ int maxmins_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, T* 
     int flags[1];
     if ((st = mcpg::read_flags(h, flags, 1))) return st;
     if ((st = key_product(h, ka, k1, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0, nullptr, 0, nullptr, 0))) return st;
+    if (after && (st = (*after)(1))) return st;
   }
-  if (dM[2] && (st = key_product(h, k1, k1, N, N, 1, dM[2], ldM[2], nullptr, 0, nullptr, 0, nullptr, 0))) return st;
+  if (dM[2]) {
+    if ((st = key_product(h, k1, k1, N, N, 1, dM[2], ldM[2], nullptr, 0, nullptr, 0, nullptr, 0))) return st;
+    if (after && (st = (*after)(2))) return st;
+  }
   if (host) {
     T* out[3] = {M1, M2, M3};
     const int64_t ldo[3] = {ld1, ld2, ld3};
@@ -1640,7 +1645,7 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
 // compute_maxmins on padded FP64 values: A padded once, M1 written both to the caller and into a
 // padded operand of the other two products
 int maxmins_f64_vals(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N, double* M1, int64_t ld1,
-                     double* M2, int64_t ld2, double* M3, int64_t ld3) {
+                     double* M2, int64_t ld2, double* M3, int64_t ld3, const std::function<int(int)>* after) {
   const size_t NN = (size_t)N * N;
   ValMat va = val_geometry(N, N), v1 = val_geometry(N, N);
   int st;
@@ -1668,6 +1673,7 @@ int maxmins_f64_vals(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64
   if ((st = prep_vals(h, dA, la, N, nullptr, va))) return st;
   if ((st = pad_vals(h, v1, N))) return st;
   if ((st = launch_maxmin_vals(h, va, va, N, N, 1, dM[0], ldM[0], v1.d, v1.ld, nullptr, 0, nullptr, 0))) return st;
+  if (after && (st = (*after)(0))) return st;
   if (dM[1]) {
     if ((st = mcpg::reset_flags(h))) return st;
     {
@@ -1678,9 +1684,12 @@ int maxmins_f64_vals(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64
     if ((st = mcpg::read_flags(h, flags, 1))) return st;
     if ((st = launch_maxmin_vals(h, va, v1, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0, nullptr, 0, nullptr, 0)))
       return st;
+    if (after && (st = (*after)(1))) return st;
   }
-  if (dM[2] && (st = launch_maxmin_vals(h, v1, v1, N, N, 1, dM[2], ldM[2], nullptr, 0, nullptr, 0, nullptr, 0)))
-    return st;
+  if (dM[2]) {
+    if ((st = launch_maxmin_vals(h, v1, v1, N, N, 1, dM[2], ldM[2], nullptr, 0, nullptr, 0, nullptr, 0))) return st;
+    if (after && (st = (*after)(2))) return st;
+  }
   if (host) {
     double* out[3] = {M1, M2, M3};
     const int64_t ldo[3] = {ld1, ld2, ld3};
@@ -1692,8 +1701,8 @@ int maxmins_f64_vals(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64
 
 // compute_maxmins on FP64 values with the register-staged tile kernel
 int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N, double* M1, int64_t ld1, double* M2,
-                int64_t ld2, double* M3, int64_t ld3) {
-  if (!f64_tile_kernel()) return maxmins_f64_vals(h, host, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
+                int64_t ld2, double* M3, int64_t ld3, const std::function<int(int)>* after = nullptr) {
+  if (!f64_tile_kernel()) return maxmins_f64_vals(h, host, A, lda, N, M1, ld1, M2, ld2, M3, ld3, after);
   const size_t NN = (size_t)N * N;
   mcpg::Sizer sz;
   if (host) { sz.add<double>(NN); for (double* p : {M2, M3}) if (p) sz.add<double>(NN); }
@@ -1715,6 +1724,7 @@ int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N,
   }
   if (host || !M1) { dM[0] = cv.take<double>(NN); ldM[0] = N; }
   if ((st = launch_maxmin<TrF64>(h, dA, la, dA, la, nullptr, N, N, N, 1, dM[0], ldM[0], nullptr, 0, nullptr, 0))) return st;
+  if (after && (st = (*after)(0))) return st;
   if (dM[1]) {
     if ((st = mcpg::reset_flags(h))) return st;
     {
@@ -1726,10 +1736,14 @@ int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N,
     if ((st = launch_maxmin<TrF64>(h, dA, la, dM[0], ldM[0], nullptr, N, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0,
                                    nullptr, 0)))
       return st;
+    if (after && (st = (*after)(1))) return st;
   }
-  if (dM[2] && (st = launch_maxmin<TrF64>(h, dM[0], ldM[0], dM[0], ldM[0], nullptr, N, N, N, 1, dM[2], ldM[2], nullptr, 0,
-                                          nullptr, 0)))
-    return st;
+  if (dM[2]) {
+    if ((st = launch_maxmin<TrF64>(h, dM[0], ldM[0], dM[0], ldM[0], nullptr, N, N, N, 1, dM[2], ldM[2], nullptr, 0,
+                                   nullptr, 0)))
+      return st;
+    if (after && (st = (*after)(2))) return st;
+  }
   if (host) {
     double* out[3] = {M1, M2, M3};
     const int64_t ldo[3] = {ld1, ld2, ld3};
@@ -1741,7 +1755,7 @@ int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N,
 
 template <typename T>
 int maxmins_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T* M1, int64_t ld1, T* M2, int64_t ld2,
-                  T* M3, int64_t ld3) {
+                  T* M3, int64_t ld3, const std::function<int(int)>* after = nullptr) {
   MCPG_CHECK_HANDLE(h);
   mcpg::begin_call(h);
   if (N < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
@@ -1752,14 +1766,14 @@ int maxmins_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T* 
   if (ptrs != MCPG_HOST && ptrs != MCPG_DEVICE) return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad ptrs kind");
   MCPG_HIP(h, hipSetDevice(h->device));
   if constexpr (std::is_same<T, float>::value) {
-    return maxmins_keys<float>(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
+    return maxmins_keys<float>(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3, nullptr, after);
   } else {
     bool done = false;
     if (f64_keys_enabled()) {
-      const int st = maxmins_keys<double>(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3, &done);
+      const int st = maxmins_keys<double>(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3, &done, after);
       if (st || done) return st;
     }
-    return maxmins_f64(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3);
+    return maxmins_f64(h, ptrs == MCPG_HOST, A, lda, N, M1, ld1, M2, ld2, M3, ld3, after);
   }
 }
 
@@ -2246,6 +2260,15 @@ void init_scratch(mcpg_ctx* h) {
     g_scratch_ptr[h->device] = p;
   }
   h->scratch = g_scratch_ptr[h->device];
+}
+
+int maxmins_chain(mcpg_ctx* h, const float* A, int64_t N, float* M1, float* M2, float* M3,
+                  const std::function<int(int)>* after) {
+  return maxmins_entry<float>(h, MCPG_DEVICE, A, N, N, M1, N, M2, N, M3, N, after);
+}
+int maxmins_chain(mcpg_ctx* h, const double* A, int64_t N, double* M1, double* M2, double* M3,
+                  const std::function<int(int)>* after) {
+  return maxmins_entry<double>(h, MCPG_DEVICE, A, N, N, M1, N, M2, N, M3, N, after);
 }
 
 }  // namespace mcpg

@@ -84,9 +84,6 @@ struct Stage<float> {
   static int mcp(mcpg_ctx* h, const float* A, int64_t lda, int64_t N, int k, float* out, int64_t ldo) {
     return mcpg_mcp_f32(h, MCPG_DEVICE, A, lda, N, k, nullptr, 0, out, ldo);
   }
-  static int maxmins(mcpg_ctx* h, const float* A, int64_t N, float* M1, float* M2, float* M3) {
-    return mcpg_maxmins_f32(h, MCPG_DEVICE, A, N, N, M1, N, M2, N, M3, N);
-  }
 };
 template <>
 struct Stage<double> {
@@ -100,9 +97,6 @@ struct Stage<double> {
   }
   static int mcp(mcpg_ctx* h, const double* A, int64_t lda, int64_t N, int k, double* out, int64_t ldo) {
     return mcpg_mcp_f64(h, MCPG_DEVICE, A, lda, N, k, nullptr, 0, out, ldo);
-  }
-  static int maxmins(mcpg_ctx* h, const double* A, int64_t N, double* M1, double* M2, double* M3) {
-    return mcpg_maxmins_f64(h, MCPG_DEVICE, A, N, N, M1, N, M2, N, M3, N);
   }
 };
 
@@ -266,7 +260,7 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
     return st;
 
   // ---- MCP2/3/4 (compute_mcp{2,3,4}, or compute_mcp{2,3,4}_tfs with tf_gene_transition k - 1)
-  // With host outputs each order's scores come back on the side stream while the next order is
+  // With host outputs each order's scores come back on the side stream while the next product is
   // computed; the downloads run one after another (they share the side stream's pinned buffers),
   // and each order's device copy lives until the end of the call.
   Buf bm1;
@@ -278,22 +272,20 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
   // MCP_k = ratio(A, M_{k-1}); each compute_mcp{k} alone would form A (x) A again.
   Buf bmx[3];
   T* dmx[3] = {nullptr, nullptr, nullptr};
-  if (!tfs && (mcp2 ? 1 : 0) + (mcp3 ? 1 : 0) + (mcp4 ? 1 : 0) >= 2) {
-    for (int q = 0; q < 3; ++q)
-      if ((q == 0 || outs[q + 1]) && (st = alloc(h, bmx[q], (size_t)G * G, &dmx[q]))) return st;
-    if ((st = Stage<T>::maxmins(h, dmi, G, dmx[0], dmx[1], dmx[2]))) return st;
-  }
-  for (int k = 2; k <= 4; ++k) {
+  const bool shared = !tfs && (mcp2 ? 1 : 0) + (mcp3 ? 1 : 0) + (mcp4 ? 1 : 0) >= 2;
+  // Order k's scores from what is on the stream so far, then their download on the side stream.
+  auto emit = [&](int k) -> int {
     T* out = outs[k - 1];
-    if (!out) continue;
+    if (!out) return MCPG_OK;
     Buf& bout = bouts[k - 2];
     T* dout = out;
     int64_t ldo = lds[k - 1];
+    int st;
     if (host) {
       if ((st = alloc(h, bout, (size_t)R * G, &dout))) return st;
       ldo = G;
     }
-    if (!tfs && dmx[0]) {
+    if (shared) {
       if ((st = Stage<T>::ratio(h, dmi, G, dmx[k - 2], G, G, G, dout, ldo))) return st;
     } else if (!tfs) {
       if ((st = Stage<T>::mcp(h, dmi, G, G, k, dout, ldo))) return st;
@@ -326,7 +318,18 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
         if ((st = out_dl[q].wait(h))) return st;
       if ((st = out_dl[k - 2].start(h, out, lds[k - 1] * sizeof(T), dout, G * sizeof(T), G * sizeof(T), R))) return st;
     }
-    if ((st = mcpg::finish(h, "mcpg_mcpnet"))) return st;
+    return mcpg::finish(h, "mcpg_mcpnet");
+  };
+  if (shared) {
+    for (int q = 0; q < 3; ++q)
+      if ((q == 0 || outs[q + 1]) && (st = alloc(h, bmx[q], (size_t)G * G, &dmx[q]))) return st;
+    // order k is emitted as soon as M_{k-1} is on the stream, so that its ratio and download
+    // overlap the next product
+    const std::function<int(int)> after = [&](int q) { return emit(q + 2); };
+    if ((st = mcpg::maxmins_chain(h, dmi, G, dmx[0], dmx[1], dmx[2], &after))) return st;
+  } else {
+    for (int k = 2; k <= 4; ++k)
+      if ((st = emit(k))) return st;
   }
   if ((st = mi_dl.wait(h))) return st;
   for (int q = 0; q < 3; ++q)
