@@ -142,14 +142,15 @@ const uint32_t* tile_order(mcpg_ctx* h, int64_t ni, int64_t nj, int ratio, int g
 const uint32_t* tri_order(mcpg_ctx* h, int64_t nb, int group, int xcd_deal, int* status);
 
 // compute_maxmins on device matrices (M1 = A (x) A, M2 = A (x) M1, M3 = M1 (x) M1; M1 is
-// required, M2 and M3 are formed when set).  `after(q)` runs once each product q (0: M1, 1: M2,
-// 2: M3) is on the handle's stream, before the next one is launched, so that its consumers (a
-// ratio, a download on the side stream) overlap the next product; a nonzero return ends the
-// chain with that status.  It must not use the handle's workspace.
-int maxmins_chain(mcpg_ctx* h, const float* A, int64_t N, float* M1, float* M2, float* M3,
-                  const std::function<int(int)>* after);
-int maxmins_chain(mcpg_ctx* h, const double* A, int64_t N, double* M1, double* M2, double* M3,
-                  const std::function<int(int)>* after);
+// required, M2 and M3 are formed when set).  `after(q, r0, r1)` runs once rows [r0, r1) of
+// product q (0: M1, 1: M2, 2: M3) are on the handle's stream, before anything later is launched,
+// so that their consumers (a ratio, a download on the side stream) overlap the next product or
+// row panel: every product but the last whole, the last one (a triangle on the FP32-key kernel)
+// in a few row panels.  A nonzero return ends the chain with that status.  It must not use the
+// handle's workspace.
+using ChainHook = std::function<int(int, int64_t, int64_t)>;
+int maxmins_chain(mcpg_ctx* h, const float* A, int64_t N, float* M1, float* M2, float* M3, const ChainHook* after);
+int maxmins_chain(mcpg_ctx* h, const double* A, int64_t N, double* M1, double* M2, double* M3, const ChainHook* after);
 
 // Tile-group size: the value of environment variable `name` if set (1..256), else `def`.
 int env_group(const char* name, int def);

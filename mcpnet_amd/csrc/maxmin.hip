@@ -1101,7 +1101,8 @@ struct WideOut {
 // caller's buffers and nothing is allocated.
 int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M, int64_t N, int tri, float* C,
                        int64_t ldc, int* Ck, int64_t ldck, const float* R, int64_t ldr, float* O, int64_t ldo,
-                       const MMScratch* sc = nullptr, const WideOut* wide = nullptr) {
+                       const MMScratch* sc = nullptr, const WideOut* wide = nullptr, int64_t b0 = 0,
+                       int64_t b1 = -1) {
   if (M <= 0 || N <= 0) return MCPG_OK;
   MKArgs a;
   a.A = A.d; a.lda = A.ldk; a.B = B.d; a.ldb = B.ldk; a.a_bytes = A.bytes(); a.b_bytes = B.bytes();
@@ -1118,6 +1119,17 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
   a.nbn = ceil_div(N, kBN);
   a.ntiles = tri ? a.nbm * (a.nbm + 1) / 2 : a.nbm * a.nbn;
   if (a.ntiles > (int64_t)INT32_MAX) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "max-min grid too large");
+  // Row panel [b0, b1) of a triangle launch: the tiles with b0 <= bm < b1, a contiguous stretch of
+  // the blocked order when b0 and b1 fall on its row groups (and tiles are not dealt over XCDs).
+  // After the panels in increasing order the rows of every panel so far are complete.
+  const bool panel = b0 != 0 || (b1 >= 0 && b1 != a.nbm);
+  if (panel) {
+    const int64_t group = std::max(mcpg::env_group("MCPG_MM_GROUP", 8), 1);
+    if (!tri || sc || mcpg::env_group("MCPG_MM_XCD", 0) != 0 || b0 < 0 || b1 <= b0 || b1 > a.nbm ||
+        b0 % group != 0 || (b1 % group != 0 && b1 != a.nbm))
+      return mcpg::set_error(h, MCPG_ERR_INVALID_ARG, "bad row panel of a triangle max-min launch");
+    a.ntiles = (b1 * a.nbm - b1 * (b1 - 1) / 2) - (b0 * a.nbm - b0 * (b0 - 1) / 2);
+  }
   a.order = nullptr;
   if (tri && sc) {
     if (a.ntiles > sc->order_cap || a.nbm > 65535)
@@ -1130,6 +1142,7 @@ int launch_maxmin_keys(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M,
     int st;
     a.order = mcpg::tri_order(h, a.nbm, mcpg::env_group("MCPG_MM_GROUP", 8), mcpg::env_group("MCPG_MM_XCD", 0), &st);
     if (!a.order) return st;
+    if (panel) a.order += b0 * a.nbm - b0 * (b0 - 1) / 2;
   }
   const int kc = mcpg::env_group("MCPG_MM_KC", 16) == 32 ? 32 : 16;
   const void* kfns[2][2] = {{reinterpret_cast<const void*>(&maxmin_key_kernel<16, false>),
@@ -1267,16 +1280,34 @@ int keys_exact(mcpg_ctx* h, bool* exact) {
 // One key product into outputs of the operands' type (see launch_maxmin_keys).
 int key_product(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M, int64_t N, int tri, float* C, int64_t ldc,
                 int* Ck, int64_t ldck, const float* R, int64_t ldr, float* O, int64_t ldo,
-                const MMScratch* sc = nullptr) {
-  return launch_maxmin_keys(h, A, B, M, N, tri, C, ldc, Ck, ldck, R, ldr, O, ldo, sc);
+                const MMScratch* sc = nullptr, int64_t b0 = 0, int64_t b1 = -1) {
+  return launch_maxmin_keys(h, A, B, M, N, tri, C, ldc, Ck, ldck, R, ldr, O, ldo, sc, nullptr, b0, b1);
 }
 int key_product(mcpg_ctx* h, const KeyMat& A, const KeyMat& B, int64_t M, int64_t N, int tri, double* C, int64_t ldc,
                 int* Ck, int64_t ldck, const double* R, int64_t ldr, double* O, int64_t ldo,
-                const MMScratch* sc = nullptr) {
-  if (!C && !O) return launch_maxmin_keys(h, A, B, M, N, tri, nullptr, 0, Ck, ldck, nullptr, 0, nullptr, 0, sc);
+                const MMScratch* sc = nullptr, int64_t b0 = 0, int64_t b1 = -1) {
+  if (!C && !O)
+    return launch_maxmin_keys(h, A, B, M, N, tri, nullptr, 0, Ck, ldck, nullptr, 0, nullptr, 0, sc, nullptr, b0, b1);
   WideOut w;
   w.C = C; w.ldc = ldc; w.R = R; w.ldr = ldr; w.O = O; w.ldo = ldo;
-  return launch_maxmin_keys(h, A, B, M, N, tri, nullptr, 0, Ck, ldck, nullptr, 0, nullptr, 0, sc, &w);
+  return launch_maxmin_keys(h, A, B, M, N, tri, nullptr, 0, Ck, ldck, nullptr, 0, nullptr, 0, sc, &w, b0, b1);
+}
+
+// Row-block ends of the panels in which a hooked chain runs its last triangle product: 1/2, 3/4
+// and 7/8 of the row groups, then the rest, so that each panel's rows (its download) fit under
+// the next panel's product, whose share of the triangle shrinks with the row index.  One panel
+// below four row groups, with tiles dealt over XCDs, or with MCPG_MM_PANELS=0.
+std::vector<int64_t> row_panels(int64_t nb) {
+  const int64_t group = std::max(mcpg::env_group("MCPG_MM_GROUP", 8), 1);
+  const int64_t ng = ceil_div(nb, group);
+  std::vector<int64_t> ends;
+  if (mcpg::env_group("MCPG_MM_PANELS", 1) != 0 && mcpg::env_group("MCPG_MM_XCD", 0) == 0 && ng >= 4)
+    for (int64_t eighths : {4, 6, 7}) {
+      const int64_t e = std::min(nb, ceil_div(ng * eighths, 8) * group);
+      if (e < nb && (ends.empty() || e > ends.back())) ends.push_back(e);
+    }
+  ends.push_back(nb);
+  return ends;
 }
 
 // ---------------------------------------------------------------- FP64 padded-value path, host side
@@ -1446,7 +1477,7 @@ int run_vals(mcpg_ctx* h, const ValPlan& p, const double* A, int64_t lda, const 
 // fails the key test, and the caller runs the FP64 kernel.
 template <typename T>
 int maxmins_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, T* M1, int64_t ld1, T* M2, int64_t ld2,
-                 T* M3, int64_t ld3, bool* done = nullptr, const std::function<int(int)>* after = nullptr) {
+                 T* M3, int64_t ld3, bool* done = nullptr, const mcpg::ChainHook* after = nullptr) {
   constexpr bool f64 = std::is_same<T, double>::value;
   const size_t NN = (size_t)N * N;
   KeyMat ka = key_geometry(N, N), k1 = key_geometry(N, N);
@@ -1480,8 +1511,23 @@ int maxmins_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, T* 
     if ((st = keys_exact(h, &exact)) || !exact) return st;
   }
   if ((st = pad_keys(h, k1, N))) return st;
-  if ((st = key_product(h, ka, ka, N, N, 1, dM[0], ldM[0], k1.d, k1.ldk, nullptr, 0, nullptr, 0))) return st;
-  if (after && (st = (*after)(0))) return st;
+  // Product q; with a hook, the last product of a triangle runs in row panels, each handed on as
+  // soon as it is on the stream.
+  const int qlast = dM[2] ? 2 : dM[1] ? 1 : 0;
+  auto product = [&](int q, const KeyMat& x, const KeyMat& y, int tri, int* ck, int64_t ldck) -> int {
+    const int64_t nb = ceil_div(N, kBM);
+    const std::vector<int64_t> ends = after && tri && q == qlast ? row_panels(nb) : std::vector<int64_t>{nb};
+    int64_t b0 = 0;
+    for (const int64_t b1 : ends) {
+      int rc;
+      if ((rc = key_product(h, x, y, N, N, tri, dM[q], ldM[q], ck, ldck, nullptr, 0, nullptr, 0, nullptr, b0, b1)))
+        return rc;
+      if (after && (rc = (*after)(q, b0 * kBM, std::min(b1 * kBM, N)))) return rc;
+      b0 = b1;
+    }
+    return MCPG_OK;
+  };
+  if ((st = product(0, ka, ka, 1, k1.d, k1.ldk))) return st;
   if (dM[1]) {
     if ((st = mcpg::reset_flags(h))) return st;
     {
@@ -1490,13 +1536,9 @@ int maxmins_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, T* 
     }
     int flags[1];
     if ((st = mcpg::read_flags(h, flags, 1))) return st;
-    if ((st = key_product(h, ka, k1, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0, nullptr, 0, nullptr, 0))) return st;
-    if (after && (st = (*after)(1))) return st;
+    if ((st = product(1, ka, k1, flags[0] == 0, nullptr, 0))) return st;
   }
-  if (dM[2]) {
-    if ((st = key_product(h, k1, k1, N, N, 1, dM[2], ldM[2], nullptr, 0, nullptr, 0, nullptr, 0))) return st;
-    if (after && (st = (*after)(2))) return st;
-  }
+  if (dM[2] && (st = product(2, k1, k1, 1, nullptr, 0))) return st;
   if (host) {
     T* out[3] = {M1, M2, M3};
     const int64_t ldo[3] = {ld1, ld2, ld3};
@@ -1645,7 +1687,7 @@ int dpi_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T toler
 // compute_maxmins on padded FP64 values: A padded once, M1 written both to the caller and into a
 // padded operand of the other two products
 int maxmins_f64_vals(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N, double* M1, int64_t ld1,
-                     double* M2, int64_t ld2, double* M3, int64_t ld3, const std::function<int(int)>* after) {
+                     double* M2, int64_t ld2, double* M3, int64_t ld3, const mcpg::ChainHook* after) {
   const size_t NN = (size_t)N * N;
   ValMat va = val_geometry(N, N), v1 = val_geometry(N, N);
   int st;
@@ -1673,7 +1715,7 @@ int maxmins_f64_vals(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64
   if ((st = prep_vals(h, dA, la, N, nullptr, va))) return st;
   if ((st = pad_vals(h, v1, N))) return st;
   if ((st = launch_maxmin_vals(h, va, va, N, N, 1, dM[0], ldM[0], v1.d, v1.ld, nullptr, 0, nullptr, 0))) return st;
-  if (after && (st = (*after)(0))) return st;
+  if (after && (st = (*after)(0, 0, N))) return st;
   if (dM[1]) {
     if ((st = mcpg::reset_flags(h))) return st;
     {
@@ -1684,11 +1726,11 @@ int maxmins_f64_vals(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64
     if ((st = mcpg::read_flags(h, flags, 1))) return st;
     if ((st = launch_maxmin_vals(h, va, v1, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0, nullptr, 0, nullptr, 0)))
       return st;
-    if (after && (st = (*after)(1))) return st;
+    if (after && (st = (*after)(1, 0, N))) return st;
   }
   if (dM[2]) {
     if ((st = launch_maxmin_vals(h, v1, v1, N, N, 1, dM[2], ldM[2], nullptr, 0, nullptr, 0, nullptr, 0))) return st;
-    if (after && (st = (*after)(2))) return st;
+    if (after && (st = (*after)(2, 0, N))) return st;
   }
   if (host) {
     double* out[3] = {M1, M2, M3};
@@ -1701,7 +1743,7 @@ int maxmins_f64_vals(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64
 
 // compute_maxmins on FP64 values with the register-staged tile kernel
 int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N, double* M1, int64_t ld1, double* M2,
-                int64_t ld2, double* M3, int64_t ld3, const std::function<int(int)>* after = nullptr) {
+                int64_t ld2, double* M3, int64_t ld3, const mcpg::ChainHook* after = nullptr) {
   if (!f64_tile_kernel()) return maxmins_f64_vals(h, host, A, lda, N, M1, ld1, M2, ld2, M3, ld3, after);
   const size_t NN = (size_t)N * N;
   mcpg::Sizer sz;
@@ -1724,7 +1766,7 @@ int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N,
   }
   if (host || !M1) { dM[0] = cv.take<double>(NN); ldM[0] = N; }
   if ((st = launch_maxmin<TrF64>(h, dA, la, dA, la, nullptr, N, N, N, 1, dM[0], ldM[0], nullptr, 0, nullptr, 0))) return st;
-  if (after && (st = (*after)(0))) return st;
+  if (after && (st = (*after)(0, 0, N))) return st;
   if (dM[1]) {
     if ((st = mcpg::reset_flags(h))) return st;
     {
@@ -1736,13 +1778,13 @@ int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N,
     if ((st = launch_maxmin<TrF64>(h, dA, la, dM[0], ldM[0], nullptr, N, N, N, flags[0] == 0, dM[1], ldM[1], nullptr, 0,
                                    nullptr, 0)))
       return st;
-    if (after && (st = (*after)(1))) return st;
+    if (after && (st = (*after)(1, 0, N))) return st;
   }
   if (dM[2]) {
     if ((st = launch_maxmin<TrF64>(h, dM[0], ldM[0], dM[0], ldM[0], nullptr, N, N, N, 1, dM[2], ldM[2], nullptr, 0,
                                    nullptr, 0)))
       return st;
-    if (after && (st = (*after)(2))) return st;
+    if (after && (st = (*after)(2, 0, N))) return st;
   }
   if (host) {
     double* out[3] = {M1, M2, M3};
@@ -1755,7 +1797,7 @@ int maxmins_f64(mcpg_ctx* h, bool host, const double* A, int64_t lda, int64_t N,
 
 template <typename T>
 int maxmins_entry(mcpg_ctx* h, int ptrs, const T* A, int64_t lda, int64_t N, T* M1, int64_t ld1, T* M2, int64_t ld2,
-                  T* M3, int64_t ld3, const std::function<int(int)>* after = nullptr) {
+                  T* M3, int64_t ld3, const mcpg::ChainHook* after = nullptr) {
   MCPG_CHECK_HANDLE(h);
   mcpg::begin_call(h);
   if (N < 0) return mcpg::set_error(h, MCPG_ERR_SHAPE, "negative dimension");
@@ -2263,11 +2305,11 @@ void init_scratch(mcpg_ctx* h) {
 }
 
 int maxmins_chain(mcpg_ctx* h, const float* A, int64_t N, float* M1, float* M2, float* M3,
-                  const std::function<int(int)>* after) {
+                  const mcpg::ChainHook* after) {
   return maxmins_entry<float>(h, MCPG_DEVICE, A, N, N, M1, N, M2, N, M3, N, after);
 }
 int maxmins_chain(mcpg_ctx* h, const double* A, int64_t N, double* M1, double* M2, double* M3,
-                  const std::function<int(int)>* after) {
+                  const mcpg::ChainHook* after) {
   return maxmins_entry<double>(h, MCPG_DEVICE, A, N, N, M1, N, M2, N, M3, N, after);
 }
 

@@ -260,33 +260,44 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
     return st;
 
   // ---- MCP2/3/4 (compute_mcp{2,3,4}, or compute_mcp{2,3,4}_tfs with tf_gene_transition k - 1)
-  // With host outputs each order's scores come back on the side stream while the next product is
-  // computed; the downloads run one after another (they share the side stream's pinned buffers),
-  // and each order's device copy lives until the end of the call.
+  // With host outputs each order's scores (or row panel of them) come back on the side stream
+  // while the next product (or panel) is computed; the downloads run one after another (they share
+  // the side stream's pinned buffers), and each order's device copy lives until the end of the call.
   Buf bm1;
   T* dm1 = nullptr;   // masked A (x) A (gene-TF-gene), shared by the TF orders 3 and 4
   Buf bouts[3], btmp[3][2];
-  mcpg::Download out_dl[3];
+  constexpr int kMaxDownloads = 8;   // two orders whole, the last in at most four row panels
+  mcpg::Download dls[kMaxDownloads];
+  int ndl = 0;
+  auto download = [&](T* dst, int64_t ld, const T* src, int64_t rows) -> int {
+    int rc;
+    if (ndl == kMaxDownloads) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "too many chain downloads");
+    if ((rc = mi_dl.wait(h)) || (ndl > 0 && (rc = dls[ndl - 1].wait(h)))) return rc;
+    return dls[ndl++].start(h, dst, ld * sizeof(T), src, G * sizeof(T), G * sizeof(T), rows);
+  };
   // Without TFs and with two or more orders the max-min matrices are shared: M1 = A (x) A once,
   // M2 = A (x) M1 and M3 = M1 (x) M1 as asked (compute_maxmins, mcpnet_helper.hpp:436-460), and
   // MCP_k = ratio(A, M_{k-1}); each compute_mcp{k} alone would form A (x) A again.
   Buf bmx[3];
   T* dmx[3] = {nullptr, nullptr, nullptr};
   const bool shared = !tfs && (mcp2 ? 1 : 0) + (mcp3 ? 1 : 0) + (mcp4 ? 1 : 0) >= 2;
-  // Order k's scores from what is on the stream so far, then their download on the side stream.
-  auto emit = [&](int k) -> int {
+  // Rows [r0, r1) of order k's scores from what is on the stream so far, then their download on
+  // the side stream.  Only the shared chain hands on row panels; the other forms whole orders.
+  auto emit = [&](int k, int64_t r0, int64_t r1) -> int {
     T* out = outs[k - 1];
     if (!out) return MCPG_OK;
     Buf& bout = bouts[k - 2];
+    int st;
     T* dout = out;
     int64_t ldo = lds[k - 1];
-    int st;
-    if (host) {
-      if ((st = alloc(h, bout, (size_t)R * G, &dout))) return st;
+    if (host) {   // one device copy per order, filled panel by panel
+      if (!bout.p && (st = alloc(h, bout, (size_t)R * G, &dout))) return st;
+      dout = static_cast<T*>(bout.p);
       ldo = G;
     }
     if (shared) {
-      if ((st = Stage<T>::ratio(h, dmi, G, dmx[k - 2], G, G, G, dout, ldo))) return st;
+      if ((st = Stage<T>::ratio(h, dmi + r0 * G, G, dmx[k - 2] + r0 * G, G, r1 - r0, G, dout + r0 * ldo, ldo)))
+        return st;
     } else if (!tfs) {
       if ((st = Stage<T>::mcp(h, dmi, G, G, k, dout, ldo))) return st;
     } else {
@@ -312,28 +323,26 @@ int mcpnet_entry(mcpg_ctx* h, int ptrs, const T* X, int64_t G, int64_t S, int64_
       }
       if (st || (st = Stage<T>::ratio(h, dmit, G, dmm, G, R, G, dout, ldo))) return st;
     }
-    if (host) {
-      if ((st = mi_dl.wait(h))) return st;
-      for (int q = 0; q < k - 2; ++q)
-        if ((st = out_dl[q].wait(h))) return st;
-      if ((st = out_dl[k - 2].start(h, out, lds[k - 1] * sizeof(T), dout, G * sizeof(T), G * sizeof(T), R))) return st;
-    }
+    if (host && (st = download(out + r0 * lds[k - 1], lds[k - 1], dout + r0 * ldo, r1 - r0))) return st;
     return mcpg::finish(h, "mcpg_mcpnet");
   };
   if (shared) {
     for (int q = 0; q < 3; ++q)
       if ((q == 0 || outs[q + 1]) && (st = alloc(h, bmx[q], (size_t)G * G, &dmx[q]))) return st;
-    // order k is emitted as soon as M_{k-1} is on the stream, so that its ratio and download
-    // overlap the next product
-    const std::function<int(int)> after = [&](int q) { return emit(q + 2); };
-    if ((st = mcpg::maxmins_chain(h, dmi, G, dmx[0], dmx[1], dmx[2], &after))) return st;
+    // host outputs: order k is emitted as soon as M_{k-1} (or a row panel of the last product) is
+    // on the stream, so that its download overlaps the next product or panel; device outputs have
+    // nothing to overlap, and the ratios follow the whole products
+    const mcpg::ChainHook after = [&](int q, int64_t r0, int64_t r1) { return emit(q + 2, r0, r1); };
+    if ((st = mcpg::maxmins_chain(h, dmi, G, dmx[0], dmx[1], dmx[2], host ? &after : nullptr))) return st;
+    for (int k = 2; !host && k <= 4; ++k)
+      if ((st = emit(k, 0, G))) return st;
   } else {
     for (int k = 2; k <= 4; ++k)
-      if ((st = emit(k))) return st;
+      if ((st = emit(k, 0, R))) return st;
   }
   if ((st = mi_dl.wait(h))) return st;
-  for (int q = 0; q < 3; ++q)
-    if ((st = out_dl[q].wait(h))) return st;
+  for (int q = 0; q < ndl; ++q)
+    if ((st = dls[q].wait(h))) return st;
   return mcpg::finish(h, "mcpg_mcpnet");
 }
 

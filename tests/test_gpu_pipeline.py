@@ -78,6 +78,49 @@ def test_double_ap_chain_keeps_the_mi_in_double(handle):
     assert bits_equal(res[3], handle.mcp(MI, 3)[0])
 
 
+@pytest.mark.parametrize("G,orders,env", [
+    (4000, (2, 3, 4), {}),                                            # panels of 16, 8, 8 row blocks
+    (600, (2, 3), {"MCPG_MM_GROUP": "1"}),                            # panels of 3, 1, 1 row blocks
+    (600, (3, 4), {"MCPG_MM_GROUP": "1", "MCPG_MM_PANELS": "0"}),     # the last product whole
+])
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_last_product_in_row_panels(handle, torch_cuda, monkeypatch, G, orders, env, dt):
+    """The shared chain's last triangle product runs in row panels, each panel's scores formed and
+    (host buffers) sent down while the next panel runs; host and device buffers give the per-order
+    entry points' bits, including the partial last row block."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    X = grn.grn_expression(G, 64, 38)[0].astype(dt)
+    host = handle.mcpnet(X, mi_method=0, orders=orders, precision="bf16x3")
+    dev = handle.mcpnet(torch_cuda.from_numpy(X).cuda(), mi_method=0, orders=orders, precision="bf16x3")
+    MI = handle.bspline_mi(X.astype(np.float32), precision="bf16x3").astype(dt)
+    np.fill_diagonal(MI, 0.0)
+    assert bits_equal(host["mi"], MI)
+    for k in orders:
+        want = handle.mcp(MI, k)[0]
+        assert bits_equal(host[k], want), k
+        assert bits_equal(dev[k].cpu().numpy(), want), k
+
+
+@pytest.mark.parametrize("env", [{}, {"MCPG_MM64_TILE": "1"}])
+def test_double_ap_chain_shares_fp64_maxmin(handle, torch_cuda, monkeypatch, env):
+    """The double AP MI is not float-exact, so the shared chain of MCP2/3/4 runs the FP64 max-min
+    kernels (padded values, or the register-staged tile kernel) with each order formed after its
+    product; host and device buffers give the per-order entry points' bits."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    X = grn.grn_expression(70, 256, 37)[0].astype(np.float64)
+    host = handle.mcpnet(X, mi_method=1, noise_seed=5, orders=(2, 3, 4))
+    dev = handle.mcpnet(torch_cuda.from_numpy(X).cuda(), mi_method=1, noise_seed=5, orders=(2, 3, 4))
+    MI = host["mi"]
+    assert not np.array_equal(MI.astype(np.float32).astype(np.float64), MI)
+    assert bits_equal(dev["mi"].cpu().numpy(), MI)
+    for k in (2, 3, 4):
+        want = handle.mcp(MI, k)[0]
+        assert bits_equal(host[k], want), k
+        assert bits_equal(dev[k].cpu().numpy(), want), k
+
+
 def test_double_chain_widens_the_mi(handle):
     X = grn.grn_expression(60, 200, 33)[0].astype(np.float64)
     res = handle.mcpnet(X, mi_method=0, diagonal=0.5)
