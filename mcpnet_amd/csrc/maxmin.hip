@@ -2066,8 +2066,36 @@ int mcp_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, int k, 
     bool exact;
     if ((st = keys_exact(h, &exact)) || !exact) return st;
   }
+  // The last product (ratio fused).  Host outputs: a triangle runs in row panels, and each
+  // panel's rows go down on the side stream (one download after another: they share its pinned
+  // buffers) while the next panel runs; every download ends before the call returns.
+  mcpg::Download dls[8];
+  int ndl = 0;
+  auto last = [&](const KeyMat& x, const KeyMat& y, int tri) -> int {
+    const int64_t nb = ceil_div(N, kBM);
+    const std::vector<int64_t> ends = host && !sc && tri ? row_panels(nb) : std::vector<int64_t>{nb};
+    int64_t b0 = 0;
+    int rc;
+    for (const int64_t b1 : ends) {
+      if ((rc = key_product(h, x, y, N, N, tri, dM, lm, nullptr, 0, dA, la, dO, lo, sc, b0, b1))) return rc;
+      const int64_t r0 = b0 * kBM, rows = std::min(b1 * kBM, N) - r0;
+      for (int o = 0; host && o < (mm_out ? 2 : 1); ++o) {
+        if (ndl == 8) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "too many MCP downloads");
+        if (ndl > 0 && (rc = dls[ndl - 1].wait(h))) return rc;
+        T* dst = o ? mm_out : out;
+        const int64_t ld = o ? ldm : ldo;
+        const T* src = (o ? dM : dO) + r0 * N;
+        if ((rc = dls[ndl++].start(h, dst + r0 * ld, ld * sizeof(T), src, N * sizeof(T), N * sizeof(T), rows)))
+          return rc;
+      }
+      b0 = b1;
+    }
+    for (int q = 0; q < ndl; ++q)
+      if ((rc = dls[q].wait(h))) return rc;
+    return MCPG_OK;
+  };
   if (k == 2) {
-    if ((st = key_product(h, ka, ka, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
+    if ((st = last(ka, ka, 1))) return st;
   } else {
     k1.d = cv.take<int>(k1.rows_pad * k1.ldk);
     if ((st = pad_keys(h, k1, N))) return st;
@@ -2088,18 +2116,10 @@ int mcp_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, int k, 
       }
       int flags[1];
       if ((st = mcpg::read_flags(h, flags, 1))) return st;
-      const int sym = flags[0] == 0;
-      if ((st = key_product(h, ka, k1, N, N, sym, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
+      if ((st = last(ka, k1, flags[0] == 0))) return st;
     } else {
-      if ((st = key_product(h, k1, k1, N, N, 1, dM, lm, nullptr, 0, dA, la, dO, lo, sc))) return st;
+      if ((st = last(k1, k1, 1))) return st;
     }
-  }
-  if (host) {
-    if ((st = mcpg::copy_2d(h, out, ldo * sizeof(T), dO, N * sizeof(T), N * sizeof(T), N, hipMemcpyDeviceToHost)))
-      return st;
-    if (mm_out && (st = mcpg::copy_2d(h, mm_out, ldm * sizeof(T), dM, N * sizeof(T), N * sizeof(T), N,
-                                      hipMemcpyDeviceToHost)))
-      return st;
   }
   if (done) *done = true;
   return mcpg::finish(h, "mcpg_mcp");

@@ -381,3 +381,24 @@ def test_f64_key_route_tf_chain(handle):
     want = {2: mm(T, Ad, tfs), 3: mm(T, G1, tfs), 4: mm(mm(T, Ad, tfs), G1, tfs)}
     for k in (2, 3, 4):
         assert bits_equal(res[k], oracle.ratio(T, want[k])), k
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("k,sym", [(2, True), (3, True), (3, False), (4, True)])
+def test_mcp_host_row_panels(handle, torch_cuda, dt, k, sym):
+    """Host buffers at 4000 genes: a triangle last product runs in three row panels (16, 8 and 8
+    row blocks, the last one partial), the scores and the max-min matrix of each panel downloaded
+    while the next runs; an asymmetric A makes MCP3's last product a full grid, downloaded whole.
+    Both equal the bits of the device-buffer call, whose products run whole."""
+    X = grn.grn_expression(4000, 64, 39)[0]
+    A = handle.bspline_mi(X, precision="bf16x3")
+    np.fill_diagonal(A, 0.0)
+    if not sym:
+        A[5, 9] = np.nextafter(A[5, 9], np.float32(1))
+    A = A.astype(dt)
+    got, got_mm = handle.mcp(A, k, want_maxmin=True)
+    dev, dev_mm = handle.mcp(torch_cuda.from_numpy(A).cuda(), k, want_maxmin=True)
+    assert bits_equal(got, dev.cpu().numpy())
+    assert bits_equal(got_mm, dev_mm.cpu().numpy())
+    only = handle.mcp(A, k)[0]                      # scores alone: one download per panel
+    assert bits_equal(only, got)
