@@ -1511,6 +1511,21 @@ int maxmins_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, T* 
     if ((st = keys_exact(h, &exact)) || !exact) return st;
   }
   if ((st = pad_keys(h, k1, N))) return st;
+  // Host outputs: each matrix (the last one row panel by row panel) goes down on the side stream
+  // while the next product runs, one download after another (they share its pinned buffers).
+  mcpg::Download dls[8];
+  int ndl = 0;
+  const mcpg::ChainHook down = [&](int q, int64_t r0, int64_t r1) -> int {
+    T* const hout[3] = {M1, M2, M3};
+    const int64_t ldh[3] = {ld1, ld2, ld3};
+    int rc;
+    if (!hout[q]) return MCPG_OK;
+    if (ndl == 8) return mcpg::set_error(h, MCPG_ERR_UNSUPPORTED, "too many max-min downloads");
+    if (ndl > 0 && (rc = dls[ndl - 1].wait(h))) return rc;
+    return dls[ndl++].start(h, hout[q] + r0 * ldh[q], ldh[q] * sizeof(T), dM[q] + r0 * N, N * sizeof(T),
+                            N * sizeof(T), r1 - r0);
+  };
+  if (host) after = &down;
   // Product q; with a hook, the last product of a triangle runs in row panels, each handed on as
   // soon as it is on the stream.
   const int qlast = dM[2] ? 2 : dM[1] ? 1 : 0;
@@ -1539,14 +1554,8 @@ int maxmins_keys(mcpg_ctx* h, bool host, const T* A, int64_t lda, int64_t N, T* 
     if ((st = product(1, ka, k1, flags[0] == 0, nullptr, 0))) return st;
   }
   if (dM[2] && (st = product(2, k1, k1, 1, nullptr, 0))) return st;
-  if (host) {
-    T* out[3] = {M1, M2, M3};
-    const int64_t ldo[3] = {ld1, ld2, ld3};
-    for (int q = 0; q < 3; ++q)
-      if (out[q] && (st = mcpg::copy_2d(h, out[q], ldo[q] * sizeof(T), dM[q], N * sizeof(T), N * sizeof(T), N,
-                                        hipMemcpyDeviceToHost)))
-        return st;
-  }
+  for (int q = 0; q < ndl; ++q)
+    if ((st = dls[q].wait(h))) return st;
   if (done) *done = true;
   return mcpg::finish(h, "mcpg_maxmins");
 }

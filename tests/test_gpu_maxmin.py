@@ -402,3 +402,21 @@ def test_mcp_host_row_panels(handle, torch_cuda, dt, k, sym):
     assert bits_equal(got_mm, dev_mm.cpu().numpy())
     only = handle.mcp(A, k)[0]                      # scores alone: one download per panel
     assert bits_equal(only, got)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("want", [(True, True, True), (False, True, True), (True, False, True), (True, True, False),
+                                  (True, False, False)])
+def test_maxmins_host_downloads_overlap(handle, torch_cuda, dt, want):
+    """Host buffers at 4000 genes: each wanted matrix goes down while the next product runs and
+    the last product runs in row panels; every matrix equals the device-buffer call's bits."""
+    X = grn.grn_expression(4000, 64, 40)[0]
+    A = handle.bspline_mi(X, precision="bf16x3")
+    np.fill_diagonal(A, 0.0)
+    A = A.astype(dt)
+    got = handle.maxmins(A, want)
+    dev = handle.maxmins(torch_cuda.from_numpy(A).cuda(), want)
+    for g, d, w in zip(got, dev, want):
+        assert (g is None) == (not w) and (d is None) == (not w)
+        if w:
+            assert bits_equal(g, d.cpu().numpy())
