@@ -10,7 +10,9 @@
  * Conventions
  *   - Matrices are row-major, rows = genes; leading dimensions (ld*) are in ELEMENTS.
  *   - `ptrs` says where the matrix pointers live: MCPG_HOST (pageable or pinned host memory;
- *     the call copies in, computes and copies out before returning) or MCPG_DEVICE
+ *     the call copies in, computes and copies out before returning; the MCP, maxmins and
+ *     mcpnet entries copy each output out on a second stream while the next product, or row
+ *     panel of the last product, runs, and return once every copy has ended) or MCPG_DEVICE
  *     (device memory on the handle's device; work is ordered on the handle's stream and the
  *     call returns after the stream has drained, so errors are reported synchronously).
  *   - Return value: MCPG_OK (0) or a negative mcpg_status; nothing is thrown across the ABI.
